@@ -1,0 +1,174 @@
+/*
+ * mfl.h - C ABI of libmfl.so, the MI355X (gfx950) implementation of libCL's 3D SPH step
+ * (oclFluid3D) and key/value radix sort (oclRadixSort).
+ *
+ * Every entry point names the reference interface it replaces (paths relative to the reference root).
+ * Conventions, following the reference's (libCL/oclCommon.cpp:19,39-46, libCL/oclProgram.h:146-150):
+ *   - int-returning calls return 1 on success and 0 on failure; the failure text is available from
+ *     mfl_last_error() (thread local) instead of the global sStatusCL + Log(ERR) stream.
+ *   - device pointers are plain `void*` / `uint32_t*` into device memory of the program's device;
+ *     `stream` is a hipStream_t (NULL = the null stream). compute calls only enqueue work, the
+ *     caller synchronises (mfl_device_finish is clFinish).
+ *   - element layouts are the reference's: float4 (16 B, AoS) positions / velocities / forces,
+ *     uint32 cell keys and indices, two uint32[262144] bucket tables.
+ */
+#ifndef MFL_H
+#define MFL_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define MFL_ABI_VERSION 1
+
+/* libCL/phys/oclFluid3D.h:48-63 (13 x fp32, this order; 52 bytes) */
+typedef struct mfl_params {
+    float deltaTime;
+    float particleRadius;
+    float cellSize;
+    float mass;
+    float viscosity;
+    float pressure;
+    float density;
+    float spacing;
+    float stiffness;
+    float viscosityConstant;
+    float pressureConstant;
+    float kernelConstant;
+    float velocitylimit;
+} mfl_params;
+
+typedef struct mfl_fluid3d mfl_fluid3d;
+typedef struct mfl_radixsort mfl_radixsort;
+
+/* Buffer selectors for mfl_fluid3d_{get,set,read,write}_buffer.
+   POSITION/VELOCITY/FORCE: libCL/phys/oclFluid3D.h:34-40 (get/set{Position,Velocity,Force}Buffer);
+   SORTED_*, INDEX, PARAMS: oclFluid3D.h:42-46; CELL, CELL_START, CELL_END: the internal bfCell /
+   bfCellStart / bfCellEnd of oclFluid3D.h:92-94, exposed read-only for parity checks. */
+enum mfl_buffer_kind {
+    MFL_POSITION = 0,
+    MFL_VELOCITY = 1,
+    MFL_FORCE = 2,
+    MFL_SORTED_POSITION = 3,
+    MFL_SORTED_VELOCITY = 4,
+    MFL_INDEX = 5,
+    MFL_CELL = 6,
+    MFL_CELL_START = 7,
+    MFL_CELL_END = 8,
+    MFL_PARAMS = 9
+};
+
+/* ---- runtime ------------------------------------------------------------------------------------ */
+
+/* Last failure text of this thread ("" if none). Replaces sStatusCL/oclError()
+   (libCL/oclCommon.cpp:19,39-46) and the sticky Log(ERR) (libCL/oclObject.cpp:55-58). */
+const char* mfl_last_error(void);
+void mfl_clear_error(void);
+int mfl_abi_version(void);
+/* Selects the HIP device later create calls bind to (oclContext::getDevice(i), libCL/oclContext.h). */
+int mfl_set_device(int ordinal);
+/* clFinish on the program's queue (libCL/TEST/main.cpp:217): hipDeviceSynchronize. */
+int mfl_device_finish(void);
+
+/* ---- params ------------------------------------------------------------------------------------- */
+
+/* Defaults of clInitFluid (libCL/phys/oclFluid3D.cl:130-148), evaluated on the host. */
+void mfl_params_default(mfl_params* p);
+/* kernelConstant, viscosityConstant, pressureConstant from cellSize and viscosity, with the
+   expressions of libCL/phys/oclFluid3D.cl:124-128,145-147. Never applied implicitly. */
+void mfl_params_derive(mfl_params* p);
+
+/* ---- oclRadixSort (libCL/sort/oclRadixSort.h:26-28) ---------------------------------------------- */
+
+/* oclRadixSort::oclRadixSort(oclContext&, oclProgram*) (libCL/sort/oclRadixSort.cpp:27-48) */
+mfl_radixsort* mfl_radixsort_create(void);
+void mfl_radixsort_destroy(mfl_radixsort* s);
+/* oclProgram::compile (libCL/oclProgram.cpp:58-120): kernels ship precompiled for gfx950; this
+   verifies the device can run them. */
+int mfl_radixsort_compile(mfl_radixsort* s);
+/* oclRadixSort::compute(dev, key, val, startBit, endBit) (libCL/sort/oclRadixSort.cpp:62-144):
+   stable sort of n (key, value) pairs in place by bits [start_bit, end_bit) of the key. Rejects
+   (end_bit - start_bit) % 4 != 0 like the reference (cpp:76-80); the reference's n < 4,194,304 cap
+   (cpp:70-74) is lifted (n < 2^32). n == 0 or start_bit >= end_bit is a no-op. */
+int mfl_radixsort_compute(mfl_radixsort* s, uint32_t* keys, uint32_t* vals, size_t n, int start_bit,
+                          int end_bit, void* stream);
+
+/* ---- oclFluid3D (libCL/phys/oclFluid3D.h:25-74) -------------------------------------------------- */
+
+/* oclFluid3D::oclFluid3D (libCL/phys/oclFluid3D.cpp:21-83): owns position/velocity/force buffers for
+   the default 256 particles, bucket tables initialised to 0xFFFFFFFF (the reference leaves them
+   uninitialised), reference box [-1,1]^3 and gravity (0,0,1,0) (cpp:256-268). */
+mfl_fluid3d* mfl_fluid3d_create(void);
+/* oclFluid3D::~oclFluid3D (cpp:85-90): frees program-owned buffers only. */
+void mfl_fluid3d_destroy(mfl_fluid3d* f);
+/* oclFluid3D::compile (cpp:273-285): loads clInitFluid's default Params (host copy + device
+   param buffer). */
+int mfl_fluid3d_compile(mfl_fluid3d* f);
+/* oclFluid3D::compute (cpp:293-342): one SPH step enqueued on `stream`: hash, sort, reorder, bucket
+   clear, bounds, density, forces, then either the EVT_INTEGRATE callback or clip-box + integrate +
+   gravity + integrate-velocity. Fails (like a rejected NDRange) unless the particle count is a
+   non-zero multiple of 256. */
+int mfl_fluid3d_compute(mfl_fluid3d* f, void* stream);
+/* oclFluid3D::setParticleCount (cpp:92-124): re-creates every program-owned buffer (contents lost);
+   user buffers are not resized (the reference resizes them with a WARN): a later compute fails if a
+   user buffer is smaller than the new count. */
+int mfl_fluid3d_set_particle_count(mfl_fluid3d* f, size_t n);
+/* oclFluid3D::getParticleCount (cpp:126-129) */
+size_t mfl_fluid3d_get_particle_count(mfl_fluid3d* f);
+/* set{Position,Velocity,Force}Buffer (cpp:131-165): `ptr` is caller-owned device memory of `bytes`
+   bytes (>= 16 * particle count); ownership stays with the caller. Only POSITION, VELOCITY, FORCE. */
+int mfl_fluid3d_set_buffer(mfl_fluid3d* f, int kind, void* ptr, size_t bytes);
+/* get*Buffer (cpp:171-201): device pointer and byte size of any buffer kind. */
+int mfl_fluid3d_get_buffer(mfl_fluid3d* f, int kind, void** ptr, size_t* bytes);
+/* oclBuffer::write / read of a program buffer (libCL/oclBuffer.cpp:47-83), blocking. */
+int mfl_fluid3d_write_buffer(mfl_fluid3d* f, int kind, const void* host, size_t bytes);
+int mfl_fluid3d_read_buffer(mfl_fluid3d* f, int kind, void* host, size_t bytes);
+/* oclFluid3D::getParameters (cpp:204-207): a host struct, valid at any time, read by every compute
+   call (the device param buffer is refreshed from it on each step). */
+mfl_params* mfl_fluid3d_params(mfl_fluid3d* f);
+/* Addition: the clip box (hard-coded [-1,1]^3 in bindBuffers, cpp:260-268). */
+int mfl_fluid3d_set_box(mfl_fluid3d* f, const float* min4, const float* max4);
+/* Addition: the clGravity direction argument (hard-coded (0,0,1,0), cpp:256-258). */
+int mfl_fluid3d_set_gravity(mfl_fluid3d* f, const float* dir4);
+/* Addition: how many leading bucket entries are reset to 0xFFFFFFFF each step. Default 65,536,
+   which is what the reference's byte-sized fill of an element count resets (cpp:309-313);
+   262,144 resets the whole table. */
+int mfl_fluid3d_set_bucket_clear(mfl_fluid3d* f, uint32_t count);
+/* Addition: replay compute() from a captured hipGraph (re-captured whenever the particle count,
+   buffers, params, box, gravity or bucket clear change). Ignored while an integrate callback is set. */
+int mfl_fluid3d_set_use_graph(mfl_fluid3d* f, int enable);
+
+/* Per-stage kernel timing with HIP events on the compute stream: the analogue of oclKernel::profile(true)
+   + getStartTime/getEndTime (libCL/oclKernel.cpp:84-102,167-203). `stage_mask` has bit s set for each
+   MFL_STAGE_* to time (0 disables). While a mask is set, compute() launches eagerly (no graph replay). */
+enum mfl_stage {
+    MFL_STAGE_HASH = 0,
+    MFL_STAGE_SORT = 1,
+    MFL_STAGE_REORDER = 2,
+    MFL_STAGE_BOUNDS = 3,
+    MFL_STAGE_DENSITY = 4,
+    MFL_STAGE_FORCES = 5,
+    MFL_STAGE_COUNT = 6
+};
+int mfl_fluid3d_set_profiling(mfl_fluid3d* f, uint32_t stage_mask);
+/* Waits for the recorded events and returns the accumulated time (ms) and launch count of a stage
+   since the last reset; reset != 0 clears the accumulators afterwards. */
+int mfl_fluid3d_stage_time(mfl_fluid3d* f, int stage, double* total_ms, uint64_t* launches, int reset);
+
+/* addEventHandler(srtEvent named EVT_INTEGRATE = "OnIntegrate") (cpp:19,287-291,325-328): when set,
+   compute() calls fn after the force pass instead of clip-box + integration. NULL removes it. The
+   callback's return value is ignored, as in the reference. */
+typedef int (*mfl_integrate_fn)(mfl_fluid3d* f, void* stream, void* user);
+int mfl_fluid3d_set_integrate_callback(mfl_fluid3d* f, mfl_integrate_fn fn, void* user);
+/* getKernel(name) + clEnqueueNDRangeKernel with the program's bound arguments (libCL/oclProgram.h:78,
+   libCL/TEST/main.cpp:282-302). Names: clHash, clReorder, clFindBounds, clCalculateDensity,
+   clCalculateForces, clClipBox, clIntegrateForce, clGravity, clIntegrateVelocity. */
+int mfl_fluid3d_enqueue_kernel(mfl_fluid3d* f, const char* name, void* stream);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* MFL_H */

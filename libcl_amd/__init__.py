@@ -1,0 +1,10 @@
+"""libcl_amd: MI355X (gfx950) implementation of libCL's 3D SPH step (oclFluid3D) and key/value
+radix sort (oclRadixSort), behind the reference's program-object API.
+
+Native code: libcl_amd/csrc/*.hip -> libcl_amd/libmfl.so (C ABI in include/mfl.h).
+"""
+from ._native import Params, default_params, derive, last_error, load, params_from  # noqa: F401
+from .programs import DeviceBuffer, oclFluid3D, oclRadixSort  # noqa: F401
+
+__all__ = ["Params", "default_params", "derive", "last_error", "load", "params_from", "DeviceBuffer",
+           "oclFluid3D", "oclRadixSort"]

@@ -1,0 +1,137 @@
+"""ctypes binding of libmfl.so (the C ABI declared in include/mfl.h).
+
+There is no fallback: if the library is missing or cannot be loaded, every program object raises.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libmfl.so")
+
+MFL_POSITION, MFL_VELOCITY, MFL_FORCE = 0, 1, 2
+MFL_SORTED_POSITION, MFL_SORTED_VELOCITY, MFL_INDEX = 3, 4, 5
+MFL_CELL, MFL_CELL_START, MFL_CELL_END, MFL_PARAMS = 6, 7, 8, 9
+
+PARAM_FIELDS = (
+    "deltaTime", "particleRadius", "cellSize", "mass", "viscosity", "pressure", "density", "spacing",
+    "stiffness", "viscosityConstant", "pressureConstant", "kernelConstant", "velocitylimit",
+)
+
+# every symbol include/mfl.h declares
+EXPORTS = (
+    "mfl_last_error", "mfl_clear_error", "mfl_abi_version", "mfl_set_device", "mfl_device_finish",
+    "mfl_params_default", "mfl_params_derive",
+    "mfl_radixsort_create", "mfl_radixsort_destroy", "mfl_radixsort_compile", "mfl_radixsort_compute",
+    "mfl_fluid3d_create", "mfl_fluid3d_destroy", "mfl_fluid3d_compile", "mfl_fluid3d_compute",
+    "mfl_fluid3d_set_particle_count", "mfl_fluid3d_get_particle_count", "mfl_fluid3d_set_buffer",
+    "mfl_fluid3d_get_buffer", "mfl_fluid3d_write_buffer", "mfl_fluid3d_read_buffer", "mfl_fluid3d_params",
+    "mfl_fluid3d_set_box", "mfl_fluid3d_set_gravity", "mfl_fluid3d_set_bucket_clear",
+    "mfl_fluid3d_set_use_graph", "mfl_fluid3d_set_profiling", "mfl_fluid3d_stage_time",
+    "mfl_fluid3d_set_integrate_callback", "mfl_fluid3d_enqueue_kernel",
+)
+
+STAGES = ("hash", "sort", "reorder", "bounds", "density", "forces")
+
+
+class Params(ctypes.Structure):
+    """mfl_params == libCL/phys/oclFluid3D.h:48-63 (13 x fp32)."""
+    _fields_ = [(name, ctypes.c_float) for name in PARAM_FIELDS]
+
+    def as_dict(self) -> dict:
+        return {k: float(getattr(self, k)) for k in PARAM_FIELDS}
+
+    def values(self) -> list:
+        return [float(getattr(self, k)) for k in PARAM_FIELDS]
+
+
+INTEGRATE_FN = ctypes.CFUNCTYPE(ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p)
+
+_lib = None
+
+
+def load() -> ctypes.CDLL:
+    """Loads libmfl.so (built by libcl_amd.build); raises if it is absent."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise RuntimeError(f"{LIB_PATH} is missing: build it with `python -m libcl_amd.build` "
+                           "(there is no CPU fallback)")
+    # torch wheels bundle their own libamdhip64.so. Loading torch first makes libmfl.so bind to that
+    # same HIP runtime, so device pointers and streams from torch tensors are valid in libmfl; the
+    # other order leaves two HIP runtimes in the process and torch's device init fails.
+    try:
+        import torch  # noqa: F401
+    except ImportError:
+        pass
+    L = ctypes.CDLL(LIB_PATH)
+    vp, sz, u32p = ctypes.c_void_p, ctypes.c_size_t, ctypes.POINTER(ctypes.c_uint32)
+    pp = ctypes.POINTER(Params)
+    sig = {
+        "mfl_last_error": (ctypes.c_char_p, []),
+        "mfl_clear_error": (None, []),
+        "mfl_abi_version": (ctypes.c_int, []),
+        "mfl_set_device": (ctypes.c_int, [ctypes.c_int]),
+        "mfl_device_finish": (ctypes.c_int, []),
+        "mfl_params_default": (None, [pp]),
+        "mfl_params_derive": (None, [pp]),
+        "mfl_radixsort_create": (vp, []),
+        "mfl_radixsort_destroy": (None, [vp]),
+        "mfl_radixsort_compile": (ctypes.c_int, [vp]),
+        "mfl_radixsort_compute": (ctypes.c_int, [vp, vp, vp, sz, ctypes.c_int, ctypes.c_int, vp]),
+        "mfl_fluid3d_create": (vp, []),
+        "mfl_fluid3d_destroy": (None, [vp]),
+        "mfl_fluid3d_compile": (ctypes.c_int, [vp]),
+        "mfl_fluid3d_compute": (ctypes.c_int, [vp, vp]),
+        "mfl_fluid3d_set_particle_count": (ctypes.c_int, [vp, sz]),
+        "mfl_fluid3d_get_particle_count": (sz, [vp]),
+        "mfl_fluid3d_set_buffer": (ctypes.c_int, [vp, ctypes.c_int, vp, sz]),
+        "mfl_fluid3d_get_buffer": (ctypes.c_int, [vp, ctypes.c_int, ctypes.POINTER(vp), ctypes.POINTER(sz)]),
+        "mfl_fluid3d_write_buffer": (ctypes.c_int, [vp, ctypes.c_int, vp, sz]),
+        "mfl_fluid3d_read_buffer": (ctypes.c_int, [vp, ctypes.c_int, vp, sz]),
+        "mfl_fluid3d_params": (pp, [vp]),
+        "mfl_fluid3d_set_box": (ctypes.c_int, [vp, ctypes.POINTER(ctypes.c_float), ctypes.POINTER(ctypes.c_float)]),
+        "mfl_fluid3d_set_gravity": (ctypes.c_int, [vp, ctypes.POINTER(ctypes.c_float)]),
+        "mfl_fluid3d_set_bucket_clear": (ctypes.c_int, [vp, ctypes.c_uint32]),
+        "mfl_fluid3d_set_use_graph": (ctypes.c_int, [vp, ctypes.c_int]),
+        "mfl_fluid3d_set_profiling": (ctypes.c_int, [vp, ctypes.c_uint32]),
+        "mfl_fluid3d_stage_time": (ctypes.c_int, [vp, ctypes.c_int, ctypes.POINTER(ctypes.c_double),
+                                                  ctypes.POINTER(ctypes.c_uint64), ctypes.c_int]),
+        "mfl_fluid3d_set_integrate_callback": (ctypes.c_int, [vp, INTEGRATE_FN, vp]),
+        "mfl_fluid3d_enqueue_kernel": (ctypes.c_int, [vp, ctypes.c_char_p, vp]),
+    }
+    for name, (res, args) in sig.items():
+        fn = getattr(L, name)
+        fn.restype = res
+        fn.argtypes = args
+    _lib = L
+    return L
+
+
+def last_error() -> str:
+    return load().mfl_last_error().decode()
+
+
+def default_params() -> Params:
+    p = Params()
+    load().mfl_params_default(ctypes.byref(p))
+    return p
+
+
+def derive(p: Params) -> Params:
+    q = Params(*p.values())
+    load().mfl_params_derive(ctypes.byref(q))
+    return q
+
+
+def params_from(values: dict) -> Params:
+    """Params from the base fields in `values`; the three kernel constants are derived from cellSize
+    and viscosity unless present in `values`."""
+    p = default_params()
+    for k, v in values.items():
+        setattr(p, k, float(v))
+    if not {"viscosityConstant", "pressureConstant", "kernelConstant"} & set(values):
+        p = derive(p)
+    return p

@@ -1,0 +1,693 @@
+// The 3D SPH step of oclFluid3D (libCL/phys/oclFluid3D.{h,cpp,cl}) as hand-written gfx950 kernels.
+//
+// Data layout in HBM is the reference's (libCL/phys/oclFluid3D.cpp:52-80): float4 AoS position,
+// velocity, force in the caller's particle order; float4 sorted copies whose w lanes carry density
+// (position) and pressure (velocity); uint32 cell keys and sorted indices; two uint32[2^18] bucket
+// tables (cell start / end) that keep their contents between steps.
+//
+// Step (oclFluid3D::compute, cpp:293-342):
+//   k_hash          cell key per particle                                   (cl:57-64)
+//   sort            stable key/index sort over bits [0, 18)                 (cpp:299; keys < 2^18)
+//   k_reorder_clear gather sorted pos/vel + reset the first clear_count buckets (cl:66-72, cpp:309-313)
+//   k_find_bounds   start/end of each occupied bucket                       (cl:85-113)
+//   k_density       27-bucket neighbourhood, poly6, pressure                (cl:150-200)
+//   k_forces_step   27-bucket pressure + viscosity forces, clip box, clamp, Euler, gravity
+//                   (cl:202-255, 315-361, 258-271, 294-298, 284-292 fused; each particle's update
+//                    only touches its own force/velocity/position, so fusion is exact)
+// With an EVT_INTEGRATE callback the force pass runs alone (k_forces) and the callback follows.
+#include "fluid3d.hpp"
+
+#include <cstring>
+#include <sstream>
+
+namespace mfl {
+namespace {
+
+constexpr int kThreads = 256;
+
+__device__ inline float4 f4_add(float4 a, float4 b) { return make_float4(a.x + b.x, a.y + b.y, a.z + b.z, a.w + b.w); }
+__device__ inline float4 f4_scale(float4 a, float s) { return make_float4(a.x * s, a.y * s, a.z * s, a.w * s); }
+
+template <bool WRITE_INDEX>
+__global__ __launch_bounds__(kThreads) void k_hash(const float4* __restrict__ pos, uint32_t n, float h,
+                                                   uint32_t* __restrict__ cell, uint32_t* __restrict__ index,
+                                                   mfl_params* __restrict__ params_buf, mfl_params p)
+{
+    const uint32_t i = blockIdx.x * kThreads + threadIdx.x;
+    if (i == 0 && params_buf) *params_buf = p;  // keeps getParamBuffer() in step with the host Params
+    if (i >= n) return;
+    const float4 q = pos[i];
+    cell[i] = cell_hash(grid_coord(q.x, h), grid_coord(q.y, h), grid_coord(q.z, h));
+    if (WRITE_INDEX) index[i] = i;
+}
+
+// Sorted gather (cl:66-72) over [0, n) and the per-step bucket reset (cpp:309-313) over [0, clear).
+__global__ __launch_bounds__(kThreads) void k_reorder_clear(const uint32_t* __restrict__ index,
+                                                            const float4* __restrict__ pos,
+                                                            const float4* __restrict__ vel,
+                                                            float4* __restrict__ spos, float4* __restrict__ svel,
+                                                            uint32_t n, uint32_t* __restrict__ cell_start,
+                                                            uint32_t* __restrict__ cell_end, uint32_t clear)
+{
+    const uint32_t i = blockIdx.x * kThreads + threadIdx.x;
+    if (i < n) {
+        const uint32_t s = index[i];
+        if (s < n) {  // index is a permutation of [0, n)
+            spos[i] = pos[s];
+            svel[i] = vel[s];
+        }
+    }
+    if (i < clear) {
+        cell_start[i] = kEmpty;
+        cell_end[i] = kEmpty;
+    }
+}
+
+__global__ __launch_bounds__(kThreads) void k_fill_buckets(uint32_t* __restrict__ cell_start,
+                                                           uint32_t* __restrict__ cell_end, uint32_t count)
+{
+    const uint32_t i = blockIdx.x * kThreads + threadIdx.x;
+    if (i < count) {
+        cell_start[i] = kEmpty;
+        cell_end[i] = kEmpty;
+    }
+}
+
+// cl:85-113. Every occupied bucket gets both its start and its end written; empty buckets keep
+// whatever they held (0xFFFFFFFF if reset this step, a stale range otherwise).
+__global__ __launch_bounds__(kThreads) void k_find_bounds(const uint32_t* __restrict__ cell, uint32_t n,
+                                                          uint32_t* __restrict__ cell_start,
+                                                          uint32_t* __restrict__ cell_end)
+{
+    const uint32_t i = blockIdx.x * kThreads + threadIdx.x;
+    if (i >= n) return;
+    const uint32_t h = cell[i] & (kBuckets - 1u);  // keys are < 2^18 by construction
+    if (i == 0) {
+        cell_start[h] = 0;
+        return;
+    }
+    const uint32_t prev = cell[i - 1] & (kBuckets - 1u);
+    if (h != prev) {
+        cell_end[prev] = i;
+        cell_start[h] = i;
+    }
+    if (i == n - 1) cell_end[h] = n;
+}
+
+// Distance test of cl:177-178 / cl:232-233: vlen = sqrt(x^2+y^2+z^2) < h. A candidate whose squared
+// distance exceeds h^2 by more than the sqrt rounding can affect is rejected without the sqrt.
+__device__ inline bool within(float r2, float h, float h2_hi, float* vlen)
+{
+    if (r2 > h2_hi) return false;
+    *vlen = sqrtf(r2);
+    return *vlen < h;
+}
+
+__global__ __launch_bounds__(kThreads) void k_density(float4* __restrict__ spos, float4* __restrict__ svel,
+                                                      const uint32_t* __restrict__ cell_start,
+                                                      const uint32_t* __restrict__ cell_end, uint32_t n,
+                                                      mfl_params p)
+{
+    const uint32_t i = blockIdx.x * kThreads + threadIdx.x;
+    if (i >= n) return;
+    const float h = p.cellSize;
+    const float h2_hi = h * h * 1.0000005f;
+    const float4 pi = spos[i];
+    const int gx = grid_coord(pi.x, h), gy = grid_coord(pi.y, h), gz = grid_coord(pi.z, h);
+    float acc = 0.f;
+    int count = 0;
+    for (int z = -1; z <= 1; z++)
+        for (int y = -1; y <= 1; y++)
+            for (int x = -1; x <= 1; x++) {
+                const uint32_t c = cell_hash(gx + x, gy + y, gz + z);
+                const uint32_t start = cell_start[c];
+                if (start == kEmpty) continue;
+                const uint32_t end = min(cell_end[c], n);  // stale or written ranges always lie in [0, n]
+                for (uint32_t j = start; j < end; j++) {
+                    if (j == i) continue;
+                    const float4 pj = spos[j];
+                    const float dx = pi.x - pj.x, dy = pi.y - pj.y, dz = pi.z - pj.z;
+                    float vlen;
+                    if (within(dx * dx + dy * dy + dz * dz, h, h2_hi, &vlen)) {
+                        const float t = h * h - vlen * vlen;
+                        acc += t * t * t;
+                        count++;
+                    }
+                }
+            }
+    float rho, prs;
+    if (count > 2) {
+        rho = fmaxf(1.0f, p.mass * p.kernelConstant * acc);
+        prs = p.pressure + p.stiffness * (rho - p.density);
+    } else {
+        rho = p.density;
+        prs = p.pressure;
+    }
+    spos[i].w = rho;
+    svel[i].w = prs;
+}
+
+// Sum of the pressure and viscosity terms of cl:213-249 for sorted particle i.
+__device__ inline void sph_force_sum(const float4* __restrict__ spos, const float4* __restrict__ svel,
+                                     const uint32_t* __restrict__ cell_start, const uint32_t* __restrict__ cell_end,
+                                     uint32_t i, uint32_t n, const float4& pi, const float4& vi, float h, float3& P,
+                                     float3& V)
+{
+    const float h2_hi = h * h * 1.0000005f;
+    const int gx = grid_coord(pi.x, h), gy = grid_coord(pi.y, h), gz = grid_coord(pi.z, h);
+    P = make_float3(0.f, 0.f, 0.f);
+    V = make_float3(0.f, 0.f, 0.f);
+    for (int z = -1; z <= 1; z++)
+        for (int y = -1; y <= 1; y++)
+            for (int x = -1; x <= 1; x++) {
+                const uint32_t c = cell_hash(gx + x, gy + y, gz + z);
+                const uint32_t start = cell_start[c];
+                if (start == kEmpty) continue;
+                const uint32_t end = min(cell_end[c], n);  // stale or written ranges always lie in [0, n]
+                for (uint32_t j = start; j < end; j++) {
+                    if (j == i) continue;
+                    const float4 pj = spos[j];
+                    const float dx = pi.x - pj.x, dy = pi.y - pj.y, dz = pi.z - pj.z;
+                    float vlen;
+                    if (within(dx * dx + dy * dy + dz * dz, h, h2_hi, &vlen)) {
+                        const float4 vj = svel[j];
+                        const float density2 = pi.w * pj.w;
+                        const float q = (vi.w + vj.w) / density2;
+                        const float hr = h - vlen;
+                        P.x += q * ((dx / vlen) * hr * hr);
+                        P.y += q * ((dy / vlen) * hr * hr);
+                        P.z += q * ((dz / vlen) * hr * hr);
+                        V.x += ((vj.x - vi.x) / density2) * hr;
+                        V.y += ((vj.y - vi.y) / density2) * hr;
+                        V.z += ((vj.z - vi.z) / density2) * hr;
+                    }
+                }
+            }
+}
+
+__device__ inline float4 add_sph_force(float4 f, const float3& P, const float3& V, const mfl_params& p)
+{
+    f.x += (p.pressureConstant * P.x + p.viscosityConstant * V.x) * p.mass;
+    f.y += (p.pressureConstant * P.y + p.viscosityConstant * V.y) * p.mass;
+    f.z += (p.pressureConstant * P.z + p.viscosityConstant * V.z) * p.mass;
+    f.w = 0.f;  // cl:253
+    return f;
+}
+
+// repulsionForce (cl:309-312) accumulated as in cl:327-357.
+__device__ inline void add_repulsion(float4& f, float4 nrm, float4 vel, float dist)
+{
+    const float dot = nrm.x * vel.x + nrm.y * vel.y + nrm.z * vel.z + nrm.w * vel.w;
+    const float s = 2000.0f * dist - 256.0f * dot;
+    f = f4_add(f, f4_scale(nrm, s));
+}
+
+// clClipBox body (cl:319-358) on a private force value.
+__device__ inline float4 clip_box(float4 f, float4 pos, float4 vel, float r, float4 mn, float4 mx)
+{
+    constexpr float eps = 0.00001f;  // cl:304
+    float diff = r - (pos.y - mn.y);
+    if (diff > eps) add_repulsion(f, make_float4(0.f, 1.f, 0.f, 0.f), vel, diff);
+    diff = r - (mx.y - pos.y);
+    if (diff > eps) add_repulsion(f, make_float4(0.f, -1.f, 0.f, 0.f), vel, diff);
+    diff = r - (pos.z - mn.z);
+    if (diff > eps) add_repulsion(f, make_float4(0.f, 0.f, 1.f, 0.f), vel, diff);
+    diff = r - (mx.z - pos.z);
+    if (diff > eps) add_repulsion(f, make_float4(0.f, 0.f, -1.f, 0.f), vel, diff);
+    diff = r - (pos.x - mn.x);
+    if (diff > eps) add_repulsion(f, make_float4(1.f, 0.f, 0.f, 0.f), vel, diff);
+    diff = r - (mx.x - pos.x);
+    if (diff > eps) add_repulsion(f, make_float4(-1.f, 0.f, 0.f, 0.f), vel, diff);
+    return f;
+}
+
+// clIntegrateForce body (cl:262-270): clamp a private copy of the force, then v += f * dt.
+__device__ inline float4 integrate_force(float4 vel, float4 f, const mfl_params& p)
+{
+    const float speed = sqrtf(f.x * f.x + f.y * f.y + f.z * f.z + f.w * f.w);
+    if (speed > p.velocitylimit) f = f4_scale(f, p.velocitylimit / speed);
+    return f4_add(vel, f4_scale(f, p.deltaTime));
+}
+
+__global__ __launch_bounds__(kThreads) void k_forces(const float4* __restrict__ spos, const float4* __restrict__ svel,
+                                                     float4* __restrict__ force, const uint32_t* __restrict__ index,
+                                                     const uint32_t* __restrict__ cell_start,
+                                                     const uint32_t* __restrict__ cell_end, uint32_t n, mfl_params p)
+{
+    const uint32_t i = blockIdx.x * kThreads + threadIdx.x;
+    if (i >= n) return;
+    const float4 pi = spos[i], vi = svel[i];
+    float3 P, V;
+    sph_force_sum(spos, svel, cell_start, cell_end, i, n, pi, vi, p.cellSize, P, V);
+    const uint32_t o = index[i];
+    if (o >= n) return;  // index is a permutation of [0, n)
+    force[o] = add_sph_force(force[o], P, V, p);
+}
+
+// Forces + clip box + integrate force + gravity + integrate velocity for sorted particle i, written
+// to the caller-order slot o = index[i].
+__global__ __launch_bounds__(kThreads) void k_forces_step(const float4* __restrict__ spos,
+                                                          const float4* __restrict__ svel, float4* __restrict__ pos,
+                                                          float4* __restrict__ vel, float4* __restrict__ force,
+                                                          const uint32_t* __restrict__ index,
+                                                          const uint32_t* __restrict__ cell_start,
+                                                          const uint32_t* __restrict__ cell_end, uint32_t n,
+                                                          mfl_params p, float4 box_min, float4 box_max,
+                                                          float4 gravity_force)
+{
+    const uint32_t i = blockIdx.x * kThreads + threadIdx.x;
+    if (i >= n) return;
+    const float4 pi = spos[i], vi = svel[i];
+    float3 P, V;
+    sph_force_sum(spos, svel, cell_start, cell_end, i, n, pi, vi, p.cellSize, P, V);
+    const uint32_t o = index[i];
+    if (o >= n) return;
+    float4 f = add_sph_force(force[o], P, V, p);
+    f = clip_box(f, pi, vi, p.particleRadius, box_min, box_max);
+    const float4 v = integrate_force(vel[o], f, p);
+    force[o] = gravity_force;
+    vel[o] = v;
+    pos[o] = f4_add(pos[o], f4_scale(v, p.deltaTime));
+}
+
+__global__ __launch_bounds__(kThreads) void k_clip_box(const float4* __restrict__ spos, const float4* __restrict__ svel,
+                                                       float4* __restrict__ force, const uint32_t* __restrict__ index,
+                                                       uint32_t n, mfl_params p, float4 box_min, float4 box_max)
+{
+    const uint32_t i = blockIdx.x * kThreads + threadIdx.x;
+    if (i >= n) return;
+    const uint32_t o = index[i];
+    if (o >= n) return;
+    force[o] = clip_box(force[o], spos[i], svel[i], p.particleRadius, box_min, box_max);
+}
+
+__global__ __launch_bounds__(kThreads) void k_integrate_force(float4* __restrict__ vel, const float4* __restrict__ force,
+                                                              uint32_t n, mfl_params p)
+{
+    const uint32_t i = blockIdx.x * kThreads + threadIdx.x;
+    if (i >= n) return;
+    vel[i] = integrate_force(vel[i], force[i], p);
+}
+
+__global__ __launch_bounds__(kThreads) void k_gravity(float4* __restrict__ force, uint32_t n, float4 gravity_force)
+{
+    const uint32_t i = blockIdx.x * kThreads + threadIdx.x;
+    if (i < n) force[i] = gravity_force;
+}
+
+__global__ __launch_bounds__(kThreads) void k_integrate_velocity(float4* __restrict__ pos,
+                                                                 const float4* __restrict__ vel, uint32_t n,
+                                                                 mfl_params p)
+{
+    const uint32_t i = blockIdx.x * kThreads + threadIdx.x;
+    if (i >= n) return;
+    pos[i] = f4_add(pos[i], f4_scale(vel[i], p.deltaTime));
+}
+
+inline dim3 grid_for(size_t n) { return dim3((unsigned)((n + kThreads - 1) / kThreads)); }
+
+inline float4 to_f4(const float* v) { return make_float4(v[0], v[1], v[2], v[3]); }
+
+// clGravity writes -9.8f * gravity (cl:297)
+inline float4 gravity_force_of(const float* g)
+{
+    return make_float4(-9.8f * g[0], -9.8f * g[1], -9.8f * g[2], -9.8f * g[3]);
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------------------------------------------
+
+StageTimer::~StageTimer()
+{
+    drain();
+    for (hipEvent_t e : pool_) (void)hipEventDestroy(e);
+    for (hipEvent_t e : open_)
+        if (e) (void)hipEventDestroy(e);
+}
+
+hipEvent_t StageTimer::take()
+{
+    if (!pool_.empty()) {
+        hipEvent_t e = pool_.back();
+        pool_.pop_back();
+        return e;
+    }
+    hipEvent_t e = nullptr;
+    if (hipEventCreate(&e) != hipSuccess) return nullptr;
+    return e;
+}
+
+bool StageTimer::begin(int stage, hipStream_t s)
+{
+    if (!(mask & (1u << stage))) return false;
+    hipEvent_t e = take();
+    if (!e) return false;
+    (void)hipEventRecord(e, s);
+    open_[stage] = e;
+    return true;
+}
+
+void StageTimer::end(int stage, hipStream_t s)
+{
+    hipEvent_t a = open_[stage];
+    if (!a) return;
+    open_[stage] = nullptr;
+    hipEvent_t b = take();
+    if (!b) {
+        pool_.push_back(a);
+        return;
+    }
+    (void)hipEventRecord(b, s);
+    pending_.push_back({stage, a, b});
+}
+
+bool StageTimer::drain()
+{
+    bool ok = true;
+    for (const Pending& p : pending_) {
+        float ms = 0.f;
+        if (hipEventSynchronize(p.b) == hipSuccess && hipEventElapsedTime(&ms, p.a, p.b) == hipSuccess) {
+            ms_[p.stage] += ms;
+            count_[p.stage] += 1;
+        } else {
+            ok = false;
+        }
+        pool_.push_back(p.a);
+        pool_.push_back(p.b);
+    }
+    pending_.clear();
+    return ok;
+}
+
+int StageTimer::read(int stage, double* ms, uint64_t* launches, bool reset)
+{
+    if (stage < 0 || stage >= MFL_STAGE_COUNT) {
+        set_error("unknown stage " + std::to_string(stage));
+        return 0;
+    }
+    if (!drain()) {
+        set_error("stage timing events failed");
+        return 0;
+    }
+    if (ms) *ms = ms_[stage];
+    if (launches) *launches = count_[stage];
+    if (reset) {
+        for (int i = 0; i < MFL_STAGE_COUNT; i++) {
+            ms_[i] = 0.0;
+            count_[i] = 0;
+        }
+    }
+    return 1;
+}
+
+Fluid3D::Fluid3D()
+{
+    mfl_params_default(&params);
+    if (!params_buf_.allocate(64)) return;
+    allocate_internal();
+}
+
+Fluid3D::~Fluid3D()
+{
+    drop_graph();
+    for (DeviceBuffer* b : {&pos_, &vel_, &force_, &cell_, &index_, &spos_, &svel_, &cell_start_, &cell_end_,
+                            &params_buf_})
+        b->release();
+}
+
+bool Fluid3D::allocate_internal()
+{
+    drop_graph();
+    const size_t v4 = n_ * 16, u1 = n_ * 4;
+    if (!cell_.allocate(u1) || !index_.allocate(u1) || !spos_.allocate(v4) || !svel_.allocate(v4)) return false;
+    if (!cell_start_.allocate(kBuckets * 4) || !cell_end_.allocate(kBuckets * 4)) return false;
+    // the reference leaves the tables uninitialised (cpp:53-54); here they start empty
+    if (!hip_ok(hipMemset(cell_start_.ptr, 0xFF, kBuckets * 4), "hipMemset") ||
+        !hip_ok(hipMemset(cell_end_.ptr, 0xFF, kBuckets * 4), "hipMemset"))
+        return false;
+    for (DeviceBuffer* b : {&pos_, &vel_, &force_}) {
+        if (!b->owned) continue;
+        if (!b->allocate(v4)) return false;
+        if (v4 && !hip_ok(hipMemset(b->ptr, 0, v4), "hipMemset")) return false;
+    }
+    if (!hip_ok(hipMemset(spos_.ptr, 0, v4), "hipMemset") || !hip_ok(hipMemset(svel_.ptr, 0, v4), "hipMemset"))
+        return false;
+    return hip_ok(hipDeviceSynchronize(), "hipDeviceSynchronize");
+}
+
+int Fluid3D::compile()
+{
+    if (!sort.compile()) return 0;
+    mfl_params_default(&params);  // clInitFluid (cl:130-148)
+    MFL_HIP(hipMemcpy(params_buf_.ptr, &params, sizeof(mfl_params), hipMemcpyHostToDevice));
+    return 1;
+}
+
+int Fluid3D::set_particle_count(size_t n)
+{
+    if (n > 0xFFFFFFFFull / 16) {
+        set_error("particle count too large");
+        return 0;
+    }
+    n_ = n;
+    return allocate_internal() ? 1 : 0;
+}
+
+DeviceBuffer* Fluid3D::select(int kind)
+{
+    switch (kind) {
+    case MFL_POSITION: return &pos_;
+    case MFL_VELOCITY: return &vel_;
+    case MFL_FORCE: return &force_;
+    case MFL_SORTED_POSITION: return &spos_;
+    case MFL_SORTED_VELOCITY: return &svel_;
+    case MFL_INDEX: return &index_;
+    case MFL_CELL: return &cell_;
+    case MFL_CELL_START: return &cell_start_;
+    case MFL_CELL_END: return &cell_end_;
+    case MFL_PARAMS: return &params_buf_;
+    default: set_error("unknown buffer kind " + std::to_string(kind)); return nullptr;
+    }
+}
+
+int Fluid3D::set_buffer(int kind, void* ptr, size_t bytes)
+{
+    if (kind != MFL_POSITION && kind != MFL_VELOCITY && kind != MFL_FORCE) {
+        set_error("only the position, velocity and force buffers can be replaced");
+        return 0;
+    }
+    if (!ptr) {
+        set_error("null buffer");
+        return 0;
+    }
+    if (bytes < n_ * 16) {
+        set_error("buffer of " + std::to_string(bytes) + " bytes is smaller than " + std::to_string(n_) +
+                  " float4 particles");
+        return 0;
+    }
+    DeviceBuffer* b = select(kind);
+    drop_graph();
+    b->release();  // frees a program-owned buffer only (deleteBuffer, libCL/phys/oclFluid3D.h:106-112)
+    b->ptr = ptr;
+    b->bytes = bytes;
+    b->owned = false;
+    return 1;
+}
+
+int Fluid3D::get_buffer(int kind, void** ptr, size_t* bytes)
+{
+    DeviceBuffer* b = select(kind);
+    if (!b) return 0;
+    if (ptr) *ptr = b->ptr;
+    if (bytes) *bytes = kind == MFL_PARAMS ? sizeof(mfl_params) : b->bytes;
+    return 1;
+}
+
+int Fluid3D::write_buffer(int kind, const void* host, size_t bytes)
+{
+    DeviceBuffer* b = select(kind);
+    if (!b) return 0;
+    if (bytes > b->bytes) {
+        set_error("write of " + std::to_string(bytes) + " bytes exceeds the buffer");
+        return 0;
+    }
+    MFL_HIP(hipMemcpy(b->ptr, host, bytes, hipMemcpyHostToDevice));
+    return 1;
+}
+
+int Fluid3D::read_buffer(int kind, void* host, size_t bytes)
+{
+    DeviceBuffer* b = select(kind);
+    if (!b) return 0;
+    if (bytes > b->bytes) {
+        set_error("read of " + std::to_string(bytes) + " bytes exceeds the buffer");
+        return 0;
+    }
+    MFL_HIP(hipDeviceSynchronize());
+    MFL_HIP(hipMemcpy(host, b->ptr, bytes, hipMemcpyDeviceToHost));
+    return 1;
+}
+
+bool Fluid3D::check_user_buffers()
+{
+    if (n_ == 0 || n_ % kGroup != 0) {
+        set_error("particle count " + std::to_string(n_) + " is not a non-zero multiple of 256 (cLocalSize)");
+        return false;
+    }
+    for (DeviceBuffer* b : {&pos_, &vel_, &force_}) {
+        if (!b->ptr || b->bytes < n_ * 16) {
+            set_error("a position/velocity/force buffer is smaller than the particle count");
+            return false;
+        }
+    }
+    return true;
+}
+
+int Fluid3D::enqueue_step(hipStream_t s)
+{
+    const uint32_t n = (uint32_t)n_;
+    const dim3 g = grid_for(n), b(kThreads);
+    const float4 mn = to_f4(box_min), mx = to_f4(box_max), gf = gravity_force_of(gravity);
+    auto* pos = pos_.as<float4>();
+    auto* vel = vel_.as<float4>();
+    auto* force = force_.as<float4>();
+    auto* spos = spos_.as<float4>();
+    auto* svel = svel_.as<float4>();
+    auto* cell = cell_.as<uint32_t>();
+    auto* index = index_.as<uint32_t>();
+    auto* cs = cell_start_.as<uint32_t>();
+    auto* ce = cell_end_.as<uint32_t>();
+    const uint32_t clear = clear_count < kBuckets ? clear_count : kBuckets;
+
+    timer.begin(MFL_STAGE_HASH, s);
+    hipLaunchKernelGGL(k_hash<false>, g, b, 0, s, pos, n, params.cellSize, cell, nullptr,
+                       params_buf_.as<mfl_params>(), params);
+    timer.end(MFL_STAGE_HASH, s);
+    MFL_HIP(hipGetLastError());
+    // the index buffer starts as the identity (cl:63) and is generated inside the first pass
+    timer.begin(MFL_STAGE_SORT, s);
+    if (!sort.sort_bits(cell, index, n, 0, 18, s, true)) return 0;
+    timer.end(MFL_STAGE_SORT, s);
+    timer.begin(MFL_STAGE_REORDER, s);
+    hipLaunchKernelGGL(k_reorder_clear, grid_for(n > clear ? n : clear), b, 0, s, index, pos, vel, spos, svel, n, cs,
+                       ce, clear);
+    timer.end(MFL_STAGE_REORDER, s);
+    timer.begin(MFL_STAGE_BOUNDS, s);
+    hipLaunchKernelGGL(k_find_bounds, g, b, 0, s, cell, n, cs, ce);
+    timer.end(MFL_STAGE_BOUNDS, s);
+    timer.begin(MFL_STAGE_DENSITY, s);
+    hipLaunchKernelGGL(k_density, g, b, 0, s, spos, svel, cs, ce, n, params);
+    timer.end(MFL_STAGE_DENSITY, s);
+    timer.begin(MFL_STAGE_FORCES, s);
+    if (integrate_cb) {
+        hipLaunchKernelGGL(k_forces, g, b, 0, s, spos, svel, force, index, cs, ce, n, params);
+        timer.end(MFL_STAGE_FORCES, s);
+        MFL_HIP(hipGetLastError());
+        (void)integrate_cb(handle, (void*)s, integrate_user);  // return value ignored, cpp:325-328
+        return 1;
+    }
+    hipLaunchKernelGGL(k_forces_step, g, b, 0, s, spos, svel, pos, vel, force, index, cs, ce, n, params, mn, mx, gf);
+    timer.end(MFL_STAGE_FORCES, s);
+    MFL_HIP(hipGetLastError());
+    return 1;
+}
+
+void Fluid3D::drop_graph()
+{
+    if (graph_exec_) (void)hipGraphExecDestroy(graph_exec_);
+    if (graph_) (void)hipGraphDestroy(graph_);
+    graph_exec_ = nullptr;
+    graph_ = nullptr;
+    graph_sig_.clear();
+}
+
+int Fluid3D::launch_graph(hipStream_t s)
+{
+    std::ostringstream sig;
+    sig << n_ << ':' << pos_.ptr << ':' << vel_.ptr << ':' << force_.ptr << ':' << clear_count << ':'
+        << sort.tuning.items << ':';
+    const unsigned char* raw[] = {reinterpret_cast<const unsigned char*>(&params),
+                                  reinterpret_cast<const unsigned char*>(box_min),
+                                  reinterpret_cast<const unsigned char*>(box_max),
+                                  reinterpret_cast<const unsigned char*>(gravity)};
+    const size_t len[] = {sizeof(params), sizeof(box_min), sizeof(box_max), sizeof(gravity)};
+    for (int k = 0; k < 4; k++)
+        for (size_t i = 0; i < len[k]; i++) sig << std::hex << (int)raw[k][i] << ',';
+    if (!graph_exec_ || sig.str() != graph_sig_) {
+        drop_graph();
+        if (!sort.reserve(n_, 18)) return 0;
+        MFL_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeRelaxed));
+        const int ok = enqueue_step(s);
+        hipGraph_t g = nullptr;
+        const hipError_t e = hipStreamEndCapture(s, &g);
+        if (!ok || e != hipSuccess) {
+            if (g) (void)hipGraphDestroy(g);
+            if (ok) hip_ok(e, "hipStreamEndCapture");
+            return 0;
+        }
+        graph_ = g;
+        MFL_HIP(hipGraphInstantiate(&graph_exec_, graph_, nullptr, nullptr, 0));
+        graph_sig_ = sig.str();
+    }
+    MFL_HIP(hipGraphLaunch(graph_exec_, s));
+    return 1;
+}
+
+int Fluid3D::compute(hipStream_t s)
+{
+    if (!check_user_buffers()) return 0;
+    if (use_graph && !integrate_cb && s != nullptr && timer.mask == 0) return launch_graph(s);
+    return enqueue_step(s);
+}
+
+int Fluid3D::enqueue_kernel(const char* name, hipStream_t s)
+{
+    if (!name) {
+        set_error("null kernel name");
+        return 0;
+    }
+    if (!check_user_buffers()) return 0;
+    const uint32_t n = (uint32_t)n_;
+    const dim3 g = grid_for(n), b(kThreads);
+    const float4 mn = to_f4(box_min), mx = to_f4(box_max), gf = gravity_force_of(gravity);
+    auto* pos = pos_.as<float4>();
+    auto* vel = vel_.as<float4>();
+    auto* force = force_.as<float4>();
+    auto* spos = spos_.as<float4>();
+    auto* svel = svel_.as<float4>();
+    auto* cell = cell_.as<uint32_t>();
+    auto* index = index_.as<uint32_t>();
+    auto* cs = cell_start_.as<uint32_t>();
+    auto* ce = cell_end_.as<uint32_t>();
+    const std::string k(name);
+    if (k == "clHash")
+        hipLaunchKernelGGL(k_hash<true>, g, b, 0, s, pos, n, params.cellSize, cell, index,
+                           params_buf_.as<mfl_params>(), params);
+    else if (k == "clReorder")
+        hipLaunchKernelGGL(k_reorder_clear, g, b, 0, s, index, pos, vel, spos, svel, n, cs, ce, 0u);
+    else if (k == "clInitBounds")
+        hipLaunchKernelGGL(k_fill_buckets, grid_for(kBuckets), b, 0, s, cs, ce, kBuckets);
+    else if (k == "clFindBounds")
+        hipLaunchKernelGGL(k_find_bounds, g, b, 0, s, cell, n, cs, ce);
+    else if (k == "clCalculateDensity")
+        hipLaunchKernelGGL(k_density, g, b, 0, s, spos, svel, cs, ce, n, params);
+    else if (k == "clCalculateForces")
+        hipLaunchKernelGGL(k_forces, g, b, 0, s, spos, svel, force, index, cs, ce, n, params);
+    else if (k == "clClipBox")
+        hipLaunchKernelGGL(k_clip_box, g, b, 0, s, spos, svel, force, index, n, params, mn, mx);
+    else if (k == "clIntegrateForce")
+        hipLaunchKernelGGL(k_integrate_force, g, b, 0, s, vel, force, n, params);
+    else if (k == "clGravity")
+        hipLaunchKernelGGL(k_gravity, g, b, 0, s, force, n, gf);
+    else if (k == "clIntegrateVelocity")
+        hipLaunchKernelGGL(k_integrate_velocity, g, b, 0, s, pos, vel, n, params);
+    else {
+        set_error("no kernel named " + k);  // KERNEL_VALIDATE, libCL/oclKernel.h:89-94
+        return 0;
+    }
+    MFL_HIP(hipGetLastError());
+    return 1;
+}
+
+}  // namespace mfl

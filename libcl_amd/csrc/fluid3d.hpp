@@ -1,0 +1,78 @@
+// The oclFluid3D program (libCL/phys/oclFluid3D.h:21-123) on gfx950.
+#pragma once
+
+#include "mfl_common.hpp"
+#include "radix_sort.hpp"
+
+#include <vector>
+
+namespace mfl {
+
+// HIP-event timing of the step's stages (oclKernel::profile / getStartTime / getEndTime,
+// libCL/oclKernel.cpp:84-102,167-203).
+class StageTimer {
+public:
+    ~StageTimer();
+    uint32_t mask = 0;
+    bool begin(int stage, hipStream_t s);   // records a start event when `stage` is enabled
+    void end(int stage, hipStream_t s);
+    int read(int stage, double* ms, uint64_t* launches, bool reset);
+
+private:
+    struct Pending { int stage; hipEvent_t a, b; };
+    hipEvent_t take();
+    std::vector<Pending> pending_;
+    std::vector<hipEvent_t> pool_;
+    hipEvent_t open_[MFL_STAGE_COUNT] = {};
+    double ms_[MFL_STAGE_COUNT] = {};
+    uint64_t count_[MFL_STAGE_COUNT] = {};
+    bool drain();
+};
+
+class Fluid3D {
+public:
+    Fluid3D();
+    ~Fluid3D();
+    Fluid3D(const Fluid3D&) = delete;
+    Fluid3D& operator=(const Fluid3D&) = delete;
+
+    int compile();                                  // oclFluid3D::compile, cpp:273-285
+    int compute(hipStream_t s);                     // oclFluid3D::compute, cpp:293-342
+    int set_particle_count(size_t n);               // cpp:92-124
+    size_t particle_count() const { return n_; }    // cpp:126-129
+    int set_buffer(int kind, void* ptr, size_t bytes);  // cpp:131-165
+    int get_buffer(int kind, void** ptr, size_t* bytes);
+    int write_buffer(int kind, const void* host, size_t bytes);
+    int read_buffer(int kind, void* host, size_t bytes);
+    int enqueue_kernel(const char* name, hipStream_t s);  // getKernel(name) + enqueue
+
+    mfl_params params{};
+    float box_min[4] = {-1.f, -1.f, -1.f, 0.f};     // bindBuffers, cpp:260
+    float box_max[4] = {1.f, 1.f, 1.f, 0.f};        // cpp:261
+    float gravity[4] = {0.f, 0.f, 1.f, 0.f};        // cpp:256
+    uint32_t clear_count = kReferenceClear;          // cpp:309-313
+    mfl_integrate_fn integrate_cb = nullptr;
+    void* integrate_user = nullptr;
+    mfl_fluid3d* handle = nullptr;                   // C handle passed back to the callback
+    bool use_graph = false;
+    RadixSort sort;
+    StageTimer timer;
+
+private:
+    DeviceBuffer* select(int kind);
+    bool allocate_internal();
+    bool check_user_buffers();
+    int enqueue_step(hipStream_t s);
+    int launch_graph(hipStream_t s);
+    void drop_graph();
+
+    size_t n_ = kGroup;                              // mParticleCount default, cpp:49
+    DeviceBuffer pos_, vel_, force_;
+    DeviceBuffer cell_, index_, spos_, svel_, cell_start_, cell_end_, params_buf_;
+
+    hipGraph_t graph_ = nullptr;
+    hipGraphExec_t graph_exec_ = nullptr;
+    std::string graph_sig_;
+};
+
+}  // namespace mfl

@@ -1,0 +1,52 @@
+// Shared host/device definitions of libmfl (gfx950, wave64).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstddef>
+#include <cstdint>
+#include <string>
+
+#include "../../include/mfl.h"
+
+namespace mfl {
+
+constexpr uint32_t kBuckets = 262144u;         // libCL/phys/oclFluid3D.cl:32, oclFluid3D.cpp:17
+constexpr uint32_t kEmpty = 0xFFFFFFFFu;       // libCL/phys/oclFluid3D.cpp:309
+constexpr uint32_t kReferenceClear = 65536u;   // 262,144-byte fill of uint entries, cpp:310-312
+constexpr uint32_t kGroup = 256u;              // cLocalSize, cpp:16
+constexpr int kWave = 64;
+
+// ---- error reporting (mfl_last_error) ---------------------------------------------------------
+void set_error(const std::string& msg);
+bool hip_ok(hipError_t e, const char* what);
+
+#define MFL_HIP(call)                         \
+    do {                                      \
+        if (!::mfl::hip_ok((call), #call)) return 0; \
+    } while (0)
+
+// ---- device memory owned by a program ----------------------------------------------------------
+struct DeviceBuffer {
+    void* ptr = nullptr;
+    size_t bytes = 0;
+    bool owned = true;
+
+    bool allocate(size_t nbytes);   // (re)allocates when owned; contents lost
+    bool fit(size_t nbytes);        // grow-only (oclRadixSort::fit, libCL/sort/oclRadixSort.cpp:50-59)
+    void release();
+    template <class T> T* as() const { return static_cast<T*>(ptr); }
+};
+
+// ---- cell hash (libCL/phys/oclFluid3D.cl:34-52) ------------------------------------------------
+// (int)floor(p / h) with the IEEE divide (HIP's default fp32 divide is correctly rounded), and the
+// uint-wrapping XOR of three prime products reduced mod 2^18.
+__host__ __device__ inline int grid_coord(float p, float h) { return (int)floorf(p / h); }
+
+__host__ __device__ inline uint32_t cell_hash(int gx, int gy, int gz)
+{
+    const uint32_t n = (73856093u * (uint32_t)gx) ^ (19349663u * (uint32_t)gy) ^ (83492791u * (uint32_t)gz);
+    return n & (kBuckets - 1u);
+}
+
+}  // namespace mfl

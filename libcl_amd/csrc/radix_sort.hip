@@ -1,0 +1,329 @@
+// Stable LSD key/value radix sort, gfx950 / wave64.
+//
+// Same contract as oclRadixSort::compute (libCL/sort/oclRadixSort.cpp:62-144): after the call the
+// pairs are stably ordered by bits [start_bit, end_bit) of the key. The reference walks 4-bit digits
+// with a serial 256-long scan per 256-item block (libCL/sort/oclRadixSort.cl:20-222); here each pass
+// takes up to 8 bits and has three kernels:
+//   k_tile_counts : per-tile digit histogram in LDS            (reads 4 B/key)
+//   k_scan_digits : per-digit exclusive scan over tiles          (tiny)
+//   k_scatter     : stable in-tile ranking with wave64 ballot matching (no serial scans), tile
+//                   reordered by digit in LDS, then runs written contiguously (reads 8, writes 8 B/key)
+#include "radix_sort.hpp"
+
+namespace mfl {
+namespace {
+
+constexpr int kThreads = 256;
+constexpr int kWaves = kThreads / kWave;
+
+// Exclusive scan of one value per thread over a 256-thread block. All threads must call it.
+__device__ inline uint32_t block_exclusive_scan(uint32_t v, uint32_t* s_wave, uint32_t* total)
+{
+    const int lane = threadIdx.x & (kWave - 1);
+    const int wave = threadIdx.x / kWave;
+    uint32_t x = v;
+#pragma unroll
+    for (int d = 1; d < kWave; d <<= 1) {
+        const uint32_t y = __shfl_up(x, d, kWave);
+        if (lane >= d) x += y;
+    }
+    if (lane == kWave - 1) s_wave[wave] = x;
+    __syncthreads();
+    uint32_t before = 0, all = 0;
+#pragma unroll
+    for (int w = 0; w < kWaves; w++) {
+        const uint32_t s = s_wave[w];
+        before += (w < wave) ? s : 0u;
+        all += s;
+    }
+    __syncthreads();
+    *total = all;
+    return before + x - v;
+}
+
+template <int ITEMS>
+__global__ __launch_bounds__(kThreads) void k_tile_counts(const uint32_t* __restrict__ keys, uint32_t n, int shift,
+                                                          int bits, uint32_t* __restrict__ counts, uint32_t ntiles)
+{
+    __shared__ uint32_t hist[256];
+    const uint32_t radix = 1u << bits, mask = radix - 1u;
+    hist[threadIdx.x] = 0;
+    __syncthreads();
+    const size_t base = (size_t)blockIdx.x * (kThreads * ITEMS);
+#pragma unroll
+    for (int k = 0; k < ITEMS; k++) {
+        const size_t i = base + (size_t)k * kThreads + threadIdx.x;
+        if (i < n) atomicAdd(&hist[(keys[i] >> shift) & mask], 1u);
+    }
+    __syncthreads();
+    if (threadIdx.x < radix) counts[(size_t)threadIdx.x * ntiles + blockIdx.x] = hist[threadIdx.x];
+}
+
+// One block per digit: row d of the digit-major [radix x ntiles] table becomes its exclusive scan.
+__global__ __launch_bounds__(kThreads) void k_scan_digits(uint32_t* __restrict__ counts, uint32_t ntiles,
+                                                          uint32_t* __restrict__ totals)
+{
+    __shared__ uint32_t s_wave[kWaves];
+    uint32_t* row = counts + (size_t)blockIdx.x * ntiles;
+    uint32_t carry = 0;
+    for (uint32_t base = 0; base < ntiles; base += kThreads * 4) {
+        const uint32_t i0 = base + threadIdx.x * 4;
+        uint32_t c[4], sum = 0;
+#pragma unroll
+        for (int k = 0; k < 4; k++) {
+            c[k] = (i0 + k < ntiles) ? row[i0 + k] : 0u;
+            sum += c[k];
+        }
+        uint32_t tot;
+        uint32_t run = carry + block_exclusive_scan(sum, s_wave, &tot);
+#pragma unroll
+        for (int k = 0; k < 4; k++) {
+            if (i0 + k < ntiles) row[i0 + k] = run;
+            run += c[k];
+        }
+        carry += tot;
+    }
+    if (threadIdx.x == 0) totals[blockIdx.x] = carry;
+}
+
+// Stable scatter of one tile. Keys of wave w, round r, lane l sit at tile offset (w*ITEMS + r)*64 + l,
+// so (wave, round, lane) order is input order; a key's rank is
+//   [same digit in earlier waves] + [same digit in earlier rounds of its wave] + [lower lanes, same round].
+template <int ITEMS, bool IDENTITY_VALS>
+__global__ __launch_bounds__(kThreads) void k_scatter(const uint32_t* __restrict__ kin, const uint32_t* __restrict__ vin,
+                                                      uint32_t* __restrict__ kout, uint32_t* __restrict__ vout,
+                                                      uint32_t n, int shift, int bits,
+                                                      const uint32_t* __restrict__ counts,
+                                                      const uint32_t* __restrict__ totals, uint32_t ntiles)
+{
+    constexpr int TILE = kThreads * ITEMS;
+    __shared__ uint32_t s_keys[TILE];
+    __shared__ uint32_t s_vals[TILE];
+    __shared__ uint32_t s_cnt[kWaves * 256];
+    __shared__ uint32_t s_start[256];
+    __shared__ uint32_t s_base[256];
+    __shared__ uint32_t s_wave[kWaves];
+
+    const int t = threadIdx.x;
+    const int lane = t & (kWave - 1);
+    const int wave = t / kWave;
+    const uint32_t radix = 1u << bits, mask = radix - 1u;
+    const uint32_t tile = blockIdx.x;
+    const size_t tile_base = (size_t)tile * TILE;
+
+#pragma unroll
+    for (int i = 0; i < kWaves; i++) s_cnt[i * 256 + t] = 0;
+    {
+        const uint32_t tot = (uint32_t)t < radix ? totals[t] : 0u;
+        uint32_t all;
+        const uint32_t digit_base = block_exclusive_scan(tot, s_wave, &all);
+        if ((uint32_t)t < radix) s_base[t] = digit_base + counts[(size_t)t * ntiles + tile];
+    }
+
+    uint32_t key[ITEMS], val[ITEMS], rank[ITEMS];
+#pragma unroll
+    for (int r = 0; r < ITEMS; r++) {
+        const size_t idx = tile_base + ((size_t)(wave * ITEMS + r) << 6) + lane;
+        const bool valid = idx < n;
+        key[r] = valid ? kin[idx] : 0u;
+        if (IDENTITY_VALS)
+            val[r] = (uint32_t)idx;
+        else
+            val[r] = valid ? vin[idx] : 0u;
+    }
+    __syncthreads();
+
+    const uint64_t lower_lanes = (1ull << lane) - 1ull;
+#pragma unroll
+    for (int r = 0; r < ITEMS; r++) {
+        const size_t idx = tile_base + ((size_t)(wave * ITEMS + r) << 6) + lane;
+        const bool valid = idx < n;
+        const uint32_t d = (key[r] >> shift) & mask;
+        uint64_t same = __ballot(valid);
+        for (int b = 0; b < bits; b++) {
+            const bool bit = (d >> b) & 1u;
+            const uint64_t bal = __ballot(bit);
+            same &= bit ? bal : ~bal;
+        }
+        const int leader = __ffsll((unsigned long long)same) - 1;
+        uint32_t base = 0;
+        if (valid && lane == leader) {
+            base = s_cnt[wave * 256 + d];
+            s_cnt[wave * 256 + d] = base + (uint32_t)__popcll(same);
+        }
+        base = __shfl(base, leader & (kWave - 1), kWave);
+        rank[r] = base + (uint32_t)__popcll(same & lower_lanes);
+    }
+    __syncthreads();
+
+    uint32_t tile_cnt = 0;
+    if ((uint32_t)t < radix) {
+        uint32_t run = 0;
+#pragma unroll
+        for (int w = 0; w < kWaves; w++) {
+            const uint32_t c = s_cnt[w * 256 + t];
+            s_cnt[w * 256 + t] = run;
+            run += c;
+        }
+        tile_cnt = run;
+    }
+    {
+        uint32_t all;
+        const uint32_t st = block_exclusive_scan(tile_cnt, s_wave, &all);
+        if ((uint32_t)t < radix) s_start[t] = st;
+    }
+    __syncthreads();
+
+#pragma unroll
+    for (int r = 0; r < ITEMS; r++) {
+        const size_t idx = tile_base + ((size_t)(wave * ITEMS + r) << 6) + lane;
+        if (idx < n) {
+            const uint32_t d = (key[r] >> shift) & mask;
+            const uint32_t pos = s_start[d] + s_cnt[wave * 256 + d] + rank[r];
+            s_keys[pos] = key[r];
+            s_vals[pos] = val[r];
+        }
+    }
+    __syncthreads();
+
+    const size_t remain = n - tile_base;
+    const uint32_t tile_n = remain < (size_t)TILE ? (uint32_t)remain : (uint32_t)TILE;
+    for (uint32_t p = t; p < tile_n; p += kThreads) {
+        const uint32_t k = s_keys[p];
+        const uint32_t d = (k >> shift) & mask;
+        const uint32_t o = s_base[d] + (p - s_start[d]);
+        if (o < n) {  // always true for a consistent count table; keeps a corrupt one in bounds
+            kout[o] = k;
+            vout[o] = s_vals[p];
+        }
+    }
+}
+
+template <int ITEMS>
+bool run_pass(const uint32_t* kin, const uint32_t* vin, uint32_t* kout, uint32_t* vout, uint32_t n, int shift,
+              int bits, uint32_t* counts, uint32_t* totals, bool identity, hipStream_t s)
+{
+    constexpr uint32_t TILE = kThreads * ITEMS;
+    const uint32_t ntiles = (uint32_t)(((size_t)n + TILE - 1) / TILE);
+    const uint32_t radix = 1u << bits;
+    hipLaunchKernelGGL(k_tile_counts<ITEMS>, dim3(ntiles), dim3(kThreads), 0, s, kin, n, shift, bits, counts, ntiles);
+    hipLaunchKernelGGL(k_scan_digits, dim3(radix), dim3(kThreads), 0, s, counts, ntiles, totals);
+    if (identity)
+        hipLaunchKernelGGL((k_scatter<ITEMS, true>), dim3(ntiles), dim3(kThreads), 0, s, kin, vin, kout, vout, n,
+                           shift, bits, counts, totals, ntiles);
+    else
+        hipLaunchKernelGGL((k_scatter<ITEMS, false>), dim3(ntiles), dim3(kThreads), 0, s, kin, vin, kout, vout, n,
+                           shift, bits, counts, totals, ntiles);
+    return hip_ok(hipGetLastError(), "radix sort pass launch");
+}
+
+int pick_items(size_t n, int requested)
+{
+    if (requested == 4 || requested == 8 || requested == 16) return requested;
+    if (n >= (size_t)8 << 20) return 16;
+    if (n >= (size_t)1 << 20) return 8;
+    return 4;
+}
+
+}  // namespace
+
+RadixSort::~RadixSort()
+{
+    for (auto& b : tmp_keys_) b.release();
+    for (auto& b : tmp_vals_) b.release();
+    counts_.release();
+    totals_.release();
+}
+
+int RadixSort::compile()
+{
+    int dev = 0;
+    MFL_HIP(hipGetDevice(&dev));
+    hipDeviceProp_t prop;
+    MFL_HIP(hipGetDeviceProperties(&prop, dev));
+    if (std::string(prop.gcnArchName).rfind("gfx950", 0) != 0) {
+        set_error(std::string("libmfl kernels are built for gfx950, device is ") + prop.gcnArchName);
+        return 0;
+    }
+    return 1;
+}
+
+int RadixSort::compute(uint32_t* keys, uint32_t* vals, size_t n, int start_bit, int end_bit, hipStream_t stream,
+                       bool identity_vals)
+{
+    // argument checks of libCL/sort/oclRadixSort.cpp:64-80 (the 4,194,304 cap is lifted)
+    if ((end_bit - start_bit) % 4 != 0) {
+        set_error("end bit(" + std::to_string(end_bit) + ") - start bit(" + std::to_string(start_bit) +
+                  ") must be divisible by 4");
+        return 0;
+    }
+    return sort_bits(keys, vals, n, start_bit, end_bit, stream, identity_vals);
+}
+
+int RadixSort::reserve(size_t n, int nbits)
+{
+    if (n == 0 || nbits <= 0) return 1;
+    const int passes = (nbits + 7) / 8;
+    const int items = pick_items(n, tuning.items);
+    const size_t ntiles = (n + kThreads * items - 1) / (kThreads * items);
+    const size_t bytes = n * sizeof(uint32_t);
+    const int nbuf = passes == 1 ? 1 : 2;
+    for (int i = 0; i < nbuf; i++)
+        if (!tmp_keys_[i].fit(bytes) || !tmp_vals_[i].fit(bytes)) return 0;
+    if (!counts_.fit(ntiles * 256 * sizeof(uint32_t)) || !totals_.fit(256 * sizeof(uint32_t))) return 0;
+    return 1;
+}
+
+int RadixSort::sort_bits(uint32_t* keys, uint32_t* vals, size_t n, int start_bit, int end_bit, hipStream_t stream,
+                         bool identity_vals)
+{
+    if (start_bit < 0 || end_bit > 32) {
+        set_error("bit range must lie in [0, 32]");
+        return 0;
+    }
+    if (n == 0 || start_bit >= end_bit) return 1;
+    if (n > 0xFFFFFFFFull) {
+        set_error("maximum sortable array size = 4294967295");
+        return 0;
+    }
+    if (!keys || !vals) {
+        set_error("radix sort: null key or value buffer");
+        return 0;
+    }
+    const int nbits = end_bit - start_bit;
+    const int passes = (nbits + 7) / 8;
+    const int items = pick_items(n, tuning.items);
+    const size_t bytes = n * sizeof(uint32_t);
+    if (!reserve(n, nbits)) return 0;
+
+    const uint32_t* kin = keys;
+    const uint32_t* vin = vals;
+    int shift = start_bit;
+    for (int p = 0; p < passes; p++) {
+        const int bits = nbits / passes + (p < nbits % passes ? 1 : 0);
+        const bool last = p == passes - 1 && passes > 1;
+        uint32_t* kout = last ? keys : tmp_keys_[p & 1].as<uint32_t>();
+        uint32_t* vout = last ? vals : tmp_vals_[p & 1].as<uint32_t>();
+        const bool ident = identity_vals && p == 0;
+        bool ok;
+        switch (items) {
+        case 16: ok = run_pass<16>(kin, vin, kout, vout, (uint32_t)n, shift, bits, counts_.as<uint32_t>(),
+                                   totals_.as<uint32_t>(), ident, stream); break;
+        case 8: ok = run_pass<8>(kin, vin, kout, vout, (uint32_t)n, shift, bits, counts_.as<uint32_t>(),
+                                 totals_.as<uint32_t>(), ident, stream); break;
+        default: ok = run_pass<4>(kin, vin, kout, vout, (uint32_t)n, shift, bits, counts_.as<uint32_t>(),
+                                  totals_.as<uint32_t>(), ident, stream); break;
+        }
+        if (!ok) return 0;
+        kin = kout;
+        vin = vout;
+        shift += bits;
+    }
+    if (passes == 1) {
+        MFL_HIP(hipMemcpyAsync(keys, tmp_keys_[0].ptr, bytes, hipMemcpyDeviceToDevice, stream));
+        MFL_HIP(hipMemcpyAsync(vals, tmp_vals_[0].ptr, bytes, hipMemcpyDeviceToDevice, stream));
+    }
+    return 1;
+}
+
+}  // namespace mfl

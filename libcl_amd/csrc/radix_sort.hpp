@@ -1,0 +1,45 @@
+// Stable LSD key/value radix sort for gfx950 (the oclRadixSort program, libCL/sort/oclRadixSort.h).
+#pragma once
+
+#include "mfl_common.hpp"
+
+namespace mfl {
+
+// Tile shape of one pass: 256 threads (4 wave64s) x `items` keys per thread.
+struct SortTuning {
+    int items = 0;        // 0 = pick by size
+};
+
+class RadixSort {
+public:
+    RadixSort() = default;
+    ~RadixSort();
+    RadixSort(const RadixSort&) = delete;
+    RadixSort& operator=(const RadixSort&) = delete;
+
+    // oclProgram::compile analogue: checks that the current device is a gfx950 HIP device.
+    int compile();
+
+    // oclRadixSort::compute (libCL/sort/oclRadixSort.cpp:62-144): stable in-place sort of n pairs by
+    // bits [start_bit, end_bit). `identity_vals` treats vals[i] == i on input without reading them
+    // (the clHash index, libCL/phys/oclFluid3D.cl:63).
+    int compute(uint32_t* keys, uint32_t* vals, size_t n, int start_bit, int end_bit, hipStream_t stream,
+                bool identity_vals = false);
+
+    // Same sort for any bit range in [0, 32] (no multiple-of-4 rule); used by the fluid step, whose
+    // keys are below 2^18 so that bits [0, 18) order them exactly as the reference's [0, 24).
+    int sort_bits(uint32_t* keys, uint32_t* vals, size_t n, int start_bit, int end_bit, hipStream_t stream,
+                  bool identity_vals);
+
+    // Allocates every temporary a sort of n keys over `nbits` bits needs (so that a later sort
+    // enqueues no allocation, e.g. under hipGraph capture).
+    int reserve(size_t n, int nbits);
+
+    SortTuning tuning;
+
+private:
+    DeviceBuffer tmp_keys_[2], tmp_vals_[2];
+    DeviceBuffer counts_, totals_;
+};
+
+}  // namespace mfl

@@ -1,0 +1,242 @@
+"""GPU parity of the SPH step (oclFluid3D::compute, libCL/phys/oclFluid3D.cpp:293-342).
+
+Sort keys, permutation and bucket tables are compared bit-exact; density/pressure and force fields
+within 1e-5 (relative; forces normwise and per particle floored by the RMS force, because interior
+pressure forces are near-cancelling sums). Every comparison starts from an identical snapshot, so
+chaotic divergence of trajectories never enters a tolerance.
+"""
+import numpy as np
+import pytest
+
+from conftest import STEP_CASES, force_errors, load_step_case
+from libcl_amd import scenes
+
+pytestmark = pytest.mark.gpu
+
+TOL = 1e-5
+
+
+def make_fluid(pos, vel, force, params, box_min, box_max, clear=65536, tables=None):
+    import libcl_amd as L
+    f = L.oclFluid3D()
+    assert f.compile() == 1, L.last_error()
+    assert f.setParticleCount(pos.shape[0]) == 1, L.last_error()
+    f.setParameters(params)
+    assert f.setBox(box_min, box_max) == 1
+    assert f.setBucketClear(clear) == 1
+    assert f.upload(pos, vel, force) == 1, L.last_error()
+    if tables is not None:
+        assert f.bfCellStart.write(tables[0]) == 1
+        assert f.bfCellEnd.write(tables[1]) == 1
+    return f
+
+
+def params_of(values):
+    import libcl_amd as L
+    return L.Params(*[float(x) for x in values])
+
+
+def run_forces_only(f):
+    """compute() with an EVT_INTEGRATE handler that does nothing: leaves the force pass result."""
+    import libcl_amd as L
+    assert f.addEventHandler(f.EVT_INTEGRATE, lambda prog, stream: None) == 1
+    assert f.compute() == 1, L.last_error()
+    L.load().mfl_device_finish()
+    f.removeEventHandler()
+
+
+@pytest.mark.parametrize("name", STEP_CASES)
+def test_step_matches_golden(name):
+    """One step from the committed fp64 all-pairs fixture (tests/golden/make_golden.py)."""
+    print(f"golden step {name}", flush=True)
+    z = load_step_case(name)
+    prm = params_of(z["params"])
+    f = make_fluid(z["pos"], z["vel"], z["force"], prm, z["box_min"], z["box_max"])
+    run_forces_only(f)
+    it = f.internals()
+    np.testing.assert_array_equal(it["cell"], z["key"])
+    np.testing.assert_array_equal(it["index"], z["perm"])
+    np.testing.assert_array_equal(it["cell_start"][z["bucket"]], z["start"])
+    np.testing.assert_array_equal(it["cell_end"][z["bucket"]], z["end"])
+    empty = np.ones(262144, bool)
+    empty[z["bucket"]] = False
+    assert np.all(it["cell_start"][empty] == 0xFFFFFFFF) and np.all(it["cell_end"][empty] == 0xFFFFFFFF)
+    rel = np.abs(it["spos"][:, 3] - z["rho"]) / np.abs(z["rho"])
+    assert rel.max() <= TOL
+    prs_scale = np.abs(z["prs"]) + np.sqrt(np.mean(z["prs"] ** 2))
+    assert (np.abs(it["svel"][:, 3] - z["prs"]) / prs_scale).max() <= TOL
+    st = f.state()
+    nw, per = force_errors(st["force"], z["force_sph"])
+    assert nw <= TOL and per <= TOL, (nw, per)
+    assert np.all(st["force"][:, 3] == 0)
+
+    # full step (clip box, clamp, Euler, gravity) from the same snapshot
+    f2 = make_fluid(z["pos"], z["vel"], z["force"], prm, z["box_min"], z["box_max"])
+    assert f2.compute() == 1
+    st2 = f2.state()
+    np.testing.assert_array_equal(st2["force"], np.tile(scenes.GRAVITY_FORCE, (z["pos"].shape[0], 1)))
+    dt = float(prm.deltaTime)
+    fmag = np.linalg.norm(z["force_clip"][:, :3], axis=1)
+    vtol = TOL * (fmag + np.sqrt(np.mean(fmag ** 2))) * dt + 4e-7 * (np.abs(z["vel_new"][:, :3]).max(1) + 1e-6)
+    assert np.all(np.linalg.norm(st2["vel"][:, :3] - z["vel_new"][:, :3], axis=1) <= vtol)
+    ptol = vtol * dt + 2.5e-7 * np.abs(z["pos_new"][:, :3]).max(1)
+    assert np.all(np.linalg.norm(st2["pos"][:, :3] - z["pos_new"][:, :3], axis=1) <= ptol)
+
+
+def _oracle_compare(f, fo, check_fields=True):
+    it = f.internals()
+    np.testing.assert_array_equal(it["cell"], fo.cell)
+    np.testing.assert_array_equal(it["index"], fo.index)
+    np.testing.assert_array_equal(it["cell_start"], fo.cell_start)
+    np.testing.assert_array_equal(it["cell_end"], fo.cell_end)
+    if check_fields:
+        rel = np.abs(it["spos"][:, 3] - fo.spos[:, 3]) / np.abs(fo.spos[:, 3])
+        assert rel.max() <= TOL, rel.max()
+        p = fo.svel[:, 3].astype(np.float64)
+        assert (np.abs(it["svel"][:, 3] - p) / (np.abs(p) + np.sqrt(np.mean(p ** 2)))).max() <= TOL
+    return it
+
+
+@pytest.mark.parametrize("clear", [65536, 262144])
+def test_small_dam_break_vs_oracle(oracle, clear):
+    """16,384-particle dam break: snapshots along the oracle trajectory (steps 0, 1, 10, 60) are
+    stepped once on both sides, bucket tables included (stale ranges of the reference's partial
+    clear are part of the state)."""
+    sc = scenes.config_small()
+    import libcl_amd as L
+    prm = L.params_from(sc.params)
+    op = oracle.Params(*prm.values())
+    fo = oracle.Fluid(sc.pos, sc.vel, sc.force, op, box_min=sc.box_min, box_max=sc.box_max, clear_count=clear)
+    done = 0
+    for target in (0, 1, 10, 60):
+        fo.steps(target - done)
+        done = target
+        print(f"16k snapshot step {target} clear {clear}", flush=True)
+        snap = (fo.pos.copy(), fo.vel.copy(), fo.force.copy(), fo.cell_start.copy(), fo.cell_end.copy())
+        f = make_fluid(snap[0], snap[1], snap[2], prm, sc.box_min, sc.box_max, clear, tables=(snap[3], snap[4]))
+        run_forces_only(f)
+        fo2 = oracle.Fluid(*snap[:3], op, box_min=sc.box_min, box_max=sc.box_max, clear_count=clear,
+                           cell_start=snap[3], cell_end=snap[4])
+        fo2.step(integrate=False)
+        _oracle_compare(f, fo2)
+        nw, per = force_errors(f.state()["force"], fo2.force)
+        assert nw <= TOL and per <= TOL, (target, nw, per)
+
+
+def test_1m_step_vs_oracle(oracle):
+    """1,048,576 particles in [0,4]^3: keys, permutation, bucket tables bit-exact; density, pressure
+    and forces of one step within 1e-5."""
+    sc = scenes.config_1m()
+    import libcl_amd as L
+    prm = L.params_from(sc.params)
+    op = oracle.Params(*prm.values())
+    f = make_fluid(sc.pos, sc.vel, sc.force, prm, sc.box_min, sc.box_max)
+    run_forces_only(f)
+    fo = oracle.Fluid(sc.pos, sc.vel, sc.force, op, box_min=sc.box_min, box_max=sc.box_max)
+    fo.step(integrate=False)
+    _oracle_compare(f, fo)
+    nw, per = force_errors(f.state()["force"], fo.force)
+    assert nw <= TOL and per <= TOL, (nw, per)
+
+
+def test_multistep_tables_and_graph(oracle):
+    """20 chained steps at 4,096 particles: eager and hipGraph replay give bit-identical state; the
+    bucket tables (with their stale entries) match the oracle fed the same per-step positions."""
+    import torch
+    import libcl_amd as L
+    sc = scenes.dam_break(16, 16, 16, 1.0, seed=31, steps=20)
+    prm = L.params_from(sc.params)
+    fa = make_fluid(sc.pos, sc.vel, sc.force, prm, sc.box_min, sc.box_max)
+    fb = make_fluid(sc.pos, sc.vel, sc.force, prm, sc.box_min, sc.box_max)
+    assert fb.setUseGraph(True) == 1
+    stream = torch.cuda.Stream()
+    op = oracle.Params(*prm.values())
+    for step in range(20):
+        before = fa.state()
+        tables = fa.internals()
+        assert fa.compute() == 1
+        assert fb.compute(stream) == 1, L.last_error()
+        stream.synchronize()
+        L.load().mfl_device_finish()
+        sa, sb = fa.state(), fb.state()
+        for k in ("pos", "vel", "force"):
+            np.testing.assert_array_equal(sa[k], sb[k])
+        fo = oracle.Fluid(before["pos"], before["vel"], before["force"], op, box_min=sc.box_min,
+                          box_max=sc.box_max, cell_start=tables["cell_start"], cell_end=tables["cell_end"])
+        fo.step(integrate=False)
+        _oracle_compare(fa, fo, check_fields=(step % 5 == 0))
+
+
+def test_integrate_callback_path(oracle):
+    """libCL/TEST/main.cpp:246-343: an EVT_INTEGRATE handler enqueues clIntegrateForce, clGravity,
+    clIntegrateVelocity itself (no clip box)."""
+    import libcl_amd as L
+    z = load_step_case("random512")
+    prm = params_of(z["params"])
+    f = make_fluid(z["pos"], z["vel"], z["force"], prm, z["box_min"], z["box_max"])
+    calls = []
+
+    def handler(prog, stream):
+        calls.append(1)
+        for k in ("clIntegrateForce", "clGravity", "clIntegrateVelocity"):
+            assert prog.enqueueKernel(k, stream) == 1
+
+    assert f.addEventHandler(f.EVT_INTEGRATE, handler) == 1
+    assert f.compute() == 1
+    L.load().mfl_device_finish()
+    assert calls == [1]
+    fo = oracle.Fluid(z["pos"], z["vel"], z["force"], oracle.Params(*prm.values()), box_min=z["box_min"],
+                      box_max=z["box_max"])
+    fo.step(integrate=False)
+    oracle.integrate_force(fo.vel, fo.force, fo.params)
+    oracle.gravity(fo.force)
+    oracle.integrate_velocity(fo.pos, fo.vel, fo.params)
+    st = f.state()
+    np.testing.assert_array_equal(st["force"], fo.force)
+    assert np.max(np.abs(st["vel"] - fo.vel)) <= 1e-5 * (np.abs(fo.vel).max() + 1.0)
+    assert np.max(np.abs(st["pos"] - fo.pos)) <= 1e-6
+
+
+def test_stage_kernels_individually(oracle):
+    """Each named kernel reachable through getKernel(name) matches its oracle restatement."""
+    import libcl_amd as L
+    z = load_step_case("compressed512")
+    prm = params_of(z["params"])
+    op = oracle.Params(*prm.values())
+    f = make_fluid(z["pos"], z["vel"], z["force"], prm, z["box_min"], z["box_max"])
+    for k in ("clHash",):
+        assert f.enqueueKernel(k) == 1
+    L.load().mfl_device_finish()
+    it = f.internals()
+    cell, index = oracle.hash_cells(z["pos"], float(prm.cellSize))
+    np.testing.assert_array_equal(it["cell"], cell)
+    np.testing.assert_array_equal(it["index"], index)
+    assert f.enqueueKernel("clNoSuchKernel") == 0
+    assert "clNoSuchKernel" in L.last_error()
+    # clip box alone on a force field, in sorted order through the index buffer
+    run_forces_only(f)
+    it = f.internals()
+    before = f.state()["force"]
+    assert f.enqueueKernel("clClipBox") == 1
+    L.load().mfl_device_finish()
+    ref = before.copy()
+    oracle.clip_box(it["spos"], it["svel"], ref, it["index"], op, z["box_min"], z["box_max"])
+    np.testing.assert_allclose(f.state()["force"], ref, rtol=1e-6, atol=1e-6 * np.abs(ref).max())
+
+
+def test_particle_count_rules():
+    """N must be a non-zero multiple of cLocalSize = 256 (every NDRange uses local size 256)."""
+    import libcl_amd as L
+    f = L.oclFluid3D()
+    assert f.compile() == 1
+    assert f.getParticleCount() == 256
+    assert f.setParticleCount(300) == 1
+    assert f.compute() == 0 and "multiple of 256" in L.last_error()
+    assert f.setParticleCount(512) == 1
+    import torch
+    small = torch.zeros(256, 4, dtype=torch.float32, device="cuda")
+    assert f.setPositionBuffer(small) == 0
+    big = torch.zeros(512, 4, dtype=torch.float32, device="cuda")
+    assert f.setPositionBuffer(big) == 1
+    assert f.compute() == 1
+    L.load().mfl_device_finish()
