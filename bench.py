@@ -60,6 +60,7 @@ def main():
     ap.add_argument("--config", default="1m", choices=["1m", "small"])
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-steps", type=int, default=20)
+    ap.add_argument("--order", default="morton", choices=["sorted", "caller", "morton"])
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for {args.gpus}); the step is not sharded across GPUs",
@@ -81,6 +82,7 @@ def main():
     f.setParameters(prm)
     f.setBox(sc.box_min, sc.box_max)
     assert f.upload(sc.pos, sc.vel, sc.force) == 1
+    assert f.setTraversal(args.order, 16) == 1
     stream = torch.cuda.Stream()
 
     for _ in range(args.warmup):
@@ -143,7 +145,7 @@ def main():
         "dtype": "f32",
         "data": "synthetic (seeded jittered lattice, SplitMix64)",
         "config": {"workload": sc.name, "particles": n, "cell_size": 0.02, "box": [0.0, 4.0],
-                   "buckets": 262144, "bucket_clear": 65536},
+                   "buckets": 262144, "bucket_clear": 65536, "traversal": args.order},
         "roofline": {"bound": "hbm", "kernel": dominant, "achieved": achieved, "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": None,
                      "avg_launch_us": avg_s * 1e6, "alg_bytes_per_launch": alg_bytes},

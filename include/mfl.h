@@ -141,6 +141,13 @@ int mfl_fluid3d_set_bucket_clear(mfl_fluid3d* f, uint32_t count);
    buffers, params, box, gravity or bucket clear change). Ignored while an integrate callback is set. */
 int mfl_fluid3d_set_use_graph(mfl_fluid3d* f, int enable);
 
+/* Addition: the order in which the density and force passes visit particles. It changes no result
+   (every particle's sums are formed by one thread in the reference's visit order), only cache
+   locality: SORTED walks the hash-sorted arrays, CALLER walks the caller's particle order, MORTON
+   walks a Morton order of the particles' cells rebuilt every `refresh_steps` steps (0 = once). */
+enum mfl_traversal { MFL_TRAVERSAL_SORTED = 0, MFL_TRAVERSAL_CALLER = 1, MFL_TRAVERSAL_MORTON = 2 };
+int mfl_fluid3d_set_traversal(mfl_fluid3d* f, int order, uint32_t refresh_steps);
+
 /* Per-stage kernel timing with HIP events on the compute stream: the analogue of oclKernel::profile(true)
    + getStartTime/getEndTime (libCL/oclKernel.cpp:84-102,167-203). `stage_mask` has bit s set for each
    MFL_STAGE_* to time (0 disables). While a mask is set, compute() launches eagerly (no graph replay). */
@@ -151,7 +158,8 @@ enum mfl_stage {
     MFL_STAGE_BOUNDS = 3,
     MFL_STAGE_DENSITY = 4,
     MFL_STAGE_FORCES = 5,
-    MFL_STAGE_COUNT = 6
+    MFL_STAGE_ORDER = 6,
+    MFL_STAGE_COUNT = 7
 };
 int mfl_fluid3d_set_profiling(mfl_fluid3d* f, uint32_t stage_mask);
 /* Waits for the recorded events and returns the accumulated time (ms) and launch count of a stage

@@ -28,11 +28,13 @@ EXPORTS = (
     "mfl_fluid3d_set_particle_count", "mfl_fluid3d_get_particle_count", "mfl_fluid3d_set_buffer",
     "mfl_fluid3d_get_buffer", "mfl_fluid3d_write_buffer", "mfl_fluid3d_read_buffer", "mfl_fluid3d_params",
     "mfl_fluid3d_set_box", "mfl_fluid3d_set_gravity", "mfl_fluid3d_set_bucket_clear",
-    "mfl_fluid3d_set_use_graph", "mfl_fluid3d_set_profiling", "mfl_fluid3d_stage_time",
+    "mfl_fluid3d_set_use_graph", "mfl_fluid3d_set_traversal", "mfl_fluid3d_set_profiling",
+    "mfl_fluid3d_stage_time",
     "mfl_fluid3d_set_integrate_callback", "mfl_fluid3d_enqueue_kernel",
 )
 
-STAGES = ("hash", "sort", "reorder", "bounds", "density", "forces")
+STAGES = ("hash", "sort", "reorder", "bounds", "density", "forces", "order")
+TRAVERSALS = ("sorted", "caller", "morton")
 
 
 class Params(ctypes.Structure):
@@ -96,6 +98,7 @@ def load() -> ctypes.CDLL:
         "mfl_fluid3d_set_gravity": (ctypes.c_int, [vp, ctypes.POINTER(ctypes.c_float)]),
         "mfl_fluid3d_set_bucket_clear": (ctypes.c_int, [vp, ctypes.c_uint32]),
         "mfl_fluid3d_set_use_graph": (ctypes.c_int, [vp, ctypes.c_int]),
+        "mfl_fluid3d_set_traversal": (ctypes.c_int, [vp, ctypes.c_int, ctypes.c_uint32]),
         "mfl_fluid3d_set_profiling": (ctypes.c_int, [vp, ctypes.c_uint32]),
         "mfl_fluid3d_stage_time": (ctypes.c_int, [vp, ctypes.c_int, ctypes.POINTER(ctypes.c_double),
                                                   ctypes.POINTER(ctypes.c_uint64), ctypes.c_int]),

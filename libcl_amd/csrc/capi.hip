@@ -235,6 +235,18 @@ int mfl_fluid3d_set_use_graph(mfl_fluid3d* f, int enable)
     return 1;
 }
 
+int mfl_fluid3d_set_traversal(mfl_fluid3d* f, int order, uint32_t refresh_steps)
+{
+    if (!need(f, "fluid")) return 0;
+    if (order < MFL_TRAVERSAL_SORTED || order > MFL_TRAVERSAL_MORTON) {
+        mfl::set_error("unknown traversal order " + std::to_string(order));
+        return 0;
+    }
+    f->impl.traversal = order;
+    f->impl.refresh_interval = refresh_steps;
+    return 1;
+}
+
 int mfl_fluid3d_set_profiling(mfl_fluid3d* f, uint32_t stage_mask)
 {
     if (!need(f, "fluid")) return 0;

@@ -47,7 +47,8 @@ __global__ __launch_bounds__(kThreads) void k_reorder_clear(const uint32_t* __re
                                                             const float4* __restrict__ vel,
                                                             float4* __restrict__ spos, float4* __restrict__ svel,
                                                             uint32_t n, uint32_t* __restrict__ cell_start,
-                                                            uint32_t* __restrict__ cell_end, uint32_t clear)
+                                                            uint32_t* __restrict__ cell_end, uint32_t clear,
+                                                            uint32_t* __restrict__ inv)
 {
     const uint32_t i = blockIdx.x * kThreads + threadIdx.x;
     if (i < n) {
@@ -55,6 +56,7 @@ __global__ __launch_bounds__(kThreads) void k_reorder_clear(const uint32_t* __re
         if (s < n) {  // index is a permutation of [0, n)
             spos[i] = pos[s];
             svel[i] = vel[s];
+            if (inv) inv[s] = i;
         }
     }
     if (i < clear) {
@@ -244,23 +246,305 @@ __global__ __launch_bounds__(kThreads) void k_forces(const float4* __restrict__ 
     force[o] = add_sph_force(force[o], P, V, p);
 }
 
-// Forces + clip box + integrate force + gravity + integrate velocity for sorted particle i, written
-// to the caller-order slot o = index[i].
+// ---- neighbour-list path of compute() -----------------------------------------------------------
+//
+// The density pass walks the reference's 27-bucket neighbourhood exactly (z, y, x order, ascending j,
+// duplicate buckets and stale ranges included) and records every accepted visit j of particle i in a
+// slot-major list nbr[k * n + i], k < count[i] <= kNbrCap. The force pass uses the same acceptance
+// (positions do not change between the two passes), so it sums over the recorded visits instead of
+// re-testing ~27 buckets of candidates; a particle with more than kNbrCap visits falls back to the
+// full traversal. Visits repeated by the reference (a bucket appearing twice among the 27) appear
+// twice in the list, so multiplicities are kept.
+
+constexpr uint32_t kNbrCap = 64;
+
+// Exact form of the reference test fast_length(d) < h (cl:177-178): accepted when sqrt(r2) < h.
+// r2 below h2_lo (h^2 (1 - 2^-22)) or above h2_hi is decided without the square root.
+__device__ inline bool accept_r2(float r2, float h, float h2_lo, float h2_hi)
+{
+    if (r2 < h2_lo) return true;
+    if (r2 > h2_hi) return false;
+    return sqrtf(r2) < h;
+}
+
+// Traversal order of the neighbour passes. Results do not depend on it (each particle's sums are
+// computed by one thread in a fixed visit order); it only decides which particles share a wave, an
+// L1 and an XCD's L2. Hash order scatters neighbouring cells over the whole sorted array, so the
+// passes walk particles in caller order or in a periodically refreshed Morton order of their cells
+// and reach their sorted slot through the inverse permutation written by k_reorder_clear.
+enum : int { kOrderSorted = 0, kOrderCaller = 1, kOrderListed = 2 };
+
+template <int ORDER>
+__device__ inline uint32_t sorted_slot(uint32_t t, uint32_t n, const uint32_t* __restrict__ porder,
+                                       const uint32_t* __restrict__ inv)
+{
+    if (ORDER == kOrderSorted) return t;
+    if (ORDER == kOrderCaller) return inv[t];
+    const uint32_t o = porder[t];
+    return o < n ? inv[o] : n;
+}
+
+template <int ORDER>
+__device__ inline uint32_t caller_slot(uint32_t t, uint32_t i, const uint32_t* __restrict__ porder,
+                                       const uint32_t* __restrict__ index)
+{
+    if (ORDER == kOrderSorted) return index[i];
+    if (ORDER == kOrderCaller) return t;
+    return porder[t];
+}
+
+template <int ORDER>
+__global__ __launch_bounds__(kThreads) void k_density_nbr(float4* __restrict__ spos, float4* __restrict__ svel,
+                                                          const uint32_t* __restrict__ cell_start,
+                                                          const uint32_t* __restrict__ cell_end, uint32_t n,
+                                                          mfl_params p, uint32_t* __restrict__ nbr,
+                                                          uint32_t* __restrict__ ncount,
+                                                          const uint32_t* __restrict__ porder,
+                                                          const uint32_t* __restrict__ inv)
+{
+    const uint32_t t = blockIdx.x * kThreads + threadIdx.x;
+    if (t >= n) return;
+    const uint32_t i = sorted_slot<ORDER>(t, n, porder, inv);
+    if (i >= n) return;
+    const float h = p.cellSize;
+    const float h2 = h * h;
+    const float h2_lo = h2 * 0.9999996f, h2_hi = h2 * 1.0000005f;
+    const float4 pi = spos[i];
+    const int gx = grid_coord(pi.x, h), gy = grid_coord(pi.y, h), gz = grid_coord(pi.z, h);
+    float acc = 0.f;
+    uint32_t count = 0;
+    for (int z = -1; z <= 1; z++) {
+        // the nine bucket ranges of one z slab are fetched together (one memory latency, not nine)
+        uint32_t st[9], en[9];
+#pragma unroll
+        for (int k = 0; k < 9; k++) {
+            const uint32_t c = cell_hash(gx + (k % 3) - 1, gy + (k / 3) - 1, gz + z);
+            st[k] = cell_start[c];
+            en[k] = cell_end[c];
+        }
+#pragma unroll
+        for (int k = 0; k < 9; k++) {
+            if (st[k] == kEmpty) continue;
+            const uint32_t end = min(en[k], n);
+            for (uint32_t j = st[k]; j < end; j++) {
+                const float4 pj = spos[j];
+                const float dx = pi.x - pj.x, dy = pi.y - pj.y, dz = pi.z - pj.z;
+                const float r2 = dx * dx + dy * dy + dz * dz;
+                if (j != i && accept_r2(r2, h, h2_lo, h2_hi)) {
+                    const float w = h2 - r2;  // h^2 - vlen^2 of cl:123
+                    acc += w * w * w;
+                    if (count < kNbrCap) nbr[(size_t)count * n + t] = j;
+                    count++;
+                }
+            }
+        }
+    }
+    float rho, prs;
+    if (count > 2) {
+        rho = fmaxf(1.0f, p.mass * p.kernelConstant * acc);
+        prs = p.pressure + p.stiffness * (rho - p.density);
+    } else {
+        rho = p.density;
+        prs = p.pressure;
+    }
+    spos[i].w = rho;
+    svel[i].w = prs;
+    ncount[t] = count;
+}
+
+// Density pass with eight lanes per particle. A wave takes eight consecutive entries of the traversal
+// order (in Morton or hash order usually the particles of one cell), and the eight lanes of a group
+// split each bucket range of their particle into interleaved slots, so bucket trip counts are the
+// same for the whole wave. When all eight particles share a cell, the 27 bucket hashes and range
+// loads are wave-uniform (scalar). Accepted visits are appended in the reference's visit order
+// (bucket by bucket, ascending j) with a group ballot.
+constexpr uint32_t kGroupLanes = 8;
+
+__device__ inline void density_visits(int gx, int gy, int gz, const float4& pi, uint32_t i, uint32_t t, uint32_t slot,
+                                      uint32_t grp_shift, bool live, const float4* __restrict__ spos,
+                                      const uint32_t* __restrict__ cell_start, const uint32_t* __restrict__ cell_end,
+                                      uint32_t n, float h, float h2, float h2_lo, float h2_hi,
+                                      uint32_t* __restrict__ nbr, float& acc, uint32_t& count)
+{
+    const uint32_t below = (1u << slot) - 1u;
+    for (int z = -1; z <= 1; z++) {
+        uint32_t st[9], en[9];
+#pragma unroll
+        for (int k = 0; k < 9; k++) {
+            const uint32_t c = cell_hash(gx + (k % 3) - 1, gy + (k / 3) - 1, gz + z);
+            st[k] = cell_start[c];
+            en[k] = cell_end[c];
+        }
+#pragma unroll
+        for (int k = 0; k < 9; k++) {
+            if (st[k] == kEmpty) continue;
+            const uint32_t end = min(en[k], n);
+            for (uint32_t j0 = st[k]; j0 < end; j0 += kGroupLanes) {
+                const uint32_t j = j0 + slot;
+                bool ok = false;
+                float r2 = 0.f;
+                if (j < end) {
+                    const float4 pj = spos[j];
+                    const float dx = pi.x - pj.x, dy = pi.y - pj.y, dz = pi.z - pj.z;
+                    r2 = dx * dx + dy * dy + dz * dz;
+                    ok = live && j != i && accept_r2(r2, h, h2_lo, h2_hi);
+                }
+                const uint32_t mask = (uint32_t)(__ballot(ok) >> grp_shift) & 0xFFu;
+                if (ok) {
+                    const float w = h2 - r2;  // h^2 - vlen^2 of cl:123
+                    acc += w * w * w;
+                    const uint32_t at = count + (uint32_t)__popc(mask & below);
+                    if (at < kNbrCap) nbr[(size_t)at * n + t] = j;
+                }
+                count += (uint32_t)__popc(mask);
+            }
+        }
+    }
+}
+
+template <int ORDER>
+__global__ __launch_bounds__(kThreads) void k_density_grp(float4* __restrict__ spos, float4* __restrict__ svel,
+                                                          const uint32_t* __restrict__ cell_start,
+                                                          const uint32_t* __restrict__ cell_end, uint32_t n,
+                                                          mfl_params p, uint32_t* __restrict__ nbr,
+                                                          uint32_t* __restrict__ ncount,
+                                                          const uint32_t* __restrict__ porder,
+                                                          const uint32_t* __restrict__ inv)
+{
+    const uint32_t lane = threadIdx.x & (kWave - 1);
+    const uint32_t grp = lane / kGroupLanes, slot = lane % kGroupLanes;
+    const uint32_t wave = (blockIdx.x * kThreads + threadIdx.x) / kWave;
+    const uint32_t t = wave * (kWave / kGroupLanes) + grp;
+    uint32_t i = t < n ? sorted_slot<ORDER>(t, n, porder, inv) : n;
+    const bool live = i < n;
+    const float h = p.cellSize;
+    const float h2 = h * h;
+    const float h2_lo = h2 * 0.9999996f, h2_hi = h2 * 1.0000005f;
+    const float4 pi = live ? spos[i] : make_float4(0.f, 0.f, 0.f, 0.f);
+    int gx = grid_coord(pi.x, h), gy = grid_coord(pi.y, h), gz = grid_coord(pi.z, h);
+    // the first live lane's cell; the wave is uniform when every live lane shares it
+    const uint64_t live_mask = __ballot(live);
+    if (live_mask == 0) return;
+    const int src = __ffsll((unsigned long long)live_mask) - 1;
+    const int gx0 = __shfl(gx, src, kWave), gy0 = __shfl(gy, src, kWave), gz0 = __shfl(gz, src, kWave);
+    const bool same = !live || (gx == gx0 && gy == gy0 && gz == gz0);
+    float acc = 0.f;
+    uint32_t count = 0;
+    const uint32_t grp_shift = grp * kGroupLanes;
+    if (__ballot(!same) == 0) {
+        density_visits(__builtin_amdgcn_readfirstlane(gx0), __builtin_amdgcn_readfirstlane(gy0),
+                       __builtin_amdgcn_readfirstlane(gz0), pi, i, t, slot, grp_shift, live, spos, cell_start,
+                       cell_end, n, h, h2, h2_lo, h2_hi, nbr, acc, count);
+    } else {
+        density_visits(gx, gy, gz, pi, i, t, slot, grp_shift, live, spos, cell_start, cell_end, n, h, h2, h2_lo,
+                       h2_hi, nbr, acc, count);
+    }
+    acc += __shfl_xor(acc, 1, kWave);
+    acc += __shfl_xor(acc, 2, kWave);
+    acc += __shfl_xor(acc, 4, kWave);
+    if (!live || slot != 0) return;
+    float rho, prs;
+    if (count > 2) {
+        rho = fmaxf(1.0f, p.mass * p.kernelConstant * acc);
+        prs = p.pressure + p.stiffness * (rho - p.density);
+    } else {
+        rho = p.density;
+        prs = p.pressure;
+    }
+    spos[i].w = rho;
+    svel[i].w = prs;
+    ncount[t] = count;
+}
+
+// Pressure and viscosity sums of cl:237-242 over the visits recorded for thread t (sorted particle i).
+__device__ inline void sph_force_sum_nbr(const float4* __restrict__ spos, const float4* __restrict__ svel,
+                                         const uint32_t* __restrict__ nbr, uint32_t count, uint32_t t, uint32_t n,
+                                         const float4& pi, const float4& vi, float h, float3& P, float3& V)
+{
+    P = make_float3(0.f, 0.f, 0.f);
+    V = make_float3(0.f, 0.f, 0.f);
+    for (uint32_t k = 0; k < count; k++) {
+        const uint32_t j = nbr[(size_t)k * n + t];
+        const float4 pj = spos[j];
+        const float4 vj = svel[j];
+        const float dx = pi.x - pj.x, dy = pi.y - pj.y, dz = pi.z - pj.z;
+        const float r2 = dx * dx + dy * dy + dz * dz;
+        const float inv_r = __builtin_amdgcn_rsqf(r2);
+        const float hr = h - r2 * inv_r;
+        const float inv_d2 = __builtin_amdgcn_rcpf(pi.w * pj.w);
+        // (p_i + p_j) / (rho_i rho_j) * (d / |d|) * (h - |d|)^2
+        const float s = (vi.w + vj.w) * inv_d2 * hr * hr * inv_r;
+        P.x += s * dx;
+        P.y += s * dy;
+        P.z += s * dz;
+        // (v_j - v_i) / (rho_i rho_j) * (h - |d|)
+        const float w = hr * inv_d2;
+        V.x += w * (vj.x - vi.x);
+        V.y += w * (vj.y - vi.y);
+        V.z += w * (vj.z - vi.z);
+    }
+}
+
+template <int ORDER>
+__device__ inline void forces_of(uint32_t t, uint32_t i, const float4* __restrict__ spos,
+                                 const float4* __restrict__ svel, const uint32_t* __restrict__ cell_start,
+                                 const uint32_t* __restrict__ cell_end, const uint32_t* __restrict__ nbr,
+                                 const uint32_t* __restrict__ ncount, uint32_t n, float h, const float4& pi,
+                                 const float4& vi, float3& P, float3& V)
+{
+    const uint32_t count = ncount[t];
+    if (count <= kNbrCap)
+        sph_force_sum_nbr(spos, svel, nbr, count, t, n, pi, vi, h, P, V);
+    else
+        sph_force_sum(spos, svel, cell_start, cell_end, i, n, pi, vi, h, P, V);
+}
+
+template <int ORDER>
+__global__ __launch_bounds__(kThreads) void k_forces_nbr(const float4* __restrict__ spos,
+                                                         const float4* __restrict__ svel, float4* __restrict__ force,
+                                                         const uint32_t* __restrict__ index,
+                                                         const uint32_t* __restrict__ cell_start,
+                                                         const uint32_t* __restrict__ cell_end,
+                                                         const uint32_t* __restrict__ nbr,
+                                                         const uint32_t* __restrict__ ncount, uint32_t n, mfl_params p,
+                                                         const uint32_t* __restrict__ porder,
+                                                         const uint32_t* __restrict__ inv)
+{
+    const uint32_t t = blockIdx.x * kThreads + threadIdx.x;
+    if (t >= n) return;
+    const uint32_t i = sorted_slot<ORDER>(t, n, porder, inv);
+    if (i >= n) return;
+    const float4 pi = spos[i], vi = svel[i];
+    float3 P, V;
+    forces_of<ORDER>(t, i, spos, svel, cell_start, cell_end, nbr, ncount, n, p.cellSize, pi, vi, P, V);
+    const uint32_t o = caller_slot<ORDER>(t, i, porder, index);
+    if (o >= n) return;
+    force[o] = add_sph_force(force[o], P, V, p);
+}
+
+// Forces + clip box + integrate force + gravity + integrate velocity for one particle: sorted slot i,
+// caller-order slot o = index[i].
+template <int ORDER>
 __global__ __launch_bounds__(kThreads) void k_forces_step(const float4* __restrict__ spos,
                                                           const float4* __restrict__ svel, float4* __restrict__ pos,
                                                           float4* __restrict__ vel, float4* __restrict__ force,
                                                           const uint32_t* __restrict__ index,
                                                           const uint32_t* __restrict__ cell_start,
-                                                          const uint32_t* __restrict__ cell_end, uint32_t n,
+                                                          const uint32_t* __restrict__ cell_end,
+                                                          const uint32_t* __restrict__ nbr,
+                                                          const uint32_t* __restrict__ ncount, uint32_t n,
                                                           mfl_params p, float4 box_min, float4 box_max,
-                                                          float4 gravity_force)
+                                                          float4 gravity_force, const uint32_t* __restrict__ porder,
+                                                          const uint32_t* __restrict__ inv)
 {
-    const uint32_t i = blockIdx.x * kThreads + threadIdx.x;
+    const uint32_t t = blockIdx.x * kThreads + threadIdx.x;
+    if (t >= n) return;
+    const uint32_t i = sorted_slot<ORDER>(t, n, porder, inv);
     if (i >= n) return;
     const float4 pi = spos[i], vi = svel[i];
     float3 P, V;
-    sph_force_sum(spos, svel, cell_start, cell_end, i, n, pi, vi, p.cellSize, P, V);
-    const uint32_t o = index[i];
+    forces_of<ORDER>(t, i, spos, svel, cell_start, cell_end, nbr, ncount, n, p.cellSize, pi, vi, P, V);
+    const uint32_t o = caller_slot<ORDER>(t, i, porder, index);
     if (o >= n) return;
     float4 f = add_sph_force(force[o], P, V, p);
     f = clip_box(f, pi, vi, p.particleRadius, box_min, box_max);
@@ -268,6 +552,27 @@ __global__ __launch_bounds__(kThreads) void k_forces_step(const float4* __restri
     force[o] = gravity_force;
     vel[o] = v;
     pos[o] = f4_add(pos[o], f4_scale(v, p.deltaTime));
+}
+
+// Morton code of each particle's cell (10 bits per axis, wrapped), for the listed traversal order.
+__device__ inline uint32_t spread_bits3(uint32_t v)
+{
+    v &= 0x3FFu;
+    v = (v | (v << 16)) & 0x030000FFu;
+    v = (v | (v << 8)) & 0x0300F00Fu;
+    v = (v | (v << 4)) & 0x030C30C3u;
+    v = (v | (v << 2)) & 0x09249249u;
+    return v;
+}
+
+__global__ __launch_bounds__(kThreads) void k_morton_keys(const float4* __restrict__ pos, uint32_t n, float h,
+                                                          uint32_t* __restrict__ key)
+{
+    const uint32_t o = blockIdx.x * kThreads + threadIdx.x;
+    if (o >= n) return;
+    const float4 q = pos[o];
+    key[o] = spread_bits3((uint32_t)grid_coord(q.x, h)) | (spread_bits3((uint32_t)grid_coord(q.y, h)) << 1) |
+             (spread_bits3((uint32_t)grid_coord(q.z, h)) << 2);
 }
 
 __global__ __launch_bounds__(kThreads) void k_clip_box(const float4* __restrict__ spos, const float4* __restrict__ svel,
@@ -412,7 +717,7 @@ Fluid3D::~Fluid3D()
 {
     drop_graph();
     for (DeviceBuffer* b : {&pos_, &vel_, &force_, &cell_, &index_, &spos_, &svel_, &cell_start_, &cell_end_,
-                            &params_buf_})
+                            &params_buf_, &nbr_, &ncount_, &inv_, &porder_, &mkey_})
         b->release();
 }
 
@@ -422,6 +727,10 @@ bool Fluid3D::allocate_internal()
     const size_t v4 = n_ * 16, u1 = n_ * 4;
     if (!cell_.allocate(u1) || !index_.allocate(u1) || !spos_.allocate(v4) || !svel_.allocate(v4)) return false;
     if (!cell_start_.allocate(kBuckets * 4) || !cell_end_.allocate(kBuckets * 4)) return false;
+    if (!nbr_.allocate((size_t)kNbrCap * u1) || !ncount_.allocate(u1)) return false;
+    if (!inv_.allocate(u1) || !porder_.allocate(u1) || !mkey_.allocate(u1)) return false;
+    porder_valid_ = false;
+    steps_since_refresh_ = 0;
     // the reference leaves the tables uninitialised (cpp:53-54); here they start empty
     if (!hip_ok(hipMemset(cell_start_.ptr, 0xFF, kBuckets * 4), "hipMemset") ||
         !hip_ok(hipMemset(cell_end_.ptr, 0xFF, kBuckets * 4), "hipMemset"))
@@ -571,23 +880,72 @@ int Fluid3D::enqueue_step(hipStream_t s)
     timer.end(MFL_STAGE_SORT, s);
     timer.begin(MFL_STAGE_REORDER, s);
     hipLaunchKernelGGL(k_reorder_clear, grid_for(n > clear ? n : clear), b, 0, s, index, pos, vel, spos, svel, n, cs,
-                       ce, clear);
+                       ce, clear, inv_.as<uint32_t>());
     timer.end(MFL_STAGE_REORDER, s);
     timer.begin(MFL_STAGE_BOUNDS, s);
     hipLaunchKernelGGL(k_find_bounds, g, b, 0, s, cell, n, cs, ce);
     timer.end(MFL_STAGE_BOUNDS, s);
+    auto* nbr = nbr_.as<uint32_t>();
+    auto* ncount = ncount_.as<uint32_t>();
+    auto* inv = inv_.as<uint32_t>();
+    auto* porder = porder_.as<uint32_t>();
+    const int order = (traversal == MFL_TRAVERSAL_MORTON && !porder_valid_) ? MFL_TRAVERSAL_CALLER : traversal;
     timer.begin(MFL_STAGE_DENSITY, s);
-    hipLaunchKernelGGL(k_density, g, b, 0, s, spos, svel, cs, ce, n, params);
+    if (density_lanes == 1) {
+        switch (order) {
+        case MFL_TRAVERSAL_SORTED:
+            hipLaunchKernelGGL(k_density_nbr<kOrderSorted>, g, b, 0, s, spos, svel, cs, ce, n, params, nbr, ncount,
+                               porder, inv);
+            break;
+        case MFL_TRAVERSAL_CALLER:
+            hipLaunchKernelGGL(k_density_nbr<kOrderCaller>, g, b, 0, s, spos, svel, cs, ce, n, params, nbr, ncount,
+                               porder, inv);
+            break;
+        default:
+            hipLaunchKernelGGL(k_density_nbr<kOrderListed>, g, b, 0, s, spos, svel, cs, ce, n, params, nbr, ncount,
+                               porder, inv);
+        }
+    } else {
+        const dim3 gg = grid_for((size_t)n * kGroupLanes);
+        switch (order) {
+        case MFL_TRAVERSAL_SORTED:
+            hipLaunchKernelGGL(k_density_grp<kOrderSorted>, gg, b, 0, s, spos, svel, cs, ce, n, params, nbr, ncount,
+                               porder, inv);
+            break;
+        case MFL_TRAVERSAL_CALLER:
+            hipLaunchKernelGGL(k_density_grp<kOrderCaller>, gg, b, 0, s, spos, svel, cs, ce, n, params, nbr, ncount,
+                               porder, inv);
+            break;
+        default:
+            hipLaunchKernelGGL(k_density_grp<kOrderListed>, gg, b, 0, s, spos, svel, cs, ce, n, params, nbr, ncount,
+                               porder, inv);
+        }
+    }
     timer.end(MFL_STAGE_DENSITY, s);
     timer.begin(MFL_STAGE_FORCES, s);
     if (integrate_cb) {
-        hipLaunchKernelGGL(k_forces, g, b, 0, s, spos, svel, force, index, cs, ce, n, params);
+#define MFL_FORCES_NBR(O) \
+    hipLaunchKernelGGL(k_forces_nbr<O>, g, b, 0, s, spos, svel, force, index, cs, ce, nbr, ncount, n, params, porder, inv)
+        switch (order) {
+        case MFL_TRAVERSAL_SORTED: MFL_FORCES_NBR(kOrderSorted); break;
+        case MFL_TRAVERSAL_CALLER: MFL_FORCES_NBR(kOrderCaller); break;
+        default: MFL_FORCES_NBR(kOrderListed);
+        }
+#undef MFL_FORCES_NBR
         timer.end(MFL_STAGE_FORCES, s);
         MFL_HIP(hipGetLastError());
         (void)integrate_cb(handle, (void*)s, integrate_user);  // return value ignored, cpp:325-328
         return 1;
     }
-    hipLaunchKernelGGL(k_forces_step, g, b, 0, s, spos, svel, pos, vel, force, index, cs, ce, n, params, mn, mx, gf);
+#define MFL_FORCES_STEP(O)                                                                                        \
+    hipLaunchKernelGGL(k_forces_step<O>, g, b, 0, s, spos, svel, pos, vel, force, index, cs, ce, nbr, ncount, n, \
+                       params, mn, mx, gf, porder, inv)
+    switch (order) {
+    case MFL_TRAVERSAL_SORTED: MFL_FORCES_STEP(kOrderSorted); break;
+    case MFL_TRAVERSAL_CALLER: MFL_FORCES_STEP(kOrderCaller); break;
+    default: MFL_FORCES_STEP(kOrderListed);
+    }
+#undef MFL_FORCES_STEP
     timer.end(MFL_STAGE_FORCES, s);
     MFL_HIP(hipGetLastError());
     return 1;
@@ -606,7 +964,7 @@ int Fluid3D::launch_graph(hipStream_t s)
 {
     std::ostringstream sig;
     sig << n_ << ':' << pos_.ptr << ':' << vel_.ptr << ':' << force_.ptr << ':' << clear_count << ':'
-        << sort.tuning.items << ':';
+        << sort.tuning.items << ':' << traversal << ':' << porder_valid_ << ':' << density_lanes << ':';
     const unsigned char* raw[] = {reinterpret_cast<const unsigned char*>(&params),
                                   reinterpret_cast<const unsigned char*>(box_min),
                                   reinterpret_cast<const unsigned char*>(box_max),
@@ -634,9 +992,28 @@ int Fluid3D::launch_graph(hipStream_t s)
     return 1;
 }
 
+int Fluid3D::refresh_order(hipStream_t s)
+{
+    const uint32_t n = (uint32_t)n_;
+    timer.begin(MFL_STAGE_ORDER, s);
+    hipLaunchKernelGGL(k_morton_keys, grid_for(n), dim3(kThreads), 0, s, pos_.as<float4>(), n, params.cellSize,
+                       mkey_.as<uint32_t>());
+    MFL_HIP(hipGetLastError());
+    if (!sort.sort_bits(mkey_.as<uint32_t>(), porder_.as<uint32_t>(), n, 0, 30, s, true)) return 0;
+    timer.end(MFL_STAGE_ORDER, s);
+    porder_valid_ = true;
+    return 1;
+}
+
 int Fluid3D::compute(hipStream_t s)
 {
     if (!check_user_buffers()) return 0;
+    if (traversal == MFL_TRAVERSAL_MORTON &&
+        (!porder_valid_ || (refresh_interval > 0 && steps_since_refresh_ >= refresh_interval))) {
+        if (!refresh_order(s)) return 0;
+        steps_since_refresh_ = 0;
+    }
+    steps_since_refresh_++;
     if (use_graph && !integrate_cb && s != nullptr && timer.mask == 0) return launch_graph(s);
     return enqueue_step(s);
 }
@@ -665,7 +1042,7 @@ int Fluid3D::enqueue_kernel(const char* name, hipStream_t s)
         hipLaunchKernelGGL(k_hash<true>, g, b, 0, s, pos, n, params.cellSize, cell, index,
                            params_buf_.as<mfl_params>(), params);
     else if (k == "clReorder")
-        hipLaunchKernelGGL(k_reorder_clear, g, b, 0, s, index, pos, vel, spos, svel, n, cs, ce, 0u);
+        hipLaunchKernelGGL(k_reorder_clear, g, b, 0, s, index, pos, vel, spos, svel, n, cs, ce, 0u, inv_.as<uint32_t>());
     else if (k == "clInitBounds")
         hipLaunchKernelGGL(k_fill_buckets, grid_for(kBuckets), b, 0, s, cs, ce, kBuckets);
     else if (k == "clFindBounds")

@@ -55,6 +55,9 @@ public:
     void* integrate_user = nullptr;
     mfl_fluid3d* handle = nullptr;                   // C handle passed back to the callback
     bool use_graph = false;
+    int traversal = MFL_TRAVERSAL_MORTON;   // order in which the neighbour passes visit particles
+    uint32_t refresh_interval = 16;         // steps between Morton order refreshes
+    int density_lanes = 8;                  // lanes per particle in the density pass (1 or 8)
     RadixSort sort;
     StageTimer timer;
 
@@ -69,6 +72,12 @@ private:
     size_t n_ = kGroup;                              // mParticleCount default, cpp:49
     DeviceBuffer pos_, vel_, force_;
     DeviceBuffer cell_, index_, spos_, svel_, cell_start_, cell_end_, params_buf_;
+    DeviceBuffer nbr_, ncount_;  // per-step neighbour visits recorded by the density pass
+    DeviceBuffer inv_;           // inverse of the sort permutation: inv[index[i]] = i
+    DeviceBuffer porder_, mkey_; // Morton traversal order (caller indices) and its sort keys
+    bool porder_valid_ = false;
+    uint32_t steps_since_refresh_ = 0;
+    int refresh_order(hipStream_t s);
 
     hipGraph_t graph_ = nullptr;
     hipGraphExec_t graph_exec_ = nullptr;

@@ -202,6 +202,10 @@ class oclFluid3D:
     def setUseGraph(self, enable: bool) -> int:
         return int(N.load().mfl_fluid3d_set_use_graph(self._h, 1 if enable else 0))
 
+    def setTraversal(self, order: str, refresh_steps: int = 16) -> int:
+        """Visit order of the neighbour passes: "sorted", "caller" or "morton" (results unchanged)."""
+        return int(N.load().mfl_fluid3d_set_traversal(self._h, N.TRAVERSALS.index(order), refresh_steps))
+
     def setProfiling(self, stages) -> int:
         """Times the named stages (subset of _native.STAGES) with HIP events (oclKernel::profile)."""
         mask = 0

@@ -240,3 +240,25 @@ def test_particle_count_rules():
     assert f.setPositionBuffer(big) == 1
     assert f.compute() == 1
     L.load().mfl_device_finish()
+
+
+def test_traversal_orders_bit_identical():
+    """The visit order of the neighbour passes changes no result: sorted, caller and Morton
+    traversal give bit-identical state over several steps (each particle's sums keep the reference's
+    visit order whatever thread computes them)."""
+    import libcl_amd as L
+    sc = scenes.dam_break(32, 16, 16, 1.0, seed=41, steps=5)
+    prm = L.params_from(sc.params)
+    states = []
+    for order in ("sorted", "caller", "morton"):
+        f = make_fluid(sc.pos, sc.vel, sc.force, prm, sc.box_min, sc.box_max)
+        assert f.setTraversal(order, 2) == 1
+        for _ in range(5):
+            assert f.compute() == 1, L.last_error()
+        L.load().mfl_device_finish()
+        states.append((f.state(), f.internals()))
+    for st, it in states[1:]:
+        for k in ("pos", "vel", "force"):
+            np.testing.assert_array_equal(st[k], states[0][0][k])
+        for k in ("spos", "svel", "cell_start", "cell_end", "index"):
+            np.testing.assert_array_equal(it[k], states[0][1][k])
