@@ -331,7 +331,9 @@ void orc_density(float* spos4, float* svel4, const uint32_t* cell_start, const u
                     uint32_t c = orc_hash_key(g[0] + x, g[1] + y, g[2] + z);
                     uint32_t start = cell_start[c];
                     if (start == ORC_EMPTY) continue;
-                    uint32_t end = cell_end[c];
+                    /* ranges lie in [0, n] for any n the reference accepts (a multiple of 256);
+                       the clamp only matters for n == 1, where cl:100-104 never writes the end */
+                    uint32_t end = cell_end[c] < n ? cell_end[c] : n;
                     for (uint32_t j = start; j < end; j++) {
                         if (j == (uint32_t)i) continue;
                         const float* pj = spos4 + 4 * (size_t)j;
@@ -379,7 +381,9 @@ void orc_forces(const float* spos4, const float* svel4, float* force4, const uin
                     uint32_t c = orc_hash_key(g[0] + x, g[1] + y, g[2] + z);
                     uint32_t start = cell_start[c];
                     if (start == ORC_EMPTY) continue;
-                    uint32_t end = cell_end[c];
+                    /* ranges lie in [0, n] for any n the reference accepts (a multiple of 256);
+                       the clamp only matters for n == 1, where cl:100-104 never writes the end */
+                    uint32_t end = cell_end[c] < n ? cell_end[c] : n;
                     for (uint32_t j = start; j < end; j++) {
                         if (j == (uint32_t)i) continue;
                         const float* pj = spos4 + 4 * (size_t)j;
