@@ -52,6 +52,118 @@ def cpu_baseline(sc, prm, steps: int, threads: int) -> dict:
                       f"oracle/sph_oracle.c with {used} OpenMP threads ({dt:.1f} s)"}
 
 
+def batched_leg(steps: int, seq_steps: int, num_scenes: int = 256) -> dict:
+    """256 independent scenes (log-uniform 2,048..65,536 particles) advanced together, against the same
+    scenes stepped one after another by single-scene programs on the same stream."""
+    import numpy as np
+    import torch
+
+    import libcl_amd as L
+    from libcl_amd import scenes
+
+    scs = scenes.batched_scenes(num=num_scenes, seed=5, steps=steps)
+    prm = L.params_from(scs[0].params)
+    total = sum(sc.n for sc in scs)
+    stream = torch.cuda.Stream()
+    b = L.FluidBatch([sc.n for sc in scs])
+    assert b.setParameters(prm) == 1 and b.setBox((0, 0, 0, 0), (1, 1, 1, 0)) == 1
+    assert b.upload(np.concatenate([sc.pos for sc in scs]), np.concatenate([sc.vel for sc in scs]),
+                    np.concatenate([sc.force for sc in scs])) == 1
+    b.setUseGraph(True)
+    for _ in range(3):
+        assert b.compute(stream) == 1, L.last_error()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        b.compute(stream)
+    torch.cuda.synchronize()
+    batched_ms = (time.perf_counter() - t0) * 1e3 / steps
+
+    progs = []
+    for sc in scs:
+        f = L.oclFluid3D()
+        assert f.compile() == 1 and f.setParticleCount(sc.n) == 1
+        f.setParameters(prm)
+        f.setBox(sc.box_min, sc.box_max)
+        f.setTraversal("sorted", 0)
+        assert f.upload(sc.pos, sc.vel, sc.force) == 1
+        progs.append(f)
+    out = {}
+    for mode in ("eager", "graph"):
+        for f in progs:
+            f.setUseGraph(mode == "graph")
+            f.compute(stream)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(seq_steps):
+            for f in progs:
+                f.compute(stream)
+        torch.cuda.synchronize()
+        out[mode] = (time.perf_counter() - t0) * 1e3 / seq_steps
+    del progs
+    return {"scenes": num_scenes, "particles": total, "steps": steps, "ms_per_step": batched_ms,
+            "particle_steps_per_s": total / (batched_ms / 1e3),
+            "sequential_ms_per_step": out["eager"], "sequential_graph_ms_per_step": out["graph"],
+            "sequential_steps": seq_steps, "gain": out["eager"] / batched_ms,
+            "gain_vs_graphed_sequential": out["graph"] / batched_ms}
+
+
+def adjoint_leg(steps: int = 200, every: int = 20) -> dict:
+    """Differentiable case: 65,536 particles (64x32x32), 200 steps, checkpoints every 20. Times the
+    checkpointed forward rollout and the backward pass (segment recompute + reverse steps) on one
+    stream; the loss is mean |x_T - target|^2 in FP64."""
+    import numpy as np
+    import torch
+
+    import libcl_amd as L
+    from libcl_amd import scenes
+
+    sc, target = scenes.config_grad()
+    prm = L.params_from(sc.params)
+    f = L.oclFluid3D()
+    assert f.compile() == 1 and f.setParticleCount(sc.n) == 1
+    f.setParameters(prm)
+    f.setBox(sc.box_min, sc.box_max)
+    adj = L.Adjoint(f, steps, every, 192)
+    stream = torch.cuda.Stream()
+    tgt = torch.from_numpy(target).cuda()
+
+    def run():
+        assert f.upload(sc.pos, sc.vel, sc.force) == 1
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        assert adj.forward(stream) == 1, L.last_error()
+        torch.cuda.synchronize()
+        t1 = time.perf_counter()
+        xt = torch.from_numpy(f.state()["pos"]).cuda()
+        xd = xt[:, :3].double()
+        loss = ((xd - tgt) ** 2).sum(1).mean()
+        gbar = torch.zeros_like(xt)
+        gbar[:, :3] = (2.0 * (xd - tgt) / sc.n).float()
+        torch.cuda.synchronize()
+        t2 = time.perf_counter()
+        assert adj.backward(gbar.contiguous(), None, stream) == 1, L.last_error()
+        torch.cuda.synchronize()
+        t3 = time.perf_counter()
+        return t1 - t0, t3 - t2, loss.item()
+
+    run()  # warm-up (allocations, first launches)
+    fwd, bwd, loss = run()
+    pg = adj.param_grads()
+    # plain steps of the same program without checkpoints, for reference
+    assert f.upload(sc.pos, sc.vel, sc.force) == 1
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        f.compute(stream)
+    torch.cuda.synchronize()
+    plain = time.perf_counter() - t0
+    return {"particles": sc.n, "steps": steps, "checkpoint_every": every, "loss": loss,
+            "forward_ms_per_step": fwd * 1e3 / steps, "backward_ms_per_step": bwd * 1e3 / steps,
+            "plain_ms_per_step": plain * 1e3 / steps, "adjoint_over_forward": bwd / fwd,
+            "grad_viscosity": pg["viscosity"], "grad_stiffness": pg["stiffness"], "grad_density": pg["density"]}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -61,17 +173,30 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-steps", type=int, default=20)
     ap.add_argument("--order", default="morton", choices=["sorted", "caller", "morton"])
+    ap.add_argument("--no-batched", action="store_true")
+    ap.add_argument("--no-adjoint", action="store_true")
+    ap.add_argument("--only", default=None, choices=["batched", "adjoint"],
+                    help="run one secondary leg alone and print its JSON (profiling)")
+    ap.add_argument("--batched-steps", type=int, default=100)
+    ap.add_argument("--sequential-steps", type=int, default=5)
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for {args.gpus}); the step is not sharded across GPUs",
               file=sys.stderr)
         sys.exit(2)
 
-    import numpy as np
+    import numpy as np  # noqa: F401
     import torch
 
     import libcl_amd as L
     from libcl_amd import scenes
+
+    if args.only == "batched":
+        print(json.dumps(batched_leg(args.batched_steps, args.sequential_steps)), flush=True)
+        return
+    if args.only == "adjoint":
+        print(json.dumps(adjoint_leg()), flush=True)
+        return
 
     sc = scenes.config_1m() if args.config == "1m" else scenes.config_small()
     prm = L.params_from(sc.params)
@@ -154,6 +279,10 @@ def main():
         "graph_ms_per_step": graph_ms,
         "graph_particle_steps_per_s": n / (graph_ms / 1e3),
     }
+    if not args.no_batched:
+        out["batched"] = batched_leg(args.batched_steps, args.sequential_steps)
+    if not args.no_adjoint:
+        out["adjoint"] = adjoint_leg()
     if not args.no_cpu:
         nthreads = min(os.cpu_count() or 1, 16)
         out["cpu_baseline"] = cpu_baseline(sc, prm, args.cpu_steps, nthreads)

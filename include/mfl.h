@@ -72,6 +72,9 @@ int mfl_abi_version(void);
 int mfl_set_device(int ordinal);
 /* clFinish on the program's queue (libCL/TEST/main.cpp:217): hipDeviceSynchronize. */
 int mfl_device_finish(void);
+/* Blocking copy between any two host / device addresses (oclBuffer::read / write,
+   libCL/oclBuffer.cpp:47-83, for buffers that are not program-owned). */
+int mfl_memcpy(void* dst, const void* src, size_t bytes);
 
 /* ---- params ------------------------------------------------------------------------------------- */
 
@@ -175,6 +178,51 @@ int mfl_fluid3d_set_integrate_callback(mfl_fluid3d* f, mfl_integrate_fn fn, void
    libCL/TEST/main.cpp:282-302). Names: clHash, clReorder, clFindBounds, clCalculateDensity,
    clCalculateForces, clClipBox, clIntegrateForce, clGravity, clIntegrateVelocity. */
 int mfl_fluid3d_enqueue_kernel(mfl_fluid3d* f, const char* name, void* stream);
+
+/* ---- batched scenes (addition; no counterpart in the reference) --------------------------------- */
+
+/* Many independent oclFluid3D systems advanced together: one set of launches per step for all scenes
+   (optionally one hipGraph replay), scene-offset keys and one segmented sort. Scene s owns particles
+   [offset_s, offset_s + count_s) of the concatenated float4 position / velocity / force buffers, its
+   own Params, box and pair of 2^18-entry bucket tables (scene-local indices, as a single-scene program
+   holds them), and its state after every step equals that of a single-scene program run alone.
+   counts[s] must be non-zero multiples of 256. Buffer kinds as for mfl_fluid3d_*: CELL_START and
+   CELL_END are num_scenes * 262144 entries, INDEX holds batch-global caller indices. */
+typedef struct mfl_fluid_batch mfl_fluid_batch;
+mfl_fluid_batch* mfl_batch_create(uint32_t num_scenes, const uint32_t* counts);
+void mfl_batch_destroy(mfl_fluid_batch* b);
+int mfl_batch_scene_range(mfl_fluid_batch* b, uint32_t scene, uint32_t* offset, uint32_t* count);
+/* scene < 0 applies to every scene */
+int mfl_batch_set_params(mfl_fluid_batch* b, int scene, const mfl_params* p);
+int mfl_batch_get_params(mfl_fluid_batch* b, uint32_t scene, mfl_params* p);
+int mfl_batch_set_box(mfl_fluid_batch* b, int scene, const float* min4, const float* max4);
+int mfl_batch_set_gravity(mfl_fluid_batch* b, const float* dir4);
+int mfl_batch_set_bucket_clear(mfl_fluid_batch* b, uint32_t count);
+int mfl_batch_set_use_graph(mfl_fluid_batch* b, int enable);
+int mfl_batch_get_buffer(mfl_fluid_batch* b, int kind, void** ptr, size_t* bytes);
+int mfl_batch_write_buffer(mfl_fluid_batch* b, int kind, const void* host, size_t bytes);
+int mfl_batch_read_buffer(mfl_fluid_batch* b, int kind, void* host, size_t bytes);
+int mfl_batch_compute(mfl_fluid_batch* b, void* stream);
+
+/* ---- adjoint of a rollout (addition; no counterpart in the reference) ---------------------------- */
+
+/* Reverse-mode derivative of `steps` compute() steps of a program, with checkpoints every
+   `checkpoint_every` steps and recomputation in between. The sort permutation and the neighbour visits
+   of each step (duplicates and stale-range visits included) are replayed from the forward as fixed
+   data. list_cap bounds the recorded visits per particle (backward fails with an error beyond it).
+   Gradients: dL/dx_0 and dL/dv_0 (caller-order float4 device buffers), and dL/d viscosity,
+   stiffness, rest density (FP64). The program must keep its particle count and buffers meanwhile. */
+enum mfl_adjoint_buffer { MFL_ADJ_POSITION_GRAD = 0, MFL_ADJ_VELOCITY_GRAD = 1 };
+typedef struct mfl_adjoint mfl_adjoint;
+mfl_adjoint* mfl_adjoint_create(mfl_fluid3d* f, uint32_t steps, uint32_t checkpoint_every, uint32_t list_cap);
+void mfl_adjoint_destroy(mfl_adjoint* a);
+/* runs the rollout from the program's current state (the program ends in the final state) */
+int mfl_adjoint_forward(mfl_adjoint* a, void* stream);
+/* pos_bar / vel_bar: dL/dx_T, dL/dv_T as caller-order float4 device arrays (NULL = zero) */
+int mfl_adjoint_backward(mfl_adjoint* a, const void* pos_bar, const void* vel_bar, void* stream);
+int mfl_adjoint_get_buffer(mfl_adjoint* a, int which, void** ptr, size_t* bytes);
+/* out[0..3] = dL/d viscosity, dL/d stiffness, dL/d rest density, dL/d viscosityConstant */
+int mfl_adjoint_param_grads(mfl_adjoint* a, double* out4);
 
 #ifdef __cplusplus
 }

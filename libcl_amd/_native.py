@@ -21,7 +21,7 @@ PARAM_FIELDS = (
 
 # every symbol include/mfl.h declares
 EXPORTS = (
-    "mfl_last_error", "mfl_clear_error", "mfl_abi_version", "mfl_set_device", "mfl_device_finish",
+    "mfl_last_error", "mfl_clear_error", "mfl_abi_version", "mfl_set_device", "mfl_device_finish", "mfl_memcpy",
     "mfl_params_default", "mfl_params_derive",
     "mfl_radixsort_create", "mfl_radixsort_destroy", "mfl_radixsort_compile", "mfl_radixsort_compute",
     "mfl_fluid3d_create", "mfl_fluid3d_destroy", "mfl_fluid3d_compile", "mfl_fluid3d_compute",
@@ -31,7 +31,14 @@ EXPORTS = (
     "mfl_fluid3d_set_use_graph", "mfl_fluid3d_set_traversal", "mfl_fluid3d_set_profiling",
     "mfl_fluid3d_stage_time",
     "mfl_fluid3d_set_integrate_callback", "mfl_fluid3d_enqueue_kernel",
+    "mfl_batch_create", "mfl_batch_destroy", "mfl_batch_scene_range", "mfl_batch_set_params",
+    "mfl_batch_get_params", "mfl_batch_set_box", "mfl_batch_set_gravity", "mfl_batch_set_bucket_clear",
+    "mfl_batch_set_use_graph", "mfl_batch_get_buffer", "mfl_batch_write_buffer", "mfl_batch_read_buffer",
+    "mfl_batch_compute",
+    "mfl_adjoint_create", "mfl_adjoint_destroy", "mfl_adjoint_forward", "mfl_adjoint_backward",
+    "mfl_adjoint_get_buffer", "mfl_adjoint_param_grads",
 )
+MFL_ADJ_POSITION_GRAD, MFL_ADJ_VELOCITY_GRAD = 0, 1
 
 STAGES = ("hash", "sort", "reorder", "bounds", "density", "forces", "order")
 TRAVERSALS = ("sorted", "caller", "morton")
@@ -77,6 +84,7 @@ def load() -> ctypes.CDLL:
         "mfl_abi_version": (ctypes.c_int, []),
         "mfl_set_device": (ctypes.c_int, [ctypes.c_int]),
         "mfl_device_finish": (ctypes.c_int, []),
+        "mfl_memcpy": (ctypes.c_int, [vp, vp, sz]),
         "mfl_params_default": (None, [pp]),
         "mfl_params_derive": (None, [pp]),
         "mfl_radixsort_create": (vp, []),
@@ -104,6 +112,26 @@ def load() -> ctypes.CDLL:
                                                   ctypes.POINTER(ctypes.c_uint64), ctypes.c_int]),
         "mfl_fluid3d_set_integrate_callback": (ctypes.c_int, [vp, INTEGRATE_FN, vp]),
         "mfl_fluid3d_enqueue_kernel": (ctypes.c_int, [vp, ctypes.c_char_p, vp]),
+        "mfl_batch_create": (vp, [ctypes.c_uint32, u32p]),
+        "mfl_batch_destroy": (None, [vp]),
+        "mfl_batch_scene_range": (ctypes.c_int, [vp, ctypes.c_uint32, u32p, u32p]),
+        "mfl_batch_set_params": (ctypes.c_int, [vp, ctypes.c_int, pp]),
+        "mfl_batch_get_params": (ctypes.c_int, [vp, ctypes.c_uint32, pp]),
+        "mfl_batch_set_box": (ctypes.c_int, [vp, ctypes.c_int, ctypes.POINTER(ctypes.c_float),
+                                             ctypes.POINTER(ctypes.c_float)]),
+        "mfl_batch_set_gravity": (ctypes.c_int, [vp, ctypes.POINTER(ctypes.c_float)]),
+        "mfl_batch_set_bucket_clear": (ctypes.c_int, [vp, ctypes.c_uint32]),
+        "mfl_batch_set_use_graph": (ctypes.c_int, [vp, ctypes.c_int]),
+        "mfl_batch_get_buffer": (ctypes.c_int, [vp, ctypes.c_int, ctypes.POINTER(vp), ctypes.POINTER(sz)]),
+        "mfl_batch_write_buffer": (ctypes.c_int, [vp, ctypes.c_int, vp, sz]),
+        "mfl_batch_read_buffer": (ctypes.c_int, [vp, ctypes.c_int, vp, sz]),
+        "mfl_batch_compute": (ctypes.c_int, [vp, vp]),
+        "mfl_adjoint_create": (vp, [vp, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint32]),
+        "mfl_adjoint_destroy": (None, [vp]),
+        "mfl_adjoint_forward": (ctypes.c_int, [vp, vp]),
+        "mfl_adjoint_backward": (ctypes.c_int, [vp, vp, vp, vp]),
+        "mfl_adjoint_get_buffer": (ctypes.c_int, [vp, ctypes.c_int, ctypes.POINTER(vp), ctypes.POINTER(sz)]),
+        "mfl_adjoint_param_grads": (ctypes.c_int, [vp, ctypes.POINTER(ctypes.c_double)]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(L, name)

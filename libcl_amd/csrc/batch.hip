@@ -1,0 +1,442 @@
+// Batched scenes on gfx950: S independent oclFluid3D systems (libCL/phys/oclFluid3D.cpp:293-342 each)
+// advanced by one set of launches per step, optionally replayed from one hipGraph.
+//
+// Layout: the scenes' particles are concatenated scene by scene (offsets off[s], counts multiples of
+// 256); each scene keeps its own pair of 2^18-entry bucket tables, holding scene-local indices exactly
+// as a single-scene program would. Keys carry the scene in their high bits, key = s << 18 | hash, so
+// one stable sort of the concatenation is the segmented sort: it orders every scene by hash and keeps
+// the scenes in place. Every per-particle computation reuses the single-scene arithmetic
+// (sph_device.hpp) on scene-local pointers, so each scene's state is bit-identical to running it alone.
+#include "batch.hpp"
+#include "sph_device.hpp"
+
+#include <cstring>
+#include <sstream>
+
+namespace mfl {
+namespace {
+
+using namespace sph;
+
+__global__ __launch_bounds__(kThreads) void k_b_hash(const float4* __restrict__ pos, const uint32_t* __restrict__ pscene,
+                                                     const mfl_params* __restrict__ params, uint32_t N,
+                                                     uint32_t* __restrict__ cell)
+{
+    const uint32_t g = blockIdx.x * kThreads + threadIdx.x;
+    if (g >= N) return;
+    const uint32_t s = pscene[g];
+    const float h = params[s].cellSize;
+    const float4 q = pos[g];
+    cell[g] = (s << 18) | cell_hash(grid_coord(q.x, h), grid_coord(q.y, h), grid_coord(q.z, h));
+}
+
+__global__ __launch_bounds__(kThreads) void k_b_reorder(const uint32_t* __restrict__ index,
+                                                        const float4* __restrict__ pos, const float4* __restrict__ vel,
+                                                        float4* __restrict__ spos, float4* __restrict__ svel, uint32_t N)
+{
+    const uint32_t i = blockIdx.x * kThreads + threadIdx.x;
+    if (i >= N) return;
+    const uint32_t g = index[i];
+    if (g >= N) return;
+    spos[i] = pos[g];
+    svel[i] = vel[g];
+}
+
+// the reference's per-step reset of the first `clear` entries of each scene's tables (cpp:309-313)
+__global__ __launch_bounds__(kThreads) void k_b_clear(uint32_t* __restrict__ cs_all, uint32_t* __restrict__ ce_all,
+                                                      uint32_t S, uint32_t clear)
+{
+    const size_t total = (size_t)S * clear;
+    for (size_t e = (size_t)blockIdx.x * kThreads + threadIdx.x; e < total; e += (size_t)gridDim.x * kThreads) {
+        const size_t s = e / clear, b = e % clear;
+        cs_all[s * kBuckets + b] = kEmpty;
+        ce_all[s * kBuckets + b] = kEmpty;
+    }
+}
+
+// clFindBounds (cl:85-113) per scene, in scene-local indices
+__global__ __launch_bounds__(kThreads) void k_b_find_bounds(const uint32_t* __restrict__ cell,
+                                                            const uint32_t* __restrict__ off, uint32_t N,
+                                                            uint32_t* __restrict__ cs_all, uint32_t* __restrict__ ce_all)
+{
+    const uint32_t i = blockIdx.x * kThreads + threadIdx.x;
+    if (i >= N) return;
+    const uint32_t key = cell[i];
+    const uint32_t s = key >> 18, h = key & (kBuckets - 1u);
+    const uint32_t lo = off[s], n_s = off[s + 1] - lo, li = i - lo;
+    uint32_t* cs = cs_all + (size_t)s * kBuckets;
+    uint32_t* ce = ce_all + (size_t)s * kBuckets;
+    if (li == 0) {
+        cs[h] = 0;
+        return;
+    }
+    const uint32_t prev = cell[i - 1] & (kBuckets - 1u);
+    if (h != prev) {
+        ce[prev] = li;
+        cs[h] = li;
+    }
+    if (li == n_s - 1) ce[h] = n_s;
+}
+
+// One group of eight lanes per particle (k_density_grp of the single-scene program) on scene s.
+__device__ inline void batch_density_home(uint32_t s, int gx, int gy, int gz, const float4& pi, uint32_t i, uint32_t slot,
+                                          uint32_t grp_shift, bool live, const float4* __restrict__ spos,
+                                          const uint32_t* __restrict__ cs_all, const uint32_t* __restrict__ ce_all,
+                                          const uint32_t* __restrict__ off, const mfl_params* __restrict__ params,
+                                          uint32_t N, uint32_t* __restrict__ nbr, float& acc, uint32_t& count, float& h)
+{
+    const uint32_t lo = off[s], n_s = off[s + 1] - lo;
+    h = params[s].cellSize;
+    const float h2 = h * h;
+    const float h2_lo = h2 * 0.9999996f, h2_hi = h2 * 1.0000005f;
+    density_visits(gx, gy, gz, pi, i - lo, i, slot, grp_shift, live, spos + lo, cs_all + (size_t)s * kBuckets,
+                   ce_all + (size_t)s * kBuckets, n_s, h, h2, h2_lo, h2_hi, nbr, N, acc, count);
+}
+
+__global__ __launch_bounds__(kThreads) void k_b_density(float4* __restrict__ spos, float4* __restrict__ svel,
+                                                        const uint32_t* __restrict__ cs_all,
+                                                        const uint32_t* __restrict__ ce_all,
+                                                        const uint32_t* __restrict__ cell,
+                                                        const uint32_t* __restrict__ off,
+                                                        const mfl_params* __restrict__ params, uint32_t N,
+                                                        uint32_t* __restrict__ nbr, uint32_t* __restrict__ ncount)
+{
+    const uint32_t lane = threadIdx.x & (kWave - 1);
+    const uint32_t grp = lane / kGroupLanes, slot = lane % kGroupLanes;
+    const uint32_t wave = (blockIdx.x * kThreads + threadIdx.x) / kWave;
+    const uint32_t i = wave * (kWave / kGroupLanes) + grp;  // sorted (scene-major) index
+    const bool live = i < N;
+    const uint32_t s = live ? cell[i] >> 18 : 0u;
+    const float h0 = params[s].cellSize;
+    const float4 pi = live ? spos[i] : make_float4(0.f, 0.f, 0.f, 0.f);
+    const int gx = grid_coord(pi.x, h0), gy = grid_coord(pi.y, h0), gz = grid_coord(pi.z, h0);
+    const uint64_t live_mask = __ballot(live);
+    if (live_mask == 0) return;
+    const int src = __ffsll((unsigned long long)live_mask) - 1;
+    const uint32_t s0 = __shfl(s, src, kWave);
+    const int gx0 = __shfl(gx, src, kWave), gy0 = __shfl(gy, src, kWave), gz0 = __shfl(gz, src, kWave);
+    const bool same = !live || (s == s0 && gx == gx0 && gy == gy0 && gz == gz0);
+    float acc = 0.f, h = h0;
+    uint32_t count = 0;
+    const uint32_t grp_shift = grp * kGroupLanes;
+    if (__ballot(!same) == 0)
+        batch_density_home(__builtin_amdgcn_readfirstlane(s0), __builtin_amdgcn_readfirstlane(gx0),
+                           __builtin_amdgcn_readfirstlane(gy0), __builtin_amdgcn_readfirstlane(gz0), pi, i, slot,
+                           grp_shift, live, spos, cs_all, ce_all, off, params, N, nbr, acc, count, h);
+    else
+        batch_density_home(s, gx, gy, gz, pi, i, slot, grp_shift, live, spos, cs_all, ce_all, off, params, N, nbr, acc,
+                           count, h);
+    acc += __shfl_xor(acc, 1, kWave);
+    acc += __shfl_xor(acc, 2, kWave);
+    acc += __shfl_xor(acc, 4, kWave);
+    if (!live || slot != 0) return;
+    const mfl_params& p = params[s];
+    float rho, prs;
+    if (count > 2) {
+        rho = fmaxf(1.0f, p.mass * p.kernelConstant * acc);
+        prs = p.pressure + p.stiffness * (rho - p.density);
+    } else {
+        rho = p.density;
+        prs = p.pressure;
+    }
+    spos[i].w = rho;
+    svel[i].w = prs;
+    ncount[i] = count;
+}
+
+__global__ __launch_bounds__(kThreads) void k_b_forces_step(const float4* __restrict__ spos,
+                                                            const float4* __restrict__ svel, float4* __restrict__ pos,
+                                                            float4* __restrict__ vel, float4* __restrict__ force,
+                                                            const uint32_t* __restrict__ index,
+                                                            const uint32_t* __restrict__ cell,
+                                                            const uint32_t* __restrict__ cs_all,
+                                                            const uint32_t* __restrict__ ce_all,
+                                                            const uint32_t* __restrict__ off,
+                                                            const mfl_params* __restrict__ params,
+                                                            const float4* __restrict__ boxes,
+                                                            const uint32_t* __restrict__ nbr,
+                                                            const uint32_t* __restrict__ ncount, uint32_t N,
+                                                            float4 gravity_force)
+{
+    const uint32_t i = blockIdx.x * kThreads + threadIdx.x;
+    if (i >= N) return;
+    const uint32_t s = cell[i] >> 18;
+    const uint32_t lo = off[s], n_s = off[s + 1] - lo;
+    const mfl_params p = params[s];
+    const float4 pi = spos[i], vi = svel[i];
+    float3 P, V;
+    const uint32_t count = ncount[i];
+    if (count <= kNbrCap)
+        sph_force_sum_nbr(spos + lo, svel + lo, nbr, N, count, i, pi, vi, p.cellSize, P, V);
+    else
+        sph_force_sum(spos + lo, svel + lo, cs_all + (size_t)s * kBuckets, ce_all + (size_t)s * kBuckets, i - lo, n_s,
+                      pi, vi, p.cellSize, P, V);
+    const uint32_t o = index[i];
+    if (o >= N) return;
+    float4 f = add_sph_force(force[o], P, V, p);
+    f = clip_box(f, pi, vi, p.particleRadius, boxes[2 * s], boxes[2 * s + 1]);
+    const float4 v = integrate_force(vel[o], f, p);
+    force[o] = gravity_force;
+    vel[o] = v;
+    pos[o] = f4_add(pos[o], f4_scale(v, p.deltaTime));
+}
+
+}  // namespace
+
+FluidBatch::FluidBatch(uint32_t num_scenes, const uint32_t* counts)
+{
+    if (num_scenes == 0 || num_scenes > (1u << 14) || !counts) {
+        set_error("a batch holds between 1 and 16384 scenes");
+        return;
+    }
+    S_ = num_scenes;
+    off_host_.assign(S_ + 1, 0);
+    uint64_t total = 0;
+    for (uint32_t s = 0; s < S_; s++) {
+        if (counts[s] == 0 || counts[s] % kGroup != 0) {
+            set_error("scene " + std::to_string(s) + " has " + std::to_string(counts[s]) +
+                      " particles, not a non-zero multiple of 256 (cLocalSize)");
+            return;
+        }
+        off_host_[s] = (uint32_t)total;
+        total += counts[s];
+    }
+    if (total >= (1ull << 31)) {
+        set_error("batch too large");
+        return;
+    }
+    N_ = (uint32_t)total;
+    off_host_[S_] = N_;
+    scene_bits_ = 0;
+    while ((1u << scene_bits_) < S_) scene_bits_++;
+    mfl_params def;
+    mfl_params_default(&def);
+    params_host_.assign(S_, def);
+    box_host_.assign((size_t)S_ * 8, 0.f);
+    for (uint32_t s = 0; s < S_; s++) {
+        const float mn[4] = {-1.f, -1.f, -1.f, 0.f}, mx[4] = {1.f, 1.f, 1.f, 0.f};
+        std::memcpy(&box_host_[(size_t)s * 8], mn, 16);
+        std::memcpy(&box_host_[(size_t)s * 8 + 4], mx, 16);
+    }
+    const size_t v4 = (size_t)N_ * 16, u1 = (size_t)N_ * 4;
+    std::vector<uint32_t> pscene(N_);
+    for (uint32_t s = 0; s < S_; s++)
+        for (uint32_t g = off_host_[s]; g < off_host_[s + 1]; g++) pscene[g] = s;
+    const size_t tables = (size_t)S_ * kBuckets * 4;
+    if (!off_.allocate((S_ + 1) * 4) || !pscene_.allocate(u1) || !params_.allocate(S_ * sizeof(mfl_params)) ||
+        !boxes_.allocate((size_t)S_ * 32) || !pos_.allocate(v4) || !vel_.allocate(v4) || !force_.allocate(v4) ||
+        !cell_.allocate(u1) || !index_.allocate(u1) || !spos_.allocate(v4) || !svel_.allocate(v4) ||
+        !tables_.allocate(2 * tables) || !nbr_.allocate((size_t)kNbrCap * u1) || !ncount_.allocate(u1))
+        return;
+    if (!hip_ok(hipMemcpy(off_.ptr, off_host_.data(), (S_ + 1) * 4, hipMemcpyHostToDevice), "hipMemcpy") ||
+        !hip_ok(hipMemcpy(pscene_.ptr, pscene.data(), u1, hipMemcpyHostToDevice), "hipMemcpy") ||
+        !hip_ok(hipMemset(tables_.ptr, 0xFF, 2 * tables), "hipMemset") ||
+        !hip_ok(hipMemset(pos_.ptr, 0, v4), "hipMemset") || !hip_ok(hipMemset(vel_.ptr, 0, v4), "hipMemset") ||
+        !hip_ok(hipMemset(force_.ptr, 0, v4), "hipMemset") || !hip_ok(hipMemset(spos_.ptr, 0, v4), "hipMemset") ||
+        !hip_ok(hipMemset(svel_.ptr, 0, v4), "hipMemset") || !hip_ok(hipDeviceSynchronize(), "hipDeviceSynchronize"))
+        return;
+    ok_ = true;
+}
+
+FluidBatch::~FluidBatch()
+{
+    drop_graph();
+    for (DeviceBuffer* b : {&off_, &pscene_, &params_, &boxes_, &pos_, &vel_, &force_, &cell_, &index_, &spos_, &svel_,
+                            &tables_, &nbr_, &ncount_})
+        b->release();
+}
+
+DeviceBuffer* FluidBatch::select(int kind)
+{
+    switch (kind) {
+    case MFL_POSITION: return &pos_;
+    case MFL_VELOCITY: return &vel_;
+    case MFL_FORCE: return &force_;
+    case MFL_SORTED_POSITION: return &spos_;
+    case MFL_SORTED_VELOCITY: return &svel_;
+    case MFL_INDEX: return &index_;
+    case MFL_CELL: return &cell_;
+    case MFL_CELL_START: return &tables_;
+    case MFL_CELL_END: return &tables_;
+    case MFL_PARAMS: return &params_;
+    default: set_error("unknown buffer kind " + std::to_string(kind)); return nullptr;
+    }
+}
+
+int FluidBatch::get_buffer(int kind, void** ptr, size_t* bytes)
+{
+    DeviceBuffer* b = select(kind);
+    if (!b) return 0;
+    const size_t tables = (size_t)S_ * kBuckets * 4;
+    char* base = static_cast<char*>(b->ptr);
+    if (kind == MFL_CELL_END) base += tables;
+    if (ptr) *ptr = base;
+    if (bytes) *bytes = (kind == MFL_CELL_START || kind == MFL_CELL_END) ? tables : b->bytes;
+    return 1;
+}
+
+int FluidBatch::write_buffer(int kind, const void* host, size_t bytes)
+{
+    void* ptr = nullptr;
+    size_t cap = 0;
+    if (!get_buffer(kind, &ptr, &cap)) return 0;
+    if (bytes > cap) {
+        set_error("write of " + std::to_string(bytes) + " bytes exceeds the buffer");
+        return 0;
+    }
+    MFL_HIP(hipMemcpy(ptr, host, bytes, hipMemcpyHostToDevice));
+    return 1;
+}
+
+int FluidBatch::read_buffer(int kind, void* host, size_t bytes)
+{
+    void* ptr = nullptr;
+    size_t cap = 0;
+    if (!get_buffer(kind, &ptr, &cap)) return 0;
+    if (bytes > cap) {
+        set_error("read of " + std::to_string(bytes) + " bytes exceeds the buffer");
+        return 0;
+    }
+    MFL_HIP(hipDeviceSynchronize());
+    MFL_HIP(hipMemcpy(host, ptr, bytes, hipMemcpyDeviceToHost));
+    return 1;
+}
+
+int FluidBatch::scene_range(uint32_t scene, uint32_t* offset, uint32_t* count) const
+{
+    if (scene >= S_) {
+        set_error("scene index out of range");
+        return 0;
+    }
+    if (offset) *offset = off_host_[scene];
+    if (count) *count = off_host_[scene + 1] - off_host_[scene];
+    return 1;
+}
+
+int FluidBatch::set_params(int scene, const mfl_params& p)
+{
+    if (scene >= (int)S_) {
+        set_error("scene index out of range");
+        return 0;
+    }
+    for (uint32_t s = 0; s < S_; s++)
+        if (scene < 0 || (uint32_t)scene == s) params_host_[s] = p;
+    params_dirty_ = true;
+    return 1;
+}
+
+int FluidBatch::get_params(uint32_t scene, mfl_params* p) const
+{
+    if (scene >= S_ || !p) {
+        set_error("scene index out of range");
+        return 0;
+    }
+    *p = params_host_[scene];
+    return 1;
+}
+
+int FluidBatch::set_box(int scene, const float* mn, const float* mx)
+{
+    if (scene >= (int)S_ || !mn || !mx) {
+        set_error("scene index out of range or null box");
+        return 0;
+    }
+    for (uint32_t s = 0; s < S_; s++)
+        if (scene < 0 || (uint32_t)scene == s) {
+            std::memcpy(&box_host_[(size_t)s * 8], mn, 16);
+            std::memcpy(&box_host_[(size_t)s * 8 + 4], mx, 16);
+        }
+    params_dirty_ = true;
+    return 1;
+}
+
+int FluidBatch::enqueue_step(hipStream_t s)
+{
+    const dim3 b(kThreads), g(grid_for(N_));
+    auto* pos = pos_.as<float4>();
+    auto* vel = vel_.as<float4>();
+    auto* force = force_.as<float4>();
+    auto* spos = spos_.as<float4>();
+    auto* svel = svel_.as<float4>();
+    auto* cell = cell_.as<uint32_t>();
+    auto* index = index_.as<uint32_t>();
+    auto* cs_all = tables_.as<uint32_t>();
+    auto* ce_all = cs_all + (size_t)S_ * kBuckets;
+    auto* off = off_.as<uint32_t>();
+    auto* params = params_.as<mfl_params>();
+    const uint32_t clear = clear_count < kBuckets ? clear_count : kBuckets;
+
+    hipLaunchKernelGGL(k_b_hash, g, b, 0, s, pos, pscene_.as<uint32_t>(), params, N_, cell);
+    MFL_HIP(hipGetLastError());
+    if (!sort.sort_bits(cell, index, N_, 0, 18 + (int)scene_bits_, s, true)) return 0;
+    hipLaunchKernelGGL(k_b_reorder, g, b, 0, s, index, pos, vel, spos, svel, N_);
+    if (clear) {
+        const size_t total = (size_t)S_ * clear;
+        const unsigned blocks = (unsigned)std::min<size_t>((total + kThreads - 1) / kThreads, 65536);
+        hipLaunchKernelGGL(k_b_clear, dim3(blocks), b, 0, s, cs_all, ce_all, S_, clear);
+    }
+    hipLaunchKernelGGL(k_b_find_bounds, g, b, 0, s, cell, off, N_, cs_all, ce_all);
+    hipLaunchKernelGGL(k_b_density, grid_for((size_t)N_ * kGroupLanes), b, 0, s, spos, svel, cs_all, ce_all, cell, off,
+                       params, N_, nbr_.as<uint32_t>(), ncount_.as<uint32_t>());
+    hipLaunchKernelGGL(k_b_forces_step, g, b, 0, s, spos, svel, pos, vel, force, index, cell, cs_all, ce_all, off, params,
+                       boxes_.as<float4>(), nbr_.as<uint32_t>(), ncount_.as<uint32_t>(), N_,
+                       gravity_force_of(gravity));
+    MFL_HIP(hipGetLastError());
+    return 1;
+}
+
+void FluidBatch::drop_graph()
+{
+    if (graph_exec_) (void)hipGraphExecDestroy(graph_exec_);
+    if (graph_) (void)hipGraphDestroy(graph_);
+    graph_exec_ = nullptr;
+    graph_ = nullptr;
+    graph_sig_.clear();
+}
+
+int FluidBatch::launch_graph(hipStream_t s)
+{
+    std::ostringstream sig;
+    sig << clear_count << ':' << sort.tuning.items << ':' << gravity[0] << ',' << gravity[1] << ',' << gravity[2] << ','
+        << gravity[3];
+    if (!graph_exec_ || sig.str() != graph_sig_) {
+        drop_graph();
+        if (!sort.reserve(N_, 18 + (int)scene_bits_)) return 0;
+        MFL_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeRelaxed));
+        const int ok = enqueue_step(s);
+        hipGraph_t gr = nullptr;
+        const hipError_t e = hipStreamEndCapture(s, &gr);
+        if (!ok || e != hipSuccess) {
+            if (gr) (void)hipGraphDestroy(gr);
+            if (ok) hip_ok(e, "hipStreamEndCapture");
+            return 0;
+        }
+        graph_ = gr;
+        MFL_HIP(hipGraphInstantiate(&graph_exec_, graph_, nullptr, nullptr, 0));
+        graph_sig_ = sig.str();
+    }
+    MFL_HIP(hipGraphLaunch(graph_exec_, s));
+    return 1;
+}
+
+int FluidBatch::compute(hipStream_t s)
+{
+    if (!ok_) {
+        set_error("batch was not created");
+        return 0;
+    }
+    if (params_dirty_) {
+        // the scene Params and boxes are read from device arrays, so their values are not baked into a
+        // captured graph; refreshing them needs no re-capture
+        std::vector<float4> boxes((size_t)S_ * 2);
+        std::memcpy(boxes.data(), box_host_.data(), box_host_.size() * 4);
+        MFL_HIP(hipStreamSynchronize(s));  // steps in flight may still read the previous values
+        MFL_HIP(hipMemcpy(params_.ptr, params_host_.data(), S_ * sizeof(mfl_params), hipMemcpyHostToDevice));
+        MFL_HIP(hipMemcpy(boxes_.ptr, boxes.data(), boxes.size() * 16, hipMemcpyHostToDevice));
+        params_dirty_ = false;
+    }
+    if (use_graph && s != nullptr) return launch_graph(s);
+    return enqueue_step(s);
+}
+
+}  // namespace mfl

@@ -3,6 +3,8 @@
 #include <cstring>
 #include <new>
 
+#include "adjoint.hpp"
+#include "batch.hpp"
 #include "fluid3d.hpp"
 #include "radix_sort.hpp"
 
@@ -17,6 +19,16 @@ struct mfl_fluid3d {
 
 struct mfl_radixsort {
     mfl::RadixSort impl;
+};
+
+struct mfl_adjoint {
+    mfl_adjoint(mfl::Fluid3D& f, uint32_t steps, uint32_t every, uint32_t cap) : impl(f, steps, every, cap) {}
+    mfl::Adjoint impl;
+};
+
+struct mfl_fluid_batch {
+    mfl_fluid_batch(uint32_t s, const uint32_t* c) : impl(s, c) {}
+    mfl::FluidBatch impl;
 };
 
 namespace {
@@ -63,6 +75,18 @@ int mfl_set_device(int ordinal)
 int mfl_device_finish(void)
 {
     MFL_HIP(hipDeviceSynchronize());
+    return 1;
+}
+
+int mfl_memcpy(void* dst, const void* src, size_t bytes)
+{
+    if (bytes == 0) return 1;
+    if (!dst || !src) {
+        mfl::set_error("null copy address");
+        return 0;
+    }
+    MFL_HIP(hipDeviceSynchronize());
+    MFL_HIP(hipMemcpy(dst, src, bytes, hipMemcpyDefault));
     return 1;
 }
 
@@ -272,6 +296,149 @@ int mfl_fluid3d_enqueue_kernel(mfl_fluid3d* f, const char* name, void* stream)
 {
     if (!need(f, "fluid")) return 0;
     return guarded([&] { return f->impl.enqueue_kernel(name, (hipStream_t)stream); });
+}
+
+// ---- batched scenes ---------------------------------------------------------------------------
+
+mfl_fluid_batch* mfl_batch_create(uint32_t num_scenes, const uint32_t* counts)
+{
+    mfl::clear_error();
+    mfl_fluid_batch* b = nullptr;
+    try {
+        b = new mfl_fluid_batch(num_scenes, counts);
+    } catch (...) {
+        mfl::set_error("out of host memory");
+        return nullptr;
+    }
+    if (!b->impl.ok()) {
+        delete b;
+        return nullptr;
+    }
+    return b;
+}
+
+void mfl_batch_destroy(mfl_fluid_batch* b) { delete b; }
+
+int mfl_batch_scene_range(mfl_fluid_batch* b, uint32_t scene, uint32_t* offset, uint32_t* count)
+{
+    if (!need(b, "batch")) return 0;
+    return b->impl.scene_range(scene, offset, count);
+}
+
+int mfl_batch_set_params(mfl_fluid_batch* b, int scene, const mfl_params* p)
+{
+    if (!need(b, "batch") || !need(p, "params")) return 0;
+    return b->impl.set_params(scene, *p);
+}
+
+int mfl_batch_get_params(mfl_fluid_batch* b, uint32_t scene, mfl_params* p)
+{
+    if (!need(b, "batch")) return 0;
+    return b->impl.get_params(scene, p);
+}
+
+int mfl_batch_set_box(mfl_fluid_batch* b, int scene, const float* min4, const float* max4)
+{
+    if (!need(b, "batch")) return 0;
+    return b->impl.set_box(scene, min4, max4);
+}
+
+int mfl_batch_set_gravity(mfl_fluid_batch* b, const float* dir4)
+{
+    if (!need(b, "batch") || !need(dir4, "gravity")) return 0;
+    std::memcpy(b->impl.gravity, dir4, 16);
+    return 1;
+}
+
+int mfl_batch_set_bucket_clear(mfl_fluid_batch* b, uint32_t count)
+{
+    if (!need(b, "batch")) return 0;
+    if (count > mfl::kBuckets) {
+        mfl::set_error("bucket clear count exceeds 262144");
+        return 0;
+    }
+    b->impl.clear_count = count;
+    return 1;
+}
+
+int mfl_batch_set_use_graph(mfl_fluid_batch* b, int enable)
+{
+    if (!need(b, "batch")) return 0;
+    b->impl.use_graph = enable != 0;
+    return 1;
+}
+
+int mfl_batch_get_buffer(mfl_fluid_batch* b, int kind, void** ptr, size_t* bytes)
+{
+    if (!need(b, "batch")) return 0;
+    return guarded([&] { return b->impl.get_buffer(kind, ptr, bytes); });
+}
+
+int mfl_batch_write_buffer(mfl_fluid_batch* b, int kind, const void* host, size_t bytes)
+{
+    if (!need(b, "batch")) return 0;
+    return guarded([&] { return b->impl.write_buffer(kind, host, bytes); });
+}
+
+int mfl_batch_read_buffer(mfl_fluid_batch* b, int kind, void* host, size_t bytes)
+{
+    if (!need(b, "batch")) return 0;
+    return guarded([&] { return b->impl.read_buffer(kind, host, bytes); });
+}
+
+int mfl_batch_compute(mfl_fluid_batch* b, void* stream)
+{
+    if (!need(b, "batch")) return 0;
+    return guarded([&] { return b->impl.compute((hipStream_t)stream); });
+}
+
+// ---- adjoint --------------------------------------------------------------------------------------
+
+mfl_adjoint* mfl_adjoint_create(mfl_fluid3d* f, uint32_t steps, uint32_t checkpoint_every, uint32_t list_cap)
+{
+    mfl::clear_error();
+    if (!need(f, "fluid")) return nullptr;
+    mfl_adjoint* a = nullptr;
+    try {
+        a = new mfl_adjoint(f->impl, steps, checkpoint_every, list_cap);
+    } catch (...) {
+        mfl::set_error("out of host memory");
+        return nullptr;
+    }
+    if (!a->impl.ok()) {
+        delete a;
+        return nullptr;
+    }
+    return a;
+}
+
+void mfl_adjoint_destroy(mfl_adjoint* a) { delete a; }
+
+int mfl_adjoint_forward(mfl_adjoint* a, void* stream)
+{
+    if (!need(a, "adjoint")) return 0;
+    return guarded([&] { return a->impl.forward((hipStream_t)stream); });
+}
+
+int mfl_adjoint_backward(mfl_adjoint* a, const void* pos_bar, const void* vel_bar, void* stream)
+{
+    if (!need(a, "adjoint")) return 0;
+    return guarded([&] {
+        return a->impl.backward(static_cast<const float4*>(pos_bar), static_cast<const float4*>(vel_bar),
+                                (hipStream_t)stream);
+    });
+}
+
+int mfl_adjoint_get_buffer(mfl_adjoint* a, int which, void** ptr, size_t* bytes)
+{
+    if (!need(a, "adjoint")) return 0;
+    return a->impl.get_buffer(which, ptr, bytes);
+}
+
+int mfl_adjoint_param_grads(mfl_adjoint* a, double* out4)
+{
+    if (!need(a, "adjoint") || !need(out4, "output")) return 0;
+    return guarded([&] { return a->impl.param_grads(out4); });
 }
 
 }  // extern "C"

@@ -16,6 +16,7 @@
 //                    only touches its own force/velocity/position, so fusion is exact)
 // With an EVT_INTEGRATE callback the force pass runs alone (k_forces) and the callback follows.
 #include "fluid3d.hpp"
+#include "sph_device.hpp"
 
 #include <cstring>
 #include <sstream>
@@ -23,10 +24,7 @@
 namespace mfl {
 namespace {
 
-constexpr int kThreads = 256;
-
-__device__ inline float4 f4_add(float4 a, float4 b) { return make_float4(a.x + b.x, a.y + b.y, a.z + b.z, a.w + b.w); }
-__device__ inline float4 f4_scale(float4 a, float s) { return make_float4(a.x * s, a.y * s, a.z * s, a.w * s); }
+using namespace sph;
 
 template <bool WRITE_INDEX>
 __global__ __launch_bounds__(kThreads) void k_hash(const float4* __restrict__ pos, uint32_t n, float h,
@@ -96,15 +94,6 @@ __global__ __launch_bounds__(kThreads) void k_find_bounds(const uint32_t* __rest
     if (i == n - 1) cell_end[h] = n;
 }
 
-// Distance test of cl:177-178 / cl:232-233: vlen = sqrt(x^2+y^2+z^2) < h. A candidate whose squared
-// distance exceeds h^2 by more than the sqrt rounding can affect is rejected without the sqrt.
-__device__ inline bool within(float r2, float h, float h2_hi, float* vlen)
-{
-    if (r2 > h2_hi) return false;
-    *vlen = sqrtf(r2);
-    return *vlen < h;
-}
-
 __global__ __launch_bounds__(kThreads) void k_density(float4* __restrict__ spos, float4* __restrict__ svel,
                                                       const uint32_t* __restrict__ cell_start,
                                                       const uint32_t* __restrict__ cell_end, uint32_t n,
@@ -149,88 +138,6 @@ __global__ __launch_bounds__(kThreads) void k_density(float4* __restrict__ spos,
     svel[i].w = prs;
 }
 
-// Sum of the pressure and viscosity terms of cl:213-249 for sorted particle i.
-__device__ inline void sph_force_sum(const float4* __restrict__ spos, const float4* __restrict__ svel,
-                                     const uint32_t* __restrict__ cell_start, const uint32_t* __restrict__ cell_end,
-                                     uint32_t i, uint32_t n, const float4& pi, const float4& vi, float h, float3& P,
-                                     float3& V)
-{
-    const float h2_hi = h * h * 1.0000005f;
-    const int gx = grid_coord(pi.x, h), gy = grid_coord(pi.y, h), gz = grid_coord(pi.z, h);
-    P = make_float3(0.f, 0.f, 0.f);
-    V = make_float3(0.f, 0.f, 0.f);
-    for (int z = -1; z <= 1; z++)
-        for (int y = -1; y <= 1; y++)
-            for (int x = -1; x <= 1; x++) {
-                const uint32_t c = cell_hash(gx + x, gy + y, gz + z);
-                const uint32_t start = cell_start[c];
-                if (start == kEmpty) continue;
-                const uint32_t end = min(cell_end[c], n);  // stale or written ranges always lie in [0, n]
-                for (uint32_t j = start; j < end; j++) {
-                    if (j == i) continue;
-                    const float4 pj = spos[j];
-                    const float dx = pi.x - pj.x, dy = pi.y - pj.y, dz = pi.z - pj.z;
-                    float vlen;
-                    if (within(dx * dx + dy * dy + dz * dz, h, h2_hi, &vlen)) {
-                        const float4 vj = svel[j];
-                        const float density2 = pi.w * pj.w;
-                        const float q = (vi.w + vj.w) / density2;
-                        const float hr = h - vlen;
-                        P.x += q * ((dx / vlen) * hr * hr);
-                        P.y += q * ((dy / vlen) * hr * hr);
-                        P.z += q * ((dz / vlen) * hr * hr);
-                        V.x += ((vj.x - vi.x) / density2) * hr;
-                        V.y += ((vj.y - vi.y) / density2) * hr;
-                        V.z += ((vj.z - vi.z) / density2) * hr;
-                    }
-                }
-            }
-}
-
-__device__ inline float4 add_sph_force(float4 f, const float3& P, const float3& V, const mfl_params& p)
-{
-    f.x += (p.pressureConstant * P.x + p.viscosityConstant * V.x) * p.mass;
-    f.y += (p.pressureConstant * P.y + p.viscosityConstant * V.y) * p.mass;
-    f.z += (p.pressureConstant * P.z + p.viscosityConstant * V.z) * p.mass;
-    f.w = 0.f;  // cl:253
-    return f;
-}
-
-// repulsionForce (cl:309-312) accumulated as in cl:327-357.
-__device__ inline void add_repulsion(float4& f, float4 nrm, float4 vel, float dist)
-{
-    const float dot = nrm.x * vel.x + nrm.y * vel.y + nrm.z * vel.z + nrm.w * vel.w;
-    const float s = 2000.0f * dist - 256.0f * dot;
-    f = f4_add(f, f4_scale(nrm, s));
-}
-
-// clClipBox body (cl:319-358) on a private force value.
-__device__ inline float4 clip_box(float4 f, float4 pos, float4 vel, float r, float4 mn, float4 mx)
-{
-    constexpr float eps = 0.00001f;  // cl:304
-    float diff = r - (pos.y - mn.y);
-    if (diff > eps) add_repulsion(f, make_float4(0.f, 1.f, 0.f, 0.f), vel, diff);
-    diff = r - (mx.y - pos.y);
-    if (diff > eps) add_repulsion(f, make_float4(0.f, -1.f, 0.f, 0.f), vel, diff);
-    diff = r - (pos.z - mn.z);
-    if (diff > eps) add_repulsion(f, make_float4(0.f, 0.f, 1.f, 0.f), vel, diff);
-    diff = r - (mx.z - pos.z);
-    if (diff > eps) add_repulsion(f, make_float4(0.f, 0.f, -1.f, 0.f), vel, diff);
-    diff = r - (pos.x - mn.x);
-    if (diff > eps) add_repulsion(f, make_float4(1.f, 0.f, 0.f, 0.f), vel, diff);
-    diff = r - (mx.x - pos.x);
-    if (diff > eps) add_repulsion(f, make_float4(-1.f, 0.f, 0.f, 0.f), vel, diff);
-    return f;
-}
-
-// clIntegrateForce body (cl:262-270): clamp a private copy of the force, then v += f * dt.
-__device__ inline float4 integrate_force(float4 vel, float4 f, const mfl_params& p)
-{
-    const float speed = sqrtf(f.x * f.x + f.y * f.y + f.z * f.z + f.w * f.w);
-    if (speed > p.velocitylimit) f = f4_scale(f, p.velocitylimit / speed);
-    return f4_add(vel, f4_scale(f, p.deltaTime));
-}
-
 __global__ __launch_bounds__(kThreads) void k_forces(const float4* __restrict__ spos, const float4* __restrict__ svel,
                                                      float4* __restrict__ force, const uint32_t* __restrict__ index,
                                                      const uint32_t* __restrict__ cell_start,
@@ -255,17 +162,6 @@ __global__ __launch_bounds__(kThreads) void k_forces(const float4* __restrict__ 
 // re-testing ~27 buckets of candidates; a particle with more than kNbrCap visits falls back to the
 // full traversal. Visits repeated by the reference (a bucket appearing twice among the 27) appear
 // twice in the list, so multiplicities are kept.
-
-constexpr uint32_t kNbrCap = 64;
-
-// Exact form of the reference test fast_length(d) < h (cl:177-178): accepted when sqrt(r2) < h.
-// r2 below h2_lo (h^2 (1 - 2^-22)) or above h2_hi is decided without the square root.
-__device__ inline bool accept_r2(float r2, float h, float h2_lo, float h2_hi)
-{
-    if (r2 < h2_lo) return true;
-    if (r2 > h2_hi) return false;
-    return sqrtf(r2) < h;
-}
 
 // Traversal order of the neighbour passes. Results do not depend on it (each particle's sums are
 // computed by one thread in a fixed visit order); it only decides which particles share a wave, an
@@ -352,56 +248,6 @@ __global__ __launch_bounds__(kThreads) void k_density_nbr(float4* __restrict__ s
     ncount[t] = count;
 }
 
-// Density pass with eight lanes per particle. A wave takes eight consecutive entries of the traversal
-// order (in Morton or hash order usually the particles of one cell), and the eight lanes of a group
-// split each bucket range of their particle into interleaved slots, so bucket trip counts are the
-// same for the whole wave. When all eight particles share a cell, the 27 bucket hashes and range
-// loads are wave-uniform (scalar). Accepted visits are appended in the reference's visit order
-// (bucket by bucket, ascending j) with a group ballot.
-constexpr uint32_t kGroupLanes = 8;
-
-__device__ inline void density_visits(int gx, int gy, int gz, const float4& pi, uint32_t i, uint32_t t, uint32_t slot,
-                                      uint32_t grp_shift, bool live, const float4* __restrict__ spos,
-                                      const uint32_t* __restrict__ cell_start, const uint32_t* __restrict__ cell_end,
-                                      uint32_t n, float h, float h2, float h2_lo, float h2_hi,
-                                      uint32_t* __restrict__ nbr, float& acc, uint32_t& count)
-{
-    const uint32_t below = (1u << slot) - 1u;
-    for (int z = -1; z <= 1; z++) {
-        uint32_t st[9], en[9];
-#pragma unroll
-        for (int k = 0; k < 9; k++) {
-            const uint32_t c = cell_hash(gx + (k % 3) - 1, gy + (k / 3) - 1, gz + z);
-            st[k] = cell_start[c];
-            en[k] = cell_end[c];
-        }
-#pragma unroll
-        for (int k = 0; k < 9; k++) {
-            if (st[k] == kEmpty) continue;
-            const uint32_t end = min(en[k], n);
-            for (uint32_t j0 = st[k]; j0 < end; j0 += kGroupLanes) {
-                const uint32_t j = j0 + slot;
-                bool ok = false;
-                float r2 = 0.f;
-                if (j < end) {
-                    const float4 pj = spos[j];
-                    const float dx = pi.x - pj.x, dy = pi.y - pj.y, dz = pi.z - pj.z;
-                    r2 = dx * dx + dy * dy + dz * dz;
-                    ok = live && j != i && accept_r2(r2, h, h2_lo, h2_hi);
-                }
-                const uint32_t mask = (uint32_t)(__ballot(ok) >> grp_shift) & 0xFFu;
-                if (ok) {
-                    const float w = h2 - r2;  // h^2 - vlen^2 of cl:123
-                    acc += w * w * w;
-                    const uint32_t at = count + (uint32_t)__popc(mask & below);
-                    if (at < kNbrCap) nbr[(size_t)at * n + t] = j;
-                }
-                count += (uint32_t)__popc(mask);
-            }
-        }
-    }
-}
-
 template <int ORDER>
 __global__ __launch_bounds__(kThreads) void k_density_grp(float4* __restrict__ spos, float4* __restrict__ svel,
                                                           const uint32_t* __restrict__ cell_start,
@@ -434,10 +280,10 @@ __global__ __launch_bounds__(kThreads) void k_density_grp(float4* __restrict__ s
     if (__ballot(!same) == 0) {
         density_visits(__builtin_amdgcn_readfirstlane(gx0), __builtin_amdgcn_readfirstlane(gy0),
                        __builtin_amdgcn_readfirstlane(gz0), pi, i, t, slot, grp_shift, live, spos, cell_start,
-                       cell_end, n, h, h2, h2_lo, h2_hi, nbr, acc, count);
+                       cell_end, n, h, h2, h2_lo, h2_hi, nbr, n, acc, count);
     } else {
         density_visits(gx, gy, gz, pi, i, t, slot, grp_shift, live, spos, cell_start, cell_end, n, h, h2, h2_lo,
-                       h2_hi, nbr, acc, count);
+                       h2_hi, nbr, n, acc, count);
     }
     acc += __shfl_xor(acc, 1, kWave);
     acc += __shfl_xor(acc, 2, kWave);
@@ -456,35 +302,6 @@ __global__ __launch_bounds__(kThreads) void k_density_grp(float4* __restrict__ s
     ncount[t] = count;
 }
 
-// Pressure and viscosity sums of cl:237-242 over the visits recorded for thread t (sorted particle i).
-__device__ inline void sph_force_sum_nbr(const float4* __restrict__ spos, const float4* __restrict__ svel,
-                                         const uint32_t* __restrict__ nbr, uint32_t count, uint32_t t, uint32_t n,
-                                         const float4& pi, const float4& vi, float h, float3& P, float3& V)
-{
-    P = make_float3(0.f, 0.f, 0.f);
-    V = make_float3(0.f, 0.f, 0.f);
-    for (uint32_t k = 0; k < count; k++) {
-        const uint32_t j = nbr[(size_t)k * n + t];
-        const float4 pj = spos[j];
-        const float4 vj = svel[j];
-        const float dx = pi.x - pj.x, dy = pi.y - pj.y, dz = pi.z - pj.z;
-        const float r2 = dx * dx + dy * dy + dz * dz;
-        const float inv_r = __builtin_amdgcn_rsqf(r2);
-        const float hr = h - r2 * inv_r;
-        const float inv_d2 = __builtin_amdgcn_rcpf(pi.w * pj.w);
-        // (p_i + p_j) / (rho_i rho_j) * (d / |d|) * (h - |d|)^2
-        const float s = (vi.w + vj.w) * inv_d2 * hr * hr * inv_r;
-        P.x += s * dx;
-        P.y += s * dy;
-        P.z += s * dz;
-        // (v_j - v_i) / (rho_i rho_j) * (h - |d|)
-        const float w = hr * inv_d2;
-        V.x += w * (vj.x - vi.x);
-        V.y += w * (vj.y - vi.y);
-        V.z += w * (vj.z - vi.z);
-    }
-}
-
 template <int ORDER>
 __device__ inline void forces_of(uint32_t t, uint32_t i, const float4* __restrict__ spos,
                                  const float4* __restrict__ svel, const uint32_t* __restrict__ cell_start,
@@ -494,7 +311,7 @@ __device__ inline void forces_of(uint32_t t, uint32_t i, const float4* __restric
 {
     const uint32_t count = ncount[t];
     if (count <= kNbrCap)
-        sph_force_sum_nbr(spos, svel, nbr, count, t, n, pi, vi, h, P, V);
+        sph_force_sum_nbr(spos, svel, nbr, n, count, t, pi, vi, h, P, V);
     else
         sph_force_sum(spos, svel, cell_start, cell_end, i, n, pi, vi, h, P, V);
 }
@@ -554,17 +371,6 @@ __global__ __launch_bounds__(kThreads) void k_forces_step(const float4* __restri
     pos[o] = f4_add(pos[o], f4_scale(v, p.deltaTime));
 }
 
-// Morton code of each particle's cell (10 bits per axis, wrapped), for the listed traversal order.
-__device__ inline uint32_t spread_bits3(uint32_t v)
-{
-    v &= 0x3FFu;
-    v = (v | (v << 16)) & 0x030000FFu;
-    v = (v | (v << 8)) & 0x0300F00Fu;
-    v = (v | (v << 4)) & 0x030C30C3u;
-    v = (v | (v << 2)) & 0x09249249u;
-    return v;
-}
-
 __global__ __launch_bounds__(kThreads) void k_morton_keys(const float4* __restrict__ pos, uint32_t n, float h,
                                                           uint32_t* __restrict__ key)
 {
@@ -607,16 +413,6 @@ __global__ __launch_bounds__(kThreads) void k_integrate_velocity(float4* __restr
     const uint32_t i = blockIdx.x * kThreads + threadIdx.x;
     if (i >= n) return;
     pos[i] = f4_add(pos[i], f4_scale(vel[i], p.deltaTime));
-}
-
-inline dim3 grid_for(size_t n) { return dim3((unsigned)((n + kThreads - 1) / kThreads)); }
-
-inline float4 to_f4(const float* v) { return make_float4(v[0], v[1], v[2], v[3]); }
-
-// clGravity writes -9.8f * gravity (cl:297)
-inline float4 gravity_force_of(const float* g)
-{
-    return make_float4(-9.8f * g[0], -9.8f * g[1], -9.8f * g[2], -9.8f * g[3]);
 }
 
 }  // namespace
@@ -853,14 +649,12 @@ bool Fluid3D::check_user_buffers()
     return true;
 }
 
-int Fluid3D::enqueue_step(hipStream_t s)
+int Fluid3D::enqueue_front(hipStream_t s)
 {
     const uint32_t n = (uint32_t)n_;
     const dim3 g = grid_for(n), b(kThreads);
-    const float4 mn = to_f4(box_min), mx = to_f4(box_max), gf = gravity_force_of(gravity);
     auto* pos = pos_.as<float4>();
     auto* vel = vel_.as<float4>();
-    auto* force = force_.as<float4>();
     auto* spos = spos_.as<float4>();
     auto* svel = svel_.as<float4>();
     auto* cell = cell_.as<uint32_t>();
@@ -885,6 +679,24 @@ int Fluid3D::enqueue_step(hipStream_t s)
     timer.begin(MFL_STAGE_BOUNDS, s);
     hipLaunchKernelGGL(k_find_bounds, g, b, 0, s, cell, n, cs, ce);
     timer.end(MFL_STAGE_BOUNDS, s);
+    MFL_HIP(hipGetLastError());
+    return 1;
+}
+
+int Fluid3D::enqueue_step(hipStream_t s)
+{
+    const uint32_t n = (uint32_t)n_;
+    const dim3 g = grid_for(n), b(kThreads);
+    const float4 mn = to_f4(box_min), mx = to_f4(box_max), gf = gravity_force_of(gravity);
+    auto* pos = pos_.as<float4>();
+    auto* vel = vel_.as<float4>();
+    auto* force = force_.as<float4>();
+    auto* spos = spos_.as<float4>();
+    auto* svel = svel_.as<float4>();
+    auto* index = index_.as<uint32_t>();
+    auto* cs = cell_start_.as<uint32_t>();
+    auto* ce = cell_end_.as<uint32_t>();
+    if (!enqueue_front(s)) return 0;
     auto* nbr = nbr_.as<uint32_t>();
     auto* ncount = ncount_.as<uint32_t>();
     auto* inv = inv_.as<uint32_t>();

@@ -61,10 +61,13 @@ public:
     RadixSort sort;
     StageTimer timer;
 
-private:
+    // hash, sort, reorder + bucket clear and bounds of one step (the part before the density pass)
+    int enqueue_front(hipStream_t s);
     DeviceBuffer* select(int kind);
-    bool allocate_internal();
     bool check_user_buffers();
+
+private:
+    bool allocate_internal();
     int enqueue_step(hipStream_t s);
     int launch_graph(hipStream_t s);
     void drop_graph();

@@ -1,0 +1,219 @@
+// Per-particle device arithmetic of the SPH step, shared by the single-scene program (fluid3d.hip)
+// and the batched scenes (batch.hip), so that both compute bit-identical results.
+#pragma once
+
+#include "mfl_common.hpp"
+
+namespace mfl {
+namespace sph {
+
+constexpr int kThreads = 256;
+
+__device__ inline float4 f4_add(float4 a, float4 b) { return make_float4(a.x + b.x, a.y + b.y, a.z + b.z, a.w + b.w); }
+
+__device__ inline float4 f4_scale(float4 a, float s) { return make_float4(a.x * s, a.y * s, a.z * s, a.w * s); }
+
+// Distance test of cl:177-178 / cl:232-233: vlen = sqrt(x^2+y^2+z^2) < h. A candidate whose squared
+// distance exceeds h^2 by more than the sqrt rounding can affect is rejected without the sqrt.
+__device__ inline bool within(float r2, float h, float h2_hi, float* vlen)
+{
+    if (r2 > h2_hi) return false;
+    *vlen = sqrtf(r2);
+    return *vlen < h;
+}
+
+// Pressure and viscosity terms of one accepted pair (cl:237-242):
+//   P += (p_i + p_j) / (rho_i rho_j) * (d / |d|) * (h - |d|)^2,   V += (v_j - v_i) / (rho_i rho_j) * (h - |d|)
+// with d = x_i - x_j and r2 = |d|^2. Both force paths (recorded visits and full traversal) use it, so
+// their sums are bit-identical.
+__device__ inline void pair_force(const float4& pi, const float4& vi, const float4& pj, const float4& vj, float dx,
+                                  float dy, float dz, float r2, float h, float3& P, float3& V)
+{
+    const float inv_r = __builtin_amdgcn_rsqf(r2);
+    const float hr = h - r2 * inv_r;
+    const float inv_d2 = __builtin_amdgcn_rcpf(pi.w * pj.w);
+    const float s = (vi.w + vj.w) * inv_d2 * hr * hr * inv_r;
+    P.x += s * dx;
+    P.y += s * dy;
+    P.z += s * dz;
+    const float w = hr * inv_d2;
+    V.x += w * (vj.x - vi.x);
+    V.y += w * (vj.y - vi.y);
+    V.z += w * (vj.z - vi.z);
+}
+
+// Exact form of the reference test fast_length(d) < h (cl:177-178, cl:232-233): accepted when
+// sqrt(r2) < h. r2 below h2_lo (h^2 (1 - 2^-22)) or above h2_hi (h^2 (1 + 5e-7)) is decided without
+// the square root.
+__device__ inline bool accept_r2(float r2, float h, float h2_lo, float h2_hi)
+{
+    if (r2 < h2_lo) return true;
+    if (r2 > h2_hi) return false;
+    return sqrtf(r2) < h;
+}
+
+// Pressure and viscosity sums of cl:213-249 for sorted particle i by the full 27-bucket traversal (the
+// fallback when a particle has more visits than its recorded list holds).
+__device__ inline void sph_force_sum(const float4* __restrict__ spos, const float4* __restrict__ svel,
+                                     const uint32_t* __restrict__ cell_start, const uint32_t* __restrict__ cell_end,
+                                     uint32_t i, uint32_t n, const float4& pi, const float4& vi, float h, float3& P,
+                                     float3& V)
+{
+    const float h2 = h * h;
+    const float h2_lo = h2 * 0.9999996f, h2_hi = h2 * 1.0000005f;
+    const int gx = grid_coord(pi.x, h), gy = grid_coord(pi.y, h), gz = grid_coord(pi.z, h);
+    P = make_float3(0.f, 0.f, 0.f);
+    V = make_float3(0.f, 0.f, 0.f);
+    for (int z = -1; z <= 1; z++)
+        for (int y = -1; y <= 1; y++)
+            for (int x = -1; x <= 1; x++) {
+                const uint32_t c = cell_hash(gx + x, gy + y, gz + z);
+                const uint32_t start = cell_start[c];
+                if (start == kEmpty) continue;
+                const uint32_t end = min(cell_end[c], n);  // stale or written ranges always lie in [0, n]
+                for (uint32_t j = start; j < end; j++) {
+                    if (j == i) continue;
+                    const float4 pj = spos[j];
+                    const float dx = pi.x - pj.x, dy = pi.y - pj.y, dz = pi.z - pj.z;
+                    const float r2 = dx * dx + dy * dy + dz * dz;
+                    if (accept_r2(r2, h, h2_lo, h2_hi)) pair_force(pi, vi, pj, svel[j], dx, dy, dz, r2, h, P, V);
+                }
+            }
+}
+
+__device__ inline float4 add_sph_force(float4 f, const float3& P, const float3& V, const mfl_params& p)
+{
+    f.x += (p.pressureConstant * P.x + p.viscosityConstant * V.x) * p.mass;
+    f.y += (p.pressureConstant * P.y + p.viscosityConstant * V.y) * p.mass;
+    f.z += (p.pressureConstant * P.z + p.viscosityConstant * V.z) * p.mass;
+    f.w = 0.f;  // cl:253
+    return f;
+}
+
+// repulsionForce (cl:309-312) accumulated as in cl:327-357.
+__device__ inline void add_repulsion(float4& f, float4 nrm, float4 vel, float dist)
+{
+    const float dot = nrm.x * vel.x + nrm.y * vel.y + nrm.z * vel.z + nrm.w * vel.w;
+    const float s = 2000.0f * dist - 256.0f * dot;
+    f = f4_add(f, f4_scale(nrm, s));
+}
+
+// clClipBox body (cl:319-358) on a private force value.
+__device__ inline float4 clip_box(float4 f, float4 pos, float4 vel, float r, float4 mn, float4 mx)
+{
+    constexpr float eps = 0.00001f;  // cl:304
+    float diff = r - (pos.y - mn.y);
+    if (diff > eps) add_repulsion(f, make_float4(0.f, 1.f, 0.f, 0.f), vel, diff);
+    diff = r - (mx.y - pos.y);
+    if (diff > eps) add_repulsion(f, make_float4(0.f, -1.f, 0.f, 0.f), vel, diff);
+    diff = r - (pos.z - mn.z);
+    if (diff > eps) add_repulsion(f, make_float4(0.f, 0.f, 1.f, 0.f), vel, diff);
+    diff = r - (mx.z - pos.z);
+    if (diff > eps) add_repulsion(f, make_float4(0.f, 0.f, -1.f, 0.f), vel, diff);
+    diff = r - (pos.x - mn.x);
+    if (diff > eps) add_repulsion(f, make_float4(1.f, 0.f, 0.f, 0.f), vel, diff);
+    diff = r - (mx.x - pos.x);
+    if (diff > eps) add_repulsion(f, make_float4(-1.f, 0.f, 0.f, 0.f), vel, diff);
+    return f;
+}
+
+// clIntegrateForce body (cl:262-270): clamp a private copy of the force, then v += f * dt.
+__device__ inline float4 integrate_force(float4 vel, float4 f, const mfl_params& p)
+{
+    const float speed = sqrtf(f.x * f.x + f.y * f.y + f.z * f.z + f.w * f.w);
+    if (speed > p.velocitylimit) f = f4_scale(f, p.velocitylimit / speed);
+    return f4_add(vel, f4_scale(f, p.deltaTime));
+}
+
+constexpr uint32_t kNbrCap = 64;
+
+// Density pass with eight lanes per particle. A wave takes eight consecutive entries of the traversal
+// order (in Morton or hash order usually the particles of one cell), and the eight lanes of a group
+// split each bucket range of their particle into interleaved slots, so bucket trip counts are the
+// same for the whole wave. When all eight particles share a cell, the 27 bucket hashes and range
+// loads are wave-uniform (scalar). Accepted visits are appended in the reference's visit order
+// (bucket by bucket, ascending j) with a group ballot.
+constexpr uint32_t kGroupLanes = 8;
+
+__device__ inline void density_visits(int gx, int gy, int gz, const float4& pi, uint32_t i, uint32_t t, uint32_t slot,
+                                      uint32_t grp_shift, bool live, const float4* __restrict__ spos,
+                                      const uint32_t* __restrict__ cell_start, const uint32_t* __restrict__ cell_end,
+                                      uint32_t n, float h, float h2, float h2_lo, float h2_hi,
+                                      uint32_t* __restrict__ nbr, size_t nbr_stride, float& acc, uint32_t& count,
+                                      uint32_t cap = kNbrCap)
+{
+    const uint32_t below = (1u << slot) - 1u;
+    for (int z = -1; z <= 1; z++) {
+        uint32_t st[9], en[9];
+#pragma unroll
+        for (int k = 0; k < 9; k++) {
+            const uint32_t c = cell_hash(gx + (k % 3) - 1, gy + (k / 3) - 1, gz + z);
+            st[k] = cell_start[c];
+            en[k] = cell_end[c];
+        }
+#pragma unroll
+        for (int k = 0; k < 9; k++) {
+            if (st[k] == kEmpty) continue;
+            const uint32_t end = min(en[k], n);
+            for (uint32_t j0 = st[k]; j0 < end; j0 += kGroupLanes) {
+                const uint32_t j = j0 + slot;
+                bool ok = false;
+                float r2 = 0.f;
+                if (j < end) {
+                    const float4 pj = spos[j];
+                    const float dx = pi.x - pj.x, dy = pi.y - pj.y, dz = pi.z - pj.z;
+                    r2 = dx * dx + dy * dy + dz * dz;
+                    ok = live && j != i && accept_r2(r2, h, h2_lo, h2_hi);
+                }
+                const uint32_t mask = (uint32_t)(__ballot(ok) >> grp_shift) & 0xFFu;
+                if (ok) {
+                    const float w = h2 - r2;  // h^2 - vlen^2 of cl:123
+                    acc += w * w * w;
+                    const uint32_t at = count + (uint32_t)__popc(mask & below);
+                    if (at < cap) nbr[(size_t)at * nbr_stride + t] = j;
+                }
+                count += (uint32_t)__popc(mask);
+            }
+        }
+    }
+}
+
+// Pressure and viscosity sums of cl:237-242 over the visits recorded for thread t (sorted particle i).
+__device__ inline void sph_force_sum_nbr(const float4* __restrict__ spos, const float4* __restrict__ svel,
+                                         const uint32_t* __restrict__ nbr, size_t nbr_stride, uint32_t count,
+                                         uint32_t t, const float4& pi, const float4& vi, float h, float3& P, float3& V)
+{
+    P = make_float3(0.f, 0.f, 0.f);
+    V = make_float3(0.f, 0.f, 0.f);
+    for (uint32_t k = 0; k < count; k++) {
+        const uint32_t j = nbr[(size_t)k * nbr_stride + t];
+        const float4 pj = spos[j];
+        const float4 vj = svel[j];
+        const float dx = pi.x - pj.x, dy = pi.y - pj.y, dz = pi.z - pj.z;
+        pair_force(pi, vi, pj, vj, dx, dy, dz, dx * dx + dy * dy + dz * dz, h, P, V);
+    }
+}
+
+// Morton code of each particle's cell (10 bits per axis, wrapped), for the listed traversal order.
+__device__ inline uint32_t spread_bits3(uint32_t v)
+{
+    v &= 0x3FFu;
+    v = (v | (v << 16)) & 0x030000FFu;
+    v = (v | (v << 8)) & 0x0300F00Fu;
+    v = (v | (v << 4)) & 0x030C30C3u;
+    v = (v | (v << 2)) & 0x09249249u;
+    return v;
+}
+
+inline dim3 grid_for(size_t n) { return dim3((unsigned)((n + kThreads - 1) / kThreads)); }
+
+inline float4 to_f4(const float* v) { return make_float4(v[0], v[1], v[2], v[3]); }
+
+// clGravity writes -9.8f * gravity (cl:297)
+inline float4 gravity_force_of(const float* g)
+{
+    return make_float4(-9.8f * g[0], -9.8f * g[1], -9.8f * g[2], -9.8f * g[3]);
+}
+
+}  // namespace sph
+}  // namespace mfl

@@ -71,8 +71,8 @@ class oclRadixSort:
 
     def __del__(self):
         h = getattr(self, "_h", None)
-        if h:
-            N.load().mfl_radixsort_destroy(h)
+        if h and N is not None and N._lib is not None:
+            N._lib.mfl_radixsort_destroy(h)
             self._h = None
 
     def compile(self) -> int:
@@ -120,8 +120,8 @@ class oclFluid3D:
 
     def __del__(self):
         h = getattr(self, "_h", None)
-        if h:
-            N.load().mfl_fluid3d_destroy(h)
+        if h and N is not None and N._lib is not None:
+            N._lib.mfl_fluid3d_destroy(h)
             self._h = None
 
     # ---- program interface (oclFluid3D.h:28-46) ----
@@ -264,3 +264,127 @@ class oclFluid3D:
                     spos=self.bfSortedPosition.read(np.float32, (-1, 4))[:n],
                     svel=self.bfSortedVelocity.read(np.float32, (-1, 4))[:n],
                     cell_start=self.bfCellStart.read(np.uint32), cell_end=self.bfCellEnd.read(np.uint32))
+
+
+class FluidBatch:
+    """Many independent oclFluid3D scenes advanced by one set of launches per step (an addition with no
+    reference counterpart; each scene behaves as its own oclFluid3D, libCL/phys/oclFluid3D.cpp:293-342)."""
+
+    def __init__(self, counts):
+        counts = np.ascontiguousarray(counts, np.uint32)
+        L = N.load()
+        self._h = L.mfl_batch_create(len(counts), counts.ctypes.data_as(ctypes.POINTER(ctypes.c_uint32)))
+        if not self._h:
+            raise RuntimeError(N.last_error())
+        self.counts = counts.copy()
+        self.offsets = np.concatenate([[0], np.cumsum(counts.astype(np.int64))])[:-1].astype(np.int64)
+        self.total = int(counts.astype(np.int64).sum())
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h and N is not None and N._lib is not None:
+            N._lib.mfl_batch_destroy(h)
+            self._h = None
+
+    def setParameters(self, params, scene: int = -1) -> int:
+        p = params if isinstance(params, N.Params) else N.params_from(params)
+        return int(N.load().mfl_batch_set_params(self._h, scene, ctypes.byref(p)))
+
+    def getParameters(self, scene: int) -> N.Params:
+        p = N.Params()
+        if not N.load().mfl_batch_get_params(self._h, scene, ctypes.byref(p)):
+            raise RuntimeError(N.last_error())
+        return p
+
+    def setBox(self, box_min, box_max, scene: int = -1) -> int:
+        mn = (ctypes.c_float * 4)(*[float(x) for x in box_min])
+        mx = (ctypes.c_float * 4)(*[float(x) for x in box_max])
+        return int(N.load().mfl_batch_set_box(self._h, scene, mn, mx))
+
+    def setBucketClear(self, count: int) -> int:
+        return int(N.load().mfl_batch_set_bucket_clear(self._h, count))
+
+    def setUseGraph(self, enable: bool) -> int:
+        return int(N.load().mfl_batch_set_use_graph(self._h, 1 if enable else 0))
+
+    def compute(self, stream=None) -> int:
+        return int(N.load().mfl_batch_compute(self._h, _stream_ptr(stream)))
+
+    def write(self, kind: int, host: np.ndarray) -> int:
+        host = np.ascontiguousarray(host)
+        return int(N.load().mfl_batch_write_buffer(self._h, kind, host.ctypes.data, host.nbytes))
+
+    def read(self, kind: int, dtype=np.float32, shape=None) -> np.ndarray:
+        ptr, nbytes = ctypes.c_void_p(), ctypes.c_size_t()
+        if not N.load().mfl_batch_get_buffer(self._h, kind, ctypes.byref(ptr), ctypes.byref(nbytes)):
+            raise RuntimeError(N.last_error())
+        out = np.empty(nbytes.value // np.dtype(dtype).itemsize, dtype)
+        if not N.load().mfl_batch_read_buffer(self._h, kind, out.ctypes.data, out.nbytes):
+            raise RuntimeError(N.last_error())
+        return out.reshape(shape) if shape is not None else out
+
+    def upload(self, pos, vel, force) -> int:
+        ok = self.write(N.MFL_POSITION, np.asarray(pos, np.float32))
+        ok &= self.write(N.MFL_VELOCITY, np.asarray(vel, np.float32))
+        ok &= self.write(N.MFL_FORCE, np.asarray(force, np.float32))
+        return int(ok)
+
+    def state(self) -> dict:
+        return {k: self.read(kind, np.float32, (-1, 4)) for k, kind in
+                (("pos", N.MFL_POSITION), ("vel", N.MFL_VELOCITY), ("force", N.MFL_FORCE))}
+
+    def scene_tables(self, scene: int):
+        cs = self.read(N.MFL_CELL_START, np.uint32).reshape(len(self.counts), -1)
+        ce = self.read(N.MFL_CELL_END, np.uint32).reshape(len(self.counts), -1)
+        return cs[scene], ce[scene]
+
+
+class Adjoint:
+    """Reverse-mode derivative of `steps` oclFluid3D.compute() steps of a program (an addition with no
+    reference counterpart): forward() runs the rollout with checkpoints every `checkpoint_every` steps,
+    backward(pos_bar, vel_bar) returns dL/dx_0, dL/dv_0 and the viscosity / stiffness / rest-density
+    gradients."""
+
+    def __init__(self, program: "oclFluid3D", steps: int, checkpoint_every: int = 20, list_cap: int = 192):
+        self.program = program
+        self._h = N.load().mfl_adjoint_create(program._h, steps, checkpoint_every, list_cap)
+        if not self._h:
+            raise RuntimeError(N.last_error())
+        self.steps = steps
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h and N is not None and N._lib is not None:
+            N._lib.mfl_adjoint_destroy(h)
+            self._h = None
+
+    def forward(self, stream=None) -> int:
+        return int(N.load().mfl_adjoint_forward(self._h, _stream_ptr(stream)))
+
+    def backward(self, pos_bar=None, vel_bar=None, stream=None) -> int:
+        """pos_bar / vel_bar: contiguous float32 [N, 4] device tensors (dL/dx_T, dL/dv_T) or None."""
+        pp = pos_bar.data_ptr() if pos_bar is not None else None
+        vp = vel_bar.data_ptr() if vel_bar is not None else None
+        return int(N.load().mfl_adjoint_backward(self._h, pp, vp, _stream_ptr(stream)))
+
+    def velocity_grad(self):
+        """dL/dv_0 as a float32 [N, 4] numpy array (caller order)."""
+        return self._read(N.MFL_ADJ_VELOCITY_GRAD)
+
+    def position_grad(self):
+        return self._read(N.MFL_ADJ_POSITION_GRAD)
+
+    def _read(self, which: int) -> np.ndarray:
+        ptr, nbytes = ctypes.c_void_p(), ctypes.c_size_t()
+        if not N.load().mfl_adjoint_get_buffer(self._h, which, ctypes.byref(ptr), ctypes.byref(nbytes)):
+            raise RuntimeError(N.last_error())
+        out = np.empty(nbytes.value // 4, np.float32)
+        if not N.load().mfl_memcpy(out.ctypes.data, ptr.value, nbytes.value):
+            raise RuntimeError(N.last_error())
+        return out.reshape(-1, 4)
+
+    def param_grads(self) -> dict:
+        g = (ctypes.c_double * 4)()
+        if not N.load().mfl_adjoint_param_grads(self._h, g):
+            raise RuntimeError(N.last_error())
+        return {"viscosity": g[0], "stiffness": g[1], "density": g[2], "viscosityConstant": g[3]}
