@@ -36,6 +36,19 @@ STAGE_BYTES_PER_LAUNCH = {
 }
 
 
+def measured_traffic(stage: str, workload: str):
+    """HBM bytes per launch of a stage's kernel from the committed rocprofv3 counter summary
+    (profiles/traffic.json: FETCH_SIZE + WRITE_SIZE per dispatch, scaled by the copy calibration);
+    None when that kernel and workload were not measured."""
+    path = os.path.join(ROOT, "profiles", "traffic.json")
+    try:
+        with open(path) as fh:
+            table = json.load(fh)
+        return table[workload][stage]["bytes_per_launch"]
+    except (OSError, KeyError, ValueError):
+        return None
+
+
 def cpu_baseline(sc, prm, steps: int, threads: int) -> dict:
     """The oracle (a plain-C OpenMP restatement of the reference kernels) on the host cores."""
     from oracle import oracle as O
@@ -177,6 +190,8 @@ def main():
     ap.add_argument("--no-adjoint", action="store_true")
     ap.add_argument("--only", default=None, choices=["batched", "adjoint"],
                     help="run one secondary leg alone and print its JSON (profiling)")
+    ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
+                    help="also run a plain device copy of MIB MiB three times (counter calibration)")
     ap.add_argument("--batched-steps", type=int, default=100)
     ap.add_argument("--sequential-steps", type=int, default=5)
     args = ap.parse_args()
@@ -272,13 +287,21 @@ def main():
         "config": {"workload": sc.name, "particles": n, "cell_size": 0.02, "box": [0.0, 4.0],
                    "buckets": 262144, "bucket_clear": 65536, "traversal": args.order},
         "roofline": {"bound": "hbm", "kernel": dominant, "achieved": achieved, "peak": HBM_PEAK_GBS,
-                     "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
+                     "traffic": measured_traffic(dominant, sc.name),
                      "avg_launch_us": avg_s * 1e6, "alg_bytes_per_launch": alg_bytes},
         "stage_us": {k: v * 1e3 for k, v in stage_ms.items()},
         "step_hbm_frac": step_bytes / (ms_per_step / 1e3) / 1e9 / HBM_PEAK_GBS,
         "graph_ms_per_step": graph_ms,
         "graph_particle_steps_per_s": n / (graph_ms / 1e3),
     }
+    if args.calibrate_copy:
+        src = torch.ones(args.calibrate_copy * (1 << 18), dtype=torch.float32, device="cuda")
+        dst = torch.empty_like(src)
+        for _ in range(3):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        out["calibration_copy_bytes"] = src.numel() * 4
     if not args.no_batched:
         out["batched"] = batched_leg(args.batched_steps, args.sequential_steps)
     if not args.no_adjoint:

@@ -198,7 +198,7 @@ __global__ __launch_bounds__(kThreads) void k_density_nbr(float4* __restrict__ s
                                                           const uint32_t* __restrict__ porder,
                                                           const uint32_t* __restrict__ inv)
 {
-    const uint32_t t = blockIdx.x * kThreads + threadIdx.x;
+    const uint32_t t = xcd_block() * kThreads + threadIdx.x;
     if (t >= n) return;
     const uint32_t i = sorted_slot<ORDER>(t, n, porder, inv);
     if (i >= n) return;
@@ -259,7 +259,7 @@ __global__ __launch_bounds__(kThreads) void k_density_grp(float4* __restrict__ s
 {
     const uint32_t lane = threadIdx.x & (kWave - 1);
     const uint32_t grp = lane / kGroupLanes, slot = lane % kGroupLanes;
-    const uint32_t wave = (blockIdx.x * kThreads + threadIdx.x) / kWave;
+    const uint32_t wave = (xcd_block() * kThreads + threadIdx.x) / kWave;
     const uint32_t t = wave * (kWave / kGroupLanes) + grp;
     uint32_t i = t < n ? sorted_slot<ORDER>(t, n, porder, inv) : n;
     const bool live = i < n;
@@ -327,7 +327,7 @@ __global__ __launch_bounds__(kThreads) void k_forces_nbr(const float4* __restric
                                                          const uint32_t* __restrict__ porder,
                                                          const uint32_t* __restrict__ inv)
 {
-    const uint32_t t = blockIdx.x * kThreads + threadIdx.x;
+    const uint32_t t = xcd_block() * kThreads + threadIdx.x;
     if (t >= n) return;
     const uint32_t i = sorted_slot<ORDER>(t, n, porder, inv);
     if (i >= n) return;
@@ -354,7 +354,7 @@ __global__ __launch_bounds__(kThreads) void k_forces_step(const float4* __restri
                                                           float4 gravity_force, const uint32_t* __restrict__ porder,
                                                           const uint32_t* __restrict__ inv)
 {
-    const uint32_t t = blockIdx.x * kThreads + threadIdx.x;
+    const uint32_t t = xcd_block() * kThreads + threadIdx.x;
     if (t >= n) return;
     const uint32_t i = sorted_slot<ORDER>(t, n, porder, inv);
     if (i >= n) return;
@@ -697,6 +697,7 @@ int Fluid3D::enqueue_step(hipStream_t s)
     auto* cs = cell_start_.as<uint32_t>();
     auto* ce = cell_end_.as<uint32_t>();
     if (!enqueue_front(s)) return 0;
+    const dim3 gx = grid_xcd(n);
     auto* nbr = nbr_.as<uint32_t>();
     auto* ncount = ncount_.as<uint32_t>();
     auto* inv = inv_.as<uint32_t>();
@@ -706,19 +707,19 @@ int Fluid3D::enqueue_step(hipStream_t s)
     if (density_lanes == 1) {
         switch (order) {
         case MFL_TRAVERSAL_SORTED:
-            hipLaunchKernelGGL(k_density_nbr<kOrderSorted>, g, b, 0, s, spos, svel, cs, ce, n, params, nbr, ncount,
+            hipLaunchKernelGGL(k_density_nbr<kOrderSorted>, gx, b, 0, s, spos, svel, cs, ce, n, params, nbr, ncount,
                                porder, inv);
             break;
         case MFL_TRAVERSAL_CALLER:
-            hipLaunchKernelGGL(k_density_nbr<kOrderCaller>, g, b, 0, s, spos, svel, cs, ce, n, params, nbr, ncount,
+            hipLaunchKernelGGL(k_density_nbr<kOrderCaller>, gx, b, 0, s, spos, svel, cs, ce, n, params, nbr, ncount,
                                porder, inv);
             break;
         default:
-            hipLaunchKernelGGL(k_density_nbr<kOrderListed>, g, b, 0, s, spos, svel, cs, ce, n, params, nbr, ncount,
+            hipLaunchKernelGGL(k_density_nbr<kOrderListed>, gx, b, 0, s, spos, svel, cs, ce, n, params, nbr, ncount,
                                porder, inv);
         }
     } else {
-        const dim3 gg = grid_for((size_t)n * kGroupLanes);
+        const dim3 gg = grid_xcd((size_t)n * kGroupLanes);
         switch (order) {
         case MFL_TRAVERSAL_SORTED:
             hipLaunchKernelGGL(k_density_grp<kOrderSorted>, gg, b, 0, s, spos, svel, cs, ce, n, params, nbr, ncount,
@@ -737,7 +738,7 @@ int Fluid3D::enqueue_step(hipStream_t s)
     timer.begin(MFL_STAGE_FORCES, s);
     if (integrate_cb) {
 #define MFL_FORCES_NBR(O) \
-    hipLaunchKernelGGL(k_forces_nbr<O>, g, b, 0, s, spos, svel, force, index, cs, ce, nbr, ncount, n, params, porder, inv)
+    hipLaunchKernelGGL(k_forces_nbr<O>, gx, b, 0, s, spos, svel, force, index, cs, ce, nbr, ncount, n, params, porder, inv)
         switch (order) {
         case MFL_TRAVERSAL_SORTED: MFL_FORCES_NBR(kOrderSorted); break;
         case MFL_TRAVERSAL_CALLER: MFL_FORCES_NBR(kOrderCaller); break;
@@ -750,7 +751,7 @@ int Fluid3D::enqueue_step(hipStream_t s)
         return 1;
     }
 #define MFL_FORCES_STEP(O)                                                                                        \
-    hipLaunchKernelGGL(k_forces_step<O>, g, b, 0, s, spos, svel, pos, vel, force, index, cs, ce, nbr, ncount, n, \
+    hipLaunchKernelGGL(k_forces_step<O>, gx, b, 0, s, spos, svel, pos, vel, force, index, cs, ce, nbr, ncount, n, \
                        params, mn, mx, gf, porder, inv)
     switch (order) {
     case MFL_TRAVERSAL_SORTED: MFL_FORCES_STEP(kOrderSorted); break;

@@ -9,6 +9,24 @@ namespace sph {
 
 constexpr int kThreads = 256;
 
+// XCD-aware block order. Workgroups are dealt to the 8 XCDs round robin (workgroup b runs on XCD
+// b % 8), and each XCD has its own L2. Mapping b to chunk (b % 8) * (grid / 8) + b / 8 hands every XCD
+// one contiguous run of the traversal order, so particles that share neighbours share an L2. Grids
+// launched with it are a multiple of 8 blocks (grid_xcd).
+constexpr uint32_t kXcds = 8;
+
+__device__ inline uint32_t xcd_block()
+{
+    const uint32_t per = gridDim.x / kXcds;
+    return (blockIdx.x % kXcds) * per + blockIdx.x / kXcds;
+}
+
+inline dim3 grid_xcd(size_t threads)
+{
+    const size_t blocks = (threads + kThreads - 1) / kThreads;
+    return dim3((unsigned)((blocks + kXcds - 1) / kXcds * kXcds));
+}
+
 __device__ inline float4 f4_add(float4 a, float4 b) { return make_float4(a.x + b.x, a.y + b.y, a.z + b.z, a.w + b.w); }
 
 __device__ inline float4 f4_scale(float4 a, float s) { return make_float4(a.x * s, a.y * s, a.z * s, a.w * s); }
