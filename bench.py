@@ -192,6 +192,13 @@ def main():
                     help="run one secondary leg alone and print its JSON (profiling)")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="also run a plain device copy of MIB MiB three times (counter calibration)")
+    ap.add_argument("--pass", dest="npass", default="tile_list", choices=["tile_list", "tile", "list"],
+                    help="neighbour passes: LDS-tile density + recorded-visit forces, LDS tiles for both, or "
+                         "recorded visit lists for both")
+    ap.add_argument("--tile-depth", type=int, default=256)
+    ap.add_argument("--block-waves", type=int, default=4)
+    ap.add_argument("--no-autotune", action="store_true",
+                    help="keep the given launch shape instead of timing the candidates first")
     ap.add_argument("--batched-steps", type=int, default=100)
     ap.add_argument("--sequential-steps", type=int, default=5)
     args = ap.parse_args()
@@ -223,8 +230,15 @@ def main():
     f.setBox(sc.box_min, sc.box_max)
     assert f.upload(sc.pos, sc.vel, sc.force) == 1
     assert f.setTraversal(args.order, 16) == 1
+    assert f.setTuning(neighbour_pass=args.npass, tile_depth=args.tile_depth, block_waves=args.block_waves) == 1, \
+        L.last_error()
     stream = torch.cuda.Stream()
 
+    tuned = None
+    if not args.no_autotune:
+        res = f.autotune(steps=3, stream=stream)
+        assert res is not None, L.last_error()
+        tuned = {"ms_per_step": res[1], "pick": res[0]}
     for _ in range(args.warmup):
         assert f.compute(stream) == 1, L.last_error()
     stream.synchronize()
@@ -285,7 +299,8 @@ def main():
         "dtype": "f32",
         "data": "synthetic (seeded jittered lattice, SplitMix64)",
         "config": {"workload": sc.name, "particles": n, "cell_size": 0.02, "box": [0.0, 4.0],
-                   "buckets": 262144, "bucket_clear": 65536, "traversal": args.order},
+                   "buckets": 262144, "bucket_clear": 65536, "traversal": args.order,
+                   "tuning": f.getTuning(), "autotuned": not args.no_autotune},
         "roofline": {"bound": "hbm", "kernel": dominant, "achieved": achieved, "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
                      "traffic": measured_traffic(dominant, sc.name),
@@ -295,6 +310,8 @@ def main():
         "graph_ms_per_step": graph_ms,
         "graph_particle_steps_per_s": n / (graph_ms / 1e3),
     }
+    if tuned:
+        out["autotune"] = tuned
     if args.calibrate_copy:
         src = torch.ones(args.calibrate_copy * (1 << 18), dtype=torch.float32, device="cuda")
         dst = torch.empty_like(src)

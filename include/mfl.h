@@ -151,6 +151,29 @@ int mfl_fluid3d_set_use_graph(mfl_fluid3d* f, int enable);
 enum mfl_traversal { MFL_TRAVERSAL_SORTED = 0, MFL_TRAVERSAL_CALLER = 1, MFL_TRAVERSAL_MORTON = 2 };
 int mfl_fluid3d_set_traversal(mfl_fluid3d* f, int order, uint32_t refresh_steps);
 
+/* Addition: launch shape of the neighbour passes. Results differ between shapes only in FP32
+   summation order (visit sets, counts and acceptance are identical).
+   LIST: the density pass records every particle's accepted visits, the force pass reads the record.
+   TILE: both passes stage the 27 bucket ranges of each cell's particles in LDS (`tile_depth` entries
+   per wave, `block_waves` waves per workgroup), keeping only entries that can lie within h of the
+   particles; a neighbourhood that keeps more than tile_depth entries is walked from global memory.
+   TILE_LIST: the TILE density pass also records the visits, and the LIST force pass reads them. */
+enum mfl_neighbour_pass { MFL_PASS_LIST = 0, MFL_PASS_TILE = 1, MFL_PASS_TILE_LIST = 2 };
+typedef struct mfl_tuning {
+    int32_t neighbour_pass; /* mfl_neighbour_pass */
+    int32_t density_lanes;  /* LIST: lanes per particle in the density pass, 1 or 8 */
+    uint32_t tile_depth;    /* TILE, TILE_LIST: LDS entries per wave, multiple of 16 in 64..512 */
+    uint32_t block_waves;   /* TILE, TILE_LIST: waves per workgroup, 1, 2 or 4 */
+    uint32_t sort_items;    /* keys per thread of each radix sort pass: 4, 8 or 16; 0 = by key count */
+} mfl_tuning;
+int mfl_fluid3d_get_tuning(mfl_fluid3d* f, mfl_tuning* out);
+int mfl_fluid3d_set_tuning(mfl_fluid3d* f, const mfl_tuning* t);
+/* Times `steps` eager steps of each candidate shape from the program's current state, restoring the
+   state (positions, velocities, forces, bucket tables) before each trial and at the end, and keeps the
+   fastest; `best` and `ms_per_step` (either may be NULL) receive it. Needs an allocated state
+   (compile + particle count); enqueues on `stream` and synchronises it. */
+int mfl_fluid3d_autotune(mfl_fluid3d* f, void* stream, uint32_t steps, mfl_tuning* best, double* ms_per_step);
+
 /* Per-stage kernel timing with HIP events on the compute stream: the analogue of oclKernel::profile(true)
    + getStartTime/getEndTime (libCL/oclKernel.cpp:84-102,167-203). `stage_mask` has bit s set for each
    MFL_STAGE_* to time (0 disables). While a mask is set, compute() launches eagerly (no graph replay). */
@@ -209,7 +232,7 @@ int mfl_batch_compute(mfl_fluid_batch* b, void* stream);
 /* Reverse-mode derivative of `steps` compute() steps of a program, with checkpoints every
    `checkpoint_every` steps and recomputation in between. The sort permutation and the neighbour visits
    of each step (duplicates and stale-range visits included) are replayed from the forward as fixed
-   data. list_cap bounds the recorded visits per particle (backward fails with an error beyond it).
+   data. list_cap (rounded up to a multiple of 4) bounds the recorded visits per particle (backward fails with an error beyond it).
    Gradients: dL/dx_0 and dL/dv_0 (caller-order float4 device buffers), and dL/d viscosity,
    stiffness, rest density (FP64). The program must keep its particle count and buffers meanwhile. */
 enum mfl_adjoint_buffer { MFL_ADJ_POSITION_GRAD = 0, MFL_ADJ_VELOCITY_GRAD = 1 };

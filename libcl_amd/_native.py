@@ -29,6 +29,7 @@ EXPORTS = (
     "mfl_fluid3d_get_buffer", "mfl_fluid3d_write_buffer", "mfl_fluid3d_read_buffer", "mfl_fluid3d_params",
     "mfl_fluid3d_set_box", "mfl_fluid3d_set_gravity", "mfl_fluid3d_set_bucket_clear",
     "mfl_fluid3d_set_use_graph", "mfl_fluid3d_set_traversal", "mfl_fluid3d_set_profiling",
+    "mfl_fluid3d_get_tuning", "mfl_fluid3d_set_tuning", "mfl_fluid3d_autotune",
     "mfl_fluid3d_stage_time",
     "mfl_fluid3d_set_integrate_callback", "mfl_fluid3d_enqueue_kernel",
     "mfl_batch_create", "mfl_batch_destroy", "mfl_batch_scene_range", "mfl_batch_set_params",
@@ -53,6 +54,20 @@ class Params(ctypes.Structure):
 
     def values(self) -> list:
         return [float(getattr(self, k)) for k in PARAM_FIELDS]
+
+
+PASSES = ("list", "tile", "tile_list")
+
+
+class Tuning(ctypes.Structure):
+    """mfl_tuning: launch shape of the neighbour passes and the sort."""
+    _fields_ = [("neighbour_pass", ctypes.c_int32), ("density_lanes", ctypes.c_int32),
+                ("tile_depth", ctypes.c_uint32), ("block_waves", ctypes.c_uint32), ("sort_items", ctypes.c_uint32)]
+
+    def as_dict(self) -> dict:
+        return {"neighbour_pass": PASSES[self.neighbour_pass], "density_lanes": int(self.density_lanes),
+                "tile_depth": int(self.tile_depth), "block_waves": int(self.block_waves),
+                "sort_items": int(self.sort_items)}
 
 
 INTEGRATE_FN = ctypes.CFUNCTYPE(ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p)
@@ -108,6 +123,10 @@ def load() -> ctypes.CDLL:
         "mfl_fluid3d_set_use_graph": (ctypes.c_int, [vp, ctypes.c_int]),
         "mfl_fluid3d_set_traversal": (ctypes.c_int, [vp, ctypes.c_int, ctypes.c_uint32]),
         "mfl_fluid3d_set_profiling": (ctypes.c_int, [vp, ctypes.c_uint32]),
+        "mfl_fluid3d_get_tuning": (ctypes.c_int, [vp, ctypes.POINTER(Tuning)]),
+        "mfl_fluid3d_set_tuning": (ctypes.c_int, [vp, ctypes.POINTER(Tuning)]),
+        "mfl_fluid3d_autotune": (ctypes.c_int, [vp, vp, ctypes.c_uint32, ctypes.POINTER(Tuning),
+                                                ctypes.POINTER(ctypes.c_double)]),
         "mfl_fluid3d_stage_time": (ctypes.c_int, [vp, ctypes.c_int, ctypes.POINTER(ctypes.c_double),
                                                   ctypes.POINTER(ctypes.c_uint64), ctypes.c_int]),
         "mfl_fluid3d_set_integrate_callback": (ctypes.c_int, [vp, INTEGRATE_FN, vp]),

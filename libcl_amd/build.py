@@ -17,9 +17,10 @@ SOURCES = ["runtime.hip", "radix_sort.hip", "fluid3d.hip", "batch.hip", "adjoint
 ARCH = "gfx950"
 
 # Packed FP32 (v_pk_*_f32) generation is disabled for all device code: hand-written kernels here use
-# plain scalar / float4 FP32 arithmetic only.
+# plain scalar / float4 FP32 arithmetic only. Contraction is off: fused multiply-adds appear only where
+# the source writes fmaf, so every kernel that shares a formula rounds it the same way.
 HIPCC_FLAGS = [
-    f"--offload-arch={ARCH}", "-O3", "-fPIC", "-std=c++17", "-Wall", "-Wno-unused-result",
+    f"--offload-arch={ARCH}", "-O3", "-fPIC", "-std=c++17", "-Wall", "-Wno-unused-result", "-ffp-contract=off",
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
     "-I", os.path.join(os.path.dirname(HERE), "include"),
 ]

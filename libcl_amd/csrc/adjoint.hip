@@ -228,7 +228,7 @@ __device__ inline PairAdj pair_force_adjoint(const float4& pa, const float4& va,
 {
     PairAdj o;
     const float dx = pa.x - pb_.x, dy = pa.y - pb_.y, dz = pa.z - pb_.z;
-    const float r2 = dx * dx + dy * dy + dz * dz;
+    const float r2 = dist2(dx, dy, dz);
     const float inv_r = __builtin_amdgcn_rsqf(r2);
     const float rr = r2 * inv_r;
     const float hr = h - rr;
@@ -294,7 +294,7 @@ __global__ __launch_bounds__(kThreads) void k_adj_forces(const float4* __restric
         float ux = 0.f, uy = 0.f, uz = 0.f;
         const uint32_t count = ncount[j];
         for (uint32_t k = 0; k < count; k++) {
-            const uint32_t i = nbr[(size_t)k * n + j];
+            const uint32_t i = nbr[nbr_at(j, k, n)];
             const float4 pi = spos[i], vi = svel[i];
             // own: F_j's pair with i
             const PairAdj a = pair_force_adjoint(pj, vj, pi, vi, gjx, gjy, gjz, kp, kv, h);
@@ -387,10 +387,10 @@ __global__ __launch_bounds__(kThreads) void k_adj_density(const float4* __restri
     float xbx = 0.f, xby = 0.f, xbz = 0.f;
     const uint32_t count = ncount[j];
     for (uint32_t k = 0; k < count; k++) {
-        const uint32_t i = nbr[(size_t)k * n + j];
+        const uint32_t i = nbr[nbr_at(j, k, n)];
         const float4 pi = spos[i];
         const float dx = pj.x - pi.x, dy = pj.y - pi.y, dz = pj.z - pi.z;
-        const float w = h2 - (dx * dx + dy * dy + dz * dz);
+        const float w = h2 - (dist2(dx, dy, dz));
         const float w2 = -6.0f * w * w;
         float c = aj * w2;
         if (clean_j && clean[i] != 0) {
@@ -428,7 +428,7 @@ Adjoint::Adjoint(Fluid3D& fluid, uint32_t steps, uint32_t checkpoint_every, uint
     }
     steps_ = steps;
     every_ = checkpoint_every;
-    cap_ = list_cap;
+    cap_ = (list_cap + 3u) & ~3u;  // lists are stored in chunks of four entries
     n_ = fluid.particle_count();
     const size_t ncp = (steps_ + every_ - 1) / every_ + 1;
     const size_t seg = every_ < steps_ ? every_ : steps_;

@@ -271,6 +271,28 @@ int mfl_fluid3d_set_traversal(mfl_fluid3d* f, int order, uint32_t refresh_steps)
     return 1;
 }
 
+int mfl_fluid3d_get_tuning(mfl_fluid3d* f, mfl_tuning* out)
+{
+    if (!need(f, "fluid")) return 0;
+    return f->impl.get_tuning(out);
+}
+
+int mfl_fluid3d_set_tuning(mfl_fluid3d* f, const mfl_tuning* t)
+{
+    if (!need(f, "fluid")) return 0;
+    if (!t) {
+        mfl::set_error("null tuning");
+        return 0;
+    }
+    return f->impl.set_tuning(*t);
+}
+
+int mfl_fluid3d_autotune(mfl_fluid3d* f, void* stream, uint32_t steps, mfl_tuning* best, double* ms_per_step)
+{
+    if (!need(f, "fluid")) return 0;
+    return guarded([&] { return f->impl.autotune((hipStream_t)stream, steps, best, ms_per_step); });
+}
+
 int mfl_fluid3d_set_profiling(mfl_fluid3d* f, uint32_t stage_mask)
 {
     if (!need(f, "fluid")) return 0;

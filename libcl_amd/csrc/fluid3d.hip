@@ -17,6 +17,7 @@
 // With an EVT_INTEGRATE callback the force pass runs alone (k_forces) and the callback follows.
 #include "fluid3d.hpp"
 #include "sph_device.hpp"
+#include "tile.hpp"
 
 #include <cstring>
 #include <sstream>
@@ -119,7 +120,7 @@ __global__ __launch_bounds__(kThreads) void k_density(float4* __restrict__ spos,
                     const float4 pj = spos[j];
                     const float dx = pi.x - pj.x, dy = pi.y - pj.y, dz = pi.z - pj.z;
                     float vlen;
-                    if (within(dx * dx + dy * dy + dz * dz, h, h2_hi, &vlen)) {
+                    if (within(dist2(dx, dy, dz), h, h2_hi, &vlen)) {
                         const float t = h * h - vlen * vlen;
                         acc += t * t * t;
                         count++;
@@ -225,11 +226,11 @@ __global__ __launch_bounds__(kThreads) void k_density_nbr(float4* __restrict__ s
             for (uint32_t j = st[k]; j < end; j++) {
                 const float4 pj = spos[j];
                 const float dx = pi.x - pj.x, dy = pi.y - pj.y, dz = pi.z - pj.z;
-                const float r2 = dx * dx + dy * dy + dz * dz;
+                const float r2 = dist2(dx, dy, dz);
                 if (j != i && accept_r2(r2, h, h2_lo, h2_hi)) {
                     const float w = h2 - r2;  // h^2 - vlen^2 of cl:123
-                    acc += w * w * w;
-                    if (count < kNbrCap) nbr[(size_t)count * n + t] = j;
+                    acc = fmaf(w * w, w, acc);
+                    if (count < kNbrCap) nbr[nbr_at(t, count, n)] = j;
                     count++;
                 }
             }
@@ -364,6 +365,251 @@ __global__ __launch_bounds__(kThreads) void k_forces_step(const float4* __restri
     const uint32_t o = caller_slot<ORDER>(t, i, porder, index);
     if (o >= n) return;
     float4 f = add_sph_force(force[o], P, V, p);
+    f = clip_box(f, pi, vi, p.particleRadius, box_min, box_max);
+    const float4 v = integrate_force(vel[o], f, p);
+    force[o] = gravity_force;
+    vel[o] = v;
+    pos[o] = f4_add(pos[o], f4_scale(v, p.deltaTime));
+}
+
+// Walks of a neighbourhood that overflowed the tile, from global memory. Kept out of line so that the
+// rarely taken walk does not set the register budget of the tile kernels.
+// This lane's kernel sum and its group's visit count (the same in every lane of the group); records the
+// visits of traversal slot t in nbr when nbr is not null. Returned by value so that the callers' sums
+// stay in registers.
+struct VisitSums {
+    float acc;
+    uint32_t count;
+};
+
+__device__ __noinline__ VisitSums density_fallback(int cx, int cy, int cz, float4 pi, uint32_t i, uint32_t t,
+                                                  uint32_t slot, uint32_t grp, const float4* __restrict__ spos,
+                                                  const uint32_t* __restrict__ cell_start,
+                                                  const uint32_t* __restrict__ cell_end, uint32_t n, float h,
+                                                  uint32_t* __restrict__ nbr)
+{
+    const float h2 = h * h;
+    VisitSums r{0.f, 0u};
+    density_visits(cx, cy, cz, pi, i, t, slot, grp * kGroupLanes, true, spos, cell_start, cell_end, n, h, h2,
+                   h2 * 0.9999996f, h2 * 1.0000005f, nbr, n, r.acc, r.count, nbr ? kNbrCap : 0u);
+    return r;
+}
+
+__device__ __noinline__ void force_fallback(const float4* __restrict__ spos, const float4* __restrict__ svel,
+                                           const uint32_t* __restrict__ cell_start,
+                                           const uint32_t* __restrict__ cell_end, uint32_t i, uint32_t n, float4 pi,
+                                           float4 vi, float h, float3& P, float3& V)
+{
+    sph_force_sum(spos, svel, cell_start, cell_end, i, n, pi, vi, h, P, V);
+}
+
+// Particle of traversal slot t: its sorted slot i (n when t is past the end) and its position. The
+// caller-order position pos[o] equals spos[i] (k_reorder_clear copies it), so it is read in parallel
+// with the inverse permutation instead of after it.
+template <int ORDER>
+__device__ inline float4 slot_particle(uint32_t t, uint32_t n, const uint32_t* __restrict__ porder,
+                                       const uint32_t* __restrict__ inv, const float4* __restrict__ pos,
+                                       const float4* __restrict__ spos, uint32_t& i)
+{
+    i = n;
+    if (t >= n) return make_float4(0.f, 0.f, 0.f, 0.f);
+    if (ORDER == kOrderSorted) {
+        i = t;
+        return spos[t];
+    }
+    const uint32_t o = ORDER == kOrderCaller ? t : porder[t];
+    if (o >= n) return make_float4(0.f, 0.f, 0.f, 0.f);
+    i = inv[o];
+    return pos[o];
+}
+
+// Density pass over LDS-staged neighbourhoods (cl:150-200). Block = any multiple of 64 threads up to
+// 256; each wave owns `depth` float4 of dynamic LDS plus its StageAux. A neighbourhood that keeps more
+// than `depth` entries (or holds more than 512 visits) falls back to the group traversal from global
+// memory. LIST = true also records each particle's accepted visits (nbr, ncount by traversal slot) for
+// the recorded-visit force pass.
+template <int ORDER, bool LIST>
+__global__ __launch_bounds__(kThreads) void k_density_tile(const float4* __restrict__ pos, float4* __restrict__ spos,
+                                                           float4* __restrict__ svel,
+                                                           const uint32_t* __restrict__ cell_start,
+                                                           const uint32_t* __restrict__ cell_end, uint32_t n,
+                                                           mfl_params p, uint32_t depth,
+                                                           const uint32_t* __restrict__ porder,
+                                                           const uint32_t* __restrict__ inv, uint32_t* __restrict__ nbr,
+                                                           uint32_t* __restrict__ ncount)
+{
+    extern __shared__ float4 lds[];
+    const uint32_t waves = blockDim.x / kWave, wv = threadIdx.x / kWave;
+    const uint32_t lane = threadIdx.x & (kWave - 1);
+    float4* tile = lds + (size_t)wv * (depth + kGroupLanes);
+    StageAux* aux = reinterpret_cast<StageAux*>(lds + (size_t)waves * (depth + kGroupLanes)) + wv;
+    const uint32_t grp = lane / kGroupLanes, slot = lane % kGroupLanes;
+    const uint32_t wave = (xcd_block() * blockDim.x + threadIdx.x) / kWave;
+    const uint32_t t = wave * (kWave / kGroupLanes) + grp;
+    uint32_t i;
+    const float4 pi = slot_particle<ORDER>(t, n, porder, inv, pos, spos, i);
+    const bool live = i < n;
+    const float h = p.cellSize;
+    const float h2 = h * h;
+    const float h2_lo = h2 * 0.9999996f, h2_hi = h2 * 1.0000005f;
+    const float inf = __builtin_inff();
+    const int gx = grid_coord(pi.x, h), gy = grid_coord(pi.y, h), gz = grid_coord(pi.z, h);
+    float acc = 0.f;
+    uint32_t count = 0;  // this lane's accepted visits (group-summed at the end)
+    uint64_t todo = __ballot(live);
+    while (todo) {
+        const CellGroup c = next_cell(todo, live, gx, gy, gz, pi);
+        todo &= ~__ballot(c.member);
+        const uint32_t kept = stage_neighbourhood<false>(c.cx, c.cy, c.cz, c.lo, c.hi, h2 * kKeepMargin, spos,
+                                                         nullptr, cell_start, cell_end, n, lane, depth, tile,
+                                                         nullptr, nullptr, aux);
+        if (kept <= depth) {
+            // entry 8 m + slot is tested by this lane in trip m (of `trips`); bit trips - 1 - m of `hits`
+            // records its acceptance (kept <= 512, so trips <= 64). Eight entries past the end hold
+            // positions at infinity, so the last trip needs no bound test.
+            if (lane < kGroupLanes) tile[kept + lane] = make_float4(inf, inf, inf, __uint_as_float(kEmpty));
+            wave_sync();
+            const uint32_t trips = (kept + kGroupLanes - 1) / kGroupLanes;
+            float a = 0.f;
+            uint64_t hits = 0, band = 0;
+            for (uint32_t m = 0; m < trips; m++) {
+                const float4 e = tile[m * kGroupLanes + slot];
+                const float dx = pi.x - e.x, dy = pi.y - e.y, dz = pi.z - e.z;
+                const float r2 = dist2(dx, dy, dz);
+                const bool other = __float_as_uint(e.w) != i;
+                const bool ok = other && r2 < h2_lo;
+                band |= __ballot(other && !ok && r2 <= h2_hi);
+                const float w = h2 - r2;  // h^2 - vlen^2 of cl:123
+                a = ok ? fmaf(w * w, w, a) : a;
+                hits = (hits << 1) + (ok ? 1ull : 0ull);
+            }
+            if ((band & __ballot(c.member)) != 0) {
+                // a squared distance within rounding of h^2: redo the cell with the exact test
+                a = 0.f;
+                hits = 0;
+                for (uint32_t m = 0; m < trips; m++) {
+                    const float4 e = tile[m * kGroupLanes + slot];
+                    const float dx = pi.x - e.x, dy = pi.y - e.y, dz = pi.z - e.z;
+                    const float r2 = dist2(dx, dy, dz);
+                    const bool ok = __float_as_uint(e.w) != i && accept_r2(r2, h, h2_lo, h2_hi);
+                    const float w = h2 - r2;
+                    a = ok ? fmaf(w * w, w, a) : a;
+                    hits = (hits << 1) + (ok ? 1ull : 0ull);
+                }
+            }
+            if (c.member) {
+                acc += a;
+                count += (uint32_t)__popcll(hits);
+            }
+            if (LIST && c.member) {
+                // this lane's accepted visits go after those of the lower lanes of its group
+                const uint32_t mine = (uint32_t)__popcll(hits);
+                uint32_t at = mine;
+                const uint32_t u1 = (uint32_t)__shfl_up((int)at, 1, kGroupLanes);
+                if (slot >= 1) at += u1;
+                const uint32_t u2 = (uint32_t)__shfl_up((int)at, 2, kGroupLanes);
+                if (slot >= 2) at += u2;
+                const uint32_t u4 = (uint32_t)__shfl_up((int)at, 4, kGroupLanes);
+                if (slot >= 4) at += u4;
+                at -= mine;
+                while (hits) {
+                    const uint32_t m = trips - 1u - (uint32_t)__builtin_ctzll(hits);
+                    hits &= hits - 1ull;
+                    if (at < kNbrCap) nbr[nbr_at(t, at, n)] = __float_as_uint(tile[m * kGroupLanes + slot].w);
+                    at++;
+                }
+            }
+        } else if (c.member) {
+            const VisitSums f = density_fallback(c.cx, c.cy, c.cz, pi, i, t, slot, grp, spos, cell_start, cell_end, n,
+                                                 h, LIST ? nbr : nullptr);
+            acc += f.acc;
+            count += slot == 0 ? f.count : 0u;  // every lane of the group holds the group's count
+        }
+        wave_sync();  // the next cell's neighbourhood overwrites the tile
+    }
+    acc = group_sum(acc);
+    count = group_sum_u(count);
+    if (!live || slot != 0) return;
+    float rho, prs;
+    if (count > 2) {
+        rho = fmaxf(1.0f, p.mass * p.kernelConstant * acc);
+        prs = p.pressure + p.stiffness * (rho - p.density);
+    } else {
+        rho = p.density;
+        prs = p.pressure;
+    }
+    spos[i].w = rho;
+    svel[i].w = prs;
+    if (LIST) ncount[t] = count;
+}
+
+// Force pass over LDS-staged neighbourhoods (cl:202-255). STEP = true fuses clip box, integrate force,
+// gravity and integrate velocity (cl:258-361), as k_forces_step does. Each wave owns `depth` entries of
+// 36 bytes (position, velocity, index).
+template <int ORDER, bool STEP>
+__global__ __launch_bounds__(kThreads) void k_forces_tile(const float4* __restrict__ spos,
+                                                          const float4* __restrict__ svel, float4* __restrict__ pos,
+                                                          float4* __restrict__ vel, float4* __restrict__ force,
+                                                          const uint32_t* __restrict__ index,
+                                                          const uint32_t* __restrict__ cell_start,
+                                                          const uint32_t* __restrict__ cell_end, uint32_t n,
+                                                          mfl_params p, uint32_t depth, float4 box_min, float4 box_max,
+                                                          float4 gravity_force, const uint32_t* __restrict__ porder,
+                                                          const uint32_t* __restrict__ inv)
+{
+    extern __shared__ float4 lds[];
+    const uint32_t waves = blockDim.x / kWave, wv = threadIdx.x / kWave;
+    const uint32_t lane = threadIdx.x & (kWave - 1);
+    float4* tpos = lds + (size_t)wv * 2 * depth;
+    float4* tvel = tpos + depth;
+    uint32_t* tj = reinterpret_cast<uint32_t*>(lds + (size_t)waves * 2 * depth) + (size_t)wv * depth;
+    StageAux* aux = reinterpret_cast<StageAux*>(reinterpret_cast<uint32_t*>(lds + (size_t)waves * 2 * depth) +
+                                                (size_t)waves * depth) + wv;
+    const uint32_t grp = lane / kGroupLanes, slot = lane % kGroupLanes;
+    const uint32_t wave = (xcd_block() * blockDim.x + threadIdx.x) / kWave;
+    const uint32_t t = wave * (kWave / kGroupLanes) + grp;
+    uint32_t i;
+    const float4 px = slot_particle<ORDER>(t, n, porder, inv, pos, spos, i);
+    const bool live = i < n;
+    // density and pressure of the particle itself (the w lanes), needed only after staging
+    const float4 pi = live ? spos[i] : make_float4(0.f, 0.f, 0.f, 0.f);
+    const float4 vi = live ? svel[i] : make_float4(0.f, 0.f, 0.f, 0.f);
+    const float h = p.cellSize;
+    const float h2 = h * h;
+    const float h2_lo = h2 * 0.9999996f, h2_hi = h2 * 1.0000005f;
+    const int gx = grid_coord(px.x, h), gy = grid_coord(px.y, h), gz = grid_coord(px.z, h);
+    float3 P = make_float3(0.f, 0.f, 0.f), V = make_float3(0.f, 0.f, 0.f);
+    uint64_t todo = __ballot(live);
+    while (todo) {
+        const CellGroup c = next_cell(todo, live, gx, gy, gz, px);
+        todo &= ~__ballot(c.member);
+        const uint32_t kept = stage_neighbourhood<true>(c.cx, c.cy, c.cz, c.lo, c.hi, h2 * kKeepMargin, spos, svel,
+                                                        cell_start, cell_end, n, lane, depth, tpos, tvel, tj, aux);
+        if (c.member) {
+            if (kept <= depth) {
+                for (uint32_t q = slot; q < kept; q += kGroupLanes) {
+                    const float4 pj = tpos[q];
+                    const float dx = pi.x - pj.x, dy = pi.y - pj.y, dz = pi.z - pj.z;
+                    const float r2 = dist2(dx, dy, dz);
+                    if (tj[q] != i && accept_r2(r2, h, h2_lo, h2_hi))
+                        pair_force(pi, vi, pj, tvel[q], dx, dy, dz, r2, h, P, V);
+                }
+            } else if (slot == 0) {
+                force_fallback(spos, svel, cell_start, cell_end, i, n, pi, vi, h, P, V);
+            }
+        }
+        wave_sync();
+    }
+    P = make_float3(group_sum(P.x), group_sum(P.y), group_sum(P.z));
+    V = make_float3(group_sum(V.x), group_sum(V.y), group_sum(V.z));
+    if (!live || slot != 0) return;
+    const uint32_t o = caller_slot<ORDER>(t, i, porder, index);
+    if (o >= n) return;
+    float4 f = add_sph_force(force[o], P, V, p);
+    if (!STEP) {
+        force[o] = f;
+        return;
+    }
     f = clip_box(f, pi, vi, p.particleRadius, box_min, box_max);
     const float4 v = integrate_force(vel[o], f, p);
     force[o] = gravity_force;
@@ -703,6 +949,7 @@ int Fluid3D::enqueue_step(hipStream_t s)
     auto* inv = inv_.as<uint32_t>();
     auto* porder = porder_.as<uint32_t>();
     const int order = (traversal == MFL_TRAVERSAL_MORTON && !porder_valid_) ? MFL_TRAVERSAL_CALLER : traversal;
+    if (neighbour_pass != MFL_PASS_LIST) return enqueue_tile_passes(s, order);
     timer.begin(MFL_STAGE_DENSITY, s);
     if (density_lanes == 1) {
         switch (order) {
@@ -735,6 +982,26 @@ int Fluid3D::enqueue_step(hipStream_t s)
         }
     }
     timer.end(MFL_STAGE_DENSITY, s);
+    return enqueue_list_forces(s, order);
+}
+
+int Fluid3D::enqueue_list_forces(hipStream_t s, int order)
+{
+    const uint32_t n = (uint32_t)n_;
+    const dim3 gx = grid_xcd(n), b(kThreads);
+    const float4 mn = to_f4(box_min), mx = to_f4(box_max), gf = gravity_force_of(gravity);
+    auto* pos = pos_.as<float4>();
+    auto* vel = vel_.as<float4>();
+    auto* force = force_.as<float4>();
+    auto* spos = spos_.as<float4>();
+    auto* svel = svel_.as<float4>();
+    auto* index = index_.as<uint32_t>();
+    auto* cs = cell_start_.as<uint32_t>();
+    auto* ce = cell_end_.as<uint32_t>();
+    auto* nbr = nbr_.as<uint32_t>();
+    auto* ncount = ncount_.as<uint32_t>();
+    auto* inv = inv_.as<uint32_t>();
+    auto* porder = porder_.as<uint32_t>();
     timer.begin(MFL_STAGE_FORCES, s);
     if (integrate_cb) {
 #define MFL_FORCES_NBR(O) \
@@ -764,6 +1031,205 @@ int Fluid3D::enqueue_step(hipStream_t s)
     return 1;
 }
 
+// Tile passes: 8 particles per wave, `block_waves` waves per workgroup, grid a multiple of 8 blocks.
+static size_t tile_lds_bytes(bool forces, uint32_t waves, uint32_t depth)
+{
+    const size_t entries = forces ? depth * (2 * sizeof(float4) + sizeof(uint32_t))
+                                  : (depth + kGroupLanes) * sizeof(float4);  // + 8 end markers
+    return (size_t)waves * (entries + sizeof(StageAux));
+}
+
+int Fluid3D::enqueue_tile_passes(hipStream_t s, int order)
+{
+    const uint32_t n = (uint32_t)n_;
+    const float4 mn = to_f4(box_min), mx = to_f4(box_max), gf = gravity_force_of(gravity);
+    auto* pos = pos_.as<float4>();
+    auto* vel = vel_.as<float4>();
+    auto* force = force_.as<float4>();
+    auto* spos = spos_.as<float4>();
+    auto* svel = svel_.as<float4>();
+    auto* index = index_.as<uint32_t>();
+    auto* cs = cell_start_.as<uint32_t>();
+    auto* ce = cell_end_.as<uint32_t>();
+    auto* inv = inv_.as<uint32_t>();
+    auto* porder = porder_.as<uint32_t>();
+    const size_t waves = ((size_t)n + (kWave / kGroupLanes) - 1) / (kWave / kGroupLanes);
+    size_t blocks = (waves + block_waves - 1) / block_waves;
+    blocks = (blocks + kXcds - 1) / kXcds * kXcds;
+    const dim3 g((unsigned)blocks), b(block_waves * kWave);
+    const size_t lds_d = tile_lds_bytes(false, block_waves, tile_depth);
+    const size_t lds_f = tile_lds_bytes(true, block_waves, tile_depth);
+    const uint32_t depth = tile_depth;
+    timer.begin(MFL_STAGE_DENSITY, s);
+    auto* nbr = nbr_.as<uint32_t>();
+    auto* ncount = ncount_.as<uint32_t>();
+    const bool list = neighbour_pass == MFL_PASS_TILE_LIST;
+#define MFL_DENSITY_TILE(O, LIST)                                                                              \
+    hipLaunchKernelGGL((k_density_tile<O, LIST>), g, b, lds_d, s, pos, spos, svel, cs, ce, n, params, depth, porder, \
+                       inv, nbr, ncount)
+    switch (order) {
+    case MFL_TRAVERSAL_SORTED:
+        if (list) MFL_DENSITY_TILE(kOrderSorted, true); else MFL_DENSITY_TILE(kOrderSorted, false);
+        break;
+    case MFL_TRAVERSAL_CALLER:
+        if (list) MFL_DENSITY_TILE(kOrderCaller, true); else MFL_DENSITY_TILE(kOrderCaller, false);
+        break;
+    default:
+        if (list) MFL_DENSITY_TILE(kOrderListed, true); else MFL_DENSITY_TILE(kOrderListed, false);
+    }
+#undef MFL_DENSITY_TILE
+    timer.end(MFL_STAGE_DENSITY, s);
+    if (list) return enqueue_list_forces(s, order);
+    timer.begin(MFL_STAGE_FORCES, s);
+#define MFL_FORCES_TILE(O, STEP)                                                                                  \
+    hipLaunchKernelGGL((k_forces_tile<O, STEP>), g, b, lds_f, s, spos, svel, pos, vel, force, index, cs, ce, n,    \
+                       params, depth, mn, mx, gf, porder, inv)
+    if (integrate_cb) {
+        switch (order) {
+        case MFL_TRAVERSAL_SORTED: MFL_FORCES_TILE(kOrderSorted, false); break;
+        case MFL_TRAVERSAL_CALLER: MFL_FORCES_TILE(kOrderCaller, false); break;
+        default: MFL_FORCES_TILE(kOrderListed, false);
+        }
+        timer.end(MFL_STAGE_FORCES, s);
+        MFL_HIP(hipGetLastError());
+        (void)integrate_cb(handle, (void*)s, integrate_user);  // return value ignored, cpp:325-328
+        return 1;
+    }
+    switch (order) {
+    case MFL_TRAVERSAL_SORTED: MFL_FORCES_TILE(kOrderSorted, true); break;
+    case MFL_TRAVERSAL_CALLER: MFL_FORCES_TILE(kOrderCaller, true); break;
+    default: MFL_FORCES_TILE(kOrderListed, true);
+    }
+#undef MFL_FORCES_TILE
+    timer.end(MFL_STAGE_FORCES, s);
+    MFL_HIP(hipGetLastError());
+    return 1;
+}
+
+int Fluid3D::get_tuning(mfl_tuning* out) const
+{
+    if (!out) {
+        set_error("null tuning");
+        return 0;
+    }
+    out->neighbour_pass = neighbour_pass;
+    out->density_lanes = density_lanes;
+    out->tile_depth = tile_depth;
+    out->block_waves = block_waves;
+    out->sort_items = (uint32_t)sort.tuning.items;
+    return 1;
+}
+
+int Fluid3D::set_tuning(const mfl_tuning& t)
+{
+    if (t.neighbour_pass != MFL_PASS_LIST && t.neighbour_pass != MFL_PASS_TILE &&
+        t.neighbour_pass != MFL_PASS_TILE_LIST) {
+        set_error("neighbour_pass must be MFL_PASS_LIST, MFL_PASS_TILE or MFL_PASS_TILE_LIST");
+        return 0;
+    }
+    if (t.density_lanes != 1 && t.density_lanes != 8) {
+        set_error("density_lanes must be 1 or 8");
+        return 0;
+    }
+    if (t.tile_depth < kTileMinDepth || t.tile_depth > kTileMaxDepth || t.tile_depth % 16 != 0) {
+        set_error("tile_depth must be a multiple of 16 in [64, 512]");
+        return 0;
+    }
+    if (t.block_waves != 1 && t.block_waves != 2 && t.block_waves != 4) {
+        set_error("block_waves must be 1, 2 or 4");
+        return 0;
+    }
+    if (tile_lds_bytes(true, t.block_waves, t.tile_depth) > 65536) {
+        set_error("tile_depth x block_waves needs more than 64 KiB of LDS per workgroup");
+        return 0;
+    }
+    if (t.sort_items != 0 && t.sort_items != 4 && t.sort_items != 8 && t.sort_items != 16) {
+        set_error("sort_items must be 0, 4, 8 or 16");
+        return 0;
+    }
+    neighbour_pass = t.neighbour_pass;
+    density_lanes = t.density_lanes;
+    tile_depth = t.tile_depth;
+    block_waves = t.block_waves;
+    sort.tuning.items = (int)t.sort_items;
+    return 1;
+}
+
+int Fluid3D::autotune(hipStream_t s, uint32_t steps, mfl_tuning* best, double* ms_per_step)
+{
+    if (!check_user_buffers()) return 0;
+    if (steps == 0) {
+        set_error("autotune needs at least one step per trial");
+        return 0;
+    }
+    // snapshot of everything a step reads that a step also writes
+    const size_t vb = n_ * sizeof(float4), tb = (size_t)kBuckets * sizeof(uint32_t);
+    DeviceBuffer save;
+    if (!save.allocate(3 * vb + 2 * tb)) return 0;
+    char* sp = save.as<char>();
+    auto snapshot = [&](bool restore) -> int {
+        void* dev[] = {pos_.ptr, vel_.ptr, force_.ptr, cell_start_.ptr, cell_end_.ptr};
+        const size_t len[] = {vb, vb, vb, tb, tb};
+        size_t off = 0;
+        for (int k = 0; k < 5; k++) {
+            void* dst = restore ? dev[k] : sp + off;
+            const void* src = restore ? (const void*)(sp + off) : dev[k];
+            MFL_HIP(hipMemcpyAsync(dst, src, len[k], hipMemcpyDeviceToDevice, s));
+            off += len[k];
+        }
+        return 1;
+    };
+    if (!snapshot(false)) {
+        save.release();
+        return 0;
+    }
+    mfl_tuning start{};
+    get_tuning(&start);
+    std::vector<mfl_tuning> cand;
+    for (int lanes : {1, 8}) cand.push_back({MFL_PASS_LIST, lanes, start.tile_depth, start.block_waves, start.sort_items});
+    for (uint32_t w : {1u, 2u, 4u})
+        for (uint32_t d : {128u, 192u, 256u, 384u, 512u})
+            if (tile_lds_bytes(true, w, d) <= 65536)
+                for (int pass : {MFL_PASS_TILE, MFL_PASS_TILE_LIST})
+                    cand.push_back({pass, start.density_lanes, d, w, start.sort_items});
+    const bool graph_was = use_graph;
+    const uint32_t mask_was = timer.mask;
+    const uint32_t since = steps_since_refresh_;
+    use_graph = false;
+    timer.mask = 0;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    int ok = hip_ok(hipEventCreate(&e0), "hipEventCreate") && hip_ok(hipEventCreate(&e1), "hipEventCreate");
+    double best_ms = 1e30;
+    mfl_tuning pick = start;
+    for (size_t c = 0; ok && c < cand.size(); c++) {
+        ok = set_tuning(cand[c]) && snapshot(true) && compute(s);  // one untimed step per trial
+        for (uint32_t k = 0; ok && k < steps; k++) {
+            if (k == 0) ok = hip_ok(hipEventRecord(e0, s), "hipEventRecord");
+            ok = ok && compute(s);
+        }
+        ok = ok && hip_ok(hipEventRecord(e1, s), "hipEventRecord") && hip_ok(hipEventSynchronize(e1), "sync");
+        float ms = 0.f;
+        ok = ok && hip_ok(hipEventElapsedTime(&ms, e0, e1), "hipEventElapsedTime");
+        if (ok && ms / steps < best_ms) {
+            best_ms = ms / steps;
+            pick = cand[c];
+        }
+    }
+    if (e0) (void)hipEventDestroy(e0);
+    if (e1) (void)hipEventDestroy(e1);
+    use_graph = graph_was;
+    timer.mask = mask_was;
+    const int restored = snapshot(true) && hip_ok(hipStreamSynchronize(s), "hipStreamSynchronize");
+    steps_since_refresh_ = since;
+    porder_valid_ = false;  // the trials moved the particles; rebuild the traversal order on the next step
+    set_tuning(ok ? pick : start);
+    save.release();
+    if (!ok || !restored) return 0;
+    if (best) *best = pick;
+    if (ms_per_step) *ms_per_step = best_ms;
+    return 1;
+}
+
 void Fluid3D::drop_graph()
 {
     if (graph_exec_) (void)hipGraphExecDestroy(graph_exec_);
@@ -777,7 +1243,8 @@ int Fluid3D::launch_graph(hipStream_t s)
 {
     std::ostringstream sig;
     sig << n_ << ':' << pos_.ptr << ':' << vel_.ptr << ':' << force_.ptr << ':' << clear_count << ':'
-        << sort.tuning.items << ':' << traversal << ':' << porder_valid_ << ':' << density_lanes << ':';
+        << sort.tuning.items << ':' << traversal << ':' << porder_valid_ << ':' << density_lanes << ':'
+        << neighbour_pass << ':' << tile_depth << ':' << block_waves << ':';
     const unsigned char* raw[] = {reinterpret_cast<const unsigned char*>(&params),
                                   reinterpret_cast<const unsigned char*>(box_min),
                                   reinterpret_cast<const unsigned char*>(box_max),

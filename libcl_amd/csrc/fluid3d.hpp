@@ -57,7 +57,10 @@ public:
     bool use_graph = false;
     int traversal = MFL_TRAVERSAL_MORTON;   // order in which the neighbour passes visit particles
     uint32_t refresh_interval = 16;         // steps between Morton order refreshes
-    int density_lanes = 8;                  // lanes per particle in the density pass (1 or 8)
+    int density_lanes = 8;                  // lanes per particle in the list density pass (1 or 8)
+    int neighbour_pass = MFL_PASS_TILE_LIST;  // LIST, TILE or TILE_LIST neighbour passes
+    uint32_t tile_depth = 256;              // TILE: LDS entries per wave
+    uint32_t block_waves = 4;               // TILE: waves per workgroup
     RadixSort sort;
     StageTimer timer;
 
@@ -65,10 +68,15 @@ public:
     int enqueue_front(hipStream_t s);
     DeviceBuffer* select(int kind);
     bool check_user_buffers();
+    int get_tuning(mfl_tuning* out) const;
+    int set_tuning(const mfl_tuning& t);
+    int autotune(hipStream_t s, uint32_t steps, mfl_tuning* best, double* ms_per_step);
 
 private:
     bool allocate_internal();
     int enqueue_step(hipStream_t s);
+    int enqueue_tile_passes(hipStream_t s, int order);
+    int enqueue_list_forces(hipStream_t s, int order);
     int launch_graph(hipStream_t s);
     void drop_graph();
 

@@ -49,4 +49,9 @@ __host__ __device__ inline uint32_t cell_hash(int gx, int gy, int gz)
     return n & (kBuckets - 1u);
 }
 
+// |d|^2 as fma(dz, dz, fma(dy, dy, dx * dx)). Every kernel forms squared distances with it (the
+// library is built with contraction off), so acceptance decisions and sums agree bit for bit between
+// the kernels that share a pass.
+__host__ __device__ inline float dist2(float dx, float dy, float dz) { return fmaf(dz, dz, fmaf(dy, dy, dx * dx)); }
+
 }  // namespace mfl

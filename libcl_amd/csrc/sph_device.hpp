@@ -45,19 +45,20 @@ __device__ inline bool within(float r2, float h, float h2_hi, float* vlen)
 // with d = x_i - x_j and r2 = |d|^2. Both force paths (recorded visits and full traversal) use it, so
 // their sums are bit-identical.
 __device__ inline void pair_force(const float4& pi, const float4& vi, const float4& pj, const float4& vj, float dx,
-                                  float dy, float dz, float r2, float h, float3& P, float3& V)
+                                  float dy, float dz, float r2, float h, float3& P, float3& V, bool use = true)
 {
     const float inv_r = __builtin_amdgcn_rsqf(r2);
-    const float hr = h - r2 * inv_r;
+    const float hr = fmaf(-r2, inv_r, h);
     const float inv_d2 = __builtin_amdgcn_rcpf(pi.w * pj.w);
     const float s = (vi.w + vj.w) * inv_d2 * hr * hr * inv_r;
-    P.x += s * dx;
-    P.y += s * dy;
-    P.z += s * dz;
     const float w = hr * inv_d2;
-    V.x += w * (vj.x - vi.x);
-    V.y += w * (vj.y - vi.y);
-    V.z += w * (vj.z - vi.z);
+    // `use` = false computes the terms and keeps the sums (a select, not a branch)
+    P.x = use ? fmaf(s, dx, P.x) : P.x;
+    P.y = use ? fmaf(s, dy, P.y) : P.y;
+    P.z = use ? fmaf(s, dz, P.z) : P.z;
+    V.x = use ? fmaf(w, vj.x - vi.x, V.x) : V.x;
+    V.y = use ? fmaf(w, vj.y - vi.y, V.y) : V.y;
+    V.z = use ? fmaf(w, vj.z - vi.z, V.z) : V.z;
 }
 
 // Exact form of the reference test fast_length(d) < h (cl:177-178, cl:232-233): accepted when
@@ -93,7 +94,7 @@ __device__ inline void sph_force_sum(const float4* __restrict__ spos, const floa
                     if (j == i) continue;
                     const float4 pj = spos[j];
                     const float dx = pi.x - pj.x, dy = pi.y - pj.y, dz = pi.z - pj.z;
-                    const float r2 = dx * dx + dy * dy + dz * dz;
+                    const float r2 = dist2(dx, dy, dz);
                     if (accept_r2(r2, h, h2_lo, h2_hi)) pair_force(pi, vi, pj, svel[j], dx, dy, dz, r2, h, P, V);
                 }
             }
@@ -145,6 +146,15 @@ __device__ inline float4 integrate_force(float4 vel, float4 f, const mfl_params&
 
 constexpr uint32_t kNbrCap = 64;
 
+// Recorded visit lists, slot-major in chunks of four: entry k of list t is element
+// ((k / 4) * stride + t) * 4 + k % 4, so a lane reads four consecutive entries with one 16-byte load
+// and the lanes of a wave read one contiguous run. A list of capacity cap needs ceil(cap / 4) * 4 *
+// stride elements.
+__device__ inline size_t nbr_at(uint32_t t, uint32_t k, size_t stride)
+{
+    return (((size_t)(k >> 2) * stride + t) << 2) | (k & 3u);
+}
+
 // Density pass with eight lanes per particle. A wave takes eight consecutive entries of the traversal
 // order (in Morton or hash order usually the particles of one cell), and the eight lanes of a group
 // split each bucket range of their particle into interleaved slots, so bucket trip counts are the
@@ -180,15 +190,15 @@ __device__ inline void density_visits(int gx, int gy, int gz, const float4& pi, 
                 if (j < end) {
                     const float4 pj = spos[j];
                     const float dx = pi.x - pj.x, dy = pi.y - pj.y, dz = pi.z - pj.z;
-                    r2 = dx * dx + dy * dy + dz * dz;
+                    r2 = dist2(dx, dy, dz);
                     ok = live && j != i && accept_r2(r2, h, h2_lo, h2_hi);
                 }
                 const uint32_t mask = (uint32_t)(__ballot(ok) >> grp_shift) & 0xFFu;
                 if (ok) {
                     const float w = h2 - r2;  // h^2 - vlen^2 of cl:123
-                    acc += w * w * w;
+                    acc = fmaf(w * w, w, acc);
                     const uint32_t at = count + (uint32_t)__popc(mask & below);
-                    if (at < cap) nbr[(size_t)at * nbr_stride + t] = j;
+                    if (at < cap) nbr[nbr_at(t, at, nbr_stride)] = j;
                 }
                 count += (uint32_t)__popc(mask);
             }
@@ -203,12 +213,30 @@ __device__ inline void sph_force_sum_nbr(const float4* __restrict__ spos, const 
 {
     P = make_float3(0.f, 0.f, 0.f);
     V = make_float3(0.f, 0.f, 0.f);
-    for (uint32_t k = 0; k < count; k++) {
-        const uint32_t j = nbr[(size_t)k * nbr_stride + t];
-        const float4 pj = spos[j];
-        const float4 vj = svel[j];
-        const float dx = pi.x - pj.x, dy = pi.y - pj.y, dz = pi.z - pj.z;
-        pair_force(pi, vi, pj, vj, dx, dy, dz, dx * dx + dy * dy + dz * dz, h, P, V);
+    // one 16-byte chunk of the list per trip: four visits whose positions and velocities are in flight
+    // together (the sums still run in list order)
+    if (count == 0) return;
+    const uint32_t last = (count - 1) & ~3u;  // first entry of the last chunk
+    uint4 next = *reinterpret_cast<const uint4*>(nbr + nbr_at(t, 0, nbr_stride));
+    for (uint32_t k = 0; k < count; k += 4) {
+        const uint4 c = next;
+        // the following chunk (the last one again at the end) is fetched while this one is summed
+        next = *reinterpret_cast<const uint4*>(nbr + nbr_at(t, min(k + 4, last), nbr_stride));
+        const uint32_t m = min(count - k, 4u);
+        // past the end of the list the chunk holds no visit: load the first entry again instead, so that
+        // all eight loads issue together
+        const uint32_t j[4] = {c.x, m > 1 ? c.y : c.x, m > 2 ? c.z : c.x, m > 3 ? c.w : c.x};
+        float4 pj[4], vj[4];
+#pragma unroll
+        for (uint32_t u = 0; u < 4; u++) {
+            pj[u] = spos[j[u]];
+            vj[u] = svel[j[u]];
+        }
+#pragma unroll
+        for (uint32_t u = 0; u < 4; u++) {
+            const float dx = pi.x - pj[u].x, dy = pi.y - pj[u].y, dz = pi.z - pj[u].z;
+            pair_force(pi, vi, pj[u], vj[u], dx, dy, dz, dist2(dx, dy, dz), h, P, V, u < m);
+        }
     }
 }
 

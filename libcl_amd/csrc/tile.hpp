@@ -1,0 +1,210 @@
+// LDS-staged neighbourhoods for the density and force passes (clCalculateDensity / clCalculateForces,
+// libCL/phys/oclFluid3D.cl:150-255).
+//
+// A wave holds the particles of 8 consecutive traversal slots, 8 lanes each. Particles that share a
+// cell share the 27 bucket ranges the reference walks for them, so the wave stages those ranges once:
+// lanes 0..26 read one range each (dx fastest, then dy, then dz, as cl:166-170); a prefix sum lays the
+// ranges end to end (up to 512 visits), and the wave loads them 64 at a time, keeping, in visit order,
+// every entry that can lie within h of the box spanned by the wave's particles of that cell. The
+// particles then test only the kept entries, from LDS. Pruning is exact: an entry is dropped only when
+// its distance to the box exceeds h (1 + 5e-5), where the reference's fast_length(d) < h test is false
+// for every particle in the box. Repeated buckets are staged once per visit and stale ranges as they
+// stand, so every particle sees the reference's visit multiset.
+#pragma once
+
+#include "sph_device.hpp"
+
+namespace mfl {
+namespace sph {
+
+constexpr float kKeepMargin = 1.0001f;      // h^2 (1 + 1e-4): squared-distance margin of the box prune
+constexpr uint32_t kStageRounds = 8;        // visits of one neighbourhood laid end to end: at most 8 x 64
+constexpr uint32_t kTileMinDepth = 64, kTileMaxDepth = kStageRounds * 64;  // kept <= staged <= 512
+
+// Per-wave LDS bookkeeping of a staging: bit p % 64 of starts[p / 64] marks position p where a
+// non-empty bucket range begins, and delta[c] maps a position of the c-th non-empty range to its
+// sorted index (j = p + delta[c]).
+struct StageAux {
+    unsigned long long starts[kStageRounds];
+    uint32_t delta[32];
+};
+
+__device__ inline void wave_sync()
+{
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+__device__ inline uint32_t lanes_below(uint64_t m)
+{
+    return __builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u));
+}
+
+__device__ inline float group_sum(float v)
+{
+    v += __shfl_xor(v, 1, kWave);
+    v += __shfl_xor(v, 2, kWave);
+    v += __shfl_xor(v, 4, kWave);
+    return v;
+}
+
+__device__ inline uint32_t group_sum_u(uint32_t v)
+{
+    v += (uint32_t)__shfl_xor((int)v, 1, kWave);
+    v += (uint32_t)__shfl_xor((int)v, 2, kWave);
+    v += (uint32_t)__shfl_xor((int)v, 4, kWave);
+    return v;
+}
+
+// Rounds r0 .. r0 + NR - 1 of a staging (positions 64 r + lane): loads, box prune and in-order
+// compaction of the kept entries into the tile.
+template <int NR, bool FORCES>
+__device__ inline void stage_rounds(uint32_t r0, uint32_t total, uint32_t lane, uint64_t upto, float3 lo, float3 hi,
+                                    float h2_keep, const float4* __restrict__ spos, uint32_t depth, float4* tpos,
+                                    uint32_t* tj, const StageAux* aux, uint32_t& kept, uint32_t& before)
+{
+    float4 v[NR];
+    uint32_t jv[NR];
+    bool has[NR];
+#pragma unroll
+    for (int r = 0; r < NR; r++) {
+        const uint32_t q = (r0 + r) * kWave + lane;
+        const unsigned long long sm = aux->starts[r0 + r];
+        has[r] = q < total;
+        const uint32_t c = before + (uint32_t)__popcll(sm & upto) - 1u;  // non-empty ranges starting at or before q
+        before += (uint32_t)__popcll(sm);
+        jv[r] = q + aux->delta[c & 31u];
+        if (has[r]) v[r] = spos[jv[r]];
+    }
+#pragma unroll
+    for (int r = 0; r < NR; r++) {
+        bool keep = false;
+        if (has[r]) {
+            const float ex = fmaxf(fmaxf(lo.x - v[r].x, v[r].x - hi.x), 0.f);
+            const float ey = fmaxf(fmaxf(lo.y - v[r].y, v[r].y - hi.y), 0.f);
+            const float ez = fmaxf(fmaxf(lo.z - v[r].z, v[r].z - hi.z), 0.f);
+            keep = dist2(ex, ey, ez) < h2_keep;
+        }
+        const uint64_t m = __ballot(keep);
+        if (keep) {
+            const uint32_t at = kept + lanes_below(m);
+            if (at < depth) {
+                if (FORCES) {
+                    tpos[at] = v[r];
+                    tj[at] = jv[r];
+                } else {
+                    tpos[at] = make_float4(v[r].x, v[r].y, v[r].z, __uint_as_float(jv[r]));
+                }
+            }
+        }
+        kept += (uint32_t)__popcll(m);
+    }
+}
+
+// Stages the visits of cell (cx, cy, cz)'s 27 bucket ranges that can lie within h of [lo, hi] and
+// returns how many there are (at most `depth` are written), or kStageOverflow when the ranges hold more
+// than kStageRounds x 64 visits. FORCES = false writes float4(x, y, z, bits of j) to tpos; FORCES = true
+// writes spos[j] to tpos, svel[j] to tvel and j to tj. Called by the whole wave with uniform arguments.
+constexpr uint32_t kStageOverflow = 0xFFFFFFFFu;
+
+template <bool FORCES>
+__device__ inline uint32_t stage_neighbourhood(int cx, int cy, int cz, float3 lo, float3 hi, float h2_keep,
+                                               const float4* __restrict__ spos, const float4* __restrict__ svel,
+                                               const uint32_t* __restrict__ cell_start,
+                                               const uint32_t* __restrict__ cell_end, uint32_t n, uint32_t lane,
+                                               uint32_t depth, float4* tpos, float4* tvel, uint32_t* tj,
+                                               StageAux* aux)
+{
+    uint32_t st = 0, len = 0;
+    if (lane < 27) {
+        const int dx = (int)(lane % 3) - 1, dy = (int)((lane / 3) % 3) - 1, dz = (int)(lane / 9) - 1;
+        const uint32_t c = cell_hash(cx + dx, cy + dy, cz + dz);
+        const uint32_t s = cell_start[c];
+        const uint32_t e = min(cell_end[c], n);  // stale or written ranges always lie in [0, n]
+        if (s != kEmpty) {
+            st = s;
+            len = e > s ? e - s : 0u;
+        }
+    }
+    // ranges laid end to end in visit order: range k starts at position pre
+    uint32_t incl = len;
+#pragma unroll
+    for (uint32_t o = 1; o < 32; o <<= 1) {
+        const uint32_t up = (uint32_t)__shfl_up((int)incl, o, kWave);
+        if ((lane & 31u) >= o) incl += up;
+    }
+    const uint32_t pre = incl - len;
+    const uint32_t total = __builtin_amdgcn_readlane(incl, 26);
+    if (total > kStageRounds * kWave) return kStageOverflow;
+    if (lane < kStageRounds) aux->starts[lane] = 0ull;
+    wave_sync();
+    const uint64_t nonempty = __ballot(len > 0);
+    if (len > 0) {
+        aux->delta[lanes_below(nonempty)] = st - pre;
+        atomicOr(&aux->starts[pre / kWave], 1ull << (pre % kWave));
+    }
+    wave_sync();
+    const uint64_t upto = lane == kWave - 1 ? ~0ull : (2ull << lane) - 1ull;  // bits 0..lane
+    const uint32_t rounds = (total + kWave - 1) / kWave;
+    uint32_t kept = 0, before = 0, r0 = 0;
+    // rounds of 64 positions go four (then two, then one) at a time, so that their loads are in flight
+    // together without staging rounds past the end
+    for (; r0 + 4 <= rounds; r0 += 4)
+        stage_rounds<4, FORCES>(r0, total, lane, upto, lo, hi, h2_keep, spos, depth, tpos, tj, aux, kept, before);
+    if (r0 + 2 <= rounds) {
+        stage_rounds<2, FORCES>(r0, total, lane, upto, lo, hi, h2_keep, spos, depth, tpos, tj, aux, kept, before);
+        r0 += 2;
+    }
+    if (r0 < rounds)
+        stage_rounds<1, FORCES>(r0, total, lane, upto, lo, hi, h2_keep, spos, depth, tpos, tj, aux, kept, before);
+    if (FORCES) {
+        wave_sync();
+        const uint32_t m = min(kept, depth);
+        for (uint32_t q = lane; q < m; q += kWave) tvel[q] = svel[tj[q]];
+    }
+    wave_sync();
+    return kept;
+}
+
+// Min / max over the eight particle groups of a wave (the lanes of a group hold the same value).
+__device__ inline float groups_min(float v)
+{
+    v = fminf(v, __shfl_xor(v, 8, kWave));
+    v = fminf(v, __shfl_xor(v, 16, kWave));
+    return fminf(v, __shfl_xor(v, 32, kWave));
+}
+
+__device__ inline float groups_max(float v)
+{
+    v = fmaxf(v, __shfl_xor(v, 8, kWave));
+    v = fmaxf(v, __shfl_xor(v, 16, kWave));
+    return fmaxf(v, __shfl_xor(v, 32, kWave));
+}
+
+// The cell of the first remaining lane, whether this lane's particle is in it, and the bounding box
+// of the particles that are.
+struct CellGroup {
+    int cx, cy, cz;
+    bool member;
+    float3 lo, hi;
+};
+
+__device__ inline CellGroup next_cell(uint64_t todo, bool live, int gx, int gy, int gz, const float4& pi)
+{
+    CellGroup c;
+    const int src = __ffsll((unsigned long long)todo) - 1;
+    c.cx = __builtin_amdgcn_readlane(gx, src);
+    c.cy = __builtin_amdgcn_readlane(gy, src);
+    c.cz = __builtin_amdgcn_readlane(gz, src);
+    c.member = live && gx == c.cx && gy == c.cy && gz == c.cz;
+    const float inf = __builtin_inff();
+    c.lo = make_float3(groups_min(c.member ? pi.x : inf), groups_min(c.member ? pi.y : inf),
+                       groups_min(c.member ? pi.z : inf));
+    c.hi = make_float3(groups_max(c.member ? pi.x : -inf), groups_max(c.member ? pi.y : -inf),
+                       groups_max(c.member ? pi.z : -inf));
+    return c;
+}
+
+}  // namespace sph
+}  // namespace mfl

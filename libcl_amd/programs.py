@@ -206,6 +206,31 @@ class oclFluid3D:
         """Visit order of the neighbour passes: "sorted", "caller" or "morton" (results unchanged)."""
         return int(N.load().mfl_fluid3d_set_traversal(self._h, N.TRAVERSALS.index(order), refresh_steps))
 
+    def getTuning(self) -> dict:
+        """Launch shape of the neighbour passes and the sort (mfl_tuning as a dict)."""
+        t = N.Tuning()
+        if not N.load().mfl_fluid3d_get_tuning(self._h, ctypes.byref(t)):
+            raise RuntimeError(N.load().mfl_last_error().decode())
+        return t.as_dict()
+
+    def setTuning(self, **kw) -> int:
+        """Changes fields of the tuning: neighbour_pass ("list" / "tile"), density_lanes, tile_depth,
+        block_waves, sort_items. Results differ between shapes only in FP32 summation order."""
+        cur = self.getTuning()
+        cur.update(kw)
+        t = N.Tuning(N.PASSES.index(cur["neighbour_pass"]), cur["density_lanes"], cur["tile_depth"],
+                     cur["block_waves"], cur["sort_items"])
+        return int(N.load().mfl_fluid3d_set_tuning(self._h, ctypes.byref(t)))
+
+    def autotune(self, steps: int = 3, stream=None):
+        """Times each candidate shape on the current state (left unchanged) and keeps the fastest;
+        returns (tuning dict, ms per step) or None on failure."""
+        t = N.Tuning()
+        ms = ctypes.c_double()
+        if not N.load().mfl_fluid3d_autotune(self._h, _stream_ptr(stream), steps, ctypes.byref(t), ctypes.byref(ms)):
+            return None
+        return t.as_dict(), ms.value
+
     def setProfiling(self, stages) -> int:
         """Times the named stages (subset of _native.STAGES) with HIP events (oclKernel::profile)."""
         mask = 0

@@ -12,6 +12,7 @@ def _plain(sc, prm, steps):
     import libcl_amd as L
     f = L.oclFluid3D()
     assert f.compile() == 1
+    assert f.setTuning(neighbour_pass="list") == 1  # the batch kernels split visits among lanes the same way
     assert f.setParticleCount(sc.n) == 1
     f.setParameters(prm)
     f.setBox(sc.box_min, sc.box_max)

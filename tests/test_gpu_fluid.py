@@ -16,10 +16,20 @@ pytestmark = pytest.mark.gpu
 TOL = 1e-5
 
 
-def make_fluid(pos, vel, force, params, box_min, box_max, clear=65536, tables=None):
+# launch shapes of the neighbour passes: LDS tiles (default), tiles so shallow that dense
+# neighbourhoods overflow to the global-memory walk, and the recorded-visit-list passes
+SHAPES = {"tile_list": {}, "tile": {"neighbour_pass": "tile"},
+          "tile64": {"neighbour_pass": "tile", "tile_depth": 64, "block_waves": 2},
+          "tile_list64": {"neighbour_pass": "tile_list", "tile_depth": 64, "block_waves": 1},
+          "list": {"neighbour_pass": "list"}}
+
+
+def make_fluid(pos, vel, force, params, box_min, box_max, clear=65536, tables=None, shape="tile_list"):
     import libcl_amd as L
     f = L.oclFluid3D()
     assert f.compile() == 1, L.last_error()
+    if SHAPES[shape]:
+        assert f.setTuning(**SHAPES[shape]) == 1, L.last_error()
     assert f.setParticleCount(pos.shape[0]) == 1, L.last_error()
     f.setParameters(params)
     assert f.setBox(box_min, box_max) == 1
@@ -45,13 +55,14 @@ def run_forces_only(f):
     f.removeEventHandler()
 
 
+@pytest.mark.parametrize("shape", list(SHAPES))
 @pytest.mark.parametrize("name", STEP_CASES)
-def test_step_matches_golden(name):
+def test_step_matches_golden(name, shape):
     """One step from the committed fp64 all-pairs fixture (tests/golden/make_golden.py)."""
-    print(f"golden step {name}", flush=True)
+    print(f"golden step {name} {shape}", flush=True)
     z = load_step_case(name)
     prm = params_of(z["params"])
-    f = make_fluid(z["pos"], z["vel"], z["force"], prm, z["box_min"], z["box_max"])
+    f = make_fluid(z["pos"], z["vel"], z["force"], prm, z["box_min"], z["box_max"], shape=shape)
     run_forces_only(f)
     it = f.internals()
     np.testing.assert_array_equal(it["cell"], z["key"])
@@ -71,7 +82,7 @@ def test_step_matches_golden(name):
     assert np.all(st["force"][:, 3] == 0)
 
     # full step (clip box, clamp, Euler, gravity) from the same snapshot
-    f2 = make_fluid(z["pos"], z["vel"], z["force"], prm, z["box_min"], z["box_max"])
+    f2 = make_fluid(z["pos"], z["vel"], z["force"], prm, z["box_min"], z["box_max"], shape=shape)
     assert f2.compute() == 1
     st2 = f2.state()
     np.testing.assert_array_equal(st2["force"], np.tile(scenes.GRAVITY_FORCE, (z["pos"].shape[0], 1)))
@@ -97,8 +108,9 @@ def _oracle_compare(f, fo, check_fields=True):
     return it
 
 
+@pytest.mark.parametrize("shape", list(SHAPES))
 @pytest.mark.parametrize("clear", [65536, 262144])
-def test_small_dam_break_vs_oracle(oracle, clear):
+def test_small_dam_break_vs_oracle(oracle, clear, shape):
     """16,384-particle dam break: snapshots along the oracle trajectory (steps 0, 1, 10, 60) are
     stepped once on both sides, bucket tables included (stale ranges of the reference's partial
     clear are part of the state)."""
@@ -111,9 +123,10 @@ def test_small_dam_break_vs_oracle(oracle, clear):
     for target in (0, 1, 10, 60):
         fo.steps(target - done)
         done = target
-        print(f"16k snapshot step {target} clear {clear}", flush=True)
+        print(f"16k snapshot step {target} clear {clear} {shape}", flush=True)
         snap = (fo.pos.copy(), fo.vel.copy(), fo.force.copy(), fo.cell_start.copy(), fo.cell_end.copy())
-        f = make_fluid(snap[0], snap[1], snap[2], prm, sc.box_min, sc.box_max, clear, tables=(snap[3], snap[4]))
+        f = make_fluid(snap[0], snap[1], snap[2], prm, sc.box_min, sc.box_max, clear, tables=(snap[3], snap[4]),
+                       shape=shape)
         run_forces_only(f)
         fo2 = oracle.Fluid(*snap[:3], op, box_min=sc.box_min, box_max=sc.box_max, clear_count=clear,
                            cell_start=snap[3], cell_end=snap[4])
@@ -123,14 +136,16 @@ def test_small_dam_break_vs_oracle(oracle, clear):
         assert nw <= TOL and per <= TOL, (target, nw, per)
 
 
-def test_1m_step_vs_oracle(oracle):
+@pytest.mark.parametrize("shape", ["tile_list", "tile", "list"])
+def test_1m_step_vs_oracle(oracle, shape):
     """1,048,576 particles in [0,4]^3: keys, permutation, bucket tables bit-exact; density, pressure
     and forces of one step within 1e-5."""
+    print(f"1M step {shape}", flush=True)
     sc = scenes.config_1m()
     import libcl_amd as L
     prm = L.params_from(sc.params)
     op = oracle.Params(*prm.values())
-    f = make_fluid(sc.pos, sc.vel, sc.force, prm, sc.box_min, sc.box_max)
+    f = make_fluid(sc.pos, sc.vel, sc.force, prm, sc.box_min, sc.box_max, shape=shape)
     run_forces_only(f)
     fo = oracle.Fluid(sc.pos, sc.vel, sc.force, op, box_min=sc.box_min, box_max=sc.box_max)
     fo.step(integrate=False)
@@ -243,15 +258,15 @@ def test_particle_count_rules():
 
 
 def test_traversal_orders_bit_identical():
-    """The visit order of the neighbour passes changes no result: sorted, caller and Morton
-    traversal give bit-identical state over several steps (each particle's sums keep the reference's
-    visit order whatever thread computes them)."""
+    """With the recorded-visit passes the visit order changes no result: sorted, caller and Morton
+    traversal give bit-identical state over several steps (each particle's lane partial sums take the
+    same visits in the same order whatever wave computes them)."""
     import libcl_amd as L
     sc = scenes.dam_break(32, 16, 16, 1.0, seed=41, steps=5)
     prm = L.params_from(sc.params)
     states = []
     for order in ("sorted", "caller", "morton"):
-        f = make_fluid(sc.pos, sc.vel, sc.force, prm, sc.box_min, sc.box_max)
+        f = make_fluid(sc.pos, sc.vel, sc.force, prm, sc.box_min, sc.box_max, shape="list")
         assert f.setTraversal(order, 2) == 1
         for _ in range(5):
             assert f.compute() == 1, L.last_error()
@@ -262,3 +277,51 @@ def test_traversal_orders_bit_identical():
             np.testing.assert_array_equal(st[k], states[0][0][k])
         for k in ("spos", "svel", "cell_start", "cell_end", "index"):
             np.testing.assert_array_equal(it[k], states[0][1][k])
+
+
+@pytest.mark.parametrize("order", ["sorted", "caller", "morton"])
+@pytest.mark.parametrize("shape", ["tile_list", "tile", "tile64", "tile_list64"])
+def test_tile_passes_match_list_passes(order, shape):
+    """The LDS-tile passes take the same visits as the recorded-visit passes; only the FP32 summation
+    order differs (a particle's kept entries are split among its lanes by position in the pruned
+    tile, which depends on the wave's other particles). One step from identical state: identical keys,
+    permutation and tables, densities and forces within 1e-6."""
+    import libcl_amd as L
+    sc = scenes.dam_break(32, 16, 16, 1.0, seed=41, steps=5)
+    prm = L.params_from(sc.params)
+    ref = make_fluid(sc.pos, sc.vel, sc.force, prm, sc.box_min, sc.box_max, shape="list")
+    f = make_fluid(sc.pos, sc.vel, sc.force, prm, sc.box_min, sc.box_max, shape=shape)
+    assert f.setTraversal(order, 1) == 1
+    run_forces_only(ref)
+    run_forces_only(f)
+    a, b = ref.internals(), f.internals()
+    for k in ("cell", "index", "cell_start", "cell_end"):
+        np.testing.assert_array_equal(b[k], a[k])
+    np.testing.assert_allclose(b["spos"][:, 3], a["spos"][:, 3], rtol=1e-6)
+    nw, per = force_errors(f.state()["force"], ref.state()["force"])
+    assert nw <= 1e-6 and per <= 1e-6, (nw, per)
+
+
+def test_autotune_keeps_state_and_picks_a_valid_shape():
+    import libcl_amd as L
+    sc = scenes.config_small()
+    prm = L.params_from(sc.params)
+    f = make_fluid(sc.pos, sc.vel, sc.force, prm, sc.box_min, sc.box_max)
+    for _ in range(3):
+        assert f.compute() == 1
+    L.load().mfl_device_finish()
+    before, tables = f.state(), f.internals()
+    res = f.autotune(steps=2)
+    assert res is not None, L.last_error()
+    best, ms = res
+    print("autotune", best, ms, flush=True)
+    assert ms > 0 and f.getTuning() == best
+    after, tables2 = f.state(), f.internals()
+    for k in ("pos", "vel", "force"):
+        np.testing.assert_array_equal(after[k], before[k])
+    for k in ("cell_start", "cell_end"):
+        np.testing.assert_array_equal(tables2[k], tables[k])
+    assert f.compute() == 1, L.last_error()
+    L.load().mfl_device_finish()
+    assert f.setTuning(tile_depth=100) == 0 and "tile_depth" in L.last_error()
+    assert f.setTuning(neighbour_pass="tile", tile_depth=1024, block_waves=1) == 0 and "512" in L.last_error()
