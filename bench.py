@@ -242,6 +242,14 @@ def main():
     for _ in range(args.warmup):
         assert f.compute(stream) == 1, L.last_error()
     stream.synchronize()
+    # every timed leg starts from this state (the flow evolves, and later states cost more per step)
+    snap_state, snap_tables = f.state(), f.internals()
+
+    def restore():
+        assert f.upload(snap_state["pos"], snap_state["vel"], snap_state["force"]) == 1
+        assert f.bfCellStart.write(snap_tables["cell_start"]) == 1
+        assert f.bfCellEnd.write(snap_tables["cell_end"]) == 1
+        torch.cuda.synchronize()
 
     # untimed stage profile: which kernel dominates the step
     f.setProfiling(L._native.STAGES)
@@ -257,6 +265,7 @@ def main():
     dominant = max(stage_ms, key=stage_ms.get)
 
     # timed region: K steps, HIP events around the dominant kernel on its stream
+    restore()
     f.setProfiling([dominant])
     torch.cuda.synchronize()
     t0 = time.perf_counter()
@@ -270,10 +279,10 @@ def main():
     ms_per_step = elapsed * 1e3 / args.steps
     value = n * args.steps / elapsed
 
-    # same K steps replayed from a hipGraph (no events)
+    # the same K steps from the same state, replayed from a hipGraph (no events)
     f.setUseGraph(True)
     f.compute(stream)
-    torch.cuda.synchronize()
+    restore()
     g0 = time.perf_counter()
     for _ in range(args.steps):
         f.compute(stream)

@@ -28,10 +28,36 @@ using namespace sph;
 constexpr uint32_t kFlagOverflow = 1u;
 
 // FP64 accumulation of per-lane contributions: a wave sum, then one atomic per wave.
-__device__ inline void wave_add_f64(double v, double* dst)
+// Parameter gradients are summed in FP64 without atomics: every block adds its sum to its own slot
+// partials[block * kGradSlots + which] (launches on one stream are ordered, so the slots need no
+// atomics), and k_adj_reduce_grads adds the slots up in a fixed order at the end of backward. The sums
+// are therefore reproducible run to run.
+constexpr uint32_t kGradSlots = 4;
+
+__device__ inline void block_add_f64(double v, double* __restrict__ partials, uint32_t which)
 {
+    __shared__ double red[kThreads / kWave];
     for (int o = kWave / 2; o > 0; o >>= 1) v += __shfl_xor(v, o, kWave);
-    if ((threadIdx.x & (kWave - 1)) == 0 && v != 0.0) unsafeAtomicAdd(dst, v);
+    if ((threadIdx.x & (kWave - 1)) == 0) red[threadIdx.x / kWave] = v;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double sum = 0.0;
+        for (uint32_t w = 0; w < blockDim.x / kWave; w++) sum += red[w];
+        partials[(size_t)blockIdx.x * kGradSlots + which] += sum;
+    }
+    __syncthreads();  // red is reused by the next call
+}
+
+// grads[k] = sum over blocks of partials[block * kGradSlots + k], one wave per slot, fixed order.
+__global__ __launch_bounds__(kThreads) void k_adj_reduce_grads(const double* __restrict__ partials, uint32_t nblocks,
+                                                               double* __restrict__ grads)
+{
+    const uint32_t which = threadIdx.x / kWave, lane = threadIdx.x & (kWave - 1);
+    if (which >= kGradSlots) return;
+    double v = 0.0;
+    for (uint32_t b = lane; b < nblocks; b += kWave) v += partials[(size_t)b * kGradSlots + which];
+    for (int o = kWave / 2; o > 0; o >>= 1) v += __shfl_xor(v, o, kWave);
+    if (lane == 0) grads[which] = v;
 }
 
 __device__ inline void atomic_add3(float4* p, float x, float y, float z)
@@ -67,13 +93,36 @@ __device__ inline bool neighbourhood_clean(const float4& pi, float h, const uint
     return ok;
 }
 
-// k_density_grp in hash order with a caller-chosen list capacity; overflow is flagged.
+// neighbourhood_clean for a whole wave whose particles share cell (gx, gy, gz): lanes 0..26 take one
+// bucket each, check it for a stale range and compare it with the other 26.
+__device__ inline bool cell_clean(int gx, int gy, int gz, const uint32_t* __restrict__ cell_start,
+                                  const uint32_t* __restrict__ cell, uint32_t n)
+{
+    const uint32_t lane = threadIdx.x & (kWave - 1);
+    const uint32_t k = lane < 27 ? lane : 0u;
+    const uint32_t b = cell_hash(gx + (int)(k % 3) - 1, gy + (int)((k / 3) % 3) - 1, gz + (int)(k / 9) - 1);
+    bool bad = false;
+    if (lane < 27) {
+        const uint32_t st = cell_start[b];
+        bad = st != kEmpty && (st >= n || (cell[st] & (kBuckets - 1u)) != b);
+    }
+#pragma unroll
+    for (uint32_t d = 1; d < 27; d++) {
+        const uint32_t other = (uint32_t)__shfl((int)b, (int)((k + d) % 27), kWave);
+        bad = bad || (lane < 27 && other == b);
+    }
+    return __ballot(bad) == 0;
+}
+
+// k_density_grp in hash order with a caller-chosen list capacity; overflow is flagged. Also records
+// whether each particle's neighbourhood is clean.
 __global__ __launch_bounds__(kThreads) void k_rec_density(float4* __restrict__ spos, float4* __restrict__ svel,
                                                           const uint32_t* __restrict__ cell_start,
                                                           const uint32_t* __restrict__ cell_end, uint32_t n,
                                                           mfl_params p, uint32_t* __restrict__ nbr,
                                                           uint32_t* __restrict__ ncount, uint32_t cap,
-                                                          uint32_t* __restrict__ flags)
+                                                          uint32_t* __restrict__ flags,
+                                                          const uint32_t* __restrict__ cell, uint8_t* __restrict__ clean)
 {
     const uint32_t lane = threadIdx.x & (kWave - 1);
     const uint32_t grp = lane / kGroupLanes, slot = lane % kGroupLanes;
@@ -93,13 +142,18 @@ __global__ __launch_bounds__(kThreads) void k_rec_density(float4* __restrict__ s
     float acc = 0.f;
     uint32_t count = 0;
     const uint32_t grp_shift = grp * kGroupLanes;
-    if (__ballot(!same) == 0)
-        density_visits(__builtin_amdgcn_readfirstlane(gx0), __builtin_amdgcn_readfirstlane(gy0),
-                       __builtin_amdgcn_readfirstlane(gz0), pi, i, i, slot, grp_shift, live, spos, cell_start, cell_end,
-                       n, h, h2, h2_lo, h2_hi, nbr, n, acc, count, cap);
-    else
+    bool is_clean;
+    if (__ballot(!same) == 0) {
+        const int cx = __builtin_amdgcn_readfirstlane(gx0), cy = __builtin_amdgcn_readfirstlane(gy0),
+                  cz = __builtin_amdgcn_readfirstlane(gz0);
+        density_visits(cx, cy, cz, pi, i, i, slot, grp_shift, live, spos, cell_start, cell_end, n, h, h2, h2_lo, h2_hi,
+                       nbr, n, acc, count, cap);
+        is_clean = cell_clean(cx, cy, cz, cell_start, cell, n);
+    } else {
         density_visits(gx, gy, gz, pi, i, i, slot, grp_shift, live, spos, cell_start, cell_end, n, h, h2, h2_lo, h2_hi,
                        nbr, n, acc, count, cap);
+        is_clean = live && slot == 0 && neighbourhood_clean(pi, h, cell_start, cell, n);
+    }
     acc += __shfl_xor(acc, 1, kWave);
     acc += __shfl_xor(acc, 2, kWave);
     acc += __shfl_xor(acc, 4, kWave);
@@ -115,6 +169,7 @@ __global__ __launch_bounds__(kThreads) void k_rec_density(float4* __restrict__ s
     spos[i].w = rho;
     svel[i].w = prs;
     ncount[i] = count;
+    clean[i] = is_clean ? 1 : 0;
     if (count > cap) atomicOr(flags, kFlagOverflow);
 }
 
@@ -129,13 +184,15 @@ __global__ __launch_bounds__(kThreads) void k_rec_forces_step(const float4* __re
                                                               const uint32_t* __restrict__ ncount, uint32_t cap,
                                                               uint32_t n, mfl_params p, float4 box_min, float4 box_max,
                                                               float4 gravity_force, float4* __restrict__ fpre,
-                                                              const uint32_t* __restrict__ cell,
-                                                              uint8_t* __restrict__ clean)
+                                                              uint32_t* __restrict__ tape_index,
+                                                              float4* __restrict__ tape_spos,
+                                                              float4* __restrict__ tape_svel)
 {
     const uint32_t i = blockIdx.x * kThreads + threadIdx.x;
     if (i >= n) return;
     const float4 pi = spos[i], vi = svel[i];
-    clean[i] = neighbourhood_clean(pi, p.cellSize, cell_start, cell, n) ? 1 : 0;
+    tape_spos[i] = pi;  // sorted state with density and pressure: the tape of this step
+    tape_svel[i] = vi;
     float3 P, V;
     const uint32_t count = ncount[i];
     if (count <= cap)
@@ -143,6 +200,7 @@ __global__ __launch_bounds__(kThreads) void k_rec_forces_step(const float4* __re
     else
         sph_force_sum(spos, svel, cell_start, cell_end, i, n, pi, vi, p.cellSize, P, V);
     const uint32_t o = index[i];
+    tape_index[i] = o;
     if (o >= n) return;
     float4 f = add_sph_force(force[o], P, V, p);
     f = clip_box(f, pi, vi, p.particleRadius, box_min, box_max);
@@ -338,7 +396,7 @@ __global__ __launch_bounds__(kThreads) void k_adj_forces(const float4* __restric
         // dL/dKv = sum_j m fbar_j . V_j
         kv_grad = (double)p.mass * ((double)fj.x * ux + (double)fj.y * uy + (double)fj.z * uz);
     }
-    wave_add_f64(kv_grad, grads + 0);
+    block_add_f64(kv_grad, grads, 0);
 }
 
 // Adjoint of p = p0 + k (rho - rho0) and rho = max(1, m Kc acc) (count > 2), or rho = rho0: the
@@ -366,8 +424,8 @@ __global__ __launch_bounds__(kThreads) void k_adj_pressure(const float4* __restr
         }
         accbar[i] = ab;
     }
-    wave_add_f64(k_grad, grads + 1);
-    wave_add_f64(rho0_grad, grads + 2);
+    block_add_f64(k_grad, grads, 1);
+    block_add_f64(rho0_grad, grads, 2);
 }
 
 // Adjoint of acc_a = sum_b (h^2 - r_ab^2)^3 (sorted order): with c_a = -6 accbar_a (h^2 - r^2)^2 the
@@ -440,7 +498,7 @@ Adjoint::Adjoint(Fluid3D& fluid, uint32_t steps, uint32_t checkpoint_every, uint
         !t_ncount_.allocate(seg * u1) || !t_nbr_.allocate(seg * cap_ * u1) || !t_fpre_.allocate(seg * v4) ||
         !t_clean_.allocate(seg * n_) || !accbar_.allocate(u1) ||
         !xbar_.allocate(v4) || !vbar_.allocate(v4) || !xb_s_.allocate(v4) || !vb_s_.allocate(v4) ||
-        !rp_s_.allocate(n_ * 8) || !fbar_s_.allocate(v4) || !grads_.allocate(4 * sizeof(double)) ||
+        !rp_s_.allocate(n_ * 8) || !fbar_s_.allocate(v4) || !grads_.allocate((grid_for(n_).x + 1) * kGradSlots * sizeof(double)) ||
         !flags_.allocate(4))
         return;
     ok_ = true;
@@ -516,15 +574,14 @@ int Adjoint::record_step(uint32_t slot, hipStream_t s)
     uint32_t* ncount = t_ncount_.as<uint32_t>() + (size_t)slot * n_;
     float4* fpre = t_fpre_.as<float4>() + (size_t)slot * n_;
     hipLaunchKernelGGL(k_rec_density, grid_for((size_t)n * kGroupLanes), b, 0, s, spos, svel, cs, ce, n, f_.params, nbr,
-                       ncount, cap_, flags_.as<uint32_t>());
+                       ncount, cap_, flags_.as<uint32_t>(), f_.select(MFL_CELL)->as<uint32_t>(),
+                       t_clean_.as<uint8_t>() + (size_t)slot * n_);
     hipLaunchKernelGGL(k_rec_forces_step, g, b, 0, s, spos, svel, f_.select(MFL_POSITION)->as<float4>(),
                        f_.select(MFL_VELOCITY)->as<float4>(), f_.select(MFL_FORCE)->as<float4>(), index, cs, ce, nbr,
                        ncount, cap_, n, f_.params, to_f4(f_.box_min), to_f4(f_.box_max), gravity_force_of(f_.gravity),
-                       fpre, f_.select(MFL_CELL)->as<uint32_t>(), t_clean_.as<uint8_t>() + (size_t)slot * n_);
+                       fpre, t_index_.as<uint32_t>() + (size_t)slot * n_, t_spos_.as<float4>() + (size_t)slot * n_,
+                       t_svel_.as<float4>() + (size_t)slot * n_);
     MFL_HIP(hipGetLastError());
-    MFL_HIP(hipMemcpyAsync(t_index_.as<uint32_t>() + (size_t)slot * n_, index, n_ * 4, hipMemcpyDeviceToDevice, s));
-    MFL_HIP(hipMemcpyAsync(t_spos_.as<float4>() + (size_t)slot * n_, spos, n_ * 16, hipMemcpyDeviceToDevice, s));
-    MFL_HIP(hipMemcpyAsync(t_svel_.as<float4>() + (size_t)slot * n_, svel, n_ * 16, hipMemcpyDeviceToDevice, s));
     return 1;
 }
 
@@ -569,7 +626,7 @@ int Adjoint::backward(const float4* pos_bar, const float4* vel_bar, hipStream_t 
         MFL_HIP(hipMemcpyAsync(vbar_.ptr, vel_bar, v4, hipMemcpyDeviceToDevice, s));
     else
         MFL_HIP(hipMemsetAsync(vbar_.ptr, 0, v4, s));
-    MFL_HIP(hipMemsetAsync(grads_.ptr, 0, 4 * sizeof(double), s));
+    MFL_HIP(hipMemsetAsync(grads_.ptr, 0, grads_.bytes, s));
     MFL_HIP(hipMemsetAsync(flags_.ptr, 0, 4, s));
     const uint32_t nseg = (steps_ + every_ - 1) / every_;
     for (int c = (int)nseg - 1; c >= 0; c--) {
@@ -582,6 +639,10 @@ int Adjoint::backward(const float4* pos_bar, const float4* vel_bar, hipStream_t 
             if (!reverse_step((uint32_t)l, s)) return 0;
     }
     if (!restore_checkpoint((uint32_t)ckpt_.size() - 1, s)) return 0;
+    const uint32_t nblocks = grid_for(n_).x;
+    hipLaunchKernelGGL(k_adj_reduce_grads, dim3(1), dim3(kThreads), 0, s, grads_.as<double>(), nblocks,
+                       grads_.as<double>() + (size_t)nblocks * kGradSlots);
+    MFL_HIP(hipGetLastError());
     uint32_t flags = 0;
     MFL_HIP(hipMemcpyAsync(&flags, flags_.ptr, 4, hipMemcpyDeviceToHost, s));
     MFL_HIP(hipStreamSynchronize(s));
@@ -610,7 +671,7 @@ int Adjoint::param_grads(double* out4)
 {
     double g[4];
     MFL_HIP(hipDeviceSynchronize());
-    MFL_HIP(hipMemcpy(g, grads_.ptr, sizeof(g), hipMemcpyDeviceToHost));
+    MFL_HIP(hipMemcpy(g, grads_.as<double>() + (size_t)grid_for(n_).x * kGradSlots, sizeof(g), hipMemcpyDeviceToHost));
     // [0] accumulates dL/dKv; viscosityConstant = viscosity * 45/(pi h^6) (cl:146)
     const double kv = f_.params.viscosityConstant, mu = f_.params.viscosity;
     out4[0] = mu != 0.0 ? g[0] * kv / mu : 0.0;  // dL/d viscosity
