@@ -196,6 +196,7 @@ def main():
                     help="neighbour passes: LDS-tile density + recorded-visit forces, LDS tiles for both, or "
                          "recorded visit lists for both")
     ap.add_argument("--tile-depth", type=int, default=256)
+    ap.add_argument("--lanes", type=int, default=8, help="lanes per particle in the density pass")
     ap.add_argument("--block-waves", type=int, default=4)
     ap.add_argument("--no-autotune", action="store_true",
                     help="keep the given launch shape instead of timing the candidates first")
@@ -230,7 +231,8 @@ def main():
     f.setBox(sc.box_min, sc.box_max)
     assert f.upload(sc.pos, sc.vel, sc.force) == 1
     assert f.setTraversal(args.order, 16) == 1
-    assert f.setTuning(neighbour_pass=args.npass, tile_depth=args.tile_depth, block_waves=args.block_waves) == 1, \
+    assert f.setTuning(neighbour_pass=args.npass, tile_depth=args.tile_depth, block_waves=args.block_waves,
+                       density_lanes=args.lanes) == 1, \
         L.last_error()
     stream = torch.cuda.Stream()
 

@@ -161,8 +161,8 @@ int mfl_fluid3d_set_traversal(mfl_fluid3d* f, int order, uint32_t refresh_steps)
 enum mfl_neighbour_pass { MFL_PASS_LIST = 0, MFL_PASS_TILE = 1, MFL_PASS_TILE_LIST = 2 };
 typedef struct mfl_tuning {
     int32_t neighbour_pass; /* mfl_neighbour_pass */
-    int32_t density_lanes;  /* LIST: lanes per particle in the density pass, 1 or 8 */
-    uint32_t tile_depth;    /* TILE, TILE_LIST: LDS entries per wave, multiple of 16 in 64..512 */
+    int32_t density_lanes;  /* lanes per particle in the density pass: LIST 1 or 8; TILE, TILE_LIST 4 or 8 */
+    uint32_t tile_depth;    /* TILE, TILE_LIST: LDS entries per staged cell, multiple of 16 in 64..64 x lanes */
     uint32_t block_waves;   /* TILE, TILE_LIST: waves per workgroup, 1, 2 or 4 */
     uint32_t sort_items;    /* keys per thread of each radix sort pass: 4, 8 or 16; 0 = by key count */
 } mfl_tuning;

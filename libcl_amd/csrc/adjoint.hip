@@ -546,8 +546,11 @@ int Adjoint::forward(hipStream_t s)
         set_error("adjoint was not created");
         return 0;
     }
-    if (f_.particle_count() != n_ || f_.integrate_cb || f_.density_lanes != (int)kGroupLanes) {
-        set_error("adjoint: the program changed size, has an integrate callback or a non-default density pass");
+    // The forward runs the program's own step (any launch shape); backward recomputes each segment from
+    // its checkpoint with the recording kernels, whose sums may differ from the forward's in FP32
+    // summation order only, and differentiates that recomputed trajectory.
+    if (f_.particle_count() != n_ || f_.integrate_cb) {
+        set_error("adjoint: the program changed size or has an integrate callback");
         return 0;
     }
     if (!f_.check_user_buffers()) return 0;

@@ -423,12 +423,42 @@ __device__ inline float4 slot_particle(uint32_t t, uint32_t n, const uint32_t* _
     return pos[o];
 }
 
-// Density pass over LDS-staged neighbourhoods (cl:150-200). Block = any multiple of 64 threads up to
-// 256; each wave owns `depth` float4 of dynamic LDS plus its StageAux. A neighbourhood that keeps more
-// than `depth` entries (or holds more than 512 visits) falls back to the group traversal from global
-// memory. LIST = true also records each particle's accepted visits (nbr, ncount by traversal slot) for
-// the recorded-visit force pass.
-template <int ORDER, bool LIST>
+// One lane walks particle i's 27 bucket ranges from global memory (the reference's loop, cl:161-190) and
+// records its visits for traversal slot t when nbr is not null: the overflow path of the 4-lane layout.
+__device__ __noinline__ VisitSums density_walk(int cx, int cy, int cz, float4 pi, uint32_t i, uint32_t t,
+                                              const float4* __restrict__ spos, const uint32_t* __restrict__ cell_start,
+                                              const uint32_t* __restrict__ cell_end, uint32_t n, float h,
+                                              uint32_t* __restrict__ nbr)
+{
+    const float h2 = h * h;
+    const float h2_lo = h2 * 0.9999996f, h2_hi = h2 * 1.0000005f;
+    VisitSums r{0.f, 0u};
+    for (int k = 0; k < 27; k++) {
+        const uint32_t b = cell_hash(cx + k % 3 - 1, cy + (k / 3) % 3 - 1, cz + k / 9 - 1);
+        const uint32_t st = cell_start[b];
+        if (st == kEmpty) continue;
+        const uint32_t end = min(cell_end[b], n);
+        for (uint32_t j = st; j < end; j++) {
+            const float4 pj = spos[j];
+            const float dx = pi.x - pj.x, dy = pi.y - pj.y, dz = pi.z - pj.z;
+            const float r2 = dist2(dx, dy, dz);
+            if (j != i && accept_r2(r2, h, h2_lo, h2_hi)) {
+                const float w = h2 - r2;
+                r.acc = fmaf(w * w, w, r.acc);
+                if (nbr && r.count < kNbrCap) nbr[nbr_at(t, r.count, n)] = j;
+                r.count++;
+            }
+        }
+    }
+    return r;
+}
+
+// Density pass over LDS-staged neighbourhoods (cl:150-200). The wave is split into 64 / W segments of
+// 8 particles (W / 8 lanes each); each segment stages its cell's neighbourhood in its own `depth + 8`
+// float4 of dynamic LDS plus a StageAux. A neighbourhood that keeps more than `depth` entries (or holds
+// more than 512 visits) falls back to walking the ranges from global memory. LIST = true also records
+// each particle's accepted visits (nbr, ncount by traversal slot) for the recorded-visit force pass.
+template <int ORDER, bool LIST, int W>
 __global__ __launch_bounds__(kThreads) void k_density_tile(const float4* __restrict__ pos, float4* __restrict__ spos,
                                                            float4* __restrict__ svel,
                                                            const uint32_t* __restrict__ cell_start,
@@ -438,14 +468,16 @@ __global__ __launch_bounds__(kThreads) void k_density_tile(const float4* __restr
                                                            const uint32_t* __restrict__ inv, uint32_t* __restrict__ nbr,
                                                            uint32_t* __restrict__ ncount)
 {
+    constexpr uint32_t P = W / 8;  // lanes per particle
+    constexpr uint32_t SEGS = kWave / W;
     extern __shared__ float4 lds[];
-    const uint32_t waves = blockDim.x / kWave, wv = threadIdx.x / kWave;
-    const uint32_t lane = threadIdx.x & (kWave - 1);
-    float4* tile = lds + (size_t)wv * (depth + kGroupLanes);
-    StageAux* aux = reinterpret_cast<StageAux*>(lds + (size_t)waves * (depth + kGroupLanes)) + wv;
-    const uint32_t grp = lane / kGroupLanes, slot = lane % kGroupLanes;
+    const Lanes<W> L(threadIdx.x & (kWave - 1));
+    const uint32_t nseg = (blockDim.x / kWave) * SEGS, sid = (threadIdx.x / kWave) * SEGS + L.seg;
+    float4* tile = lds + (size_t)sid * (depth + kGroupLanes);
+    StageAux* aux = reinterpret_cast<StageAux*>(lds + (size_t)nseg * (depth + kGroupLanes)) + sid;
+    const uint32_t grp = L.ls / P, slot = L.ls % P;
     const uint32_t wave = (xcd_block() * blockDim.x + threadIdx.x) / kWave;
-    const uint32_t t = wave * (kWave / kGroupLanes) + grp;
+    const uint32_t t = wave * (kWave / P) + L.seg * 8 + grp;
     uint32_t i;
     const float4 pi = slot_particle<ORDER>(t, n, porder, inv, pos, spos, i);
     const bool live = i < n;
@@ -455,25 +487,23 @@ __global__ __launch_bounds__(kThreads) void k_density_tile(const float4* __restr
     const float inf = __builtin_inff();
     const int gx = grid_coord(pi.x, h), gy = grid_coord(pi.y, h), gz = grid_coord(pi.z, h);
     float acc = 0.f;
-    uint32_t count = 0;  // this lane's accepted visits (group-summed at the end)
-    uint64_t todo = __ballot(live);
-    while (todo) {
-        const CellGroup c = next_cell(todo, live, gx, gy, gz, pi);
-        todo &= ~__ballot(c.member);
-        const uint32_t kept = stage_neighbourhood<false>(c.cx, c.cy, c.cz, c.lo, c.hi, h2 * kKeepMargin, spos,
-                                                         nullptr, cell_start, cell_end, n, lane, depth, tile,
-                                                         nullptr, nullptr, aux);
+    uint32_t count = 0;  // this lane's accepted visits (summed over the particle's lanes at the end)
+    uint64_t todo = L.bits(__ballot(live));
+    while (__ballot(todo != 0) != 0) {
+        const SegCell c = next_cell_seg<W>(L, todo, live, gx, gy, gz, pi);
+        todo &= ~L.bits(__ballot(c.member));
+        const uint32_t kept = stage_seg<W>(L, c, h2 * kKeepMargin, spos, cell_start, cell_end, n, depth, tile, aux);
         if (kept <= depth) {
-            // entry 8 m + slot is tested by this lane in trip m (of `trips`); bit trips - 1 - m of `hits`
-            // records its acceptance (kept <= 512, so trips <= 64). Eight entries past the end hold
+            // entry P m + slot is tested by this lane in trip m (of `trips`); bit trips - 1 - m of `hits`
+            // records its acceptance (kept <= 64 P, so trips <= 64). Eight entries past the end hold
             // positions at infinity, so the last trip needs no bound test.
-            if (lane < kGroupLanes) tile[kept + lane] = make_float4(inf, inf, inf, __uint_as_float(kEmpty));
+            if (L.ls < kGroupLanes) tile[kept + L.ls] = make_float4(inf, inf, inf, __uint_as_float(kEmpty));
             wave_sync();
-            const uint32_t trips = (kept + kGroupLanes - 1) / kGroupLanes;
+            const uint32_t trips = (kept + P - 1) / P;
             float a = 0.f;
             uint64_t hits = 0, band = 0;
             for (uint32_t m = 0; m < trips; m++) {
-                const float4 e = tile[m * kGroupLanes + slot];
+                const float4 e = tile[m * P + slot];
                 const float dx = pi.x - e.x, dy = pi.y - e.y, dz = pi.z - e.z;
                 const float r2 = dist2(dx, dy, dz);
                 const bool other = __float_as_uint(e.w) != i;
@@ -488,7 +518,7 @@ __global__ __launch_bounds__(kThreads) void k_density_tile(const float4* __restr
                 a = 0.f;
                 hits = 0;
                 for (uint32_t m = 0; m < trips; m++) {
-                    const float4 e = tile[m * kGroupLanes + slot];
+                    const float4 e = tile[m * P + slot];
                     const float dx = pi.x - e.x, dy = pi.y - e.y, dz = pi.z - e.z;
                     const float r2 = dist2(dx, dy, dz);
                     const bool ok = __float_as_uint(e.w) != i && accept_r2(r2, h, h2_lo, h2_hi);
@@ -502,33 +532,39 @@ __global__ __launch_bounds__(kThreads) void k_density_tile(const float4* __restr
                 count += (uint32_t)__popcll(hits);
             }
             if (LIST && c.member) {
-                // this lane's accepted visits go after those of the lower lanes of its group
+                // this lane's accepted visits go after those of the particle's lower lanes
                 const uint32_t mine = (uint32_t)__popcll(hits);
                 uint32_t at = mine;
-                const uint32_t u1 = (uint32_t)__shfl_up((int)at, 1, kGroupLanes);
-                if (slot >= 1) at += u1;
-                const uint32_t u2 = (uint32_t)__shfl_up((int)at, 2, kGroupLanes);
-                if (slot >= 2) at += u2;
-                const uint32_t u4 = (uint32_t)__shfl_up((int)at, 4, kGroupLanes);
-                if (slot >= 4) at += u4;
+#pragma unroll
+                for (uint32_t o = 1; o < P; o <<= 1) {
+                    const uint32_t up = (uint32_t)__shfl_up((int)at, o, P);
+                    if (slot >= o) at += up;
+                }
                 at -= mine;
                 while (hits) {
                     const uint32_t m = trips - 1u - (uint32_t)__builtin_ctzll(hits);
                     hits &= hits - 1ull;
-                    if (at < kNbrCap) nbr[nbr_at(t, at, n)] = __float_as_uint(tile[m * kGroupLanes + slot].w);
+                    if (at < kNbrCap) nbr[nbr_at(t, at, n)] = __float_as_uint(tile[m * P + slot].w);
                     at++;
                 }
             }
         } else if (c.member) {
-            const VisitSums f = density_fallback(c.cx, c.cy, c.cz, pi, i, t, slot, grp, spos, cell_start, cell_end, n,
-                                                 h, LIST ? nbr : nullptr);
-            acc += f.acc;
-            count += slot == 0 ? f.count : 0u;  // every lane of the group holds the group's count
+            if constexpr (W == kWave) {
+                const VisitSums f = density_fallback(c.cx, c.cy, c.cz, pi, i, t, slot, grp, spos, cell_start, cell_end,
+                                                     n, h, LIST ? nbr : nullptr);
+                acc += f.acc;
+                count += slot == 0 ? f.count : 0u;  // every lane of the group holds the group's count
+            } else if (slot == 0) {
+                const VisitSums f = density_walk(c.cx, c.cy, c.cz, pi, i, t, spos, cell_start, cell_end, n, h,
+                                                 LIST ? nbr : nullptr);
+                acc += f.acc;
+                count += f.count;
+            }
         }
         wave_sync();  // the next cell's neighbourhood overwrites the tile
     }
-    acc = group_sum(acc);
-    count = group_sum_u(count);
+    acc = particle_sum<W>(acc);
+    count = particle_sum_u<W>(count);
     if (!live || slot != 0) return;
     float rho, prs;
     if (count > 2) {
@@ -1031,12 +1067,22 @@ int Fluid3D::enqueue_list_forces(hipStream_t s, int order)
     return 1;
 }
 
-// Tile passes: 8 particles per wave, `block_waves` waves per workgroup, grid a multiple of 8 blocks.
-static size_t tile_lds_bytes(bool forces, uint32_t waves, uint32_t depth)
+// Tile passes: `block_waves` waves per workgroup, grid a multiple of 8 blocks. The density pass holds
+// 64 / lanes particles per wave in 8-particle segments (lanes = 8: one segment, lanes = 4: two); the
+// force pass holds 8 particles per wave.
+static size_t tile_lds_bytes(bool forces, uint32_t waves, uint32_t depth, uint32_t lanes = 8)
 {
-    const size_t entries = forces ? depth * (2 * sizeof(float4) + sizeof(uint32_t))
-                                  : (depth + kGroupLanes) * sizeof(float4);  // + 8 end markers
-    return (size_t)waves * (entries + sizeof(StageAux));
+    if (forces) return (size_t)waves * (depth * (2 * sizeof(float4) + sizeof(uint32_t)) + sizeof(StageAux));
+    const size_t segs = (size_t)waves * (8 / lanes);
+    return segs * ((depth + kGroupLanes) * sizeof(float4) + sizeof(StageAux));  // + 8 end markers
+}
+
+static dim3 tile_grid(size_t n, uint32_t particles_per_wave, uint32_t block_waves)
+{
+    const size_t waves = (n + particles_per_wave - 1) / particles_per_wave;
+    size_t blocks = (waves + block_waves - 1) / block_waves;
+    blocks = (blocks + kXcds - 1) / kXcds * kXcds;
+    return dim3((unsigned)blocks);
 }
 
 int Fluid3D::enqueue_tile_passes(hipStream_t s, int order)
@@ -1053,30 +1099,33 @@ int Fluid3D::enqueue_tile_passes(hipStream_t s, int order)
     auto* ce = cell_end_.as<uint32_t>();
     auto* inv = inv_.as<uint32_t>();
     auto* porder = porder_.as<uint32_t>();
-    const size_t waves = ((size_t)n + (kWave / kGroupLanes) - 1) / (kWave / kGroupLanes);
-    size_t blocks = (waves + block_waves - 1) / block_waves;
-    blocks = (blocks + kXcds - 1) / kXcds * kXcds;
-    const dim3 g((unsigned)blocks), b(block_waves * kWave);
-    const size_t lds_d = tile_lds_bytes(false, block_waves, tile_depth);
+    const uint32_t lanes = (uint32_t)density_lanes;  // 4 or 8 (set_tuning checks)
+    const dim3 b(block_waves * kWave);
+    const dim3 gd = tile_grid(n, kWave / lanes, block_waves), g = tile_grid(n, kWave / kGroupLanes, block_waves);
+    const size_t lds_d = tile_lds_bytes(false, block_waves, tile_depth, lanes);
     const size_t lds_f = tile_lds_bytes(true, block_waves, tile_depth);
     const uint32_t depth = tile_depth;
     timer.begin(MFL_STAGE_DENSITY, s);
     auto* nbr = nbr_.as<uint32_t>();
     auto* ncount = ncount_.as<uint32_t>();
     const bool list = neighbour_pass == MFL_PASS_TILE_LIST;
-#define MFL_DENSITY_TILE(O, LIST)                                                                              \
-    hipLaunchKernelGGL((k_density_tile<O, LIST>), g, b, lds_d, s, pos, spos, svel, cs, ce, n, params, depth, porder, \
-                       inv, nbr, ncount)
+#define MFL_DENSITY_TILE(O, LIST, W)                                                                             \
+    hipLaunchKernelGGL((k_density_tile<O, LIST, W>), gd, b, lds_d, s, pos, spos, svel, cs, ce, n, params, depth,  \
+                       porder, inv, nbr, ncount)
+#define MFL_DENSITY_TILE_W(O, LIST)                 \
+    if (lanes == 4) MFL_DENSITY_TILE(O, LIST, 32); \
+    else MFL_DENSITY_TILE(O, LIST, 64)
     switch (order) {
     case MFL_TRAVERSAL_SORTED:
-        if (list) MFL_DENSITY_TILE(kOrderSorted, true); else MFL_DENSITY_TILE(kOrderSorted, false);
+        if (list) { MFL_DENSITY_TILE_W(kOrderSorted, true); } else { MFL_DENSITY_TILE_W(kOrderSorted, false); }
         break;
     case MFL_TRAVERSAL_CALLER:
-        if (list) MFL_DENSITY_TILE(kOrderCaller, true); else MFL_DENSITY_TILE(kOrderCaller, false);
+        if (list) { MFL_DENSITY_TILE_W(kOrderCaller, true); } else { MFL_DENSITY_TILE_W(kOrderCaller, false); }
         break;
     default:
-        if (list) MFL_DENSITY_TILE(kOrderListed, true); else MFL_DENSITY_TILE(kOrderListed, false);
+        if (list) { MFL_DENSITY_TILE_W(kOrderListed, true); } else { MFL_DENSITY_TILE_W(kOrderListed, false); }
     }
+#undef MFL_DENSITY_TILE_W
 #undef MFL_DENSITY_TILE
     timer.end(MFL_STAGE_DENSITY, s);
     if (list) return enqueue_list_forces(s, order);
@@ -1127,8 +1176,10 @@ int Fluid3D::set_tuning(const mfl_tuning& t)
         set_error("neighbour_pass must be MFL_PASS_LIST, MFL_PASS_TILE or MFL_PASS_TILE_LIST");
         return 0;
     }
-    if (t.density_lanes != 1 && t.density_lanes != 8) {
-        set_error("density_lanes must be 1 or 8");
+    if (t.neighbour_pass == MFL_PASS_LIST ? (t.density_lanes != 1 && t.density_lanes != 8)
+                                          : (t.density_lanes != 4 && t.density_lanes != 8)) {
+        set_error(t.neighbour_pass == MFL_PASS_LIST ? "density_lanes must be 1 or 8 with MFL_PASS_LIST"
+                                                    : "density_lanes must be 4 or 8 with the tile passes");
         return 0;
     }
     if (t.tile_depth < kTileMinDepth || t.tile_depth > kTileMaxDepth || t.tile_depth % 16 != 0) {
@@ -1139,7 +1190,13 @@ int Fluid3D::set_tuning(const mfl_tuning& t)
         set_error("block_waves must be 1, 2 or 4");
         return 0;
     }
-    if (tile_lds_bytes(true, t.block_waves, t.tile_depth) > 65536) {
+    if (t.neighbour_pass != MFL_PASS_LIST && t.tile_depth > 64u * (uint32_t)t.density_lanes) {
+        set_error("tile_depth above 64 x density_lanes is not available with the tile passes");
+        return 0;
+    }
+    if (tile_lds_bytes(true, t.block_waves, t.tile_depth) > 65536 ||
+        (t.neighbour_pass != MFL_PASS_LIST &&
+         tile_lds_bytes(false, t.block_waves, t.tile_depth, (uint32_t)t.density_lanes) > 65536)) {
         set_error("tile_depth x block_waves needs more than 64 KiB of LDS per workgroup");
         return 0;
     }
@@ -1186,12 +1243,13 @@ int Fluid3D::autotune(hipStream_t s, uint32_t steps, mfl_tuning* best, double* m
     mfl_tuning start{};
     get_tuning(&start);
     std::vector<mfl_tuning> cand;
-    for (int lanes : {1, 8}) cand.push_back({MFL_PASS_LIST, lanes, start.tile_depth, start.block_waves, start.sort_items});
+    for (int lanes : {1, 8}) cand.push_back({MFL_PASS_LIST, lanes, 256u, 4u, start.sort_items});
     for (uint32_t w : {1u, 2u, 4u})
         for (uint32_t d : {128u, 192u, 256u, 384u, 512u})
-            if (tile_lds_bytes(true, w, d) <= 65536)
-                for (int pass : {MFL_PASS_TILE, MFL_PASS_TILE_LIST})
-                    cand.push_back({pass, start.density_lanes, d, w, start.sort_items});
+            for (int lanes : {4, 8})
+                if (tile_lds_bytes(true, w, d) <= 65536 && d <= 64u * (uint32_t)lanes)
+                    for (int pass : {MFL_PASS_TILE, MFL_PASS_TILE_LIST})
+                        cand.push_back({pass, lanes, d, w, start.sort_items});
     const bool graph_was = use_graph;
     const uint32_t mask_was = timer.mask;
     const uint32_t since = steps_since_refresh_;

@@ -206,5 +206,185 @@ __device__ inline CellGroup next_cell(uint64_t todo, bool live, int gx, int gy, 
     return c;
 }
 
+// ---- segmented tiles: W-lane segments of a wave, each staging the cell of its own 8 particles ----------
+//
+// With W = 32 a wave holds 16 particles (4 lanes each) of up to two cells and stages both
+// neighbourhoods at once, halving the per-cell cost of everything that runs once per staging (range
+// fetch, prefix sum, cell and box selection, reductions). W = 64 is the one-cell-per-wave layout.
+
+template <int W>
+struct Lanes {
+    static constexpr uint32_t kLpp = W / 8;  // lanes per particle
+    uint32_t lane, seg, ls;                   // lane of the wave, segment, lane within the segment
+    __device__ explicit Lanes(uint32_t l) : lane(l), seg(l / W), ls(l % W) {}
+    // this segment's bits of a wave ballot, as bits 0 .. W-1
+    __device__ uint64_t bits(uint64_t b) const
+    {
+        if constexpr (W == 64) return b;
+        else return (b >> (seg * W)) & ((1ull << W) - 1ull);
+    }
+    __device__ uint32_t below(uint64_t segbits) const { return (uint32_t)__popcll(segbits & ((1ull << ls) - 1ull)); }
+    __device__ int from(int v, uint32_t k) const { return __shfl(v, (int)(seg * W + k), kWave); }
+    __device__ uint32_t from(uint32_t v, uint32_t k) const { return (uint32_t)__shfl((int)v, (int)(seg * W + k), kWave); }
+};
+
+// min / max / sum over the 8 particles of a segment (the lanes of one particle hold equal values)
+template <int W>
+__device__ inline float seg_groups_min(float v)
+{
+    constexpr int P = W / 8;
+    v = fminf(v, __shfl_xor(v, P, kWave));
+    v = fminf(v, __shfl_xor(v, 2 * P, kWave));
+    return fminf(v, __shfl_xor(v, 4 * P, kWave));
+}
+
+template <int W>
+__device__ inline float seg_groups_max(float v)
+{
+    constexpr int P = W / 8;
+    v = fmaxf(v, __shfl_xor(v, P, kWave));
+    v = fmaxf(v, __shfl_xor(v, 2 * P, kWave));
+    return fmaxf(v, __shfl_xor(v, 4 * P, kWave));
+}
+
+// sum over the lanes of one particle
+template <int W>
+__device__ inline float particle_sum(float v)
+{
+#pragma unroll
+    for (int o = 1; o < W / 8; o <<= 1) v += __shfl_xor(v, o, kWave);
+    return v;
+}
+
+template <int W>
+__device__ inline uint32_t particle_sum_u(uint32_t v)
+{
+#pragma unroll
+    for (int o = 1; o < W / 8; o <<= 1) v += (uint32_t)__shfl_xor((int)v, o, kWave);
+    return v;
+}
+
+struct SegCell {
+    int cx, cy, cz;
+    bool active;  // the segment has a cell this round
+    bool member;  // this lane's particle is in it
+    float3 lo, hi;
+};
+
+// The cell of the segment's first remaining particle (todo: the segment's W-bit mask of lanes still to
+// serve), whether this lane's particle is in it, and the box of the particles that are.
+template <int W>
+__device__ inline SegCell next_cell_seg(const Lanes<W>& L, uint64_t todo, bool live, int gx, int gy, int gz,
+                                        const float4& pi)
+{
+    SegCell c;
+    c.active = todo != 0;
+    const uint32_t src = c.active ? (uint32_t)(__ffsll((unsigned long long)todo) - 1) : 0u;
+    c.cx = L.from(gx, src);
+    c.cy = L.from(gy, src);
+    c.cz = L.from(gz, src);
+    c.member = c.active && live && gx == c.cx && gy == c.cy && gz == c.cz;
+    const float inf = __builtin_inff();
+    c.lo = make_float3(seg_groups_min<W>(c.member ? pi.x : inf), seg_groups_min<W>(c.member ? pi.y : inf),
+                       seg_groups_min<W>(c.member ? pi.z : inf));
+    c.hi = make_float3(seg_groups_max<W>(c.member ? pi.x : -inf), seg_groups_max<W>(c.member ? pi.y : -inf),
+                       seg_groups_max<W>(c.member ? pi.z : -inf));
+    return c;
+}
+
+// Rounds r0 .. r0 + NR - 1 of a segment's staging (positions W r + ls): loads, box prune, in-order
+// compaction of the kept entries into the tile as float4(x, y, z, bits of j).
+template <int W, int NR>
+__device__ inline void stage_seg_rounds(const Lanes<W>& L, uint32_t r0, uint32_t total, uint64_t upto, float3 lo,
+                                        float3 hi, float h2_keep, const float4* __restrict__ spos, uint32_t depth,
+                                        float4* tile, const StageAux* aux, uint32_t& kept, uint32_t& before)
+{
+    float4 v[NR];
+    uint32_t jv[NR];
+    bool has[NR];
+#pragma unroll
+    for (int r = 0; r < NR; r++) {
+        const uint32_t rr = r0 + r;
+        const uint32_t q = rr * W + L.ls;
+        unsigned long long sm = aux->starts[(rr * W) / 64];
+        if constexpr (W < 64) sm = (sm >> ((rr * W) % 64)) & ((1ull << W) - 1ull);
+        has[r] = q < total;
+        const uint32_t c = before + (uint32_t)__popcll(sm & upto) - 1u;  // non-empty ranges starting at or before q
+        before += (uint32_t)__popcll(sm);
+        jv[r] = q + aux->delta[c & 31u];
+        if (has[r]) v[r] = spos[jv[r]];
+    }
+#pragma unroll
+    for (int r = 0; r < NR; r++) {
+        bool keep = false;
+        if (has[r]) {
+            const float ex = fmaxf(fmaxf(lo.x - v[r].x, v[r].x - hi.x), 0.f);
+            const float ey = fmaxf(fmaxf(lo.y - v[r].y, v[r].y - hi.y), 0.f);
+            const float ez = fmaxf(fmaxf(lo.z - v[r].z, v[r].z - hi.z), 0.f);
+            keep = dist2(ex, ey, ez) < h2_keep;
+        }
+        const uint64_t m = L.bits(__ballot(keep));
+        if (keep) {
+            const uint32_t at = kept + L.below(m);
+            if (at < depth) tile[at] = make_float4(v[r].x, v[r].y, v[r].z, __uint_as_float(jv[r]));
+        }
+        kept += (uint32_t)__popcll(m);
+    }
+}
+
+// Segmented stage_neighbourhood (density form): every segment stages the cell it was given (a segment
+// with active = false stages nothing). Returns the segment's kept count or kStageOverflow.
+template <int W>
+__device__ inline uint32_t stage_seg(const Lanes<W>& L, const SegCell& c, float h2_keep, const float4* __restrict__ spos,
+                                     const uint32_t* __restrict__ cell_start, const uint32_t* __restrict__ cell_end,
+                                     uint32_t n, uint32_t depth, float4* tile, StageAux* aux)
+{
+    uint32_t st = 0, len = 0;
+    if (c.active && L.ls < 27) {
+        const int dx = (int)(L.ls % 3) - 1, dy = (int)((L.ls / 3) % 3) - 1, dz = (int)(L.ls / 9) - 1;
+        const uint32_t b = cell_hash(c.cx + dx, c.cy + dy, c.cz + dz);
+        const uint32_t s = cell_start[b];
+        const uint32_t e = min(cell_end[b], n);  // stale or written ranges always lie in [0, n]
+        if (s != kEmpty) {
+            st = s;
+            len = e > s ? e - s : 0u;
+        }
+    }
+    // ranges laid end to end in visit order (scan over lanes 0..31 of each half wave)
+    uint32_t incl = len;
+#pragma unroll
+    for (uint32_t o = 1; o < 32; o <<= 1) {
+        const uint32_t up = (uint32_t)__shfl_up((int)incl, o, 32);
+        if ((L.lane & 31u) >= o) incl += up;
+    }
+    const uint32_t pre = incl - len;
+    const uint32_t total = L.from(incl, 26u);
+    const bool over = total > kStageRounds * kWave;
+    if (L.ls < kStageRounds) aux->starts[L.ls] = 0ull;
+    wave_sync();
+    const bool mark = len > 0 && !over;
+    const uint64_t nonempty = L.bits(__ballot(mark));
+    if (mark) {
+        aux->delta[L.below(nonempty)] = st - pre;
+        atomicOr(&aux->starts[pre / 64], 1ull << (pre % 64));
+    }
+    wave_sync();
+    const uint32_t tot = over ? 0u : total;
+    uint32_t rounds = (tot + W - 1) / W;
+    if constexpr (W < 64) rounds = max(rounds, (uint32_t)__shfl_xor((int)rounds, W, kWave));
+    rounds = __builtin_amdgcn_readfirstlane(rounds);  // the wave runs the longest segment's rounds
+    const uint64_t upto = L.ls == 63 ? ~0ull : (2ull << L.ls) - 1ull;  // bits 0..ls
+    uint32_t kept = 0, before = 0, r0 = 0;
+    for (; r0 + 4 <= rounds; r0 += 4)
+        stage_seg_rounds<W, 4>(L, r0, tot, upto, c.lo, c.hi, h2_keep, spos, depth, tile, aux, kept, before);
+    if (r0 + 2 <= rounds) {
+        stage_seg_rounds<W, 2>(L, r0, tot, upto, c.lo, c.hi, h2_keep, spos, depth, tile, aux, kept, before);
+        r0 += 2;
+    }
+    if (r0 < rounds) stage_seg_rounds<W, 1>(L, r0, tot, upto, c.lo, c.hi, h2_keep, spos, depth, tile, aux, kept, before);
+    wave_sync();
+    return over ? kStageOverflow : kept;
+}
+
 }  // namespace sph
 }  // namespace mfl

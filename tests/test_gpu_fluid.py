@@ -19,8 +19,10 @@ TOL = 1e-5
 # launch shapes of the neighbour passes: LDS tiles (default), tiles so shallow that dense
 # neighbourhoods overflow to the global-memory walk, and the recorded-visit-list passes
 SHAPES = {"tile_list": {}, "tile": {"neighbour_pass": "tile"},
+          "tile_list4": {"neighbour_pass": "tile_list", "density_lanes": 4, "tile_depth": 256},
           "tile64": {"neighbour_pass": "tile", "tile_depth": 64, "block_waves": 2},
           "tile_list64": {"neighbour_pass": "tile_list", "tile_depth": 64, "block_waves": 1},
+          "tile_list4_64": {"neighbour_pass": "tile_list", "density_lanes": 4, "tile_depth": 64, "block_waves": 1},
           "list": {"neighbour_pass": "list"}}
 
 
@@ -136,7 +138,7 @@ def test_small_dam_break_vs_oracle(oracle, clear, shape):
         assert nw <= TOL and per <= TOL, (target, nw, per)
 
 
-@pytest.mark.parametrize("shape", ["tile_list", "tile", "list"])
+@pytest.mark.parametrize("shape", ["tile_list", "tile_list4", "tile", "list"])
 def test_1m_step_vs_oracle(oracle, shape):
     """1,048,576 particles in [0,4]^3: keys, permutation, bucket tables bit-exact; density, pressure
     and forces of one step within 1e-5."""
@@ -280,7 +282,7 @@ def test_traversal_orders_bit_identical():
 
 
 @pytest.mark.parametrize("order", ["sorted", "caller", "morton"])
-@pytest.mark.parametrize("shape", ["tile_list", "tile", "tile64", "tile_list64"])
+@pytest.mark.parametrize("shape", ["tile_list", "tile_list4", "tile", "tile64", "tile_list64", "tile_list4_64"])
 def test_tile_passes_match_list_passes(order, shape):
     """The LDS-tile passes take the same visits as the recorded-visit passes; only the FP32 summation
     order differs (a particle's kept entries are split among its lanes by position in the pruned
