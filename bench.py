@@ -197,6 +197,7 @@ def main():
                          "recorded visit lists for both")
     ap.add_argument("--tile-depth", type=int, default=256)
     ap.add_argument("--lanes", type=int, default=8, help="lanes per particle in the density pass")
+    ap.add_argument("--force-chunks", type=int, default=1, help="list chunks per trip in the force pass")
     ap.add_argument("--block-waves", type=int, default=4)
     ap.add_argument("--no-autotune", action="store_true",
                     help="keep the given launch shape instead of timing the candidates first")
@@ -232,7 +233,7 @@ def main():
     assert f.upload(sc.pos, sc.vel, sc.force) == 1
     assert f.setTraversal(args.order, 16) == 1
     assert f.setTuning(neighbour_pass=args.npass, tile_depth=args.tile_depth, block_waves=args.block_waves,
-                       density_lanes=args.lanes) == 1, \
+                       density_lanes=args.lanes, force_chunks=args.force_chunks) == 1, \
         L.last_error()
     stream = torch.cuda.Stream()
 

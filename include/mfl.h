@@ -165,6 +165,7 @@ typedef struct mfl_tuning {
     uint32_t tile_depth;    /* TILE, TILE_LIST: LDS entries per staged cell, multiple of 16 in 64..64 x lanes */
     uint32_t block_waves;   /* TILE, TILE_LIST: waves per workgroup, 1, 2 or 4 */
     uint32_t sort_items;    /* keys per thread of each radix sort pass: 4, 8 or 16; 0 = by key count */
+    uint32_t force_chunks;  /* LIST, TILE_LIST force pass: visits per trip / 4, 1 or 2 */
 } mfl_tuning;
 int mfl_fluid3d_get_tuning(mfl_fluid3d* f, mfl_tuning* out);
 int mfl_fluid3d_set_tuning(mfl_fluid3d* f, const mfl_tuning* t);

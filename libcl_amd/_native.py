@@ -62,12 +62,13 @@ PASSES = ("list", "tile", "tile_list")
 class Tuning(ctypes.Structure):
     """mfl_tuning: launch shape of the neighbour passes and the sort."""
     _fields_ = [("neighbour_pass", ctypes.c_int32), ("density_lanes", ctypes.c_int32),
-                ("tile_depth", ctypes.c_uint32), ("block_waves", ctypes.c_uint32), ("sort_items", ctypes.c_uint32)]
+                ("tile_depth", ctypes.c_uint32), ("block_waves", ctypes.c_uint32), ("sort_items", ctypes.c_uint32),
+                ("force_chunks", ctypes.c_uint32)]
 
     def as_dict(self) -> dict:
         return {"neighbour_pass": PASSES[self.neighbour_pass], "density_lanes": int(self.density_lanes),
                 "tile_depth": int(self.tile_depth), "block_waves": int(self.block_waves),
-                "sort_items": int(self.sort_items)}
+                "sort_items": int(self.sort_items), "force_chunks": int(self.force_chunks)}
 
 
 INTEGRATE_FN = ctypes.CFUNCTYPE(ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p)

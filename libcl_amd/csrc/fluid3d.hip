@@ -303,7 +303,7 @@ __global__ __launch_bounds__(kThreads) void k_density_grp(float4* __restrict__ s
     ncount[t] = count;
 }
 
-template <int ORDER>
+template <int ORDER, int CH>
 __device__ inline void forces_of(uint32_t t, uint32_t i, const float4* __restrict__ spos,
                                  const float4* __restrict__ svel, const uint32_t* __restrict__ cell_start,
                                  const uint32_t* __restrict__ cell_end, const uint32_t* __restrict__ nbr,
@@ -312,12 +312,12 @@ __device__ inline void forces_of(uint32_t t, uint32_t i, const float4* __restric
 {
     const uint32_t count = ncount[t];
     if (count <= kNbrCap)
-        sph_force_sum_nbr(spos, svel, nbr, n, count, t, pi, vi, h, P, V);
+        sph_force_sum_nbr<CH>(spos, svel, nbr, n, count, t, pi, vi, h, P, V);
     else
         sph_force_sum(spos, svel, cell_start, cell_end, i, n, pi, vi, h, P, V);
 }
 
-template <int ORDER>
+template <int ORDER, int CH>
 __global__ __launch_bounds__(kThreads) void k_forces_nbr(const float4* __restrict__ spos,
                                                          const float4* __restrict__ svel, float4* __restrict__ force,
                                                          const uint32_t* __restrict__ index,
@@ -334,7 +334,7 @@ __global__ __launch_bounds__(kThreads) void k_forces_nbr(const float4* __restric
     if (i >= n) return;
     const float4 pi = spos[i], vi = svel[i];
     float3 P, V;
-    forces_of<ORDER>(t, i, spos, svel, cell_start, cell_end, nbr, ncount, n, p.cellSize, pi, vi, P, V);
+    forces_of<ORDER, CH>(t, i, spos, svel, cell_start, cell_end, nbr, ncount, n, p.cellSize, pi, vi, P, V);
     const uint32_t o = caller_slot<ORDER>(t, i, porder, index);
     if (o >= n) return;
     force[o] = add_sph_force(force[o], P, V, p);
@@ -342,7 +342,7 @@ __global__ __launch_bounds__(kThreads) void k_forces_nbr(const float4* __restric
 
 // Forces + clip box + integrate force + gravity + integrate velocity for one particle: sorted slot i,
 // caller-order slot o = index[i].
-template <int ORDER>
+template <int ORDER, int CH>
 __global__ __launch_bounds__(kThreads) void k_forces_step(const float4* __restrict__ spos,
                                                           const float4* __restrict__ svel, float4* __restrict__ pos,
                                                           float4* __restrict__ vel, float4* __restrict__ force,
@@ -361,7 +361,7 @@ __global__ __launch_bounds__(kThreads) void k_forces_step(const float4* __restri
     if (i >= n) return;
     const float4 pi = spos[i], vi = svel[i];
     float3 P, V;
-    forces_of<ORDER>(t, i, spos, svel, cell_start, cell_end, nbr, ncount, n, p.cellSize, pi, vi, P, V);
+    forces_of<ORDER, CH>(t, i, spos, svel, cell_start, cell_end, nbr, ncount, n, p.cellSize, pi, vi, P, V);
     const uint32_t o = caller_slot<ORDER>(t, i, porder, index);
     if (o >= n) return;
     float4 f = add_sph_force(force[o], P, V, p);
@@ -500,20 +500,28 @@ __global__ __launch_bounds__(kThreads) void k_density_tile(const float4* __restr
             if (L.ls < kGroupLanes) tile[kept + L.ls] = make_float4(inf, inf, inf, __uint_as_float(kEmpty));
             wave_sync();
             const uint32_t trips = (kept + P - 1) / P;
+            // the wave runs the longest segment's trip count; past its own end a lane reads an end marker
+            uint32_t wtrips = trips;
+            if constexpr (W < kWave) wtrips = max(wtrips, (uint32_t)__shfl_xor((int)wtrips, W, kWave));
+            wtrips = __builtin_amdgcn_readfirstlane(wtrips);
+            const uint32_t qmax = kept + slot;
             float a = 0.f;
-            uint64_t hits = 0, band = 0;
-            for (uint32_t m = 0; m < trips; m++) {
-                const float4 e = tile[m * P + slot];
+            uint64_t hits = 0;
+            bool band = false;
+            for (uint32_t m = 0; m < wtrips; m++) {
+                const float4 e = tile[min(m * P + slot, qmax)];
                 const float dx = pi.x - e.x, dy = pi.y - e.y, dz = pi.z - e.z;
                 const float r2 = dist2(dx, dy, dz);
                 const bool other = __float_as_uint(e.w) != i;
                 const bool ok = other && r2 < h2_lo;
-                band |= __ballot(other && !ok && r2 <= h2_hi);
+                band |= other & (r2 >= h2_lo) & (r2 <= h2_hi);
                 const float w = h2 - r2;  // h^2 - vlen^2 of cl:123
                 a = ok ? fmaf(w * w, w, a) : a;
                 hits = (hits << 1) + (ok ? 1ull : 0ull);
             }
-            if ((band & __ballot(c.member)) != 0) {
+            // entries past a segment's end were end markers (never accepted): drop their bits
+            hits >>= wtrips - trips;
+            if (__ballot(band && c.member) != 0) {
                 // a squared distance within rounding of h^2: redo the cell with the exact test
                 a = 0.f;
                 hits = 0;
@@ -1040,8 +1048,13 @@ int Fluid3D::enqueue_list_forces(hipStream_t s, int order)
     auto* porder = porder_.as<uint32_t>();
     timer.begin(MFL_STAGE_FORCES, s);
     if (integrate_cb) {
-#define MFL_FORCES_NBR(O) \
-    hipLaunchKernelGGL(k_forces_nbr<O>, gx, b, 0, s, spos, svel, force, index, cs, ce, nbr, ncount, n, params, porder, inv)
+#define MFL_FORCES_NBR(O)                                                                                     \
+    if (force_chunks == 2)                                                                                    \
+        hipLaunchKernelGGL((k_forces_nbr<O, 2>), gx, b, 0, s, spos, svel, force, index, cs, ce, nbr, ncount, n,  \
+                           params, porder, inv);                                                              \
+    else                                                                                                      \
+        hipLaunchKernelGGL((k_forces_nbr<O, 1>), gx, b, 0, s, spos, svel, force, index, cs, ce, nbr, ncount, n,  \
+                           params, porder, inv)
         switch (order) {
         case MFL_TRAVERSAL_SORTED: MFL_FORCES_NBR(kOrderSorted); break;
         case MFL_TRAVERSAL_CALLER: MFL_FORCES_NBR(kOrderCaller); break;
@@ -1053,9 +1066,13 @@ int Fluid3D::enqueue_list_forces(hipStream_t s, int order)
         (void)integrate_cb(handle, (void*)s, integrate_user);  // return value ignored, cpp:325-328
         return 1;
     }
-#define MFL_FORCES_STEP(O)                                                                                        \
-    hipLaunchKernelGGL(k_forces_step<O>, gx, b, 0, s, spos, svel, pos, vel, force, index, cs, ce, nbr, ncount, n, \
-                       params, mn, mx, gf, porder, inv)
+#define MFL_FORCES_STEP(O)                                                                                    \
+    if (force_chunks == 2)                                                                                    \
+        hipLaunchKernelGGL((k_forces_step<O, 2>), gx, b, 0, s, spos, svel, pos, vel, force, index, cs, ce, nbr,  \
+                           ncount, n, params, mn, mx, gf, porder, inv);                                       \
+    else                                                                                                      \
+        hipLaunchKernelGGL((k_forces_step<O, 1>), gx, b, 0, s, spos, svel, pos, vel, force, index, cs, ce, nbr,  \
+                           ncount, n, params, mn, mx, gf, porder, inv)
     switch (order) {
     case MFL_TRAVERSAL_SORTED: MFL_FORCES_STEP(kOrderSorted); break;
     case MFL_TRAVERSAL_CALLER: MFL_FORCES_STEP(kOrderCaller); break;
@@ -1166,6 +1183,7 @@ int Fluid3D::get_tuning(mfl_tuning* out) const
     out->tile_depth = tile_depth;
     out->block_waves = block_waves;
     out->sort_items = (uint32_t)sort.tuning.items;
+    out->force_chunks = force_chunks;
     return 1;
 }
 
@@ -1204,11 +1222,16 @@ int Fluid3D::set_tuning(const mfl_tuning& t)
         set_error("sort_items must be 0, 4, 8 or 16");
         return 0;
     }
+    if (t.force_chunks != 1 && t.force_chunks != 2) {
+        set_error("force_chunks must be 1 or 2");
+        return 0;
+    }
     neighbour_pass = t.neighbour_pass;
     density_lanes = t.density_lanes;
     tile_depth = t.tile_depth;
     block_waves = t.block_waves;
     sort.tuning.items = (int)t.sort_items;
+    force_chunks = t.force_chunks;
     return 1;
 }
 
@@ -1243,13 +1266,15 @@ int Fluid3D::autotune(hipStream_t s, uint32_t steps, mfl_tuning* best, double* m
     mfl_tuning start{};
     get_tuning(&start);
     std::vector<mfl_tuning> cand;
-    for (int lanes : {1, 8}) cand.push_back({MFL_PASS_LIST, lanes, 256u, 4u, start.sort_items});
+    for (uint32_t ch : {1u, 2u})
+        for (int lanes : {1, 8}) cand.push_back({MFL_PASS_LIST, lanes, 256u, 4u, start.sort_items, ch});
     for (uint32_t w : {1u, 2u, 4u})
         for (uint32_t d : {128u, 192u, 256u, 384u, 512u})
             for (int lanes : {4, 8})
-                if (tile_lds_bytes(true, w, d) <= 65536 && d <= 64u * (uint32_t)lanes)
-                    for (int pass : {MFL_PASS_TILE, MFL_PASS_TILE_LIST})
-                        cand.push_back({pass, lanes, d, w, start.sort_items});
+                if (tile_lds_bytes(true, w, d) <= 65536 && d <= 64u * (uint32_t)lanes) {
+                    cand.push_back({MFL_PASS_TILE, lanes, d, w, start.sort_items, 1u});
+                    for (uint32_t ch : {1u, 2u}) cand.push_back({MFL_PASS_TILE_LIST, lanes, d, w, start.sort_items, ch});
+                }
     const bool graph_was = use_graph;
     const uint32_t mask_was = timer.mask;
     const uint32_t since = steps_since_refresh_;
@@ -1302,7 +1327,7 @@ int Fluid3D::launch_graph(hipStream_t s)
     std::ostringstream sig;
     sig << n_ << ':' << pos_.ptr << ':' << vel_.ptr << ':' << force_.ptr << ':' << clear_count << ':'
         << sort.tuning.items << ':' << traversal << ':' << porder_valid_ << ':' << density_lanes << ':'
-        << neighbour_pass << ':' << tile_depth << ':' << block_waves << ':';
+        << neighbour_pass << ':' << tile_depth << ':' << block_waves << ':' << force_chunks << ':';
     const unsigned char* raw[] = {reinterpret_cast<const unsigned char*>(&params),
                                   reinterpret_cast<const unsigned char*>(box_min),
                                   reinterpret_cast<const unsigned char*>(box_max),

@@ -61,6 +61,7 @@ public:
     int neighbour_pass = MFL_PASS_TILE_LIST;  // LIST, TILE or TILE_LIST neighbour passes
     uint32_t tile_depth = 256;              // TILE: LDS entries per wave
     uint32_t block_waves = 4;               // TILE: waves per workgroup
+    uint32_t force_chunks = 1;              // list force pass: 16-byte list chunks per trip
     RadixSort sort;
     StageTimer timer;
 

@@ -207,33 +207,42 @@ __device__ inline void density_visits(int gx, int gy, int gz, const float4& pi, 
 }
 
 // Pressure and viscosity sums of cl:237-242 over the visits recorded for thread t (sorted particle i).
+// CH 16-byte chunks of the list (4 CH visits) per trip: their entries, then the positions and
+// velocities of their visits, are in flight together; the sums still run in list order.
+template <int CH = 1>
 __device__ inline void sph_force_sum_nbr(const float4* __restrict__ spos, const float4* __restrict__ svel,
                                          const uint32_t* __restrict__ nbr, size_t nbr_stride, uint32_t count,
                                          uint32_t t, const float4& pi, const float4& vi, float h, float3& P, float3& V)
 {
+    constexpr uint32_t U = 4 * CH;
     P = make_float3(0.f, 0.f, 0.f);
     V = make_float3(0.f, 0.f, 0.f);
-    // one 16-byte chunk of the list per trip: four visits whose positions and velocities are in flight
-    // together (the sums still run in list order)
     if (count == 0) return;
     const uint32_t last = (count - 1) & ~3u;  // first entry of the last chunk
-    uint4 next = *reinterpret_cast<const uint4*>(nbr + nbr_at(t, 0, nbr_stride));
-    for (uint32_t k = 0; k < count; k += 4) {
-        const uint4 c = next;
-        // the following chunk (the last one again at the end) is fetched while this one is summed
-        next = *reinterpret_cast<const uint4*>(nbr + nbr_at(t, min(k + 4, last), nbr_stride));
-        const uint32_t m = min(count - k, 4u);
-        // past the end of the list the chunk holds no visit: load the first entry again instead, so that
-        // all eight loads issue together
-        const uint32_t j[4] = {c.x, m > 1 ? c.y : c.x, m > 2 ? c.z : c.x, m > 3 ? c.w : c.x};
-        float4 pj[4], vj[4];
+    for (uint32_t k = 0; k < count; k += U) {
+        uint4 c[CH];
 #pragma unroll
-        for (uint32_t u = 0; u < 4; u++) {
+        for (int q = 0; q < CH; q++)
+            c[q] = *reinterpret_cast<const uint4*>(nbr + nbr_at(t, min(k + 4 * q, last), nbr_stride));
+        const uint32_t m = min(count - k, U);
+        // past the end of the list a chunk holds no visit: load the first entry again instead, so that
+        // all loads issue together
+        uint32_t j[U];
+#pragma unroll
+        for (int q = 0; q < CH; q++) {
+            j[4 * q + 0] = 4 * q + 0 < m ? c[q].x : c[0].x;
+            j[4 * q + 1] = 4 * q + 1 < m ? c[q].y : c[0].x;
+            j[4 * q + 2] = 4 * q + 2 < m ? c[q].z : c[0].x;
+            j[4 * q + 3] = 4 * q + 3 < m ? c[q].w : c[0].x;
+        }
+        float4 pj[U], vj[U];
+#pragma unroll
+        for (uint32_t u = 0; u < U; u++) {
             pj[u] = spos[j[u]];
             vj[u] = svel[j[u]];
         }
 #pragma unroll
-        for (uint32_t u = 0; u < 4; u++) {
+        for (uint32_t u = 0; u < U; u++) {
             const float dx = pi.x - pj[u].x, dy = pi.y - pj[u].y, dz = pi.z - pj[u].z;
             pair_force(pi, vi, pj[u], vj[u], dx, dy, dz, dist2(dx, dy, dz), h, P, V, u < m);
         }

@@ -300,18 +300,26 @@ __device__ inline void stage_seg_rounds(const Lanes<W>& L, uint32_t r0, uint32_t
                                         float4* tile, const StageAux* aux, uint32_t& kept, uint32_t& before)
 {
     float4 v[NR];
-    uint32_t jv[NR];
+    uint32_t jv[NR], cr[NR];
     bool has[NR];
+    // phases, so that the LDS reads of all NR rounds and then their global loads are in flight together:
+    // start masks, range index of each position, range offsets, positions
+    unsigned long long sm[NR];
+#pragma unroll
+    for (int r = 0; r < NR; r++) sm[r] = aux->starts[((r0 + r) * W) / 64];
 #pragma unroll
     for (int r = 0; r < NR; r++) {
-        const uint32_t rr = r0 + r;
-        const uint32_t q = rr * W + L.ls;
-        unsigned long long sm = aux->starts[(rr * W) / 64];
-        if constexpr (W < 64) sm = (sm >> ((rr * W) % 64)) & ((1ull << W) - 1ull);
+        if constexpr (W < 64) sm[r] = (sm[r] >> (((r0 + r) * W) % 64)) & ((1ull << W) - 1ull);
+        cr[r] = before + (uint32_t)__popcll(sm[r] & upto) - 1u;  // non-empty ranges starting at or before q
+        before += (uint32_t)__popcll(sm[r]);
+    }
+#pragma unroll
+    for (int r = 0; r < NR; r++) cr[r] = aux->delta[cr[r] & 31u];
+#pragma unroll
+    for (int r = 0; r < NR; r++) {
+        const uint32_t q = (r0 + r) * W + L.ls;
         has[r] = q < total;
-        const uint32_t c = before + (uint32_t)__popcll(sm & upto) - 1u;  // non-empty ranges starting at or before q
-        before += (uint32_t)__popcll(sm);
-        jv[r] = q + aux->delta[c & 31u];
+        jv[r] = q + cr[r];
         if (has[r]) v[r] = spos[jv[r]];
     }
 #pragma unroll

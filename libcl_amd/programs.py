@@ -214,12 +214,13 @@ class oclFluid3D:
         return t.as_dict()
 
     def setTuning(self, **kw) -> int:
-        """Changes fields of the tuning: neighbour_pass ("list" / "tile"), density_lanes, tile_depth,
-        block_waves, sort_items. Results differ between shapes only in FP32 summation order."""
+        """Changes fields of the tuning: neighbour_pass ("list" / "tile" / "tile_list"), density_lanes,
+        tile_depth, block_waves, sort_items, force_chunks. Results differ between shapes only in FP32
+        summation order."""
         cur = self.getTuning()
         cur.update(kw)
         t = N.Tuning(N.PASSES.index(cur["neighbour_pass"]), cur["density_lanes"], cur["tile_depth"],
-                     cur["block_waves"], cur["sort_items"])
+                     cur["block_waves"], cur["sort_items"], cur["force_chunks"])
         return int(N.load().mfl_fluid3d_set_tuning(self._h, ctypes.byref(t)))
 
     def autotune(self, steps: int = 3, stream=None):

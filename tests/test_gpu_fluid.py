@@ -23,6 +23,7 @@ SHAPES = {"tile_list": {}, "tile": {"neighbour_pass": "tile"},
           "tile64": {"neighbour_pass": "tile", "tile_depth": 64, "block_waves": 2},
           "tile_list64": {"neighbour_pass": "tile_list", "tile_depth": 64, "block_waves": 1},
           "tile_list4_64": {"neighbour_pass": "tile_list", "density_lanes": 4, "tile_depth": 64, "block_waves": 1},
+          "tile_list4_ch2": {"neighbour_pass": "tile_list", "density_lanes": 4, "force_chunks": 2},
           "list": {"neighbour_pass": "list"}}
 
 
@@ -282,7 +283,8 @@ def test_traversal_orders_bit_identical():
 
 
 @pytest.mark.parametrize("order", ["sorted", "caller", "morton"])
-@pytest.mark.parametrize("shape", ["tile_list", "tile_list4", "tile", "tile64", "tile_list64", "tile_list4_64"])
+@pytest.mark.parametrize("shape", ["tile_list", "tile_list4", "tile", "tile64", "tile_list64", "tile_list4_64",
+                                   "tile_list4_ch2"])
 def test_tile_passes_match_list_passes(order, shape):
     """The LDS-tile passes take the same visits as the recorded-visit passes; only the FP32 summation
     order differs (a particle's kept entries are split among its lanes by position in the pruned
