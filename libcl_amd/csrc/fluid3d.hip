@@ -6,15 +6,18 @@
 // tables (cell start / end) that keep their contents between steps.
 //
 // Step (oclFluid3D::compute, cpp:293-342):
-//   k_hash          cell key per particle                                   (cl:57-64)
+//   k_hash (CLEAR)  cell key per particle + reset of the first clear_count buckets
+//                   (cl:57-64, cpp:309-313; nothing between the two reads the tables, so the reset
+//                    moves to the start of the step)
 //   sort            stable key/index sort over bits [0, 18)                 (cpp:299; keys < 2^18)
-//   k_reorder_clear gather sorted pos/vel + reset the first clear_count buckets (cl:66-72, cpp:309-313)
-//   k_find_bounds   start/end of each occupied bucket                       (cl:85-113)
-//   k_density       27-bucket neighbourhood, poly6, pressure                (cl:150-200)
-//   k_forces_step   27-bucket pressure + viscosity forces, clip box, clamp, Euler, gravity
+//   k_reorder_bounds gather sorted pos/vel + start/end of each occupied bucket (cl:66-72, 85-113)
+//   density pass    27-bucket neighbourhood, poly6, pressure                (cl:150-200)
+//   force pass      27-bucket pressure + viscosity forces, clip box, clamp, Euler, gravity
 //                   (cl:202-255, 315-361, 258-271, 294-298, 284-292 fused; each particle's update
 //                    only touches its own force/velocity/position, so fusion is exact)
-// With an EVT_INTEGRATE callback the force pass runs alone (k_forces) and the callback follows.
+// With an EVT_INTEGRATE callback the force pass runs alone and the callback follows. The reference's
+// kernels one by one (getKernel + enqueue) are k_hash, k_reorder_clear, k_find_bounds, k_density,
+// k_forces and the integration kernels below.
 #include "fluid3d.hpp"
 #include "sph_device.hpp"
 #include "tile.hpp"
@@ -27,13 +30,19 @@ namespace {
 
 using namespace sph;
 
-template <bool WRITE_INDEX>
+template <bool WRITE_INDEX, bool CLEAR>
 __global__ __launch_bounds__(kThreads) void k_hash(const float4* __restrict__ pos, uint32_t n, float h,
                                                    uint32_t* __restrict__ cell, uint32_t* __restrict__ index,
-                                                   mfl_params* __restrict__ params_buf, mfl_params p)
+                                                   mfl_params* __restrict__ params_buf, mfl_params p,
+                                                   uint32_t* __restrict__ cell_start, uint32_t* __restrict__ cell_end,
+                                                   uint32_t clear)
 {
     const uint32_t i = blockIdx.x * kThreads + threadIdx.x;
     if (i == 0 && params_buf) *params_buf = p;  // keeps getParamBuffer() in step with the host Params
+    if (CLEAR && i < clear) {
+        cell_start[i] = kEmpty;
+        cell_end[i] = kEmpty;
+    }
     if (i >= n) return;
     const float4 q = pos[i];
     cell[i] = cell_hash(grid_coord(q.x, h), grid_coord(q.y, h), grid_coord(q.z, h));
@@ -62,6 +71,36 @@ __global__ __launch_bounds__(kThreads) void k_reorder_clear(const uint32_t* __re
         cell_start[i] = kEmpty;
         cell_end[i] = kEmpty;
     }
+}
+
+// k_reorder_clear without the reset (done by k_hash<..., true>) and k_find_bounds in one pass.
+__global__ __launch_bounds__(kThreads) void k_reorder_bounds(const uint32_t* __restrict__ index,
+                                                             const float4* __restrict__ pos,
+                                                             const float4* __restrict__ vel,
+                                                             float4* __restrict__ spos, float4* __restrict__ svel,
+                                                             uint32_t n, const uint32_t* __restrict__ cell,
+                                                             uint32_t* __restrict__ cell_start,
+                                                             uint32_t* __restrict__ cell_end, uint32_t* __restrict__ inv)
+{
+    const uint32_t i = blockIdx.x * kThreads + threadIdx.x;
+    if (i >= n) return;
+    const uint32_t s = index[i];
+    if (s < n) {  // index is a permutation of [0, n)
+        spos[i] = pos[s];
+        svel[i] = vel[s];
+        inv[s] = i;
+    }
+    const uint32_t h = cell[i] & (kBuckets - 1u);  // keys are < 2^18 by construction
+    if (i == 0) {
+        cell_start[h] = 0;
+        return;
+    }
+    const uint32_t prev = cell[i - 1] & (kBuckets - 1u);
+    if (h != prev) {
+        cell_end[prev] = i;
+        cell_start[h] = i;
+    }
+    if (i == n - 1) cell_end[h] = n;
 }
 
 __global__ __launch_bounds__(kThreads) void k_fill_buckets(uint32_t* __restrict__ cell_start,
@@ -954,21 +993,19 @@ int Fluid3D::enqueue_front(hipStream_t s)
     const uint32_t clear = clear_count < kBuckets ? clear_count : kBuckets;
 
     timer.begin(MFL_STAGE_HASH, s);
-    hipLaunchKernelGGL(k_hash<false>, g, b, 0, s, pos, n, params.cellSize, cell, nullptr,
-                       params_buf_.as<mfl_params>(), params);
+    hipLaunchKernelGGL((k_hash<false, true>), grid_for(n > clear ? n : clear), b, 0, s, pos, n, params.cellSize, cell,
+                       nullptr, params_buf_.as<mfl_params>(), params, cs, ce, clear);
     timer.end(MFL_STAGE_HASH, s);
     MFL_HIP(hipGetLastError());
     // the index buffer starts as the identity (cl:63) and is generated inside the first pass
     timer.begin(MFL_STAGE_SORT, s);
     if (!sort.sort_bits(cell, index, n, 0, 18, s, true)) return 0;
     timer.end(MFL_STAGE_SORT, s);
+    // bucket bounds come out of the reorder pass (the BOUNDS stage stays empty)
     timer.begin(MFL_STAGE_REORDER, s);
-    hipLaunchKernelGGL(k_reorder_clear, grid_for(n > clear ? n : clear), b, 0, s, index, pos, vel, spos, svel, n, cs,
-                       ce, clear, inv_.as<uint32_t>());
+    hipLaunchKernelGGL(k_reorder_bounds, g, b, 0, s, index, pos, vel, spos, svel, n, cell, cs, ce,
+                       inv_.as<uint32_t>());
     timer.end(MFL_STAGE_REORDER, s);
-    timer.begin(MFL_STAGE_BOUNDS, s);
-    hipLaunchKernelGGL(k_find_bounds, g, b, 0, s, cell, n, cs, ce);
-    timer.end(MFL_STAGE_BOUNDS, s);
     MFL_HIP(hipGetLastError());
     return 1;
 }
@@ -1402,8 +1439,8 @@ int Fluid3D::enqueue_kernel(const char* name, hipStream_t s)
     auto* ce = cell_end_.as<uint32_t>();
     const std::string k(name);
     if (k == "clHash")
-        hipLaunchKernelGGL(k_hash<true>, g, b, 0, s, pos, n, params.cellSize, cell, index,
-                           params_buf_.as<mfl_params>(), params);
+        hipLaunchKernelGGL((k_hash<true, false>), g, b, 0, s, pos, n, params.cellSize, cell, index,
+                           params_buf_.as<mfl_params>(), params, nullptr, nullptr, 0u);
     else if (k == "clReorder")
         hipLaunchKernelGGL(k_reorder_clear, g, b, 0, s, index, pos, vel, spos, svel, n, cs, ce, 0u, inv_.as<uint32_t>());
     else if (k == "clInitBounds")
