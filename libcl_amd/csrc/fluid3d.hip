@@ -1321,8 +1321,8 @@ int Fluid3D::autotune(hipStream_t s, uint32_t steps, mfl_tuning* best, double* m
     int ok = hip_ok(hipEventCreate(&e0), "hipEventCreate") && hip_ok(hipEventCreate(&e1), "hipEventCreate");
     double best_ms = 1e30;
     mfl_tuning pick = start;
-    for (size_t c = 0; ok && c < cand.size(); c++) {
-        ok = set_tuning(cand[c]) && snapshot(true) && compute(s);  // one untimed step per trial
+    auto trial = [&](const mfl_tuning& t) {
+        ok = set_tuning(t) && snapshot(true) && compute(s);  // one untimed step per trial
         for (uint32_t k = 0; ok && k < steps; k++) {
             if (k == 0) ok = hip_ok(hipEventRecord(e0, s), "hipEventRecord");
             ok = ok && compute(s);
@@ -1332,8 +1332,16 @@ int Fluid3D::autotune(hipStream_t s, uint32_t steps, mfl_tuning* best, double* m
         ok = ok && hip_ok(hipEventElapsedTime(&ms, e0, e1), "hipEventElapsedTime");
         if (ok && ms / steps < best_ms) {
             best_ms = ms / steps;
-            pick = cand[c];
+            pick = t;
         }
+    };
+    for (size_t c = 0; ok && c < cand.size(); c++) trial(cand[c]);
+    // then the sort's tile size, with the best neighbour passes
+    const mfl_tuning passes = pick;
+    for (uint32_t items : {4u, 8u, 16u}) {
+        mfl_tuning t = passes;
+        t.sort_items = items;
+        if (ok && items != passes.sort_items) trial(t);
     }
     if (e0) (void)hipEventDestroy(e0);
     if (e1) (void)hipEventDestroy(e1);

@@ -3,7 +3,8 @@
 // Same contract as oclRadixSort::compute (libCL/sort/oclRadixSort.cpp:62-144): after the call the
 // pairs are stably ordered by bits [start_bit, end_bit) of the key. The reference walks 4-bit digits
 // with a serial 256-long scan per 256-item block (libCL/sort/oclRadixSort.cl:20-222); here each pass
-// takes up to 8 bits and has three kernels:
+// takes up to 9 bits (so the 18-bit cell keys of the fluid step sort in two passes) and has three
+// kernels:
 //   k_tile_counts : per-tile digit histogram in LDS            (reads 4 B/key)
 //   k_scan_digits : per-digit exclusive scan over tiles          (tiny)
 //   k_scatter     : stable in-tile ranking with wave64 ballot matching (no serial scans), tile
@@ -15,6 +16,8 @@ namespace {
 
 constexpr int kThreads = 256;
 constexpr int kWaves = kThreads / kWave;
+constexpr int kMaxDigitBits = 9;
+constexpr uint32_t kMaxRadix = 1u << kMaxDigitBits;  // 512 digits: two per thread in the digit loops
 
 // Exclusive scan of one value per thread over a 256-thread block. All threads must call it.
 __device__ inline uint32_t block_exclusive_scan(uint32_t v, uint32_t* s_wave, uint32_t* total)
@@ -45,9 +48,9 @@ template <int ITEMS>
 __global__ __launch_bounds__(kThreads) void k_tile_counts(const uint32_t* __restrict__ keys, uint32_t n, int shift,
                                                           int bits, uint32_t* __restrict__ counts, uint32_t ntiles)
 {
-    __shared__ uint32_t hist[256];
+    __shared__ uint32_t hist[kMaxRadix];
     const uint32_t radix = 1u << bits, mask = radix - 1u;
-    hist[threadIdx.x] = 0;
+    for (uint32_t d = threadIdx.x; d < radix; d += kThreads) hist[d] = 0;
     __syncthreads();
     const size_t base = (size_t)blockIdx.x * (kThreads * ITEMS);
 #pragma unroll
@@ -56,7 +59,7 @@ __global__ __launch_bounds__(kThreads) void k_tile_counts(const uint32_t* __rest
         if (i < n) atomicAdd(&hist[(keys[i] >> shift) & mask], 1u);
     }
     __syncthreads();
-    if (threadIdx.x < radix) counts[(size_t)threadIdx.x * ntiles + blockIdx.x] = hist[threadIdx.x];
+    for (uint32_t d = threadIdx.x; d < radix; d += kThreads) counts[(size_t)d * ntiles + blockIdx.x] = hist[d];
 }
 
 // One block per digit: row d of the digit-major [radix x ntiles] table becomes its exclusive scan.
@@ -99,9 +102,9 @@ __global__ __launch_bounds__(kThreads) void k_scatter(const uint32_t* __restrict
     constexpr int TILE = kThreads * ITEMS;
     __shared__ uint32_t s_keys[TILE];
     __shared__ uint32_t s_vals[TILE];
-    __shared__ uint32_t s_cnt[kWaves * 256];
-    __shared__ uint32_t s_start[256];
-    __shared__ uint32_t s_base[256];
+    __shared__ uint32_t s_cnt[kWaves * kMaxRadix];
+    __shared__ uint32_t s_start[kMaxRadix];
+    __shared__ uint32_t s_base[kMaxRadix];
     __shared__ uint32_t s_wave[kWaves];
 
     const int t = threadIdx.x;
@@ -111,13 +114,15 @@ __global__ __launch_bounds__(kThreads) void k_scatter(const uint32_t* __restrict
     const uint32_t tile = blockIdx.x;
     const size_t tile_base = (size_t)tile * TILE;
 
-#pragma unroll
-    for (int i = 0; i < kWaves; i++) s_cnt[i * 256 + t] = 0;
+    // thread t owns digits 2t and 2t + 1 in the per-digit loops
+    const uint32_t d0 = 2u * (uint32_t)t, d1 = d0 + 1u;
+    for (uint32_t i = t; i < kWaves * radix; i += kThreads) s_cnt[(i / radix) * kMaxRadix + i % radix] = 0;
     {
-        const uint32_t tot = (uint32_t)t < radix ? totals[t] : 0u;
+        const uint32_t t0 = d0 < radix ? totals[d0] : 0u, t1 = d1 < radix ? totals[d1] : 0u;
         uint32_t all;
-        const uint32_t digit_base = block_exclusive_scan(tot, s_wave, &all);
-        if ((uint32_t)t < radix) s_base[t] = digit_base + counts[(size_t)t * ntiles + tile];
+        const uint32_t digit_base = block_exclusive_scan(t0 + t1, s_wave, &all);
+        if (d0 < radix) s_base[d0] = digit_base + counts[(size_t)d0 * ntiles + tile];
+        if (d1 < radix) s_base[d1] = digit_base + t0 + counts[(size_t)d1 * ntiles + tile];
     }
 
     uint32_t key[ITEMS], val[ITEMS], rank[ITEMS];
@@ -148,29 +153,40 @@ __global__ __launch_bounds__(kThreads) void k_scatter(const uint32_t* __restrict
         const int leader = __ffsll((unsigned long long)same) - 1;
         uint32_t base = 0;
         if (valid && lane == leader) {
-            base = s_cnt[wave * 256 + d];
-            s_cnt[wave * 256 + d] = base + (uint32_t)__popcll(same);
+            base = s_cnt[wave * kMaxRadix + d];
+            s_cnt[wave * kMaxRadix + d] = base + (uint32_t)__popcll(same);
         }
         base = __shfl(base, leader & (kWave - 1), kWave);
         rank[r] = base + (uint32_t)__popcll(same & lower_lanes);
     }
     __syncthreads();
 
-    uint32_t tile_cnt = 0;
-    if ((uint32_t)t < radix) {
+    uint32_t tile_cnt0 = 0, tile_cnt1 = 0;
+    if (d0 < radix) {
         uint32_t run = 0;
 #pragma unroll
         for (int w = 0; w < kWaves; w++) {
-            const uint32_t c = s_cnt[w * 256 + t];
-            s_cnt[w * 256 + t] = run;
+            const uint32_t c = s_cnt[w * kMaxRadix + d0];
+            s_cnt[w * kMaxRadix + d0] = run;
             run += c;
         }
-        tile_cnt = run;
+        tile_cnt0 = run;
+    }
+    if (d1 < radix) {
+        uint32_t run = 0;
+#pragma unroll
+        for (int w = 0; w < kWaves; w++) {
+            const uint32_t c = s_cnt[w * kMaxRadix + d1];
+            s_cnt[w * kMaxRadix + d1] = run;
+            run += c;
+        }
+        tile_cnt1 = run;
     }
     {
         uint32_t all;
-        const uint32_t st = block_exclusive_scan(tile_cnt, s_wave, &all);
-        if ((uint32_t)t < radix) s_start[t] = st;
+        const uint32_t st = block_exclusive_scan(tile_cnt0 + tile_cnt1, s_wave, &all);
+        if (d0 < radix) s_start[d0] = st;
+        if (d1 < radix) s_start[d1] = st + tile_cnt0;
     }
     __syncthreads();
 
@@ -179,7 +195,7 @@ __global__ __launch_bounds__(kThreads) void k_scatter(const uint32_t* __restrict
         const size_t idx = tile_base + ((size_t)(wave * ITEMS + r) << 6) + lane;
         if (idx < n) {
             const uint32_t d = (key[r] >> shift) & mask;
-            const uint32_t pos = s_start[d] + s_cnt[wave * 256 + d] + rank[r];
+            const uint32_t pos = s_start[d] + s_cnt[wave * kMaxRadix + d] + rank[r];
             s_keys[pos] = key[r];
             s_vals[pos] = val[r];
         }
@@ -263,14 +279,14 @@ int RadixSort::compute(uint32_t* keys, uint32_t* vals, size_t n, int start_bit, 
 int RadixSort::reserve(size_t n, int nbits)
 {
     if (n == 0 || nbits <= 0) return 1;
-    const int passes = (nbits + 7) / 8;
+    const int passes = (nbits + kMaxDigitBits - 1) / kMaxDigitBits;
     const int items = pick_items(n, tuning.items);
     const size_t ntiles = (n + kThreads * items - 1) / (kThreads * items);
     const size_t bytes = n * sizeof(uint32_t);
     const int nbuf = passes == 1 ? 1 : 2;
     for (int i = 0; i < nbuf; i++)
         if (!tmp_keys_[i].fit(bytes) || !tmp_vals_[i].fit(bytes)) return 0;
-    if (!counts_.fit(ntiles * 256 * sizeof(uint32_t)) || !totals_.fit(256 * sizeof(uint32_t))) return 0;
+    if (!counts_.fit(ntiles * kMaxRadix * sizeof(uint32_t)) || !totals_.fit(kMaxRadix * sizeof(uint32_t))) return 0;
     return 1;
 }
 
@@ -291,7 +307,7 @@ int RadixSort::sort_bits(uint32_t* keys, uint32_t* vals, size_t n, int start_bit
         return 0;
     }
     const int nbits = end_bit - start_bit;
-    const int passes = (nbits + 7) / 8;
+    const int passes = (nbits + kMaxDigitBits - 1) / kMaxDigitBits;
     const int items = pick_items(n, tuning.items);
     const size_t bytes = n * sizeof(uint32_t);
     if (!reserve(n, nbits)) return 0;
