@@ -195,10 +195,10 @@ def main():
     ap.add_argument("--pass", dest="npass", default="tile_list", choices=["tile_list", "tile", "list"],
                     help="neighbour passes: LDS-tile density + recorded-visit forces, LDS tiles for both, or "
                          "recorded visit lists for both")
-    ap.add_argument("--tile-depth", type=int, default=256)
-    ap.add_argument("--lanes", type=int, default=8, help="lanes per particle in the density pass")
-    ap.add_argument("--force-chunks", type=int, default=1, help="list chunks per trip in the force pass")
-    ap.add_argument("--block-waves", type=int, default=4)
+    ap.add_argument("--tile-depth", type=int, default=192)
+    ap.add_argument("--lanes", type=int, default=4, help="lanes per particle in the density pass")
+    ap.add_argument("--force-chunks", type=int, default=2, help="list chunks per trip in the force pass")
+    ap.add_argument("--block-waves", type=int, default=2)
     ap.add_argument("--no-autotune", action="store_true",
                     help="keep the given launch shape instead of timing the candidates first")
     ap.add_argument("--batched-steps", type=int, default=100)

@@ -57,11 +57,12 @@ public:
     bool use_graph = false;
     int traversal = MFL_TRAVERSAL_MORTON;   // order in which the neighbour passes visit particles
     uint32_t refresh_interval = 16;         // steps between Morton order refreshes
-    int density_lanes = 8;                  // lanes per particle in the list density pass (1 or 8)
+    // launch shape (mfl_tuning); the defaults are the fastest shape measured on the 1M-particle step
     int neighbour_pass = MFL_PASS_TILE_LIST;  // LIST, TILE or TILE_LIST neighbour passes
-    uint32_t tile_depth = 256;              // TILE: LDS entries per wave
-    uint32_t block_waves = 4;               // TILE: waves per workgroup
-    uint32_t force_chunks = 1;              // list force pass: 16-byte list chunks per trip
+    int density_lanes = 4;                  // lanes per particle in the density pass
+    uint32_t tile_depth = 192;              // tile passes: LDS entries per staged cell
+    uint32_t block_waves = 2;               // tile passes: waves per workgroup
+    uint32_t force_chunks = 2;              // list force pass: 16-byte list chunks per trip
     RadixSort sort;
     StageTimer timer;
 

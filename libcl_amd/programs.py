@@ -218,6 +218,11 @@ class oclFluid3D:
         tile_depth, block_waves, sort_items, force_chunks. Results differ between shapes only in FP32
         summation order."""
         cur = self.getTuning()
+        if "density_lanes" not in kw and "neighbour_pass" in kw:
+            # keep the lane count valid for the pass being selected (list: 1 or 8, tiles: 4 or 8)
+            ok = (1, 8) if kw["neighbour_pass"] == "list" else (4, 8)
+            if cur["density_lanes"] not in ok:
+                kw = dict(kw, density_lanes=8)
         cur.update(kw)
         t = N.Tuning(N.PASSES.index(cur["neighbour_pass"]), cur["density_lanes"], cur["tile_depth"],
                      cur["block_waves"], cur["sort_items"], cur["force_chunks"])

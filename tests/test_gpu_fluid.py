@@ -18,10 +18,10 @@ TOL = 1e-5
 
 # launch shapes of the neighbour passes: LDS tiles (default), tiles so shallow that dense
 # neighbourhoods overflow to the global-memory walk, and the recorded-visit-list passes
-SHAPES = {"tile_list": {}, "tile": {"neighbour_pass": "tile"},
-          "tile_list4": {"neighbour_pass": "tile_list", "density_lanes": 4, "tile_depth": 256},
-          "tile64": {"neighbour_pass": "tile", "tile_depth": 64, "block_waves": 2},
-          "tile_list64": {"neighbour_pass": "tile_list", "tile_depth": 64, "block_waves": 1},
+SHAPES = {"tile_list": {}, "tile": {"neighbour_pass": "tile", "density_lanes": 8, "tile_depth": 256},
+          "tile_list8": {"neighbour_pass": "tile_list", "density_lanes": 8, "tile_depth": 256, "force_chunks": 1},
+          "tile64": {"neighbour_pass": "tile", "density_lanes": 8, "tile_depth": 64, "block_waves": 2},
+          "tile_list64": {"neighbour_pass": "tile_list", "density_lanes": 8, "tile_depth": 64, "block_waves": 1},
           "tile_list4_64": {"neighbour_pass": "tile_list", "density_lanes": 4, "tile_depth": 64, "block_waves": 1},
           "tile_list4_ch2": {"neighbour_pass": "tile_list", "density_lanes": 4, "force_chunks": 2},
           "list": {"neighbour_pass": "list"}}
@@ -139,7 +139,7 @@ def test_small_dam_break_vs_oracle(oracle, clear, shape):
         assert nw <= TOL and per <= TOL, (target, nw, per)
 
 
-@pytest.mark.parametrize("shape", ["tile_list", "tile_list4", "tile", "list"])
+@pytest.mark.parametrize("shape", ["tile_list", "tile_list8", "tile", "list"])
 def test_1m_step_vs_oracle(oracle, shape):
     """1,048,576 particles in [0,4]^3: keys, permutation, bucket tables bit-exact; density, pressure
     and forces of one step within 1e-5."""
@@ -283,7 +283,7 @@ def test_traversal_orders_bit_identical():
 
 
 @pytest.mark.parametrize("order", ["sorted", "caller", "morton"])
-@pytest.mark.parametrize("shape", ["tile_list", "tile_list4", "tile", "tile64", "tile_list64", "tile_list4_64",
+@pytest.mark.parametrize("shape", ["tile_list", "tile_list8", "tile", "tile64", "tile_list64", "tile_list4_64",
                                    "tile_list4_ch2"])
 def test_tile_passes_match_list_passes(order, shape):
     """The LDS-tile passes take the same visits as the recorded-visit passes; only the FP32 summation
