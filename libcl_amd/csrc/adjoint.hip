@@ -60,6 +60,20 @@ __global__ __launch_bounds__(kThreads) void k_adj_reduce_grads(const double* __r
     if (lane == 0) grads[which] = v;
 }
 
+// The gathering reverse kernels split a particle's recorded visits over kAdjLanes lanes (visit k goes
+// to lane k % kAdjLanes) and add the lanes' sums in a fixed order.
+constexpr uint32_t kAdjLanes = 4;
+
+// blocks of the widest kernel that adds parameter-gradient partials (k_adj_forces)
+inline uint32_t grad_blocks(size_t n) { return grid_for(n * kAdjLanes).x; }
+
+__device__ inline float lanes_sum(float v)
+{
+    v += __shfl_xor(v, 1, kWave);
+    v += __shfl_xor(v, 2, kWave);
+    return v;
+}
+
 __device__ inline void atomic_add3(float4* p, float x, float y, float z)
 {
     float* q = reinterpret_cast<float*>(p);
@@ -196,7 +210,7 @@ __global__ __launch_bounds__(kThreads) void k_rec_forces_step(const float4* __re
     float3 P, V;
     const uint32_t count = ncount[i];
     if (count <= cap)
-        sph_force_sum_nbr(spos, svel, nbr, n, count, i, pi, vi, p.cellSize, P, V);
+        sph_force_sum_nbr<2>(spos, svel, nbr, n, count, i, pi, vi, p.cellSize, P, V);
     else
         sph_force_sum(spos, svel, cell_start, cell_end, i, n, pi, vi, p.cellSize, P, V);
     const uint32_t o = index[i];
@@ -339,7 +353,7 @@ __global__ __launch_bounds__(kThreads) void k_adj_forces(const float4* __restric
                                                          float4* __restrict__ vb_s, float2* __restrict__ rp_s,
                                                          double* __restrict__ grads)
 {
-    const uint32_t j = blockIdx.x * kThreads + threadIdx.x;
+    const uint32_t j = (blockIdx.x * kThreads + threadIdx.x) / kAdjLanes, sub = threadIdx.x % kAdjLanes;
     double kv_grad = 0.0;
     if (j < n) {
         const float h = p.cellSize;
@@ -351,7 +365,7 @@ __global__ __launch_bounds__(kThreads) void k_adj_forces(const float4* __restric
         float xbx = 0.f, xby = 0.f, xbz = 0.f, vbx = 0.f, vby = 0.f, vbz = 0.f, rb = 0.f, pbs = 0.f;
         float ux = 0.f, uy = 0.f, uz = 0.f;
         const uint32_t count = ncount[j];
-        for (uint32_t k = 0; k < count; k++) {
+        for (uint32_t k = sub; k < count; k += kAdjLanes) {
             const uint32_t i = nbr[nbr_at(j, k, n)];
             const float4 pi = spos[i], vi = svel[i];
             // own: F_j's pair with i
@@ -388,13 +402,26 @@ __global__ __launch_bounds__(kThreads) void k_adj_forces(const float4* __restric
                 unsafeAtomicAdd(rpi + 1, a.pb);
             }
         }
-        atomic_add3(&xb_s[j], xbx, xby, xbz);
-        atomic_add3(&vb_s[j], vbx, vby, vbz);
-        float* rpj = reinterpret_cast<float*>(&rp_s[j]);
-        unsafeAtomicAdd(rpj + 0, rb);
-        unsafeAtomicAdd(rpj + 1, pbs);
-        // dL/dKv = sum_j m fbar_j . V_j
-        kv_grad = (double)p.mass * ((double)fj.x * ux + (double)fj.y * uy + (double)fj.z * uz);
+        xbx = lanes_sum(xbx);
+        xby = lanes_sum(xby);
+        xbz = lanes_sum(xbz);
+        vbx = lanes_sum(vbx);
+        vby = lanes_sum(vby);
+        vbz = lanes_sum(vbz);
+        rb = lanes_sum(rb);
+        pbs = lanes_sum(pbs);
+        ux = lanes_sum(ux);
+        uy = lanes_sum(uy);
+        uz = lanes_sum(uz);
+        if (sub == 0) {
+            atomic_add3(&xb_s[j], xbx, xby, xbz);
+            atomic_add3(&vb_s[j], vbx, vby, vbz);
+            float* rpj = reinterpret_cast<float*>(&rp_s[j]);
+            unsafeAtomicAdd(rpj + 0, rb);
+            unsafeAtomicAdd(rpj + 1, pbs);
+            // dL/dKv = sum_j m fbar_j . V_j
+            kv_grad = (double)p.mass * ((double)fj.x * ux + (double)fj.y * uy + (double)fj.z * uz);
+        }
     }
     block_add_f64(kv_grad, grads, 0);
 }
@@ -436,15 +463,15 @@ __global__ __launch_bounds__(kThreads) void k_adj_density(const float4* __restri
                                                           const uint8_t* __restrict__ clean, uint32_t n, mfl_params p,
                                                           const float* __restrict__ accbar, float4* __restrict__ xb_s)
 {
-    const uint32_t j = blockIdx.x * kThreads + threadIdx.x;
-    if (j >= n) return;
+    const uint32_t j = (blockIdx.x * kThreads + threadIdx.x) / kAdjLanes, sub = threadIdx.x % kAdjLanes;
+    if (j >= n) return;  // whole groups of kAdjLanes lanes leave together
     const float h2 = p.cellSize * p.cellSize;
     const float4 pj = spos[j];
     const float aj = accbar[j];
     const bool clean_j = clean[j] != 0;
     float xbx = 0.f, xby = 0.f, xbz = 0.f;
     const uint32_t count = ncount[j];
-    for (uint32_t k = 0; k < count; k++) {
+    for (uint32_t k = sub; k < count; k += kAdjLanes) {
         const uint32_t i = nbr[nbr_at(j, k, n)];
         const float4 pi = spos[i];
         const float dx = pj.x - pi.x, dy = pj.y - pi.y, dz = pj.z - pi.z;
@@ -460,7 +487,10 @@ __global__ __launch_bounds__(kThreads) void k_adj_density(const float4* __restri
         xby += c * dy;
         xbz += c * dz;
     }
-    atomic_add3(&xb_s[j], xbx, xby, xbz);
+    xbx = lanes_sum(xbx);
+    xby = lanes_sum(xby);
+    xbz = lanes_sum(xbz);
+    if (sub == 0) atomic_add3(&xb_s[j], xbx, xby, xbz);
 }
 
 __global__ __launch_bounds__(kThreads) void k_adj_scatter(const uint32_t* __restrict__ index,
@@ -498,7 +528,7 @@ Adjoint::Adjoint(Fluid3D& fluid, uint32_t steps, uint32_t checkpoint_every, uint
         !t_ncount_.allocate(seg * u1) || !t_nbr_.allocate(seg * cap_ * u1) || !t_fpre_.allocate(seg * v4) ||
         !t_clean_.allocate(seg * n_) || !accbar_.allocate(u1) ||
         !xbar_.allocate(v4) || !vbar_.allocate(v4) || !xb_s_.allocate(v4) || !vb_s_.allocate(v4) ||
-        !rp_s_.allocate(n_ * 8) || !fbar_s_.allocate(v4) || !grads_.allocate((grid_for(n_).x + 1) * kGradSlots * sizeof(double)) ||
+        !rp_s_.allocate(n_ * 8) || !fbar_s_.allocate(v4) || !grads_.allocate((grad_blocks(n_) + 1) * kGradSlots * sizeof(double)) ||
         !flags_.allocate(4))
         return;
     ok_ = true;
@@ -602,11 +632,12 @@ int Adjoint::reverse_step(uint32_t slot, hipStream_t s)
                        f_.params, to_f4(f_.box_min), to_f4(f_.box_max), xb_s_.as<float4>(), vb_s_.as<float4>(),
                        rp_s_.as<float2>(), fbar_s_.as<float4>());
     const uint8_t* clean = t_clean_.as<uint8_t>() + (size_t)slot * n_;
-    hipLaunchKernelGGL(k_adj_forces, g, b, 0, s, spos, svel, nbr, ncount, clean, n, f_.params, fbar_s_.as<float4>(),
+    const dim3 ga(grid_for((size_t)n * kAdjLanes));
+    hipLaunchKernelGGL(k_adj_forces, ga, b, 0, s, spos, svel, nbr, ncount, clean, n, f_.params, fbar_s_.as<float4>(),
                        xb_s_.as<float4>(), vb_s_.as<float4>(), rp_s_.as<float2>(), grads_.as<double>());
     hipLaunchKernelGGL(k_adj_pressure, g, b, 0, s, spos, ncount, n, f_.params, rp_s_.as<float2>(),
                        accbar_.as<float>(), grads_.as<double>());
-    hipLaunchKernelGGL(k_adj_density, g, b, 0, s, spos, nbr, ncount, clean, n, f_.params, accbar_.as<float>(),
+    hipLaunchKernelGGL(k_adj_density, ga, b, 0, s, spos, nbr, ncount, clean, n, f_.params, accbar_.as<float>(),
                        xb_s_.as<float4>());
     hipLaunchKernelGGL(k_adj_scatter, g, b, 0, s, index, xb_s_.as<float4>(), vb_s_.as<float4>(), n, xbar_.as<float4>(),
                        vbar_.as<float4>());
@@ -642,7 +673,7 @@ int Adjoint::backward(const float4* pos_bar, const float4* vel_bar, hipStream_t 
             if (!reverse_step((uint32_t)l, s)) return 0;
     }
     if (!restore_checkpoint((uint32_t)ckpt_.size() - 1, s)) return 0;
-    const uint32_t nblocks = grid_for(n_).x;
+    const uint32_t nblocks = grad_blocks(n_);
     hipLaunchKernelGGL(k_adj_reduce_grads, dim3(1), dim3(kThreads), 0, s, grads_.as<double>(), nblocks,
                        grads_.as<double>() + (size_t)nblocks * kGradSlots);
     MFL_HIP(hipGetLastError());
@@ -674,7 +705,7 @@ int Adjoint::param_grads(double* out4)
 {
     double g[4];
     MFL_HIP(hipDeviceSynchronize());
-    MFL_HIP(hipMemcpy(g, grads_.as<double>() + (size_t)grid_for(n_).x * kGradSlots, sizeof(g), hipMemcpyDeviceToHost));
+    MFL_HIP(hipMemcpy(g, grads_.as<double>() + (size_t)grad_blocks(n_) * kGradSlots, sizeof(g), hipMemcpyDeviceToHost));
     // [0] accumulates dL/dKv; viscosityConstant = viscosity * 45/(pi h^6) (cl:146)
     const double kv = f_.params.viscosityConstant, mu = f_.params.viscosity;
     out4[0] = mu != 0.0 ? g[0] * kv / mu : 0.0;  // dL/d viscosity
