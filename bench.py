@@ -3,8 +3,8 @@
 python bench.py [--gpus 1] [--steps K] [--warmup W] [--config 1m|small] [--no-cpu]
 
 Workload: the 1,048,576-particle dam break (128x64x128 jittered lattice at 0.5h, h = 0.02, box
-[0,4]^3, fp32), inputs resident in HBM. A step is one oclFluid3D::compute(): hash, sort, reorder,
-bucket clear, bounds, density, forces + clip box + integration. Prints one JSON line.
+[0,4]^3, fp32), inputs resident in HBM. A step is one oclFluid3D::compute(): hash + bucket clear,
+sort, reorder + bounds, density, forces + clip box + integration. Prints one JSON line.
 """
 from __future__ import annotations
 
@@ -23,14 +23,15 @@ HBM_PEAK_GBS = 8000.0  # MI355X HBM3E, vendor figure
 # once; neighbour re-reads are cache traffic and not counted), plus per-launch constants.
 STAGE_BYTES_PER_PARTICLE = {
     "hash": 16 + 4,                     # R pos, W key
-    "sort": 16 + 20 + 20,               # 3 passes of 6 bits: counts R 4, scatter R 4/8 W 8 (index generated)
-    "reorder": 4 + 32 + 32,             # R index, gather pos/vel, W sorted pos/vel
-    "bounds": 4 + 1,                    # R key, W start/end (one occupied bucket per ~8 particles)
+    "sort": 16 + 20,                    # 2 passes of 9 bits: counts R 4, scatter R 4/8 W 8 (index generated)
+    "reorder": 4 + 32 + 32 + 4 + 4 + 1, # R index, gather pos/vel, W sorted pos/vel, W inverse, R key,
+                                        # W bucket start/end (one occupied bucket per ~8 particles)
+    "bounds": 0,                        # (done by the reorder kernel)
     "density": 16 + 8,                  # R sorted pos, W density/pressure lanes
     "forces": 16 + 16 + 4 + 48 + 48,    # R sorted pos/vel, index; R+W force/vel/pos in caller order
 }
 STAGE_BYTES_PER_LAUNCH = {
-    "reorder": 2 * 65536 * 4,           # reference bucket clear
+    "hash": 2 * 65536 * 4,              # reference bucket clear
     "density": 2 * 262144 * 4,          # one read of both bucket tables
     "forces": 2 * 262144 * 4,
 }
@@ -45,6 +46,17 @@ def measured_traffic(stage: str, workload: str):
         with open(path) as fh:
             table = json.load(fh)
         return table[workload][stage]["bytes_per_launch"]
+    except (OSError, KeyError, ValueError):
+        return None
+
+
+def measured_valu(stage: str, workload: str):
+    """VALU busy fraction of a stage's kernel from the committed rocprofv3 SQ counter summary
+    (profiles/valu.json); None when that kernel and workload were not measured."""
+    path = os.path.join(ROOT, "profiles", "valu.json")
+    try:
+        with open(path) as fh:
+            return json.load(fh)[workload][stage]["valu_busy"]
     except (OSError, KeyError, ValueError):
         return None
 
@@ -316,6 +328,7 @@ def main():
         "roofline": {"bound": "hbm", "kernel": dominant, "achieved": achieved, "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
                      "traffic": measured_traffic(dominant, sc.name),
+                     "valu_busy": measured_valu(dominant, sc.name),
                      "avg_launch_us": avg_s * 1e6, "alg_bytes_per_launch": alg_bytes},
         "stage_us": {k: v * 1e3 for k, v in stage_ms.items()},
         "step_hbm_frac": step_bytes / (ms_per_step / 1e3) / 1e9 / HBM_PEAK_GBS,
