@@ -78,8 +78,9 @@ def cpu_baseline(sc, prm, steps: int, threads: int) -> dict:
 
 
 def batched_leg(steps: int, seq_steps: int, num_scenes: int = 256) -> dict:
-    """256 independent scenes (log-uniform 2,048..65,536 particles) advanced together, against the same
-    scenes stepped one after another by single-scene programs on the same stream."""
+    """256 independent scenes (log-uniform 2,048..65,536 particles) advanced together (launch shape
+    autotuned, hipGraph replay), against the same scenes stepped one after another by single-scene
+    programs (default launch shape) on the same stream."""
     import numpy as np
     import torch
 
@@ -94,6 +95,8 @@ def batched_leg(steps: int, seq_steps: int, num_scenes: int = 256) -> dict:
     assert b.setParameters(prm) == 1 and b.setBox((0, 0, 0, 0), (1, 1, 1, 0)) == 1
     assert b.upload(np.concatenate([sc.pos for sc in scs]), np.concatenate([sc.vel for sc in scs]),
                     np.concatenate([sc.force for sc in scs])) == 1
+    res = b.autotune(steps=3, stream=stream)
+    assert res is not None, L.last_error()
     b.setUseGraph(True)
     for _ in range(3):
         assert b.compute(stream) == 1, L.last_error()
@@ -110,7 +113,6 @@ def batched_leg(steps: int, seq_steps: int, num_scenes: int = 256) -> dict:
         assert f.compile() == 1 and f.setParticleCount(sc.n) == 1
         f.setParameters(prm)
         f.setBox(sc.box_min, sc.box_max)
-        f.setTraversal("sorted", 0)
         assert f.upload(sc.pos, sc.vel, sc.force) == 1
         progs.append(f)
     out = {}
@@ -126,7 +128,7 @@ def batched_leg(steps: int, seq_steps: int, num_scenes: int = 256) -> dict:
         torch.cuda.synchronize()
         out[mode] = (time.perf_counter() - t0) * 1e3 / seq_steps
     del progs
-    return {"scenes": num_scenes, "particles": total, "steps": steps, "ms_per_step": batched_ms,
+    return {"scenes": num_scenes, "particles": total, "steps": steps, "tuning": res[0], "ms_per_step": batched_ms,
             "particle_steps_per_s": total / (batched_ms / 1e3),
             "sequential_ms_per_step": out["eager"], "sequential_graph_ms_per_step": out["graph"],
             "sequential_steps": seq_steps, "gain": out["eager"] / batched_ms,

@@ -35,7 +35,7 @@ EXPORTS = (
     "mfl_batch_create", "mfl_batch_destroy", "mfl_batch_scene_range", "mfl_batch_set_params",
     "mfl_batch_get_params", "mfl_batch_set_box", "mfl_batch_set_gravity", "mfl_batch_set_bucket_clear",
     "mfl_batch_set_use_graph", "mfl_batch_get_buffer", "mfl_batch_write_buffer", "mfl_batch_read_buffer",
-    "mfl_batch_compute",
+    "mfl_batch_compute", "mfl_batch_get_tuning", "mfl_batch_set_tuning", "mfl_batch_autotune",
     "mfl_adjoint_create", "mfl_adjoint_destroy", "mfl_adjoint_forward", "mfl_adjoint_backward",
     "mfl_adjoint_get_buffer", "mfl_adjoint_param_grads",
 )
@@ -146,6 +146,10 @@ def load() -> ctypes.CDLL:
         "mfl_batch_write_buffer": (ctypes.c_int, [vp, ctypes.c_int, vp, sz]),
         "mfl_batch_read_buffer": (ctypes.c_int, [vp, ctypes.c_int, vp, sz]),
         "mfl_batch_compute": (ctypes.c_int, [vp, vp]),
+        "mfl_batch_get_tuning": (ctypes.c_int, [vp, ctypes.POINTER(Tuning)]),
+        "mfl_batch_set_tuning": (ctypes.c_int, [vp, ctypes.POINTER(Tuning)]),
+        "mfl_batch_autotune": (ctypes.c_int, [vp, vp, ctypes.c_uint32, ctypes.POINTER(Tuning),
+                                              ctypes.POINTER(ctypes.c_double)]),
         "mfl_adjoint_create": (vp, [vp, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint32]),
         "mfl_adjoint_destroy": (None, [vp]),
         "mfl_adjoint_forward": (ctypes.c_int, [vp, vp]),

@@ -9,6 +9,7 @@
 // (sph_device.hpp) on scene-local pointers, so each scene's state is bit-identical to running it alone.
 #include "batch.hpp"
 #include "sph_device.hpp"
+#include "tile.hpp"
 
 #include <cstring>
 #include <sstream>
@@ -32,7 +33,8 @@ __global__ __launch_bounds__(kThreads) void k_b_hash(const float4* __restrict__ 
 
 __global__ __launch_bounds__(kThreads) void k_b_reorder(const uint32_t* __restrict__ index,
                                                         const float4* __restrict__ pos, const float4* __restrict__ vel,
-                                                        float4* __restrict__ spos, float4* __restrict__ svel, uint32_t N)
+                                                        float4* __restrict__ spos, float4* __restrict__ svel, uint32_t N,
+                                                        uint32_t* __restrict__ inv)
 {
     const uint32_t i = blockIdx.x * kThreads + threadIdx.x;
     if (i >= N) return;
@@ -40,6 +42,26 @@ __global__ __launch_bounds__(kThreads) void k_b_reorder(const uint32_t* __restri
     if (g >= N) return;
     spos[i] = pos[g];
     svel[i] = vel[g];
+    inv[g] = i;
+}
+
+// Traversal order of the tile density pass: scenes in order (scene in the high key bits), a Morton
+// order of the cells inside each scene (mbits bits per axis, wrapped).
+__global__ __launch_bounds__(kThreads) void k_b_morton_keys(const float4* __restrict__ pos,
+                                                            const uint32_t* __restrict__ pscene,
+                                                            const mfl_params* __restrict__ params, uint32_t N,
+                                                            uint32_t mbits, uint32_t* __restrict__ key)
+{
+    const uint32_t g = blockIdx.x * kThreads + threadIdx.x;
+    if (g >= N) return;
+    const uint32_t s = pscene[g];
+    const float h = params[s].cellSize;
+    const float4 q = pos[g];
+    const uint32_t m = (1u << mbits) - 1u;
+    const uint32_t code = spread_bits3((uint32_t)grid_coord(q.x, h) & m) |
+                          (spread_bits3((uint32_t)grid_coord(q.y, h) & m) << 1) |
+                          (spread_bits3((uint32_t)grid_coord(q.z, h) & m) << 2);
+    key[g] = (s << (3 * mbits)) | code;
 }
 
 // the reference's per-step reset of the first `clear` entries of each scene's tables (cpp:309-313)
@@ -144,6 +166,173 @@ __global__ __launch_bounds__(kThreads) void k_b_density(float4* __restrict__ spo
     ncount[i] = count;
 }
 
+// One lane walks particle i's 27 bucket ranges of its scene from global memory and records the visits
+// (the overflow path of the batched tile pass; scene-local indices throughout).
+__device__ __noinline__ uint32_t b_density_walk(int cx, int cy, int cz, float4 pi, uint32_t il, uint32_t ig,
+                                               const float4* __restrict__ spos, const uint32_t* __restrict__ cs,
+                                               const uint32_t* __restrict__ ce, uint32_t n_s, float h, uint32_t N,
+                                               uint32_t* __restrict__ nbr, float* acc_out)
+{
+    const float h2 = h * h;
+    const float h2_lo = h2 * 0.9999996f, h2_hi = h2 * 1.0000005f;
+    float acc = 0.f;
+    uint32_t count = 0;
+    for (int k = 0; k < 27; k++) {
+        const uint32_t b = cell_hash(cx + k % 3 - 1, cy + (k / 3) % 3 - 1, cz + k / 9 - 1);
+        const uint32_t st = cs[b];
+        if (st == kEmpty) continue;
+        const uint32_t end = min(ce[b], n_s);
+        for (uint32_t j = st; j < end; j++) {
+            const float4 pj = spos[j];
+            const float dx = pi.x - pj.x, dy = pi.y - pj.y, dz = pi.z - pj.z;
+            const float r2 = dist2(dx, dy, dz);
+            if (j != il && accept_r2(r2, h, h2_lo, h2_hi)) {
+                const float w = h2 - r2;
+                acc = fmaf(w * w, w, acc);
+                if (count < kNbrCap) nbr[nbr_at(ig, count, N)] = j;
+                count++;
+            }
+        }
+    }
+    *acc_out = acc;
+    return count;
+}
+
+// The tile density pass of the single-scene program (k_density_tile, TILE_LIST form) over all scenes:
+// traversal slot t in the batch's Morton order, each 8-particle segment inside one scene (scene counts
+// are multiples of 256 and the order keeps scenes in place), that scene's Params, tables and
+// scene-local indices; the visits are recorded by sorted slot for k_b_forces_step.
+template <int W>
+__global__ __launch_bounds__(kThreads) void k_b_density_tile(const float4* __restrict__ pos, float4* __restrict__ spos,
+                                                             float4* __restrict__ svel,
+                                                             const uint32_t* __restrict__ cs_all,
+                                                             const uint32_t* __restrict__ ce_all,
+                                                             const uint32_t* __restrict__ off,
+                                                             const uint32_t* __restrict__ pscene,
+                                                             const mfl_params* __restrict__ params, uint32_t N,
+                                                             uint32_t depth, const uint32_t* __restrict__ porder,
+                                                             const uint32_t* __restrict__ inv,
+                                                             uint32_t* __restrict__ nbr, uint32_t* __restrict__ ncount)
+{
+    constexpr uint32_t P = W / 8;
+    constexpr uint32_t SEGS = kWave / W;
+    extern __shared__ float4 lds[];
+    const Lanes<W> L(threadIdx.x & (kWave - 1));
+    const uint32_t nseg = (blockDim.x / kWave) * SEGS, sid = (threadIdx.x / kWave) * SEGS + L.seg;
+    float4* tile = lds + (size_t)sid * (depth + kGroupLanes);
+    StageAux* aux = reinterpret_cast<StageAux*>(lds + (size_t)nseg * (depth + kGroupLanes)) + sid;
+    const uint32_t grp = L.ls / P, slot = L.ls % P;
+    const uint32_t wave = (xcd_block() * blockDim.x + threadIdx.x) / kWave;
+    const uint32_t t = wave * (kWave / P) + L.seg * 8 + grp;
+    uint32_t i = N, sc = 0;
+    float4 pi = make_float4(0.f, 0.f, 0.f, 0.f);
+    if (t < N) {
+        const uint32_t o = porder[t];
+        if (o < N) {
+            i = inv[o];
+            pi = pos[o];
+            sc = pscene[o];
+        }
+    }
+    const bool live = i < N;
+    // the segment's scene (N is a multiple of 256, so a segment is all live or all past the end)
+    const uint32_t s = L.from(sc, 0u);
+    const uint32_t lo = off[s], n_s = off[s + 1] - lo;
+    const float4* sp = spos + lo;
+    const uint32_t* cs = cs_all + (size_t)s * kBuckets;
+    const uint32_t* ce = ce_all + (size_t)s * kBuckets;
+    const mfl_params p = params[s];
+    const uint32_t il = live ? i - lo : n_s;
+    const float h = p.cellSize;
+    const float h2 = h * h;
+    const float h2_lo = h2 * 0.9999996f, h2_hi = h2 * 1.0000005f;
+    const float inf = __builtin_inff();
+    const int gx = grid_coord(pi.x, h), gy = grid_coord(pi.y, h), gz = grid_coord(pi.z, h);
+    float acc = 0.f;
+    uint32_t count = 0;
+    uint64_t todo = L.bits(__ballot(live));
+    while (__ballot(todo != 0) != 0) {
+        const SegCell c = next_cell_seg<W>(L, todo, live, gx, gy, gz, pi);
+        todo &= ~L.bits(__ballot(c.member));
+        const uint32_t kept = stage_seg<W>(L, c, h2 * kKeepMargin, sp, cs, ce, n_s, depth, tile, aux);
+        if (kept <= depth) {
+            if (L.ls < kGroupLanes) tile[kept + L.ls] = make_float4(inf, inf, inf, __uint_as_float(kEmpty));
+            wave_sync();
+            const uint32_t trips = (kept + P - 1) / P;
+            uint32_t wtrips = trips;
+            if constexpr (W < kWave) wtrips = max(wtrips, (uint32_t)__shfl_xor((int)wtrips, W, kWave));
+            wtrips = __builtin_amdgcn_readfirstlane(wtrips);
+            const uint32_t qmax = kept + slot;
+            float a = 0.f;
+            uint64_t hits = 0;
+            bool band = false;
+            for (uint32_t m = 0; m < wtrips; m++) {
+                const float4 e = tile[min(m * P + slot, qmax)];
+                const float dx = pi.x - e.x, dy = pi.y - e.y, dz = pi.z - e.z;
+                const float r2 = dist2(dx, dy, dz);
+                const bool other = __float_as_uint(e.w) != il;
+                const bool ok = other && r2 < h2_lo;
+                band |= other & (r2 >= h2_lo) & (r2 <= h2_hi);
+                const float w = h2 - r2;
+                a = ok ? fmaf(w * w, w, a) : a;
+                hits = (hits << 1) + (ok ? 1ull : 0ull);
+            }
+            hits >>= wtrips - trips;
+            if (__ballot(band && c.member) != 0) {
+                a = 0.f;
+                hits = 0;
+                for (uint32_t m = 0; m < trips; m++) {
+                    const float4 e = tile[m * P + slot];
+                    const float dx = pi.x - e.x, dy = pi.y - e.y, dz = pi.z - e.z;
+                    const float r2 = dist2(dx, dy, dz);
+                    const bool ok = __float_as_uint(e.w) != il && accept_r2(r2, h, h2_lo, h2_hi);
+                    const float w = h2 - r2;
+                    a = ok ? fmaf(w * w, w, a) : a;
+                    hits = (hits << 1) + (ok ? 1ull : 0ull);
+                }
+            }
+            if (c.member) {
+                acc += a;
+                const uint32_t mine = (uint32_t)__popcll(hits);
+                count += mine;
+                uint32_t at = mine;
+#pragma unroll
+                for (uint32_t o = 1; o < P; o <<= 1) {
+                    const uint32_t up = (uint32_t)__shfl_up((int)at, o, P);
+                    if (slot >= o) at += up;
+                }
+                at -= mine;
+                while (hits) {
+                    const uint32_t m = trips - 1u - (uint32_t)__builtin_ctzll(hits);
+                    hits &= hits - 1ull;
+                    if (at < kNbrCap) nbr[nbr_at(i, at, N)] = __float_as_uint(tile[m * P + slot].w);
+                    at++;
+                }
+            }
+        } else if (c.member && slot == 0) {
+            float a = 0.f;
+            count += b_density_walk(c.cx, c.cy, c.cz, pi, il, i, sp, cs, ce, n_s, h, N, nbr, &a);
+            acc += a;
+        }
+        wave_sync();
+    }
+    acc = particle_sum<W>(acc);
+    count = particle_sum_u<W>(count);
+    if (!live || slot != 0) return;
+    float rho, prs;
+    if (count > 2) {
+        rho = fmaxf(1.0f, p.mass * p.kernelConstant * acc);
+        prs = p.pressure + p.stiffness * (rho - p.density);
+    } else {
+        rho = p.density;
+        prs = p.pressure;
+    }
+    spos[i].w = rho;
+    svel[i].w = prs;
+    ncount[i] = count;
+}
+
+template <int CH>
 __global__ __launch_bounds__(kThreads) void k_b_forces_step(const float4* __restrict__ spos,
                                                             const float4* __restrict__ svel, float4* __restrict__ pos,
                                                             float4* __restrict__ vel, float4* __restrict__ force,
@@ -167,7 +356,7 @@ __global__ __launch_bounds__(kThreads) void k_b_forces_step(const float4* __rest
     float3 P, V;
     const uint32_t count = ncount[i];
     if (count <= kNbrCap)
-        sph_force_sum_nbr(spos + lo, svel + lo, nbr, N, count, i, pi, vi, p.cellSize, P, V);
+        sph_force_sum_nbr<CH>(spos + lo, svel + lo, nbr, N, count, i, pi, vi, p.cellSize, P, V);
     else
         sph_force_sum(spos + lo, svel + lo, cs_all + (size_t)s * kBuckets, ce_all + (size_t)s * kBuckets, i - lo, n_s,
                       pi, vi, p.cellSize, P, V);
@@ -226,7 +415,8 @@ FluidBatch::FluidBatch(uint32_t num_scenes, const uint32_t* counts)
     if (!off_.allocate((S_ + 1) * 4) || !pscene_.allocate(u1) || !params_.allocate(S_ * sizeof(mfl_params)) ||
         !boxes_.allocate((size_t)S_ * 32) || !pos_.allocate(v4) || !vel_.allocate(v4) || !force_.allocate(v4) ||
         !cell_.allocate(u1) || !index_.allocate(u1) || !spos_.allocate(v4) || !svel_.allocate(v4) ||
-        !tables_.allocate(2 * tables) || !nbr_.allocate((size_t)kNbrCap * u1) || !ncount_.allocate(u1))
+        !tables_.allocate(2 * tables) || !nbr_.allocate((size_t)kNbrCap * u1) || !ncount_.allocate(u1) ||
+        !inv_.allocate(u1) || !porder_.allocate(u1) || !mkey_.allocate(u1))
         return;
     if (!hip_ok(hipMemcpy(off_.ptr, off_host_.data(), (S_ + 1) * 4, hipMemcpyHostToDevice), "hipMemcpy") ||
         !hip_ok(hipMemcpy(pscene_.ptr, pscene.data(), u1, hipMemcpyHostToDevice), "hipMemcpy") ||
@@ -242,7 +432,7 @@ FluidBatch::~FluidBatch()
 {
     drop_graph();
     for (DeviceBuffer* b : {&off_, &pscene_, &params_, &boxes_, &pos_, &vel_, &force_, &cell_, &index_, &spos_, &svel_,
-                            &tables_, &nbr_, &ncount_})
+                            &tables_, &nbr_, &ncount_, &inv_, &porder_, &mkey_})
         b->release();
 }
 
@@ -369,18 +559,38 @@ int FluidBatch::enqueue_step(hipStream_t s)
     hipLaunchKernelGGL(k_b_hash, g, b, 0, s, pos, pscene_.as<uint32_t>(), params, N_, cell);
     MFL_HIP(hipGetLastError());
     if (!sort.sort_bits(cell, index, N_, 0, 18 + (int)scene_bits_, s, true)) return 0;
-    hipLaunchKernelGGL(k_b_reorder, g, b, 0, s, index, pos, vel, spos, svel, N_);
+    hipLaunchKernelGGL(k_b_reorder, g, b, 0, s, index, pos, vel, spos, svel, N_, inv_.as<uint32_t>());
     if (clear) {
         const size_t total = (size_t)S_ * clear;
         const unsigned blocks = (unsigned)std::min<size_t>((total + kThreads - 1) / kThreads, 65536);
         hipLaunchKernelGGL(k_b_clear, dim3(blocks), b, 0, s, cs_all, ce_all, S_, clear);
     }
     hipLaunchKernelGGL(k_b_find_bounds, g, b, 0, s, cell, off, N_, cs_all, ce_all);
-    hipLaunchKernelGGL(k_b_density, grid_for((size_t)N_ * kGroupLanes), b, 0, s, spos, svel, cs_all, ce_all, cell, off,
-                       params, N_, nbr_.as<uint32_t>(), ncount_.as<uint32_t>());
-    hipLaunchKernelGGL(k_b_forces_step, g, b, 0, s, spos, svel, pos, vel, force, index, cell, cs_all, ce_all, off, params,
-                       boxes_.as<float4>(), nbr_.as<uint32_t>(), ncount_.as<uint32_t>(), N_,
-                       gravity_force_of(gravity));
+    if (neighbour_pass == MFL_PASS_TILE_LIST && porder_valid_) {
+        const uint32_t lanes = (uint32_t)density_lanes, ppw = kWave / lanes;
+        const size_t waves = ((size_t)N_ + ppw - 1) / ppw;
+        size_t blocks = (waves + block_waves - 1) / block_waves;
+        blocks = (blocks + kXcds - 1) / kXcds * kXcds;
+        const size_t lds = (size_t)block_waves * (8 / lanes) * ((tile_depth + kGroupLanes) * sizeof(float4) +
+                                                                sizeof(StageAux));
+#define MFL_B_TILE(W)                                                                                              \
+    hipLaunchKernelGGL(k_b_density_tile<W>, dim3((unsigned)blocks), dim3(block_waves * kWave), lds, s, pos, spos,  \
+                       svel, cs_all, ce_all, off, pscene_.as<uint32_t>(), params, N_, tile_depth,                     \
+                       porder_.as<uint32_t>(), inv_.as<uint32_t>(), nbr_.as<uint32_t>(), ncount_.as<uint32_t>())
+        if (lanes == 4) MFL_B_TILE(32);
+        else MFL_B_TILE(64);
+#undef MFL_B_TILE
+    } else {
+        hipLaunchKernelGGL(k_b_density, grid_for((size_t)N_ * kGroupLanes), b, 0, s, spos, svel, cs_all, ce_all, cell,
+                           off, params, N_, nbr_.as<uint32_t>(), ncount_.as<uint32_t>());
+    }
+#define MFL_B_FORCES(CH)                                                                                          \
+    hipLaunchKernelGGL(k_b_forces_step<CH>, g, b, 0, s, spos, svel, pos, vel, force, index, cell, cs_all, ce_all, off, \
+                       params, boxes_.as<float4>(), nbr_.as<uint32_t>(), ncount_.as<uint32_t>(), N_,              \
+                       gravity_force_of(gravity))
+    if (force_chunks == 2) MFL_B_FORCES(2);
+    else MFL_B_FORCES(1);
+#undef MFL_B_FORCES
     MFL_HIP(hipGetLastError());
     return 1;
 }
@@ -398,7 +608,8 @@ int FluidBatch::launch_graph(hipStream_t s)
 {
     std::ostringstream sig;
     sig << clear_count << ':' << sort.tuning.items << ':' << gravity[0] << ',' << gravity[1] << ',' << gravity[2] << ','
-        << gravity[3];
+        << gravity[3] << ':' << neighbour_pass << ':' << density_lanes << ':' << tile_depth << ':' << block_waves << ':'
+        << force_chunks << ':' << porder_valid_;
     if (!graph_exec_ || sig.str() != graph_sig_) {
         drop_graph();
         if (!sort.reserve(N_, 18 + (int)scene_bits_)) return 0;
@@ -435,8 +646,162 @@ int FluidBatch::compute(hipStream_t s)
         MFL_HIP(hipMemcpy(boxes_.ptr, boxes.data(), boxes.size() * 16, hipMemcpyHostToDevice));
         params_dirty_ = false;
     }
+    if (neighbour_pass == MFL_PASS_TILE_LIST &&
+        (!porder_valid_ || (refresh_interval > 0 && steps_since_refresh_ >= refresh_interval))) {
+        if (!refresh_order(s)) return 0;
+        steps_since_refresh_ = 0;
+    }
+    steps_since_refresh_++;
     if (use_graph && s != nullptr) return launch_graph(s);
     return enqueue_step(s);
+}
+
+int FluidBatch::refresh_order(hipStream_t s)
+{
+    const uint32_t mbits = (32u - scene_bits_) / 3u > 10u ? 10u : (32u - scene_bits_) / 3u;
+    hipLaunchKernelGGL(k_b_morton_keys, grid_for(N_), dim3(kThreads), 0, s, pos_.as<float4>(), pscene_.as<uint32_t>(),
+                       params_.as<mfl_params>(), N_, mbits, mkey_.as<uint32_t>());
+    MFL_HIP(hipGetLastError());
+    if (!sort.sort_bits(mkey_.as<uint32_t>(), porder_.as<uint32_t>(), N_, 0, (int)(3 * mbits + scene_bits_), s, true))
+        return 0;
+    porder_valid_ = true;
+    return 1;
+}
+
+int FluidBatch::get_tuning(mfl_tuning* out) const
+{
+    if (!out) {
+        set_error("null tuning");
+        return 0;
+    }
+    out->neighbour_pass = neighbour_pass;
+    out->density_lanes = density_lanes;
+    out->tile_depth = tile_depth;
+    out->block_waves = block_waves;
+    out->sort_items = (uint32_t)sort.tuning.items;
+    out->force_chunks = force_chunks;
+    return 1;
+}
+
+int FluidBatch::set_tuning(const mfl_tuning& t)
+{
+    if (t.neighbour_pass != MFL_PASS_LIST && t.neighbour_pass != MFL_PASS_TILE_LIST) {
+        set_error("batched scenes take MFL_PASS_LIST or MFL_PASS_TILE_LIST");
+        return 0;
+    }
+    const bool tile = t.neighbour_pass == MFL_PASS_TILE_LIST;
+    if (tile ? (t.density_lanes != 4 && t.density_lanes != 8) : t.density_lanes != 8) {
+        set_error(tile ? "density_lanes must be 4 or 8 with MFL_PASS_TILE_LIST"
+                       : "density_lanes must be 8 with MFL_PASS_LIST for batched scenes");
+        return 0;
+    }
+    if (t.tile_depth < kTileMinDepth || t.tile_depth > kTileMaxDepth || t.tile_depth % 16 != 0 ||
+        (tile && t.tile_depth > 64u * (uint32_t)t.density_lanes)) {
+        set_error("tile_depth must be a multiple of 16 in [64, 64 x density_lanes]");
+        return 0;
+    }
+    if (t.block_waves != 1 && t.block_waves != 2 && t.block_waves != 4) {
+        set_error("block_waves must be 1, 2 or 4");
+        return 0;
+    }
+    if (tile && (size_t)t.block_waves * (8 / t.density_lanes) *
+                        ((t.tile_depth + kGroupLanes) * sizeof(float4) + sizeof(StageAux)) > 65536) {
+        set_error("tile_depth x block_waves needs more than 64 KiB of LDS per workgroup");
+        return 0;
+    }
+    if (t.sort_items != 0 && t.sort_items != 4 && t.sort_items != 8 && t.sort_items != 16) {
+        set_error("sort_items must be 0, 4, 8 or 16");
+        return 0;
+    }
+    if (t.force_chunks != 1 && t.force_chunks != 2) {
+        set_error("force_chunks must be 1 or 2");
+        return 0;
+    }
+    neighbour_pass = t.neighbour_pass;
+    density_lanes = t.density_lanes;
+    tile_depth = t.tile_depth;
+    block_waves = t.block_waves;
+    sort.tuning.items = (int)t.sort_items;
+    force_chunks = t.force_chunks;
+    return 1;
+}
+
+int FluidBatch::autotune(hipStream_t s, uint32_t steps, mfl_tuning* best, double* ms_per_step)
+{
+    if (!ok_) {
+        set_error("batch was not created");
+        return 0;
+    }
+    if (steps == 0) {
+        set_error("autotune needs at least one step per trial");
+        return 0;
+    }
+    const size_t vb = (size_t)N_ * sizeof(float4), tb = (size_t)S_ * kBuckets * 2 * sizeof(uint32_t);
+    DeviceBuffer save;
+    if (!save.allocate(3 * vb + tb)) return 0;
+    char* sp = save.as<char>();
+    auto snapshot = [&](bool restore) -> int {
+        void* dev[] = {pos_.ptr, vel_.ptr, force_.ptr, tables_.ptr};
+        const size_t len[] = {vb, vb, vb, tb};
+        size_t o = 0;
+        for (int k = 0; k < 4; k++) {
+            void* dst = restore ? dev[k] : sp + o;
+            const void* src = restore ? (const void*)(sp + o) : dev[k];
+            MFL_HIP(hipMemcpyAsync(dst, src, len[k], hipMemcpyDeviceToDevice, s));
+            o += len[k];
+        }
+        return 1;
+    };
+    if (!snapshot(false)) {
+        save.release();
+        return 0;
+    }
+    mfl_tuning start{};
+    get_tuning(&start);
+    std::vector<mfl_tuning> cand;
+    cand.push_back({MFL_PASS_LIST, 8, 192u, 2u, start.sort_items, 2u});
+    for (uint32_t w : {1u, 2u, 4u})
+        for (uint32_t d : {128u, 192u, 256u, 384u})
+            for (int lanes : {4, 8})
+                for (uint32_t ch : {1u, 2u}) {
+                    const mfl_tuning t{MFL_PASS_TILE_LIST, lanes, d, w, start.sort_items, ch};
+                    if (d <= 64u * (uint32_t)lanes &&
+                        (size_t)w * (8 / lanes) * ((d + kGroupLanes) * sizeof(float4) + sizeof(StageAux)) <= 65536)
+                        cand.push_back(t);
+                }
+    const bool graph_was = use_graph;
+    const uint32_t since = steps_since_refresh_;
+    use_graph = false;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    int ok = hip_ok(hipEventCreate(&e0), "hipEventCreate") && hip_ok(hipEventCreate(&e1), "hipEventCreate");
+    double best_ms = 1e30;
+    mfl_tuning pick = start;
+    for (size_t c = 0; ok && c < cand.size(); c++) {
+        ok = set_tuning(cand[c]) && snapshot(true) && compute(s);  // one untimed step per trial
+        for (uint32_t k = 0; ok && k < steps; k++) {
+            if (k == 0) ok = hip_ok(hipEventRecord(e0, s), "hipEventRecord");
+            ok = ok && compute(s);
+        }
+        ok = ok && hip_ok(hipEventRecord(e1, s), "hipEventRecord") && hip_ok(hipEventSynchronize(e1), "sync");
+        float ms = 0.f;
+        ok = ok && hip_ok(hipEventElapsedTime(&ms, e0, e1), "hipEventElapsedTime");
+        if (ok && ms / steps < best_ms) {
+            best_ms = ms / steps;
+            pick = cand[c];
+        }
+    }
+    if (e0) (void)hipEventDestroy(e0);
+    if (e1) (void)hipEventDestroy(e1);
+    use_graph = graph_was;
+    const int restored = snapshot(true) && hip_ok(hipStreamSynchronize(s), "hipStreamSynchronize");
+    steps_since_refresh_ = since;
+    porder_valid_ = false;  // rebuilt from the restored positions on the next step
+    set_tuning(ok ? pick : start);
+    save.release();
+    if (!ok || !restored) return 0;
+    if (best) *best = pick;
+    if (ms_per_step) *ms_per_step = best_ms;
+    return 1;
 }
 
 }  // namespace mfl

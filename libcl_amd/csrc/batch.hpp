@@ -33,11 +33,27 @@ public:
     bool use_graph = false;
     RadixSort sort;
 
+    // launch shape (mfl_tuning): LIST (group traversal in sorted order) or TILE_LIST (the single-scene
+    // tile density pass in a per-scene Morton order); recorded-visit forces either way
+    int neighbour_pass = MFL_PASS_TILE_LIST;
+    int density_lanes = 4;
+    uint32_t tile_depth = 192;
+    uint32_t block_waves = 2;
+    uint32_t force_chunks = 2;
+    uint32_t refresh_interval = 16;  // steps between traversal order refreshes
+    int get_tuning(mfl_tuning* out) const;
+    int set_tuning(const mfl_tuning& t);
+    int autotune(hipStream_t s, uint32_t steps, mfl_tuning* best, double* ms_per_step);
+
 private:
     int enqueue_step(hipStream_t s);
     int launch_graph(hipStream_t s);
     void drop_graph();
+    int refresh_order(hipStream_t s);
     DeviceBuffer* select(int kind);
+    bool porder_valid_ = false;
+    uint32_t steps_since_refresh_ = 0;
+    DeviceBuffer inv_, porder_, mkey_;
 
     bool ok_ = false;
     uint32_t S_ = 0, N_ = 0, scene_bits_ = 0;

@@ -414,6 +414,28 @@ int mfl_batch_compute(mfl_fluid_batch* b, void* stream)
     return guarded([&] { return b->impl.compute((hipStream_t)stream); });
 }
 
+int mfl_batch_get_tuning(mfl_fluid_batch* b, mfl_tuning* out)
+{
+    if (!need(b, "batch")) return 0;
+    return b->impl.get_tuning(out);
+}
+
+int mfl_batch_set_tuning(mfl_fluid_batch* b, const mfl_tuning* t)
+{
+    if (!need(b, "batch")) return 0;
+    if (!t) {
+        mfl::set_error("null tuning");
+        return 0;
+    }
+    return b->impl.set_tuning(*t);
+}
+
+int mfl_batch_autotune(mfl_fluid_batch* b, void* stream, uint32_t steps, mfl_tuning* best, double* ms_per_step)
+{
+    if (!need(b, "batch")) return 0;
+    return guarded([&] { return b->impl.autotune((hipStream_t)stream, steps, best, ms_per_step); });
+}
+
 // ---- adjoint --------------------------------------------------------------------------------------
 
 mfl_adjoint* mfl_adjoint_create(mfl_fluid3d* f, uint32_t steps, uint32_t checkpoint_every, uint32_t list_cap)

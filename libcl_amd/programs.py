@@ -341,6 +341,29 @@ class FluidBatch:
     def compute(self, stream=None) -> int:
         return int(N.load().mfl_batch_compute(self._h, _stream_ptr(stream)))
 
+    def getTuning(self) -> dict:
+        t = N.Tuning()
+        if not N.load().mfl_batch_get_tuning(self._h, ctypes.byref(t)):
+            raise RuntimeError(N.load().mfl_last_error().decode())
+        return t.as_dict()
+
+    def setTuning(self, **kw) -> int:
+        """As oclFluid3D.setTuning; batched scenes take the "list" and "tile_list" passes."""
+        cur = self.getTuning()
+        if "density_lanes" not in kw and kw.get("neighbour_pass") == "list":
+            kw = dict(kw, density_lanes=8)
+        cur.update(kw)
+        t = N.Tuning(N.PASSES.index(cur["neighbour_pass"]), cur["density_lanes"], cur["tile_depth"],
+                     cur["block_waves"], cur["sort_items"], cur["force_chunks"])
+        return int(N.load().mfl_batch_set_tuning(self._h, ctypes.byref(t)))
+
+    def autotune(self, steps: int = 3, stream=None):
+        t = N.Tuning()
+        ms = ctypes.c_double()
+        if not N.load().mfl_batch_autotune(self._h, _stream_ptr(stream), steps, ctypes.byref(t), ctypes.byref(ms)):
+            return None
+        return t.as_dict(), ms.value
+
     def write(self, kind: int, host: np.ndarray) -> int:
         host = np.ascontiguousarray(host)
         return int(N.load().mfl_batch_write_buffer(self._h, kind, host.ctypes.data, host.nbytes))

@@ -8,11 +8,15 @@ from libcl_amd import scenes
 pytestmark = pytest.mark.gpu
 
 
-def _plain(sc, prm, steps):
+def _plain(sc, prm, steps, shape="list"):
     import libcl_amd as L
     f = L.oclFluid3D()
     assert f.compile() == 1
-    assert f.setTuning(neighbour_pass="list") == 1  # the batch kernels split visits among lanes the same way
+    if shape == "list":
+        assert f.setTuning(neighbour_pass="list") == 1
+    else:
+        assert f.setTuning(neighbour_pass="tile_list", density_lanes=4, tile_depth=192, block_waves=2,
+                           force_chunks=2) == 1
     assert f.setParticleCount(sc.n) == 1
     f.setParameters(prm)
     f.setBox(sc.box_min, sc.box_max)
@@ -24,7 +28,11 @@ def _plain(sc, prm, steps):
 
 
 @pytest.mark.parametrize("graph", [False, True])
-def test_batch_matches_single_scene_runs(graph):
+@pytest.mark.parametrize("shape", ["list", "tile_list"])
+def test_batch_matches_single_scene_runs(graph, shape):
+    """Every scene of the batch equals the same scene run alone by a single-scene program with the same
+    launch shape (LIST: group traversal; TILE_LIST: 4-lane tiles in a Morton order refreshed every 16
+    steps), bit for bit."""
     import torch
     import libcl_amd as L
     from libcl_amd import _native as N
@@ -32,6 +40,7 @@ def test_batch_matches_single_scene_runs(graph):
     prm = L.params_from(scs[0].params)
     counts = [s.n for s in scs]
     b = L.FluidBatch(counts)
+    assert b.setTuning(neighbour_pass=shape) == 1, L.last_error()
     assert b.setParameters(prm) == 1
     assert b.setBox((0, 0, 0, 0), (1, 1, 1, 0)) == 1
     assert b.upload(np.concatenate([s.pos for s in scs]), np.concatenate([s.vel for s in scs]),
@@ -47,8 +56,8 @@ def test_batch_matches_single_scene_runs(graph):
     cell = b.read(N.MFL_CELL, np.uint32)
     spos = b.read(N.MFL_SORTED_POSITION, np.float32, (-1, 4))
     for s, sc in enumerate(scs):
-        print(f"scene {s} n={sc.n}", flush=True)
-        f = _plain(sc, prm, steps)
+        print(f"scene {s} n={sc.n} {shape}", flush=True)
+        f = _plain(sc, prm, steps, shape)
         ref = f.state()
         it = f.internals()
         lo, n = int(b.offsets[s]), sc.n
@@ -69,3 +78,32 @@ def test_batch_rejects_bad_counts():
         L.FluidBatch([256, 300])
     with pytest.raises(RuntimeError):
         L.FluidBatch([])
+
+
+def test_batch_autotune_keeps_state():
+    import torch
+    import libcl_amd as L
+    from libcl_amd import _native as N
+    scs = scenes.batched_scenes(num=8, seed=7, steps=4)
+    prm = L.params_from(scs[0].params)
+    b = L.FluidBatch([s.n for s in scs])
+    assert b.setParameters(prm) == 1 and b.setBox((0, 0, 0, 0), (1, 1, 1, 0)) == 1
+    assert b.upload(np.concatenate([s.pos for s in scs]), np.concatenate([s.vel for s in scs]),
+                    np.concatenate([s.force for s in scs])) == 1
+    stream = torch.cuda.Stream()
+    for _ in range(2):
+        assert b.compute(stream) == 1
+    stream.synchronize()
+    before = b.state()
+    tables = b.read(N.MFL_CELL_START, np.uint32)
+    res = b.autotune(steps=2, stream=stream)
+    assert res is not None, L.last_error()
+    print("batch autotune", res, flush=True)
+    assert b.getTuning() == res[0]
+    after = b.state()
+    for k in ("pos", "vel", "force"):
+        np.testing.assert_array_equal(after[k], before[k])
+    np.testing.assert_array_equal(b.read(N.MFL_CELL_START, np.uint32), tables)
+    assert b.compute(stream) == 1, L.last_error()
+    stream.synchronize()
+    assert b.setTuning(neighbour_pass="tile") == 0 and "TILE_LIST" in L.last_error()
