@@ -157,8 +157,10 @@ int mfl_fluid3d_set_traversal(mfl_fluid3d* f, int order, uint32_t refresh_steps)
    TILE: both passes stage the 27 bucket ranges of each cell's particles in LDS (`tile_depth` entries
    per wave, `block_waves` waves per workgroup), keeping only entries that can lie within h of the
    particles; a neighbourhood that keeps more than tile_depth entries is walked from global memory.
-   TILE_LIST: the TILE density pass also records the visits, and the LIST force pass reads them. */
-enum mfl_neighbour_pass { MFL_PASS_LIST = 0, MFL_PASS_TILE = 1, MFL_PASS_TILE_LIST = 2 };
+   TILE_LIST: the TILE density pass also records the visits, and the LIST force pass reads them.
+   TILE_KEPT: the TILE density pass records each cell's kept entries and every lane's acceptance bits;
+   the force pass stages the kept entries in LDS and sums the accepted ones. */
+enum mfl_neighbour_pass { MFL_PASS_LIST = 0, MFL_PASS_TILE = 1, MFL_PASS_TILE_LIST = 2, MFL_PASS_TILE_KEPT = 3 };
 typedef struct mfl_tuning {
     int32_t neighbour_pass; /* mfl_neighbour_pass */
     int32_t density_lanes;  /* lanes per particle in the density pass: LIST 1 or 8; TILE, TILE_LIST 4 or 8 */

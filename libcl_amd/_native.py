@@ -56,7 +56,7 @@ class Params(ctypes.Structure):
         return [float(getattr(self, k)) for k in PARAM_FIELDS]
 
 
-PASSES = ("list", "tile", "tile_list")
+PASSES = ("list", "tile", "tile_list", "tile_kept")
 
 
 class Tuning(ctypes.Structure):

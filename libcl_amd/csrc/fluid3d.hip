@@ -434,12 +434,18 @@ __device__ __noinline__ VisitSums density_fallback(int cx, int cy, int cz, float
     return r;
 }
 
-__device__ __noinline__ void force_fallback(const float4* __restrict__ spos, const float4* __restrict__ svel,
-                                           const uint32_t* __restrict__ cell_start,
-                                           const uint32_t* __restrict__ cell_end, uint32_t i, uint32_t n, float4 pi,
-                                           float4 vi, float h, float3& P, float3& V)
+struct PairSums {
+    float3 P, V;
+};
+
+__device__ __noinline__ PairSums force_fallback(const float4* __restrict__ spos, const float4* __restrict__ svel,
+                                               const uint32_t* __restrict__ cell_start,
+                                               const uint32_t* __restrict__ cell_end, uint32_t i, uint32_t n,
+                                               float4 pi, float4 vi, float h)
 {
-    sph_force_sum(spos, svel, cell_start, cell_end, i, n, pi, vi, h, P, V);
+    PairSums r;
+    sph_force_sum(spos, svel, cell_start, cell_end, i, n, pi, vi, h, r.P, r.V);
+    return r;
 }
 
 // Particle of traversal slot t: its sorted slot i (n when t is past the end) and its position. The
@@ -492,12 +498,19 @@ __device__ __noinline__ VisitSums density_walk(int cx, int cy, int cz, float4 pi
     return r;
 }
 
+// Outputs of the tile density pass: none, recorded visit lists, or each cell's kept entries plus the
+// lanes' acceptance bits (kept buffer of kSegCap entries per 8-particle segment; meta[t] = offset | kept
+// << 16 of the particle's cell, kMetaWalk when the force pass must walk the ranges itself).
+enum : int { kOutNone = 0, kOutList = 1, kOutKept = 2 };
+constexpr uint32_t kSegCap = 512;
+constexpr uint32_t kMetaWalk = 0xFFFFFFFFu;
+
 // Density pass over LDS-staged neighbourhoods (cl:150-200). The wave is split into 64 / W segments of
 // 8 particles (W / 8 lanes each); each segment stages its cell's neighbourhood in its own `depth + 8`
 // float4 of dynamic LDS plus a StageAux. A neighbourhood that keeps more than `depth` entries (or holds
 // more than 512 visits) falls back to walking the ranges from global memory. LIST = true also records
 // each particle's accepted visits (nbr, ncount by traversal slot) for the recorded-visit force pass.
-template <int ORDER, bool LIST, int W>
+template <int ORDER, int OUT, int W>
 __global__ __launch_bounds__(kThreads) void k_density_tile(const float4* __restrict__ pos, float4* __restrict__ spos,
                                                            float4* __restrict__ svel,
                                                            const uint32_t* __restrict__ cell_start,
@@ -505,8 +518,10 @@ __global__ __launch_bounds__(kThreads) void k_density_tile(const float4* __restr
                                                            mfl_params p, uint32_t depth,
                                                            const uint32_t* __restrict__ porder,
                                                            const uint32_t* __restrict__ inv, uint32_t* __restrict__ nbr,
-                                                           uint32_t* __restrict__ ncount)
+                                                           uint32_t* __restrict__ ncount, uint32_t* __restrict__ kept_out,
+                                                           uint64_t* __restrict__ hitmask, uint32_t* __restrict__ meta)
 {
+    constexpr bool LIST = OUT == kOutList;
     constexpr uint32_t P = W / 8;  // lanes per particle
     constexpr uint32_t SEGS = kWave / W;
     extern __shared__ float4 lds[];
@@ -527,6 +542,8 @@ __global__ __launch_bounds__(kThreads) void k_density_tile(const float4* __restr
     const int gx = grid_coord(pi.x, h), gy = grid_coord(pi.y, h), gz = grid_coord(pi.z, h);
     float acc = 0.f;
     uint32_t count = 0;  // this lane's accepted visits (summed over the particle's lanes at the end)
+    uint32_t used = 0;   // OUT == kOutKept: entries of this segment's kept buffer already written
+    const uint32_t gseg = wave * SEGS + L.seg;
     uint64_t todo = L.bits(__ballot(live));
     while (__ballot(todo != 0) != 0) {
         const SegCell c = next_cell_seg<W>(L, todo, live, gx, gy, gz, pi);
@@ -578,6 +595,20 @@ __global__ __launch_bounds__(kThreads) void k_density_tile(const float4* __restr
                 acc += a;
                 count += (uint32_t)__popcll(hits);
             }
+            if constexpr (OUT == kOutKept) {
+                // the cell's kept entries once for the segment, the acceptance bits per lane
+                if (used + kept <= kSegCap) {
+                    for (uint32_t q = L.ls; q < kept; q += W)
+                        kept_out[(size_t)gseg * kSegCap + used + q] = __float_as_uint(tile[q].w);
+                    if (c.member) {
+                        hitmask[(size_t)t * P + slot] = hits;
+                        if (slot == 0) meta[t] = used | (kept << 16);
+                    }
+                    used += kept;
+                } else if (c.member && slot == 0) {
+                    meta[t] = kMetaWalk;
+                }
+            }
             if (LIST && c.member) {
                 // this lane's accepted visits go after those of the particle's lower lanes
                 const uint32_t mine = (uint32_t)__popcll(hits);
@@ -596,6 +627,7 @@ __global__ __launch_bounds__(kThreads) void k_density_tile(const float4* __restr
                 }
             }
         } else if (c.member) {
+            if (OUT == kOutKept && slot == 0) meta[t] = kMetaWalk;
             if constexpr (W == kWave) {
                 const VisitSums f = density_fallback(c.cx, c.cy, c.cz, pi, i, t, slot, grp, spos, cell_start, cell_end,
                                                      n, h, LIST ? nbr : nullptr);
@@ -678,7 +710,9 @@ __global__ __launch_bounds__(kThreads) void k_forces_tile(const float4* __restri
                         pair_force(pi, vi, pj, tvel[q], dx, dy, dz, r2, h, P, V);
                 }
             } else if (slot == 0) {
-                force_fallback(spos, svel, cell_start, cell_end, i, n, pi, vi, h, P, V);
+                const PairSums f = force_fallback(spos, svel, cell_start, cell_end, i, n, pi, vi, h);
+                P = f.P;
+                V = f.V;
             }
         }
         wave_sync();
@@ -689,6 +723,117 @@ __global__ __launch_bounds__(kThreads) void k_forces_tile(const float4* __restri
     const uint32_t o = caller_slot<ORDER>(t, i, porder, index);
     if (o >= n) return;
     float4 f = add_sph_force(force[o], P, V, p);
+    if (!STEP) {
+        force[o] = f;
+        return;
+    }
+    f = clip_box(f, pi, vi, p.particleRadius, box_min, box_max);
+    const float4 v = integrate_force(vel[o], f, p);
+    force[o] = gravity_force;
+    vel[o] = v;
+    pos[o] = f4_add(pos[o], f4_scale(v, p.deltaTime));
+}
+
+// Force pass over the kept entries of the tile density pass (cl:202-255, STEP: + cl:258-361): the same
+// waves and segments as the density pass; each segment stages the positions (with density) and
+// velocities (with pressure) of its cell's kept entries in LDS from the recorded kept list, and every
+// lane sums the pair terms of the entries its acceptance bits mark (the density pass tested exactly
+// these). Particles whose cell did not fit walk the 27 ranges.
+template <int ORDER, int W, bool STEP>
+__global__ __launch_bounds__(kThreads) void k_forces_kept(const float4* __restrict__ spos,
+                                                          const float4* __restrict__ svel, float4* __restrict__ pos,
+                                                          float4* __restrict__ vel, float4* __restrict__ force,
+                                                          const uint32_t* __restrict__ index,
+                                                          const uint32_t* __restrict__ cell_start,
+                                                          const uint32_t* __restrict__ cell_end, uint32_t n,
+                                                          mfl_params p, uint32_t depth, float4 box_min, float4 box_max,
+                                                          float4 gravity_force, const uint32_t* __restrict__ porder,
+                                                          const uint32_t* __restrict__ inv,
+                                                          const uint32_t* __restrict__ kept_in,
+                                                          const uint64_t* __restrict__ hitmask,
+                                                          const uint32_t* __restrict__ meta)
+{
+    constexpr uint32_t P = W / 8;
+    constexpr uint32_t SEGS = kWave / W;
+    extern __shared__ float4 lds[];
+    const Lanes<W> L(threadIdx.x & (kWave - 1));
+    const uint32_t sid = (threadIdx.x / kWave) * SEGS + L.seg;
+    float4* tpos = lds + (size_t)sid * 2 * depth;
+    float4* tvel = tpos + depth;
+    const uint32_t grp = L.ls / P, slot = L.ls % P;
+    const uint32_t wave = (xcd_block() * blockDim.x + threadIdx.x) / kWave;
+    const uint32_t t = wave * (kWave / P) + L.seg * 8 + grp;
+    const uint32_t gseg = wave * SEGS + L.seg;
+    const uint32_t i = t < n ? sorted_slot<ORDER>(t, n, porder, inv) : n;
+    const bool live = i < n;
+    const float h = p.cellSize;
+    const float4 pi = live ? spos[i] : make_float4(0.f, 0.f, 0.f, 0.f);
+    const float4 vi = live ? svel[i] : make_float4(0.f, 0.f, 0.f, 0.f);
+    const uint32_t mt = live ? meta[t] : kMetaWalk;
+    const bool staged = live && mt != kMetaWalk;
+    const uint64_t hm = staged ? hitmask[(size_t)t * P + slot] : 0ull;
+    float3 Pa = make_float3(0.f, 0.f, 0.f), Va = make_float3(0.f, 0.f, 0.f);
+    uint64_t todo = L.bits(__ballot(staged));
+    while (__ballot(todo != 0) != 0) {
+        const bool active = todo != 0;
+        const uint32_t src = active ? (uint32_t)(__ffsll((unsigned long long)todo) - 1) : 0u;
+        const uint32_t gm = L.from(mt, src);
+        const bool member = active && staged && mt == gm;
+        todo &= ~L.bits(__ballot(member));
+        const uint32_t off = active ? (gm & 0xFFFFu) : 0u, kept = active ? (gm >> 16) : 0u;
+        const uint32_t* kj = kept_in + (size_t)gseg * kSegCap + off;
+        uint32_t kmax = kept;
+        if constexpr (W < kWave) kmax = max(kmax, (uint32_t)__shfl_xor((int)kmax, W, kWave));
+        kmax = __builtin_amdgcn_readfirstlane(kmax);
+        for (uint32_t q0 = 0; q0 < kmax; q0 += 2 * W) {
+            // two entries per lane per trip: their indices, then their state, in flight together
+            const uint32_t qa = q0 + L.ls, qb = qa + W;
+            const bool ha = qa < kept, hb = qb < kept;
+            const uint32_t ja = ha ? kj[qa] : 0u, jb = hb ? kj[qb] : 0u;
+            float4 pa, va, pb, vb;
+            if (ha) {
+                pa = spos[ja];
+                va = svel[ja];
+            }
+            if (hb) {
+                pb = spos[jb];
+                vb = svel[jb];
+            }
+            if (ha) {
+                tpos[qa] = pa;
+                tvel[qa] = va;
+            }
+            if (hb) {
+                tpos[qb] = pb;
+                tvel[qb] = vb;
+            }
+        }
+        wave_sync();
+        if (member) {
+            const uint32_t trips = (kept + P - 1) / P;
+            uint64_t bits = hm;
+            while (bits) {
+                const uint32_t b = 63u - (uint32_t)__builtin_clzll(bits);  // earliest trip first
+                bits &= ~(1ull << b);
+                const uint32_t q = (trips - 1u - b) * P + slot;
+                const float4 pj = tpos[q], vj = tvel[q];
+                const float dx = pi.x - pj.x, dy = pi.y - pj.y, dz = pi.z - pj.z;
+                pair_force(pi, vi, pj, vj, dx, dy, dz, dist2(dx, dy, dz), h, Pa, Va);
+            }
+        }
+        wave_sync();  // the next group's entries overwrite the tile
+    }
+    if (live && !staged && slot == 0) {
+        const PairSums f = force_fallback(spos, svel, cell_start, cell_end, i, n, pi, vi, h);
+        Pa = f.P;
+        Va = f.V;
+    }
+    Pa = make_float3(particle_sum<W>(Pa.x), particle_sum<W>(Pa.y), particle_sum<W>(Pa.z));
+    Va = make_float3(particle_sum<W>(Va.x), particle_sum<W>(Va.y), particle_sum<W>(Va.z));
+    if (!live || slot != 0) return;
+    const uint32_t o = caller_slot<ORDER>(t, i, porder, index);
+    if (o >= n) return;
+    float4 f = add_sph_force(force[o], Pa, Va, p);
     if (!STEP) {
         force[o] = f;
         return;
@@ -842,7 +987,7 @@ Fluid3D::~Fluid3D()
 {
     drop_graph();
     for (DeviceBuffer* b : {&pos_, &vel_, &force_, &cell_, &index_, &spos_, &svel_, &cell_start_, &cell_end_,
-                            &params_buf_, &nbr_, &ncount_, &inv_, &porder_, &mkey_})
+                            &params_buf_, &nbr_, &ncount_, &kept_, &hitmask_, &meta_, &inv_, &porder_, &mkey_})
         b->release();
 }
 
@@ -853,6 +998,10 @@ bool Fluid3D::allocate_internal()
     if (!cell_.allocate(u1) || !index_.allocate(u1) || !spos_.allocate(v4) || !svel_.allocate(v4)) return false;
     if (!cell_start_.allocate(kBuckets * 4) || !cell_end_.allocate(kBuckets * 4)) return false;
     if (!nbr_.allocate((size_t)kNbrCap * u1) || !ncount_.allocate(u1)) return false;
+    // kept entries of the tile density pass: kSegCap per 8-particle segment, 64-bit bits per lane
+    if (!kept_.allocate((u1 / 4 + 7) / 8 * kSegCap * 4) || !hitmask_.allocate(u1 / 4 * kGroupLanes * 8) ||
+        !meta_.allocate(u1))
+        return false;
     if (!inv_.allocate(u1) || !porder_.allocate(u1) || !mkey_.allocate(u1)) return false;
     porder_valid_ = false;
     steps_since_refresh_ = 0;
@@ -1162,28 +1311,59 @@ int Fluid3D::enqueue_tile_passes(hipStream_t s, int order)
     timer.begin(MFL_STAGE_DENSITY, s);
     auto* nbr = nbr_.as<uint32_t>();
     auto* ncount = ncount_.as<uint32_t>();
-    const bool list = neighbour_pass == MFL_PASS_TILE_LIST;
-#define MFL_DENSITY_TILE(O, LIST, W)                                                                             \
-    hipLaunchKernelGGL((k_density_tile<O, LIST, W>), gd, b, lds_d, s, pos, spos, svel, cs, ce, n, params, depth,  \
-                       porder, inv, nbr, ncount)
-#define MFL_DENSITY_TILE_W(O, LIST)                 \
-    if (lanes == 4) MFL_DENSITY_TILE(O, LIST, 32); \
-    else MFL_DENSITY_TILE(O, LIST, 64)
+    auto* kept = kept_.as<uint32_t>();
+    auto* hitmask = hitmask_.as<uint64_t>();
+    auto* meta = meta_.as<uint32_t>();
+    const int out = neighbour_pass == MFL_PASS_TILE_LIST ? kOutList
+                    : neighbour_pass == MFL_PASS_TILE_KEPT ? kOutKept : kOutNone;
+#define MFL_DENSITY_TILE(O, OUT, W)                                                                              \
+    hipLaunchKernelGGL((k_density_tile<O, OUT, W>), gd, b, lds_d, s, pos, spos, svel, cs, ce, n, params, depth,   \
+                       porder, inv, nbr, ncount, kept, hitmask, meta)
+#define MFL_DENSITY_TILE_W(O, OUT)                 \
+    if (lanes == 4) MFL_DENSITY_TILE(O, OUT, 32); \
+    else MFL_DENSITY_TILE(O, OUT, 64)
+#define MFL_DENSITY_TILE_OUT(O)                                       \
+    if (out == kOutList) { MFL_DENSITY_TILE_W(O, kOutList); }         \
+    else if (out == kOutKept) { MFL_DENSITY_TILE_W(O, kOutKept); }    \
+    else { MFL_DENSITY_TILE_W(O, kOutNone); }
     switch (order) {
-    case MFL_TRAVERSAL_SORTED:
-        if (list) { MFL_DENSITY_TILE_W(kOrderSorted, true); } else { MFL_DENSITY_TILE_W(kOrderSorted, false); }
-        break;
-    case MFL_TRAVERSAL_CALLER:
-        if (list) { MFL_DENSITY_TILE_W(kOrderCaller, true); } else { MFL_DENSITY_TILE_W(kOrderCaller, false); }
-        break;
-    default:
-        if (list) { MFL_DENSITY_TILE_W(kOrderListed, true); } else { MFL_DENSITY_TILE_W(kOrderListed, false); }
+    case MFL_TRAVERSAL_SORTED: MFL_DENSITY_TILE_OUT(kOrderSorted); break;
+    case MFL_TRAVERSAL_CALLER: MFL_DENSITY_TILE_OUT(kOrderCaller); break;
+    default: MFL_DENSITY_TILE_OUT(kOrderListed);
     }
+#undef MFL_DENSITY_TILE_OUT
 #undef MFL_DENSITY_TILE_W
 #undef MFL_DENSITY_TILE
     timer.end(MFL_STAGE_DENSITY, s);
-    if (list) return enqueue_list_forces(s, order);
+    if (out == kOutList) return enqueue_list_forces(s, order);
     timer.begin(MFL_STAGE_FORCES, s);
+    if (out == kOutKept) {
+        const size_t lds_k = (size_t)block_waves * (8 / lanes) * tile_depth * 2 * sizeof(float4);
+#define MFL_FORCES_KEPT(O, W, STEP)                                                                              \
+    hipLaunchKernelGGL((k_forces_kept<O, W, STEP>), gd, b, lds_k, s, spos, svel, pos, vel, force, index, cs, ce, n, \
+                       params, depth, mn, mx, gf, porder, inv, kept, hitmask, meta)
+#define MFL_FORCES_KEPT_W(O, STEP)                 \
+    if (lanes == 4) MFL_FORCES_KEPT(O, 32, STEP); \
+    else MFL_FORCES_KEPT(O, 64, STEP)
+#define MFL_FORCES_KEPT_O(STEP)                                             \
+    switch (order) {                                                        \
+    case MFL_TRAVERSAL_SORTED: MFL_FORCES_KEPT_W(kOrderSorted, STEP); break; \
+    case MFL_TRAVERSAL_CALLER: MFL_FORCES_KEPT_W(kOrderCaller, STEP); break; \
+    default: MFL_FORCES_KEPT_W(kOrderListed, STEP);                         \
+    }
+        if (integrate_cb) {
+            MFL_FORCES_KEPT_O(false);
+        } else {
+            MFL_FORCES_KEPT_O(true);
+        }
+#undef MFL_FORCES_KEPT_O
+#undef MFL_FORCES_KEPT_W
+#undef MFL_FORCES_KEPT
+        timer.end(MFL_STAGE_FORCES, s);
+        MFL_HIP(hipGetLastError());
+        if (integrate_cb) (void)integrate_cb(handle, (void*)s, integrate_user);  // return value ignored, cpp:325-328
+        return 1;
+    }
 #define MFL_FORCES_TILE(O, STEP)                                                                                  \
     hipLaunchKernelGGL((k_forces_tile<O, STEP>), g, b, lds_f, s, spos, svel, pos, vel, force, index, cs, ce, n,    \
                        params, depth, mn, mx, gf, porder, inv)
@@ -1227,8 +1407,8 @@ int Fluid3D::get_tuning(mfl_tuning* out) const
 int Fluid3D::set_tuning(const mfl_tuning& t)
 {
     if (t.neighbour_pass != MFL_PASS_LIST && t.neighbour_pass != MFL_PASS_TILE &&
-        t.neighbour_pass != MFL_PASS_TILE_LIST) {
-        set_error("neighbour_pass must be MFL_PASS_LIST, MFL_PASS_TILE or MFL_PASS_TILE_LIST");
+        t.neighbour_pass != MFL_PASS_TILE_LIST && t.neighbour_pass != MFL_PASS_TILE_KEPT) {
+        set_error("neighbour_pass must be MFL_PASS_LIST, MFL_PASS_TILE, MFL_PASS_TILE_LIST or MFL_PASS_TILE_KEPT");
         return 0;
     }
     if (t.neighbour_pass == MFL_PASS_LIST ? (t.density_lanes != 1 && t.density_lanes != 8)
@@ -1310,6 +1490,7 @@ int Fluid3D::autotune(hipStream_t s, uint32_t steps, mfl_tuning* best, double* m
             for (int lanes : {4, 8})
                 if (tile_lds_bytes(true, w, d) <= 65536 && d <= 64u * (uint32_t)lanes) {
                     cand.push_back({MFL_PASS_TILE, lanes, d, w, start.sort_items, 1u});
+                    cand.push_back({MFL_PASS_TILE_KEPT, lanes, d, w, start.sort_items, 1u});
                     for (uint32_t ch : {1u, 2u}) cand.push_back({MFL_PASS_TILE_LIST, lanes, d, w, start.sort_items, ch});
                 }
     const bool graph_was = use_graph;

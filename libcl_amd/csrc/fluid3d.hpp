@@ -58,7 +58,7 @@ public:
     int traversal = MFL_TRAVERSAL_MORTON;   // order in which the neighbour passes visit particles
     uint32_t refresh_interval = 16;         // steps between Morton order refreshes
     // launch shape (mfl_tuning); the defaults are the fastest shape measured on the 1M-particle step
-    int neighbour_pass = MFL_PASS_TILE_LIST;  // LIST, TILE or TILE_LIST neighbour passes
+    int neighbour_pass = MFL_PASS_TILE_LIST;  // LIST, TILE, TILE_LIST or TILE_KEPT neighbour passes
     int density_lanes = 4;                  // lanes per particle in the density pass
     uint32_t tile_depth = 192;              // tile passes: LDS entries per staged cell
     uint32_t block_waves = 2;               // tile passes: waves per workgroup
@@ -86,6 +86,7 @@ private:
     DeviceBuffer pos_, vel_, force_;
     DeviceBuffer cell_, index_, spos_, svel_, cell_start_, cell_end_, params_buf_;
     DeviceBuffer nbr_, ncount_;  // per-step neighbour visits recorded by the density pass
+    DeviceBuffer kept_, hitmask_, meta_;  // per-step kept entries and acceptance bits (TILE_KEPT)
     DeviceBuffer inv_;           // inverse of the sort permutation: inv[index[i]] = i
     DeviceBuffer porder_, mkey_; // Morton traversal order (caller indices) and its sort keys
     bool porder_valid_ = false;
