@@ -206,9 +206,9 @@ def main():
                     help="run one secondary leg alone and print its JSON (profiling)")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="also run a plain device copy of MIB MiB three times (counter calibration)")
-    ap.add_argument("--pass", dest="npass", default="tile_list", choices=["tile_list", "tile", "list"],
-                    help="neighbour passes: LDS-tile density + recorded-visit forces, LDS tiles for both, or "
-                         "recorded visit lists for both")
+    ap.add_argument("--pass", dest="npass", default="tile_kept", choices=["tile_kept", "tile_list", "tile", "list"],
+                    help="neighbour passes: LDS-tile density + kept-entry forces, LDS-tile density + recorded-visit "
+                         "forces, LDS tiles for both, or recorded visit lists for both")
     ap.add_argument("--tile-depth", type=int, default=192)
     ap.add_argument("--lanes", type=int, default=4, help="lanes per particle in the density pass")
     ap.add_argument("--force-chunks", type=int, default=2, help="list chunks per trip in the force pass")

@@ -504,6 +504,7 @@ __device__ __noinline__ VisitSums density_walk(int cx, int cy, int cz, float4 pi
 enum : int { kOutNone = 0, kOutList = 1, kOutKept = 2 };
 constexpr uint32_t kSegCap = 512;
 constexpr uint32_t kMetaWalk = 0xFFFFFFFFu;
+constexpr uint32_t kKeptChunk = 64;  // kept entries per segment staged at once by the force pass (multiple of 8)
 
 // Density pass over LDS-staged neighbourhoods (cl:150-200). The wave is split into 64 / W segments of
 // 8 particles (W / 8 lanes each); each segment stages its cell's neighbourhood in its own `depth + 8`
@@ -758,8 +759,8 @@ __global__ __launch_bounds__(kThreads) void k_forces_kept(const float4* __restri
     extern __shared__ float4 lds[];
     const Lanes<W> L(threadIdx.x & (kWave - 1));
     const uint32_t sid = (threadIdx.x / kWave) * SEGS + L.seg;
-    float4* tpos = lds + (size_t)sid * 2 * depth;
-    float4* tvel = tpos + depth;
+    float4* tpos = lds + (size_t)sid * 2 * kKeptChunk;
+    float4* tvel = tpos + kKeptChunk;
     const uint32_t grp = L.ls / P, slot = L.ls % P;
     const uint32_t wave = (xcd_block() * blockDim.x + threadIdx.x) / kWave;
     const uint32_t t = wave * (kWave / P) + L.seg * 8 + grp;
@@ -785,43 +786,54 @@ __global__ __launch_bounds__(kThreads) void k_forces_kept(const float4* __restri
         uint32_t kmax = kept;
         if constexpr (W < kWave) kmax = max(kmax, (uint32_t)__shfl_xor((int)kmax, W, kWave));
         kmax = __builtin_amdgcn_readfirstlane(kmax);
-        for (uint32_t q0 = 0; q0 < kmax; q0 += 2 * W) {
-            // two entries per lane per trip: their indices, then their state, in flight together
-            const uint32_t qa = q0 + L.ls, qb = qa + W;
-            const bool ha = qa < kept, hb = qb < kept;
-            const uint32_t ja = ha ? kj[qa] : 0u, jb = hb ? kj[qb] : 0u;
-            float4 pa, va, pb, vb;
-            if (ha) {
-                pa = spos[ja];
-                va = svel[ja];
+        const uint32_t trips = (kept + P - 1) / P;
+        // kKeptChunk entries at a time (small LDS per segment keeps more waves resident)
+        for (uint32_t c0 = 0; c0 < kmax; c0 += kKeptChunk) {
+            for (uint32_t q0 = 0; q0 < kKeptChunk; q0 += 2 * W) {
+                // two entries per lane per trip: their indices, then their state, in flight together
+                const uint32_t qa = q0 + L.ls, qb = qa + W;
+                const bool ha = c0 + qa < kept, hb = qb < kKeptChunk && c0 + qb < kept;
+                const uint32_t ja = ha ? kj[c0 + qa] : 0u, jb = hb ? kj[c0 + qb] : 0u;
+                float4 pa, va, pb, vb;
+                if (ha) {
+                    pa = spos[ja];
+                    va = svel[ja];
+                }
+                if (hb) {
+                    pb = spos[jb];
+                    vb = svel[jb];
+                }
+                if (ha) {
+                    tpos[qa] = pa;
+                    tvel[qa] = va;
+                }
+                if (hb) {
+                    tpos[qb] = pb;
+                    tvel[qb] = vb;
+                }
             }
-            if (hb) {
-                pb = spos[jb];
-                vb = svel[jb];
+            wave_sync();
+            if (member) {
+                // trips [c0 / P, (c0 + chunk) / P) test entries of this chunk: their bits are
+                // [trips - hi, trips - lo) of the acceptance mask (bit trips - 1 - m is trip m)
+                const uint32_t mlo = c0 / P, mhi = min((c0 + kKeptChunk) / P, trips);
+                uint64_t bits = 0;
+                if (mlo < mhi) {
+                    const uint32_t blo = trips - mhi, bhi = trips - mlo;  // bhi <= 64
+                    const uint64_t upto = bhi >= 64 ? ~0ull : (1ull << bhi) - 1ull;
+                    bits = hm & upto & ~((1ull << blo) - 1ull);
+                }
+                while (bits) {
+                    const uint32_t b = 63u - (uint32_t)__builtin_clzll(bits);  // earliest trip first
+                    bits &= ~(1ull << b);
+                    const uint32_t q = (trips - 1u - b) * P + slot - c0;
+                    const float4 pj = tpos[q], vj = tvel[q];
+                    const float dx = pi.x - pj.x, dy = pi.y - pj.y, dz = pi.z - pj.z;
+                    pair_force(pi, vi, pj, vj, dx, dy, dz, dist2(dx, dy, dz), h, Pa, Va);
+                }
             }
-            if (ha) {
-                tpos[qa] = pa;
-                tvel[qa] = va;
-            }
-            if (hb) {
-                tpos[qb] = pb;
-                tvel[qb] = vb;
-            }
+            wave_sync();  // the next chunk overwrites the tile
         }
-        wave_sync();
-        if (member) {
-            const uint32_t trips = (kept + P - 1) / P;
-            uint64_t bits = hm;
-            while (bits) {
-                const uint32_t b = 63u - (uint32_t)__builtin_clzll(bits);  // earliest trip first
-                bits &= ~(1ull << b);
-                const uint32_t q = (trips - 1u - b) * P + slot;
-                const float4 pj = tpos[q], vj = tvel[q];
-                const float dx = pi.x - pj.x, dy = pi.y - pj.y, dz = pi.z - pj.z;
-                pair_force(pi, vi, pj, vj, dx, dy, dz, dist2(dx, dy, dz), h, Pa, Va);
-            }
-        }
-        wave_sync();  // the next group's entries overwrite the tile
     }
     if (live && !staged && slot == 0) {
         const PairSums f = force_fallback(spos, svel, cell_start, cell_end, i, n, pi, vi, h);
@@ -1338,7 +1350,7 @@ int Fluid3D::enqueue_tile_passes(hipStream_t s, int order)
     if (out == kOutList) return enqueue_list_forces(s, order);
     timer.begin(MFL_STAGE_FORCES, s);
     if (out == kOutKept) {
-        const size_t lds_k = (size_t)block_waves * (8 / lanes) * tile_depth * 2 * sizeof(float4);
+        const size_t lds_k = (size_t)block_waves * (8 / lanes) * kKeptChunk * 2 * sizeof(float4);
 #define MFL_FORCES_KEPT(O, W, STEP)                                                                              \
     hipLaunchKernelGGL((k_forces_kept<O, W, STEP>), gd, b, lds_k, s, spos, svel, pos, vel, force, index, cs, ce, n, \
                        params, depth, mn, mx, gf, porder, inv, kept, hitmask, meta)

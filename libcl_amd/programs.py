@@ -214,7 +214,8 @@ class oclFluid3D:
         return t.as_dict()
 
     def setTuning(self, **kw) -> int:
-        """Changes fields of the tuning: neighbour_pass ("list" / "tile" / "tile_list"), density_lanes,
+        """Changes fields of the tuning: neighbour_pass ("list" / "tile" / "tile_list" / "tile_kept"),
+        density_lanes,
         tile_depth, block_waves, sort_items, force_chunks. Results differ between shapes only in FP32
         summation order."""
         cur = self.getTuning()

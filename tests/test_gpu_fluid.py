@@ -16,21 +16,22 @@ pytestmark = pytest.mark.gpu
 TOL = 1e-5
 
 
-# launch shapes of the neighbour passes: LDS tiles (default), tiles so shallow that dense
-# neighbourhoods overflow to the global-memory walk, and the recorded-visit-list passes
-SHAPES = {"tile_list": {}, "tile": {"neighbour_pass": "tile", "density_lanes": 8, "tile_depth": 256},
+# launch shapes of the neighbour passes: LDS tiles (tile_kept is the default), tiles so shallow that
+# dense neighbourhoods overflow to the global-memory walk, and the recorded-visit-list passes
+SHAPES = {"tile_list": {"neighbour_pass": "tile_list", "density_lanes": 4, "tile_depth": 192},
+          "tile": {"neighbour_pass": "tile", "density_lanes": 8, "tile_depth": 256},
           "tile_list8": {"neighbour_pass": "tile_list", "density_lanes": 8, "tile_depth": 256, "force_chunks": 1},
           "tile64": {"neighbour_pass": "tile", "density_lanes": 8, "tile_depth": 64, "block_waves": 2},
           "tile_list64": {"neighbour_pass": "tile_list", "density_lanes": 8, "tile_depth": 64, "block_waves": 1},
           "tile_list4_64": {"neighbour_pass": "tile_list", "density_lanes": 4, "tile_depth": 64, "block_waves": 1},
           "tile_list4_ch2": {"neighbour_pass": "tile_list", "density_lanes": 4, "force_chunks": 2},
-          "tile_kept": {"neighbour_pass": "tile_kept", "density_lanes": 4, "tile_depth": 192},
+          "tile_kept": {},
           "tile_kept8": {"neighbour_pass": "tile_kept", "density_lanes": 8, "tile_depth": 256},
           "tile_kept64": {"neighbour_pass": "tile_kept", "density_lanes": 4, "tile_depth": 64, "block_waves": 1},
           "list": {"neighbour_pass": "list"}}
 
 
-def make_fluid(pos, vel, force, params, box_min, box_max, clear=65536, tables=None, shape="tile_list"):
+def make_fluid(pos, vel, force, params, box_min, box_max, clear=65536, tables=None, shape="tile_kept"):
     import libcl_amd as L
     f = L.oclFluid3D()
     assert f.compile() == 1, L.last_error()
