@@ -37,28 +37,33 @@ STAGE_BYTES_PER_LAUNCH = {
 }
 
 
-def measured_traffic(stage: str, workload: str):
+def _profiled(name: str, workload: str, tuning: dict):
+    """A committed counter summary's table for a workload, when it was measured with the same kernels
+    (neighbour pass and lanes per particle) as `tuning`."""
+    try:
+        with open(os.path.join(ROOT, "profiles", name)) as fh:
+            table = json.load(fh)
+    except (OSError, ValueError):
+        return None
+    shape = table.get("shape", {})
+    if any(shape.get(k) != tuning.get(k) for k in ("neighbour_pass", "density_lanes")):
+        return None
+    return table.get(workload)
+
+
+def measured_traffic(stage: str, workload: str, tuning: dict):
     """HBM bytes per launch of a stage's kernel from the committed rocprofv3 counter summary
     (profiles/traffic.json: FETCH_SIZE + WRITE_SIZE per dispatch, scaled by the copy calibration);
     None when that kernel and workload were not measured."""
-    path = os.path.join(ROOT, "profiles", "traffic.json")
-    try:
-        with open(path) as fh:
-            table = json.load(fh)
-        return table[workload][stage]["bytes_per_launch"]
-    except (OSError, KeyError, ValueError):
-        return None
+    t = _profiled("traffic.json", workload, tuning)
+    return t[stage]["bytes_per_launch"] if t and stage in t else None
 
 
-def measured_valu(stage: str, workload: str):
+def measured_valu(stage: str, workload: str, tuning: dict):
     """VALU busy fraction of a stage's kernel from the committed rocprofv3 SQ counter summary
     (profiles/valu.json); None when that kernel and workload were not measured."""
-    path = os.path.join(ROOT, "profiles", "valu.json")
-    try:
-        with open(path) as fh:
-            return json.load(fh)[workload][stage]["valu_busy"]
-    except (OSError, KeyError, ValueError):
-        return None
+    t = _profiled("valu.json", workload, tuning)
+    return t[stage]["valu_busy"] if t and stage in t else None
 
 
 def cpu_baseline(sc, prm, steps: int, threads: int) -> dict:
@@ -329,8 +334,8 @@ def main():
                    "tuning": f.getTuning(), "autotuned": not args.no_autotune},
         "roofline": {"bound": "hbm", "kernel": dominant, "achieved": achieved, "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
-                     "traffic": measured_traffic(dominant, sc.name),
-                     "valu_busy": measured_valu(dominant, sc.name),
+                     "traffic": measured_traffic(dominant, sc.name, f.getTuning()),
+                     "valu_busy": measured_valu(dominant, sc.name, f.getTuning()),
                      "avg_launch_us": avg_s * 1e6, "alg_bytes_per_launch": alg_bytes},
         "stage_us": {k: v * 1e3 for k, v in stage_ms.items()},
         "step_hbm_frac": step_bytes / (ms_per_step / 1e3) / 1e9 / HBM_PEAK_GBS,
