@@ -206,6 +206,56 @@ int mfl_fluid3d_set_integrate_callback(mfl_fluid3d* f, mfl_integrate_fn fn, void
    clCalculateForces, clClipBox, clIntegrateForce, clGravity, clIntegrateVelocity. */
 int mfl_fluid3d_enqueue_kernel(mfl_fluid3d* f, const char* name, void* stream);
 
+/* ---- triangle-mesh BVH and particle collision (oclBvhTrimesh, clCollideBVH) ---------------------- */
+
+/* BVHNode (libCL/geom/oclBvhTrimesh.h:33-41, libCL/geom/oclBvhTrimesh.cl:105-113), 48 bytes: internal
+   nodes 0 .. T-2 (bit = highest differing bit of the neighbouring sorted Morton keys, trav = 2 after
+   a build), leaves T-1 .. 2T-2 (left = right = 0, bit = triangle index, trav = 0). */
+typedef struct mfl_bvh_node {
+    float bbMin[4];
+    float bbMax[4];
+    int32_t left;
+    int32_t right;
+    uint32_t bit;
+    uint32_t trav;
+} mfl_bvh_node;
+typedef struct mfl_bvh_trimesh mfl_bvh_trimesh;
+/* oclBvhTrimesh::oclBvhTrimesh / ~oclBvhTrimesh (libCL/geom/oclBvhTrimesh.cpp:19-51) */
+mfl_bvh_trimesh* mfl_bvh_trimesh_create(void);
+void mfl_bvh_trimesh_destroy(mfl_bvh_trimesh* b);
+/* oclProgram::compile: device check */
+int mfl_bvh_trimesh_compile(mfl_bvh_trimesh* b);
+/* oclBvhTrimesh::compute(device, bfVertex, bfIndex) (cpp:53-162): T = index_count / 3 triangles of
+   `index` (uint32 vertex indices, device) over `vertex` (vertex_count float4, device): global box of
+   all vertices, 30-bit Morton key of each triangle's centroid, stable 32-bit key sort, 2T-1 nodes,
+   root. Enqueues on `stream`, then waits for it and reads the root back (the reference maps
+   bfBvhRoot, cpp:156-157). Fails for T < 2 or a triangle index not below vertex_count. The global box
+   starts empty on every build (the reference's bfAABB is never initialised and accumulates). Where
+   Morton keys repeat (the reference's linking races there) ties link to the left neighbour. */
+int mfl_bvh_trimesh_compute(mfl_bvh_trimesh* b, const void* vertex, size_t vertex_count, const uint32_t* index,
+                            size_t index_count, void* stream);
+/* getNodeBuffer / getNodeCount / getRootNode (cpp:164-177): device array of mfl_bvh_node */
+int mfl_bvh_trimesh_get_node_buffer(mfl_bvh_trimesh* b, void** ptr, size_t* bytes);
+uint32_t mfl_bvh_trimesh_get_node_count(mfl_bvh_trimesh* b);
+uint32_t mfl_bvh_trimesh_get_root_node(mfl_bvh_trimesh* b);
+/* bfMortonKey / bfMortonVal after the sort (device, T entries each) */
+int mfl_bvh_trimesh_get_morton(mfl_bvh_trimesh* b, void** keys, void** vals, size_t* count);
+
+/* clCollideBVH (libCL/phys/oclFluid3D.cl:498-550) with its kernel arguments: forceOut, position,
+   velocity (n caller-order float4 each, device), param (host copy), bvh + bvhRoot, triangle (index
+   buffer) and vertex. The counts bound every index the walk reads: a node, triangle or vertex index
+   outside them ends that particle's walk without a change (the reference reads past the buffers).
+   Breadth-first walk with the reference's 256-entry ring; the first triangle that collides moves the
+   particle, zeroes its velocity and adds the contact force. */
+int mfl_collide_bvh(void* force, void* position, void* velocity, size_t n, const mfl_params* p, const void* nodes,
+                    uint32_t node_count, uint32_t root, const uint32_t* triangle, size_t index_count,
+                    const void* vertex, size_t vertex_count, void* stream);
+/* The same kernel bound to a program's position / velocity / force buffers and Params, as an
+   EVT_INTEGRATE handler enqueues it (getKernel("clCollideBVH") + the program's buffers). */
+int mfl_fluid3d_collide_bvh(mfl_fluid3d* f, const void* nodes, uint32_t node_count, uint32_t root,
+                            const uint32_t* triangle, size_t index_count, const void* vertex, size_t vertex_count,
+                            void* stream);
+
 /* ---- batched scenes (addition; no counterpart in the reference) --------------------------------- */
 
 /* Many independent oclFluid3D systems advanced together: one set of launches per step for all scenes

@@ -38,6 +38,9 @@ EXPORTS = (
     "mfl_batch_compute", "mfl_batch_get_tuning", "mfl_batch_set_tuning", "mfl_batch_autotune",
     "mfl_adjoint_create", "mfl_adjoint_destroy", "mfl_adjoint_forward", "mfl_adjoint_backward",
     "mfl_adjoint_get_buffer", "mfl_adjoint_param_grads",
+    "mfl_bvh_trimesh_create", "mfl_bvh_trimesh_destroy", "mfl_bvh_trimesh_compile", "mfl_bvh_trimesh_compute",
+    "mfl_bvh_trimesh_get_node_buffer", "mfl_bvh_trimesh_get_node_count", "mfl_bvh_trimesh_get_root_node",
+    "mfl_bvh_trimesh_get_morton", "mfl_collide_bvh", "mfl_fluid3d_collide_bvh",
 )
 MFL_ADJ_POSITION_GRAD, MFL_ADJ_VELOCITY_GRAD = 0, 1
 
@@ -156,6 +159,18 @@ def load() -> ctypes.CDLL:
         "mfl_adjoint_backward": (ctypes.c_int, [vp, vp, vp, vp]),
         "mfl_adjoint_get_buffer": (ctypes.c_int, [vp, ctypes.c_int, ctypes.POINTER(vp), ctypes.POINTER(sz)]),
         "mfl_adjoint_param_grads": (ctypes.c_int, [vp, ctypes.POINTER(ctypes.c_double)]),
+        "mfl_bvh_trimesh_create": (vp, []),
+        "mfl_bvh_trimesh_destroy": (None, [vp]),
+        "mfl_bvh_trimesh_compile": (ctypes.c_int, [vp]),
+        "mfl_bvh_trimesh_compute": (ctypes.c_int, [vp, vp, sz, vp, sz, vp]),
+        "mfl_bvh_trimesh_get_node_buffer": (ctypes.c_int, [vp, ctypes.POINTER(vp), ctypes.POINTER(sz)]),
+        "mfl_bvh_trimesh_get_node_count": (ctypes.c_uint32, [vp]),
+        "mfl_bvh_trimesh_get_root_node": (ctypes.c_uint32, [vp]),
+        "mfl_bvh_trimesh_get_morton": (ctypes.c_int, [vp, ctypes.POINTER(vp), ctypes.POINTER(vp),
+                                                      ctypes.POINTER(sz)]),
+        "mfl_collide_bvh": (ctypes.c_int, [vp, vp, vp, sz, pp, vp, ctypes.c_uint32, ctypes.c_uint32, vp, sz, vp, sz,
+                                           vp]),
+        "mfl_fluid3d_collide_bvh": (ctypes.c_int, [vp, vp, ctypes.c_uint32, ctypes.c_uint32, vp, sz, vp, sz, vp]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(L, name)

@@ -1,0 +1,446 @@
+// Triangle-mesh BVH build (libCL/geom/oclBvhTrimesh.cpp:53-162, kernels libCL/geom/oclBvhTrimesh.cl:31-297)
+// and particle-vs-mesh collision (clCollideBVH, libCL/phys/oclFluid3D.cl:365-550) for gfx950.
+//
+// Build: global box (two-level min/max reduction) -> Morton key per triangle -> 32-bit stable radix sort
+// -> node init (split bit of neighbouring keys, leaf boxes) -> one bottom-up pass that links the tree
+// and unites the boxes. The reference links each internal node by scanning left and right for the
+// nearest larger split bit (O(T) per node) and then walks every leaf from the root to unite the boxes;
+// here every leaf climbs once: a node covering leaves [l, r] is the child of split l-1 or split r,
+// whichever has the smaller split bit (ties: r), the first of the two children to arrive stops and the
+// second unites the boxes and climbs on. The tree is the one the reference's scans define (the
+// Cartesian tree of the split bits); where split bits tie, which only duplicate keys produce and where
+// the reference's parallel writes race, the left neighbour counts as the larger, which keeps the tree
+// well formed.
+//
+// Definitions the OpenCL sources leave to the implementation (the oracle uses the same ones):
+// min(x, y) = y < x ? y : x, max(x, y) = x < y ? y : x; dot(a, b) = ((a.x b.x + a.y b.y) + a.z b.z) +
+// a.w b.w; normalize(v) = v / sqrt(dot(v, v)) (v when zero); fast_length(v) = sqrt(dot(v, v)); float
+// to uint conversion truncates and saturates with NaN -> 0; the global box starts empty on every build.
+#include "bvh.hpp"
+
+#include <algorithm>
+#include <cfloat>
+#include <string>
+
+namespace mfl {
+namespace {
+
+constexpr uint32_t kBlock = 256;
+constexpr uint32_t kBoxBlocks = 512;           // partial boxes of the global-box reduction
+constexpr uint32_t kNoBound = 0xFFFFFFFFu;     // bound[] before the first child arrives
+constexpr uint32_t kQueue = 256;               // QUEUE_SIZE, libCL/phys/oclFluid3D.cl:378
+
+__device__ inline float cl_min(float x, float y) { return y < x ? y : x; }
+__device__ inline float cl_max(float x, float y) { return x < y ? y : x; }
+__device__ inline float4 min4(float4 a, float4 b)
+{
+    return make_float4(cl_min(a.x, b.x), cl_min(a.y, b.y), cl_min(a.z, b.z), cl_min(a.w, b.w));
+}
+__device__ inline float4 max4(float4 a, float4 b)
+{
+    return make_float4(cl_max(a.x, b.x), cl_max(a.y, b.y), cl_max(a.z, b.z), cl_max(a.w, b.w));
+}
+__device__ inline float4 add4(float4 a, float4 b) { return make_float4(a.x + b.x, a.y + b.y, a.z + b.z, a.w + b.w); }
+__device__ inline float4 sub4(float4 a, float4 b) { return make_float4(a.x - b.x, a.y - b.y, a.z - b.z, a.w - b.w); }
+__device__ inline float4 scale4(float4 a, float s) { return make_float4(a.x * s, a.y * s, a.z * s, a.w * s); }
+__device__ inline float dot4(float4 a, float4 b) { return ((a.x * b.x + a.y * b.y) + a.z * b.z) + a.w * b.w; }
+__device__ inline float4 cross4(float4 a, float4 b)
+{
+    return make_float4(a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x, 0.f);
+}
+__device__ inline float4 normalize4(float4 v)
+{
+    if (v.x == 0.f && v.y == 0.f && v.z == 0.f && v.w == 0.f) return v;
+    const float l = sqrtf(dot4(v, v));
+    return make_float4(v.x / l, v.y / l, v.z / l, v.w / l);
+}
+__device__ inline uint32_t to_uint(float q)
+{
+    if (!(q == q) || q <= -1.f) return 0u;
+    if (q >= 4294967296.f) return 0xFFFFFFFFu;
+    return (uint32_t)q;
+}
+__device__ inline float4 empty_min() { return make_float4(FLT_MAX, FLT_MAX, FLT_MAX, 0.f); }   // cl:36
+__device__ inline float4 empty_max() { return make_float4(-FLT_MAX, -FLT_MAX, -FLT_MAX, 0.f); }
+
+// code(), libCL/geom/oclBvhTrimesh.cl:78-85: the low 10 bits of x spread to every third bit
+__device__ inline uint32_t morton_code(uint32_t x)
+{
+    x = (x | (x << 16)) & 0x030000FFu;
+    x = (x | (x << 8)) & 0x0300F00Fu;
+    x = (x | (x << 4)) & 0x030C30C3u;
+    x = (x | (x << 2)) & 0x09249249u;
+    return x;
+}
+
+// ---- global box (clAABB, cl:31-72) -----------------------------------------------------------------
+
+__device__ inline void block_minmax(float4& mn, float4& mx, float4* smin, float4* smax)
+{
+    const uint32_t t = threadIdx.x;
+    smin[t] = mn;
+    smax[t] = mx;
+    __syncthreads();
+    for (uint32_t s = kBlock / 2; s > 0; s >>= 1) {
+        if (t < s) {
+            smin[t] = min4(smin[t], smin[t + s]);
+            smax[t] = max4(smax[t], smax[t + s]);
+        }
+        __syncthreads();
+    }
+    mn = smin[0];
+    mx = smax[0];
+}
+
+__global__ __launch_bounds__(kBlock) void k_box_partial(const float4* __restrict__ vertex, uint32_t nv,
+                                                       float4* __restrict__ part, uint32_t* __restrict__ info)
+{
+    __shared__ float4 smin[kBlock], smax[kBlock];
+    if (blockIdx.x == 0 && threadIdx.x == 0) info[1] = 0u;  // error word of this build
+    float4 mn = empty_min(), mx = empty_max();
+    for (uint32_t i = blockIdx.x * kBlock + threadIdx.x; i < nv; i += gridDim.x * kBlock) {
+        const float4 v = vertex[i];
+        mn = min4(mn, v);
+        mx = max4(mx, v);
+    }
+    block_minmax(mn, mx, smin, smax);
+    if (threadIdx.x == 0) {
+        part[2 * blockIdx.x] = mn;
+        part[2 * blockIdx.x + 1] = mx;
+    }
+}
+
+__global__ __launch_bounds__(kBlock) void k_box_final(float4* __restrict__ part, uint32_t nparts)
+{
+    __shared__ float4 smin[kBlock], smax[kBlock];
+    float4 mn = empty_min(), mx = empty_max();
+    for (uint32_t i = threadIdx.x; i < nparts; i += kBlock) {
+        mn = min4(mn, part[2 * i]);
+        mx = max4(mx, part[2 * i + 1]);
+    }
+    block_minmax(mn, mx, smin, smax);
+    if (threadIdx.x == 0) {
+        part[2 * kBoxBlocks] = mn;  // the global box follows the partials
+        part[2 * kBoxBlocks + 1] = mx;
+    }
+}
+
+// ---- Morton keys (clMorton, cl:87-99) ----------------------------------------------------------------
+
+__device__ inline bool triangle_vertices(const uint32_t* __restrict__ index, const float4* __restrict__ vertex,
+                                         uint32_t nv, uint32_t tri, float4& v0, float4& v1, float4& v2)
+{
+    const uint32_t i0 = index[3 * (size_t)tri], i1 = index[3 * (size_t)tri + 1], i2 = index[3 * (size_t)tri + 2];
+    const bool ok = i0 < nv && i1 < nv && i2 < nv;
+    v0 = vertex[ok ? i0 : 0u];
+    v1 = vertex[ok ? i1 : 0u];
+    v2 = vertex[ok ? i2 : 0u];
+    return ok;
+}
+
+__global__ __launch_bounds__(kBlock) void k_morton(const uint32_t* __restrict__ index, const float4* __restrict__ vertex,
+                                                  uint32_t nv, uint32_t ntri, const float4* __restrict__ box,
+                                                  uint32_t* __restrict__ key, uint32_t* __restrict__ info)
+{
+    const uint32_t g = blockIdx.x * kBlock + threadIdx.x;
+    if (g >= ntri) return;
+    float4 v0, v1, v2;
+    if (!triangle_vertices(index, vertex, nv, g, v0, v1, v2)) info[1] = 1u;  // index out of range
+    const float4 mn = box[0], mx = box[1];
+    const float4 s = add4(add4(v0, v1), v2);
+    const float nx = (s.x / 3.0f - mn.x) / (mx.x - mn.x);
+    const float ny = (s.y / 3.0f - mn.y) / (mx.y - mn.y);
+    const float nz = (s.z / 3.0f - mn.z) / (mx.z - mn.z);
+    key[g] = morton_code(to_uint(nx * 1023.0f)) | (morton_code(to_uint(ny * 1023.0f)) << 1) |
+             (morton_code(to_uint(nz * 1023.0f)) << 2);
+}
+
+// ---- nodes (clCreateNodes cl:115-142, clCreateLeaves cl:198-237, leaf boxes cl:255-271) ----------------
+
+// the highest of bits 29..0 where the keys differ, 0 if none
+__device__ inline uint32_t split_bit(uint32_t k0, uint32_t k1)
+{
+    const uint32_t x = (k0 ^ k1) & 0x3FFFFFFFu;
+    return x ? 31u - (uint32_t)__clz((int)x) : 0u;
+}
+
+__global__ __launch_bounds__(kBlock) void k_nodes(const uint32_t* __restrict__ key, const uint32_t* __restrict__ val,
+                                                 const uint32_t* __restrict__ index, const float4* __restrict__ vertex,
+                                                 uint32_t nv, uint32_t T, BvhNode* __restrict__ nodes,
+                                                 uint32_t* __restrict__ split, uint32_t* __restrict__ bound)
+{
+    const uint32_t g = blockIdx.x * kBlock + threadIdx.x;
+    if (g >= T) return;
+    if (g + 1 < T) {
+        const uint32_t b = split_bit(key[g], key[g + 1]);
+        BvhNode d;
+        d.bbMin = empty_min();
+        d.bbMax = empty_max();
+        d.left = -1;
+        d.right = -1;
+        d.bit = b;
+        d.trav = 0u;
+        nodes[g] = d;
+        split[g] = b;
+        bound[g] = kNoBound;
+    }
+    const uint32_t tri = val[g];
+    float4 v0, v1, v2;
+    triangle_vertices(index, vertex, nv, tri, v0, v1, v2);
+    BvhNode leaf;
+    leaf.bbMin = min4(min4(min4(empty_min(), v0), v1), v2);
+    leaf.bbMax = max4(max4(max4(empty_max(), v0), v1), v2);
+    leaf.left = 0;
+    leaf.right = 0;
+    leaf.bit = tri;
+    leaf.trav = 0u;
+    nodes[T - 1 + g] = leaf;
+}
+
+// ---- bottom-up link + boxes (clLinkNodes cl:144-196, clComputeAABBs cl:239-297) ------------------------
+
+__global__ __launch_bounds__(kBlock) void k_link_up(BvhNode* __restrict__ nodes, const uint32_t* __restrict__ split,
+                                                   uint32_t* __restrict__ bound, uint32_t T, uint32_t* __restrict__ info)
+{
+    const uint32_t g = blockIdx.x * kBlock + threadIdx.x;
+    if (g >= T) return;
+    uint32_t cur = T - 1 + g, l = g, r = g;
+    for (;;) {
+        // parent: split r (this node is its left child) or split l-1 (right child); the smaller split
+        // bit is the lower node, ties go to r
+        const bool to_r = l == 0u || (r != T - 1u && !(split[l - 1] < split[r]));
+        const uint32_t p = to_r ? r : l - 1u;
+        if (to_r)
+            nodes[p].left = (int32_t)cur;
+        else
+            nodes[p].right = (int32_t)cur;
+        // publish this child (its box and the pointer above) and its outer bound; the second arrival
+        // gets the sibling's bound back and owns the parent
+        const uint32_t other =
+            __hip_atomic_exchange(&bound[p], to_r ? l : r, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
+        if (other == kNoBound) return;
+        if (to_r)
+            r = other;
+        else
+            l = other;
+        const int32_t lc = nodes[p].left, rc = nodes[p].right;
+        if (lc < 0 || rc < 0 || (uint32_t)lc > 2u * (T - 1u) || (uint32_t)rc > 2u * (T - 1u)) {
+            info[1] = 2u;  // a child link is missing: report instead of reading outside the nodes
+            return;
+        }
+        nodes[p].bbMin = min4(nodes[lc].bbMin, nodes[rc].bbMin);
+        nodes[p].bbMax = max4(nodes[lc].bbMax, nodes[rc].bbMax);
+        nodes[p].trav = 2u;  // cl:282-286: the first of the two arrivals leaves 2
+        cur = p;
+        if (l == 0u && r == T - 1u) {
+            info[0] = p;
+            return;
+        }
+    }
+}
+
+// ---- collision (clCollideBVH, libCL/phys/oclFluid3D.cl:365-550) ----------------------------------------
+
+// collideTriangle, cl:380-461
+__device__ inline bool collide_triangle(float4 V0, float4 V1, float4 V2, float4& pos, float4& vel, float4& force,
+                                        const mfl_params& p)
+{
+    V0.w = 0.f;
+    V1.w = 0.f;
+    V2.w = 0.f;
+    pos.w = 0.f;
+    const float4 u = sub4(V1, V0), v = sub4(V2, V0);
+    float4 n = cross4(u, v);
+    if (dot4(n, n) == 0.f) return false;
+    const float4 w0 = sub4(pos, V0);
+    const float4 disp = scale4(vel, p.deltaTime);
+    const float a = dot4(n, w0);
+    if (a < 0.f) return false;                        // other side of the triangle
+    const float b = dot4(n, disp);
+    if ((double)fabsf(b) < 0.000001) return false;    // parallel to the plane (a double literal)
+    const float r = -a / b;
+    if (r < 0.f) return false;                        // moving away
+    const float4 I = add4(pos, scale4(disp, r));
+    const float uu = dot4(u, u), uv = dot4(u, v), vv = dot4(v, v);
+    const float4 w = sub4(I, V0);
+    const float wu = dot4(w, u), wv = dot4(w, v);
+    const float D = uv * uv - uu * vv;
+    const float s = (uv * wv - vv * wu) / D;
+    if (s < 0.f || s > 1.f) return false;
+    const float t = (uv * wu - uu * wv) / D;
+    if (t < 0.f || (s + t) > 1.f) return false;
+    n = normalize4(n);
+    const float4 Vc = sub4(add4(pos, disp), V0);
+    const float dist = dot4(Vc, n);
+    const float depth = p.particleRadius - dist;
+    if (!(depth > 0.f)) return false;
+    const float4 dPos = scale4(scale4(normalize4(disp), dot4(n, disp)), depth);
+    const float4 dTangent = sub4(vel, scale4(n, dot4(n, vel)));
+    const float coef = p.pressureConstant / p.density;
+    const float hd = p.cellSize - dist;
+    const float4 pv = scale4(scale4(n, hd), hd);      // PressureVariable, cl:125
+    force = add4(force, make_float4(dTangent.x / p.deltaTime, dTangent.y / p.deltaTime, dTangent.z / p.deltaTime,
+                                    dTangent.w / p.deltaTime));
+    force = add4(force, scale4(scale4(pv, coef), p.mass));
+    pos = add4(pos, dPos);
+    vel = make_float4(0.f, 0.f, 0.f, 0.f);
+    return true;
+}
+
+// intersectBox (cl:464-484) + collideAABB (cl:486-496); only x, y, z reach the result
+__device__ inline bool collide_aabb(const BvhNode* __restrict__ node, float4 ro, float4 rd, float radius)
+{
+    const float4 bmin = node->bbMin, bmax = node->bbMax;
+    const float4 nd = normalize4(rd);
+    const float ix = 1.0f / nd.x, iy = 1.0f / nd.y, iz = 1.0f / nd.z;
+    const float bx = ix * (bmin.x - ro.x), by = iy * (bmin.y - ro.y), bz = iz * (bmin.z - ro.z);
+    const float tx = ix * (bmax.x - ro.x), ty = iy * (bmax.y - ro.y), tz = iz * (bmax.z - ro.z);
+    const float mnx = cl_min(tx, bx), mny = cl_min(ty, by), mnz = cl_min(tz, bz);
+    const float mxx = cl_max(tx, bx), mxy = cl_max(ty, by), mxz = cl_max(tz, bz);
+    const float largest_tmin = cl_max(cl_max(mnx, mny), cl_max(mnx, mnz));
+    const float smallest_tmax = cl_min(cl_min(mxx, mxy), cl_min(mxx, mxz));
+    if (!(smallest_tmax > largest_tmin)) return false;
+    return largest_tmin < cl_max(radius, sqrtf(dot4(rd, rd)));
+}
+
+// One lane per particle, breadth-first walk with the reference's 256-entry ring (a full ring makes
+// head == tail and ends the walk, as in the reference); the first triangle that collides ends it.
+__global__ __launch_bounds__(kBlock) void k_collide_bvh(float4* __restrict__ force, float4* __restrict__ pos,
+                                                       float4* __restrict__ vel, uint32_t n, mfl_params p,
+                                                       const BvhNode* __restrict__ nodes, uint32_t nnodes, uint32_t root,
+                                                       const uint32_t* __restrict__ tri, uint32_t ntri,
+                                                       const float4* __restrict__ vertex, uint32_t nv)
+{
+    const uint32_t q = blockIdx.x * kBlock + threadIdx.x;
+    if (q >= n) return;
+    float4 P = pos[q], V = vel[q], F = force[q];
+    uint32_t queue[kQueue];
+    uint32_t head = 0, tail = 1;
+    queue[0] = root;
+    while (head != tail) {
+        const uint32_t it = queue[head];
+        head = (head + 1u) % kQueue;
+        if (it >= nnodes) break;
+        const int4 link = *reinterpret_cast<const int4*>(&nodes[it].left);
+        const uint32_t left = (uint32_t)link.x, right = (uint32_t)link.y;
+        if (left == right) {
+            const uint32_t t = (uint32_t)link.z;
+            if (t >= ntri) break;
+            float4 v0, v1, v2;
+            if (!triangle_vertices(tri, vertex, nv, t, v0, v1, v2)) break;
+            if (collide_triangle(v0, v1, v2, P, V, F, p)) {
+                pos[q] = P;
+                vel[q] = V;
+                force[q] = F;
+                break;
+            }
+        } else {
+            if (left >= nnodes || right >= nnodes) break;
+            if (collide_aabb(&nodes[left], P, V, p.particleRadius)) {
+                queue[tail] = left;
+                tail = (tail + 1u) % kQueue;
+            }
+            if (collide_aabb(&nodes[right], P, V, p.particleRadius)) {
+                queue[tail] = right;
+                tail = (tail + 1u) % kQueue;
+            }
+        }
+    }
+}
+
+inline uint32_t blocks_for(size_t n) { return (uint32_t)((n + kBlock - 1) / kBlock); }
+
+}  // namespace
+
+int BvhTrimesh::compile()
+{
+    if (!sort_.compile()) return 0;
+    compiled_ = true;
+    return 1;
+}
+
+int BvhTrimesh::compute(const float4* vertex, size_t vertex_count, const uint32_t* index, size_t index_count,
+                        hipStream_t s)
+{
+    if (!compiled_) {
+        set_error("oclBvhTrimesh: compile() first");
+        return 0;
+    }
+    const size_t T = index_count / 3;  // cpp:56
+    if (T < 2) {
+        set_error("oclBvhTrimesh: at least 2 triangles are needed (index count " + std::to_string(index_count) + ")");
+        return 0;
+    }
+    if (!vertex || !index || vertex_count == 0) {
+        set_error("oclBvhTrimesh: null vertex or index buffer");
+        return 0;
+    }
+    if (T > (1u << 30) || vertex_count > 0xFFFFFFFFull) {
+        set_error("oclBvhTrimesh: mesh too large");
+        return 0;
+    }
+    const uint32_t t = (uint32_t)T, nv = (uint32_t)vertex_count;
+    if (!part_.fit((2 * kBoxBlocks + 2) * sizeof(float4)) || !keys_.fit(T * 4) || !vals_.fit(T * 4) ||
+        !nodes_.fit((2 * T - 1) * sizeof(BvhNode)) || !split_.fit(T * 4) || !bound_.fit(T * 4) ||
+        !info_.fit(2 * sizeof(uint32_t)) || !sort_.reserve(T, 32))
+        return 0;
+    float4* part = part_.as<float4>();
+    uint32_t* info = info_.as<uint32_t>();
+    const uint32_t box_blocks = std::min<uint32_t>(kBoxBlocks, std::max<uint32_t>(1u, blocks_for(nv)));
+    hipLaunchKernelGGL(k_box_partial, dim3(box_blocks), dim3(kBlock), 0, s, vertex, nv, part, info);
+    hipLaunchKernelGGL(k_box_final, dim3(1), dim3(kBlock), 0, s, part, box_blocks);
+    hipLaunchKernelGGL(k_morton, dim3(blocks_for(T)), dim3(kBlock), 0, s, index, vertex, nv, t, part + 2 * kBoxBlocks,
+                       keys_.as<uint32_t>(), info);
+    MFL_HIP(hipGetLastError());
+    // cpp:108-112: stable sort of all 32 key bits; the values start as 0..T-1
+    if (!sort_.compute(keys_.as<uint32_t>(), vals_.as<uint32_t>(), T, 0, 32, s, true)) return 0;
+    hipLaunchKernelGGL(k_nodes, dim3(blocks_for(T)), dim3(kBlock), 0, s, keys_.as<uint32_t>(), vals_.as<uint32_t>(),
+                       index, vertex, nv, t, nodes(), split_.as<uint32_t>(), bound_.as<uint32_t>());
+    hipLaunchKernelGGL(k_link_up, dim3(blocks_for(T)), dim3(kBlock), 0, s, nodes(), split_.as<uint32_t>(),
+                       bound_.as<uint32_t>(), t, info);
+    MFL_HIP(hipGetLastError());
+    uint32_t host[2] = {0, 0};
+    MFL_HIP(hipMemcpyAsync(host, info, sizeof(host), hipMemcpyDeviceToHost, s));
+    MFL_HIP(hipStreamSynchronize(s));
+    if (host[1] == 1u) {
+        set_error("oclBvhTrimesh: a triangle index is not below the vertex count");
+        return 0;
+    }
+    if (host[1] != 0u) {
+        set_error("oclBvhTrimesh: the tree links are incomplete");
+        return 0;
+    }
+    if (host[0] >= 2 * T - 1) {
+        set_error("oclBvhTrimesh: tree root out of range");
+        return 0;
+    }
+    count_ = (uint32_t)(2 * T - 1);
+    root_ = host[0];
+    return 1;
+}
+
+int collide_bvh(float4* force, float4* pos, float4* vel, size_t n, const mfl_params& p, const BvhNode* nodes,
+                uint32_t node_count, uint32_t root, const uint32_t* triangle, size_t index_count, const float4* vertex,
+                size_t vertex_count, hipStream_t s)
+{
+    if (n == 0) return 1;
+    if (!force || !pos || !vel || !nodes || !triangle || !vertex) {
+        set_error("clCollideBVH: null buffer");
+        return 0;
+    }
+    if (root >= node_count) {
+        set_error("clCollideBVH: root " + std::to_string(root) + " is not below the node count " +
+                  std::to_string(node_count));
+        return 0;
+    }
+    if (n > 0xFFFFFFFFull || vertex_count > 0xFFFFFFFFull || index_count / 3 > 0xFFFFFFFFull) {
+        set_error("clCollideBVH: sizes beyond 32-bit indices");
+        return 0;
+    }
+    hipLaunchKernelGGL(k_collide_bvh, dim3(blocks_for(n)), dim3(kBlock), 0, s, force, pos, vel, (uint32_t)n, p, nodes,
+                       node_count, root, triangle, (uint32_t)(index_count / 3), vertex, (uint32_t)vertex_count);
+    MFL_HIP(hipGetLastError());
+    return 1;
+}
+
+}  // namespace mfl

@@ -5,6 +5,7 @@
 
 #include "adjoint.hpp"
 #include "batch.hpp"
+#include "bvh.hpp"
 #include "fluid3d.hpp"
 #include "radix_sort.hpp"
 
@@ -19,6 +20,10 @@ struct mfl_fluid3d {
 
 struct mfl_radixsort {
     mfl::RadixSort impl;
+};
+
+struct mfl_bvh_trimesh {
+    mfl::BvhTrimesh impl;
 };
 
 struct mfl_adjoint {
@@ -483,6 +488,94 @@ int mfl_adjoint_param_grads(mfl_adjoint* a, double* out4)
 {
     if (!need(a, "adjoint") || !need(out4, "output")) return 0;
     return guarded([&] { return a->impl.param_grads(out4); });
+}
+
+}  // extern "C"
+
+// ---- triangle-mesh BVH and collision ------------------------------------------------------------------
+
+extern "C" {
+
+mfl_bvh_trimesh* mfl_bvh_trimesh_create(void)
+{
+    try {
+        return new mfl_bvh_trimesh();
+    } catch (...) {
+        mfl::set_error("out of host memory");
+        return nullptr;
+    }
+}
+
+void mfl_bvh_trimesh_destroy(mfl_bvh_trimesh* b) { delete b; }
+
+int mfl_bvh_trimesh_compile(mfl_bvh_trimesh* b)
+{
+    if (!need(b, "BVH")) return 0;
+    return guarded([&] { return b->impl.compile(); });
+}
+
+int mfl_bvh_trimesh_compute(mfl_bvh_trimesh* b, const void* vertex, size_t vertex_count, const uint32_t* index,
+                            size_t index_count, void* stream)
+{
+    if (!need(b, "BVH")) return 0;
+    return guarded([&] {
+        return b->impl.compute(static_cast<const float4*>(vertex), vertex_count, index, index_count,
+                               (hipStream_t)stream);
+    });
+}
+
+int mfl_bvh_trimesh_get_node_buffer(mfl_bvh_trimesh* b, void** ptr, size_t* bytes)
+{
+    if (!need(b, "BVH")) return 0;
+    if (ptr) *ptr = b->impl.nodes();
+    if (bytes) *bytes = (size_t)b->impl.node_count() * sizeof(mfl_bvh_node);
+    return 1;
+}
+
+uint32_t mfl_bvh_trimesh_get_node_count(mfl_bvh_trimesh* b) { return b ? b->impl.node_count() : 0u; }
+
+uint32_t mfl_bvh_trimesh_get_root_node(mfl_bvh_trimesh* b) { return b ? b->impl.root() : 0u; }
+
+int mfl_bvh_trimesh_get_morton(mfl_bvh_trimesh* b, void** keys, void** vals, size_t* count)
+{
+    if (!need(b, "BVH")) return 0;
+    if (keys) *keys = b->impl.morton_keys();
+    if (vals) *vals = b->impl.morton_vals();
+    if (count) *count = b->impl.node_count() ? (b->impl.node_count() + 1u) / 2u : 0u;
+    return 1;
+}
+
+int mfl_collide_bvh(void* force, void* position, void* velocity, size_t n, const mfl_params* p, const void* nodes,
+                    uint32_t node_count, uint32_t root, const uint32_t* triangle, size_t index_count,
+                    const void* vertex, size_t vertex_count, void* stream)
+{
+    if (!p) {
+        mfl::set_error("null params");
+        return 0;
+    }
+    return guarded([&] {
+        return mfl::collide_bvh(static_cast<float4*>(force), static_cast<float4*>(position),
+                                static_cast<float4*>(velocity), n, *p, static_cast<const mfl::BvhNode*>(nodes),
+                                node_count, root, triangle, index_count, static_cast<const float4*>(vertex),
+                                vertex_count, (hipStream_t)stream);
+    });
+}
+
+int mfl_fluid3d_collide_bvh(mfl_fluid3d* f, const void* nodes, uint32_t node_count, uint32_t root,
+                            const uint32_t* triangle, size_t index_count, const void* vertex, size_t vertex_count,
+                            void* stream)
+{
+    if (!need(f, "fluid")) return 0;
+    return guarded([&] {
+        void *pos = nullptr, *vel = nullptr, *force = nullptr;
+        if (!f->impl.get_buffer(MFL_POSITION, &pos, nullptr) || !f->impl.get_buffer(MFL_VELOCITY, &vel, nullptr) ||
+            !f->impl.get_buffer(MFL_FORCE, &force, nullptr))
+            return 0;
+        return mfl::collide_bvh(static_cast<float4*>(force), static_cast<float4*>(pos), static_cast<float4*>(vel),
+                                f->impl.particle_count(), f->impl.params, static_cast<const mfl::BvhNode*>(nodes),
+                                node_count, root, triangle, index_count, static_cast<const float4*>(vertex),
+                                vertex_count, (hipStream_t)stream);
+    });
 }
 
 }  // extern "C"

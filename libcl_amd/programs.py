@@ -1,7 +1,7 @@
-"""Python mirror of libCL's program objects on the HIP path: oclRadixSort and oclFluid3D.
+"""Python mirror of libCL's program objects on the HIP path: oclRadixSort, oclFluid3D, oclBvhTrimesh.
 
 Names, argument meaning and the int (1 ok / 0 fail) return convention follow
-libCL/sort/oclRadixSort.h:26-28 and libCL/phys/oclFluid3D.h:25-74; failures leave their text in
+libCL/sort/oclRadixSort.h:26-28, libCL/phys/oclFluid3D.h:25-74 and libCL/geom/oclBvhTrimesh.h:21-82; failures leave their text in
 last_error() (the reference logs through Log(ERR), libCL/oclCommon.cpp:146-162). Device memory is
 torch tensors (plumbing only); every call goes through libmfl.so's C ABI.
 """
@@ -278,6 +278,16 @@ class oclFluid3D:
         """getKernel(name) + clEnqueueNDRangeKernel with the bound arguments."""
         return int(N.load().mfl_fluid3d_enqueue_kernel(self._h, name.encode(), _stream_ptr(stream)))
 
+    def collideBVH(self, bvh: "oclBvhTrimesh", triangle, vertex, stream=None) -> int:
+        """clCollideBVH (cl:498-550) on this program's position / velocity / force buffers and Params,
+        against `bvh` built over (triangle indices, vertices): what an EVT_INTEGRATE handler enqueues."""
+        _f4_tensor(vertex, "vertex")
+        _u32_tensor(triangle, "triangle")
+        ptr, _ = bvh.getNodeBuffer()
+        return int(N.load().mfl_fluid3d_collide_bvh(self._h, ptr, bvh.getNodeCount(), bvh.getRootNode(),
+                                                    triangle.data_ptr(), triangle.numel(), vertex.data_ptr(),
+                                                    vertex.numel() // 4, _stream_ptr(stream)))
+
     # ---- host helpers ----
     def upload(self, pos: np.ndarray, vel: np.ndarray, force: np.ndarray) -> int:
         ok = self.bfPosition.write(np.asarray(pos, np.float32))
@@ -296,6 +306,104 @@ class oclFluid3D:
                     spos=self.bfSortedPosition.read(np.float32, (-1, 4))[:n],
                     svel=self.bfSortedVelocity.read(np.float32, (-1, 4))[:n],
                     cell_start=self.bfCellStart.read(np.uint32), cell_end=self.bfCellEnd.read(np.uint32))
+
+
+# BVHNode, libCL/geom/oclBvhTrimesh.h:33-41 (mfl_bvh_node, 48 bytes)
+BVH_NODE = np.dtype([("bbMin", "<f4", (4,)), ("bbMax", "<f4", (4,)), ("left", "<i4"), ("right", "<i4"),
+                     ("bit", "<u4"), ("trav", "<u4")])
+
+
+def _u32_tensor(t, what: str):
+    import torch
+    if t.dtype not in (torch.int32, torch.uint32) or not t.is_contiguous() or not t.is_cuda:
+        raise TypeError(f"{what} must be a contiguous 32-bit integer device tensor")
+
+
+def _f4_tensor(t, what: str):
+    import torch
+    if t.dtype != torch.float32 or not t.is_contiguous() or not t.is_cuda or t.numel() % 4:
+        raise TypeError(f"{what} must be a contiguous float32 device tensor of float4 rows")
+
+
+class oclBvhTrimesh:
+    """libCL/geom/oclBvhTrimesh.h:21-82 over libmfl's gfx950 BVH build."""
+
+    def __init__(self):
+        L = N.load()
+        self._h = L.mfl_bvh_trimesh_create()
+        if not self._h:
+            raise RuntimeError(N.last_error())
+        self._mesh = None
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h and N is not None and N._lib is not None:
+            N._lib.mfl_bvh_trimesh_destroy(h)
+            self._h = None
+
+    def compile(self) -> int:
+        return int(N.load().mfl_bvh_trimesh_compile(self._h))
+
+    def compute(self, vertex, index, stream=None) -> int:
+        """oclBvhTrimesh::compute(device, bfVertex, bfIndex) (cpp:53-162): vertex is a [V, 4] float32
+        device tensor, index a uint32 device tensor of 3 vertex indices per triangle."""
+        _f4_tensor(vertex, "vertex")
+        _u32_tensor(index, "index")
+        ok = int(N.load().mfl_bvh_trimesh_compute(self._h, vertex.data_ptr(), vertex.numel() // 4, index.data_ptr(),
+                                                  index.numel(), _stream_ptr(stream)))
+        if ok:
+            self._mesh = (vertex, index)
+        return ok
+
+    def getNodeBuffer(self):
+        """(device pointer, bytes) of the mfl_bvh_node array (getNodeBuffer, cpp:169-172)."""
+        ptr, nbytes = ctypes.c_void_p(), ctypes.c_size_t()
+        if not N.load().mfl_bvh_trimesh_get_node_buffer(self._h, ctypes.byref(ptr), ctypes.byref(nbytes)):
+            raise RuntimeError(N.last_error())
+        return ptr.value or 0, nbytes.value
+
+    def getNodeCount(self) -> int:
+        return int(N.load().mfl_bvh_trimesh_get_node_count(self._h))
+
+    def getRootNode(self) -> int:
+        return int(N.load().mfl_bvh_trimesh_get_root_node(self._h))
+
+    def nodes(self) -> np.ndarray:
+        """Host copy of the nodes as a BVH_NODE structured array."""
+        ptr, nbytes = self.getNodeBuffer()
+        out = np.zeros(nbytes // BVH_NODE.itemsize, BVH_NODE)
+        if nbytes and not N.load().mfl_memcpy(out.ctypes.data, ptr, nbytes):
+            raise RuntimeError(N.last_error())
+        return out
+
+    def morton(self):
+        """Host copies of the sorted Morton keys and triangle indices (bfMortonKey, bfMortonVal)."""
+        k, v, cnt = ctypes.c_void_p(), ctypes.c_void_p(), ctypes.c_size_t()
+        if not N.load().mfl_bvh_trimesh_get_morton(self._h, ctypes.byref(k), ctypes.byref(v), ctypes.byref(cnt)):
+            raise RuntimeError(N.last_error())
+        keys = np.zeros(cnt.value, np.uint32)
+        vals = np.zeros(cnt.value, np.uint32)
+        if cnt.value:
+            N.load().mfl_memcpy(keys.ctypes.data, k.value, keys.nbytes)
+            N.load().mfl_memcpy(vals.ctypes.data, v.value, vals.nbytes)
+        return keys, vals
+
+
+def collide_bvh(force, position, velocity, params: N.Params, bvh: oclBvhTrimesh, triangle, vertex,
+                stream=None) -> int:
+    """clCollideBVH (libCL/phys/oclFluid3D.cl:498-550) on float4 device tensors force / position /
+    velocity with `bvh`'s nodes and root over the mesh (triangle indices, vertices)."""
+    for t, w in ((force, "force"), (position, "position"), (velocity, "velocity"), (vertex, "vertex")):
+        _f4_tensor(t, w)
+    _u32_tensor(triangle, "triangle")
+    n = position.numel() // 4
+    if force.numel() // 4 < n or velocity.numel() // 4 < n:
+        raise ValueError("force / velocity hold fewer particles than position")
+    ptr, _ = bvh.getNodeBuffer()
+    return int(N.load().mfl_collide_bvh(force.data_ptr(), position.data_ptr(), velocity.data_ptr(), n,
+                                        ctypes.byref(params), ptr, bvh.getNodeCount(), bvh.getRootNode(),
+                                        triangle.data_ptr(), triangle.numel(), vertex.data_ptr(), vertex.numel() // 4,
+                                        _stream_ptr(stream)))
 
 
 class FluidBatch:

@@ -157,3 +157,93 @@ def sort_keys(n: int, seed: int, skew: bool = False, bits: int = 24) -> np.ndarr
         which = ((r >> np.uint64(41)) % np.uint64(hot.shape[0])).astype(np.int64)
         keys = np.where(pick == 1, hot[which], keys)
     return keys
+
+
+# ---- triangle meshes for the BVH and the particle-vs-mesh collision ---------------------------------
+
+@dataclass
+class Mesh:
+    """A triangle mesh as the BVH program takes it: float4 vertices (w = 1) and 3 uint32 vertex
+    indices per triangle."""
+    name: str
+    vertex: np.ndarray
+    index: np.ndarray
+
+    @property
+    def triangles(self) -> int:
+        return self.index.shape[0] // 3
+
+
+def floor_mesh(nx: int, ny: int, lo=(0.0, 0.0), hi=(1.0, 1.0), z: float = 0.0, jitter: float = 0.0, seed: int = 0,
+               name: str = "floor", slope=(0.0, 0.0)) -> Mesh:
+    """An nx x ny grid of quads over [lo, hi] at height z + slope . (x, y) (each vertex moved by
+    jitter * (2u - 1) in z, seed), two triangles per quad facing +z; vertices x fastest. (A triangle
+    in a plane z = const has a box of zero thickness, which the reference's box test never enters.)"""
+    xs = np.linspace(lo[0], hi[0], nx + 1)
+    ys = np.linspace(lo[1], hi[1], ny + 1)
+    gy, gx = np.meshgrid(ys, xs, indexing="ij")
+    nv = (nx + 1) * (ny + 1)
+    vertex = np.ones((nv, 4), np.float32)
+    vertex[:, 0] = gx.ravel()
+    vertex[:, 1] = gy.ravel()
+    vertex[:, 2] = (z + slope[0] * gx.ravel() + slope[1] * gy.ravel() +
+                    (jitter * (2.0 * uniform(seed, nv) - 1.0) if jitter else 0.0))
+    j, i = np.meshgrid(np.arange(ny), np.arange(nx), indexing="ij")
+    v00 = (j * (nx + 1) + i).ravel()
+    v10, v01, v11 = v00 + 1, v00 + nx + 1, v00 + nx + 2
+    index = np.stack([v00, v10, v01, v10, v11, v01], axis=1).reshape(-1).astype(np.uint32)
+    return Mesh(name, vertex, index)
+
+
+def random_triangles(nv: int = 1000, seed: int = 1, name: str = "random") -> Mesh:
+    """The shape of libCL/TEST/main.cpp:349-383: nv vertices uniform in [-0.5, 0.5)^3 with w = 1 and
+    indices 0..nv-1, so nv // 3 large, overlapping triangles (seeded here instead of rand())."""
+    vertex = np.ones((nv, 4), np.float32)
+    vertex[:, :3] = (uniform(seed, 3 * nv).reshape(nv, 3) - 0.5).astype(np.float32)
+    return Mesh(name, vertex, np.arange(nv, dtype=np.uint32))
+
+
+def sphere_mesh(n_lat: int, n_lon: int, center=(0.0, 0.0, 0.0), radius: float = 1.0, name: str = "sphere") -> Mesh:
+    """A UV sphere with outward-facing triangles (two poles, n_lat - 1 rings of n_lon vertices)."""
+    c = np.asarray(center, np.float64)
+    rings = []
+    for a in range(1, n_lat):
+        th = math.pi * a / n_lat
+        for b in range(n_lon):
+            ph = 2.0 * math.pi * b / n_lon
+            rings.append((math.sin(th) * math.cos(ph), math.sin(th) * math.sin(ph), math.cos(th)))
+    pts = np.array([(0.0, 0.0, 1.0)] + rings + [(0.0, 0.0, -1.0)])
+    south = len(pts) - 1
+    tris = []
+    ring = lambda a, b: 1 + (a - 1) * n_lon + (b % n_lon)  # noqa: E731
+    for b in range(n_lon):
+        tris.append((0, ring(1, b), ring(1, b + 1)))
+        tris.append((south, ring(n_lat - 1, b + 1), ring(n_lat - 1, b)))
+    for a in range(1, n_lat - 1):
+        for b in range(n_lon):
+            tris.append((ring(a, b), ring(a + 1, b), ring(a + 1, b + 1)))
+            tris.append((ring(a, b), ring(a + 1, b + 1), ring(a, b + 1)))
+    tri = np.array(tris, np.int64)
+    p = pts * radius + c
+    nrm = np.cross(p[tri[:, 1]] - p[tri[:, 0]], p[tri[:, 2]] - p[tri[:, 0]])
+    out = (nrm * (p[tri].mean(1) - c)).sum(1) < 0
+    tri[out] = tri[out][:, ::-1]
+    vertex = np.ones((len(p), 4), np.float32)
+    vertex[:, :3] = p.astype(np.float32)
+    return Mesh(name, vertex, tri.reshape(-1).astype(np.uint32))
+
+
+def config_collide_1m() -> tuple[Scene, Mesh]:
+    """Particle-vs-mesh workload: the 1,048,576-particle block of config_1m falling at 1.5 units/s
+    (vx, vy ~ N(0, 0.02^2), seed 3) onto a 96 x 48-quad floor (9,216 triangles) across the block at
+    z = 0.6 with 0.002 height jitter (seed 4). The collision runs with clInitFluid's Params
+    (dt = 0.005): its parallel-ray cut |n . v dt| < 1e-6 uses the unnormalised triangle normal, so a
+    contact needs |2 area x v dt| >= 1e-6, which these triangles (legs ~0.013) and speed meet."""
+    sc = config_1m()
+    noise = normal(3, 2 * sc.n).reshape(sc.n, 2)
+    sc.vel[:, 0] = (0.02 * noise[:, 0]).astype(np.float32)
+    sc.vel[:, 1] = (0.02 * noise[:, 1]).astype(np.float32)
+    sc.vel[:, 2] = -1.5
+    sc.name = "collide_1m_floor9k"
+    mesh = floor_mesh(96, 48, (0.0, 0.0), (1.28, 0.64), 0.6, 0.002, 4, name="floor9k")
+    return sc, mesh

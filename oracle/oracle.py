@@ -3,8 +3,9 @@
 TEST INFRASTRUCTURE ONLY: imported by tests/, __graft_entry__.smoke() and bench.py's cpu_baseline
 leg, where it is the checker and the timed CPU baseline. The product (libcl_amd) never imports it.
 
-The C side restates libCL/phys/oclFluid3D.cl:15-361, libCL/phys/oclFluid3D.cpp:213-342 and
-libCL/sort/oclRadixSort.{cl,cpp}; see oracle/sph_oracle.c for the per-function citations.
+The C side restates libCL/phys/oclFluid3D.cl:15-361, libCL/phys/oclFluid3D.cpp:213-342,
+libCL/sort/oclRadixSort.{cl,cpp} (oracle/sph_oracle.c), and libCL/geom/oclBvhTrimesh.{cl,cpp} with
+clCollideBVH, libCL/phys/oclFluid3D.cl:365-550 (oracle/bvh_oracle.c); each C function cites its lines.
 """
 from __future__ import annotations
 
@@ -98,6 +99,15 @@ def lib():
         L.orc_steps.restype = ctypes.c_int
         L.orc_set_threads.argtypes = [ctypes.c_int]
         L.orc_set_threads.restype = ctypes.c_int
+        L.orc_morton_code.argtypes = [ctypes.c_uint32]
+        L.orc_morton_code.restype = ctypes.c_uint32
+        L.orc_bvh_aabb.argtypes = [_f32p, ctypes.c_uint32, _f32p, _f32p]
+        L.orc_bvh_morton.argtypes = [_u32p, _f32p, ctypes.c_uint32, _f32p, _f32p, _u32p, _u32p]
+        L.orc_bvh_build.argtypes = [_f32p, ctypes.c_uint32, _u32p, ctypes.c_uint32, ctypes.c_void_p, _u32p, _u32p,
+                                    _u32p]
+        L.orc_bvh_build.restype = ctypes.c_int
+        L.orc_collide_bvh.argtypes = [_f32p, _f32p, _f32p, ctypes.c_uint32, ctypes.POINTER(Params), ctypes.c_void_p,
+                                      ctypes.c_uint32, _u32p, _f32p, _u32p, _u32p]
         _lib = L
     return _lib
 
@@ -259,3 +269,64 @@ def gravity(force, g=(0, 0, 1, 0)):
 
 def integrate_velocity(pos, vel, params: Params):
     lib().orc_integrate_velocity(_f(pos), _f(vel), pos.shape[0], ctypes.byref(params))
+
+
+# ---- triangle-mesh BVH and particle collision (oracle/bvh_oracle.c) --------------------------------
+
+# BVHNode, libCL/geom/oclBvhTrimesh.h:33-41 (48 bytes)
+BVH_NODE = np.dtype([("bbMin", "<f4", (4,)), ("bbMax", "<f4", (4,)), ("left", "<i4"), ("right", "<i4"),
+                     ("bit", "<u4"), ("trav", "<u4")])
+assert BVH_NODE.itemsize == 48
+
+
+def morton_code(x: int) -> int:
+    return int(lib().orc_morton_code(x))
+
+
+def bvh_aabb(vertex):
+    mn = np.zeros(4, np.float32)
+    mx = np.zeros(4, np.float32)
+    lib().orc_bvh_aabb(_f(vertex), vertex.shape[0], _f(mn), _f(mx))
+    return mn, mx
+
+
+def bvh_morton(index, vertex, mn, mx):
+    t = index.shape[0] // 3
+    key = np.zeros(t, np.uint32)
+    val = np.zeros(t, np.uint32)
+    lib().orc_bvh_morton(_u(index), _f(vertex), t, _f(np.asarray(mn, np.float32)), _f(np.asarray(mx, np.float32)),
+                         _u(key), _u(val))
+    return key, val
+
+
+def bvh_build(vertex, index):
+    """oclBvhTrimesh::compute: (nodes [2T-1] BVH_NODE, root, sorted keys, sorted triangle indices), or None
+    when the build is rejected (fewer than 2 triangles, an index out of range)."""
+    vertex = np.ascontiguousarray(vertex, np.float32)
+    index = np.ascontiguousarray(index, np.uint32)
+    t = index.shape[0] // 3
+    if t < 2:
+        return None
+    nodes = np.zeros(2 * t - 1, BVH_NODE)
+    root = np.zeros(1, np.uint32)
+    key = np.zeros(t, np.uint32)
+    val = np.zeros(t, np.uint32)
+    if not lib().orc_bvh_build(_f(vertex), vertex.shape[0], _u(index), index.shape[0], nodes.ctypes.data, _u(root),
+                               _u(key), _u(val)):
+        return None
+    return nodes, int(root[0]), key, val
+
+
+def collide_bvh(pos, vel, force, params: Params, nodes, root, index, vertex, filled=None):
+    """clCollideBVH over copies of pos / vel / force; returns them and the leaf that stopped each walk
+    (0xFFFFFFFF where none did). `filled` (uint32 [n], optional) receives 1 where the ring filled up."""
+    pos = np.ascontiguousarray(pos, np.float32).copy()
+    vel = np.ascontiguousarray(vel, np.float32).copy()
+    force = np.ascontiguousarray(force, np.float32).copy()
+    nodes = np.ascontiguousarray(nodes, BVH_NODE)
+    index = np.ascontiguousarray(index, np.uint32)
+    vertex = np.ascontiguousarray(vertex, np.float32)
+    hit = np.zeros(pos.shape[0], np.uint32)
+    lib().orc_collide_bvh(_f(force), _f(pos), _f(vel), pos.shape[0], ctypes.byref(params), nodes.ctypes.data, root,
+                          _u(index), _f(vertex), _u(hit), None if filled is None else _u(filled))
+    return pos, vel, force, hit
