@@ -4,7 +4,9 @@ python bench.py [--gpus 1] [--steps K] [--warmup W] [--config 1m|small] [--no-cp
 
 Workload: the 1,048,576-particle dam break (128x64x128 jittered lattice at 0.5h, h = 0.02, box
 [0,4]^3, fp32), inputs resident in HBM. A step is one oclFluid3D::compute(): hash + bucket clear,
-sort, reorder + bounds, density, forces + clip box + integration. Prints one JSON line.
+sort, reorder + bounds, density, forces + clip box + integration. Prints one JSON line; its
+secondary legs are the batched scenes, the adjoint and the particle-vs-mesh collision (BVH build +
+clCollideBVH on the same 1M particles).
 """
 from __future__ import annotations
 
@@ -196,6 +198,75 @@ def adjoint_leg(steps: int = 200, every: int = 20) -> dict:
             "grad_viscosity": pg["viscosity"], "grad_stiffness": pg["stiffness"], "grad_density": pg["density"]}
 
 
+def collide_leg(launches: int = 20, cpu: bool = True) -> dict:
+    """Particle-vs-mesh collision (oclBvhTrimesh + clCollideBVH) on the 1,048,576-particle block falling
+    onto a 9,216-triangle floor (collide_1m_floor9k), with clInitFluid's Params: BVH build time (the
+    build ends by reading the root back, so it is timed by the host clock), the build of a
+    1,048,576-triangle floor, and clCollideBVH over all particles, each launch from the same input
+    (restored between launches, outside the HIP events that time the kernel)."""
+    import numpy as np
+    import torch
+
+    import libcl_amd as L
+    from libcl_amd import scenes
+
+    sc, mesh = scenes.config_collide_1m()
+    prm = L.default_params()
+    b = L.oclBvhTrimesh()
+    assert b.compile() == 1, L.last_error()
+    stream = torch.cuda.Stream()
+
+    def build_ms(m, reps):
+        vtx = torch.from_numpy(m.vertex).cuda()
+        idx = torch.from_numpy(m.index.view(np.int32)).cuda()
+        for _ in range(2):
+            assert b.compute(vtx, idx, stream) == 1, L.last_error()
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            assert b.compute(vtx, idx, stream) == 1, L.last_error()
+        return (time.perf_counter() - t0) * 1e3 / reps, vtx, idx
+
+    big = scenes.floor_mesh(1024, 512, (0.0, 0.0), (4.0, 2.0), 0.3, 0.001, 7)
+    big_ms, _, _ = build_ms(big, 5)
+    ms, vtx, idx = build_ms(mesh, 20)
+    P0, V0 = torch.from_numpy(sc.pos).cuda(), torch.from_numpy(sc.vel).cuda()
+    F0 = torch.from_numpy(sc.force).cuda()
+    P, V, F = P0.clone(), V0.clone(), F0.clone()
+    times = []
+    for k in range(launches + 2):
+        with torch.cuda.stream(stream):
+            P.copy_(P0)
+            V.copy_(V0)
+            F.copy_(F0)
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record(stream)
+        assert L.collide_bvh(F, P, V, prm, b, idx, vtx, stream) == 1, L.last_error()
+        e1.record(stream)
+        times.append((e0, e1))
+    torch.cuda.synchronize()
+    us = [a.elapsed_time(z) * 1e3 for a, z in times[2:]]
+    avg = sum(us) / len(us)
+    contacts = int((V[:, :3] == 0).all(1).sum().item())
+    out = {"workload": sc.name, "particles": sc.n, "triangles": mesh.triangles, "build_ms": ms,
+           "build_1m_triangles_ms": big_ms, "collide_us_per_launch": avg, "particles_per_s": sc.n / (avg * 1e-6),
+           "contacts": contacts, "launches": launches}
+    if cpu:
+        from oracle import oracle as O
+        O.build()
+        threads = O.set_threads(min(os.cpu_count() or 1, 16))
+        nodes = b.nodes()
+        pos, vel, force = sc.pos.copy(), sc.vel.copy(), sc.force.copy()
+        op = O.Params(*prm.values())
+        O.collide_bvh(pos[:1024], vel[:1024], force[:1024], op, nodes, b.getRootNode(), mesh.index, mesh.vertex)
+        t0 = time.perf_counter()
+        O.collide_bvh(pos, vel, force, op, nodes, b.getRootNode(), mesh.index, mesh.vertex)
+        dt = time.perf_counter() - t0
+        out["cpu_baseline"] = {"value": pos.shape[0] / dt, "unit": "particles/s", "cores": threads, "kind": "port",
+                               "sample": f"one clCollideBVH pass over all {pos.shape[0]} particles, "
+                                         f"oracle/bvh_oracle.c with {threads} OpenMP threads ({dt:.2f} s)"}
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -207,7 +278,8 @@ def main():
     ap.add_argument("--order", default="morton", choices=["sorted", "caller", "morton"])
     ap.add_argument("--no-batched", action="store_true")
     ap.add_argument("--no-adjoint", action="store_true")
-    ap.add_argument("--only", default=None, choices=["batched", "adjoint"],
+    ap.add_argument("--no-collide", action="store_true")
+    ap.add_argument("--only", default=None, choices=["batched", "adjoint", "collide"],
                     help="run one secondary leg alone and print its JSON (profiling)")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="also run a plain device copy of MIB MiB three times (counter calibration)")
@@ -239,6 +311,9 @@ def main():
         return
     if args.only == "adjoint":
         print(json.dumps(adjoint_leg()), flush=True)
+        return
+    if args.only == "collide":
+        print(json.dumps(collide_leg(cpu=not args.no_cpu)), flush=True)
         return
 
     sc = scenes.config_1m() if args.config == "1m" else scenes.config_small()
@@ -355,6 +430,8 @@ def main():
         out["batched"] = batched_leg(args.batched_steps, args.sequential_steps)
     if not args.no_adjoint:
         out["adjoint"] = adjoint_leg()
+    if not args.no_collide:
+        out["collide"] = collide_leg(cpu=not args.no_cpu)
     if not args.no_cpu:
         nthreads = min(os.cpu_count() or 1, 16)
         out["cpu_baseline"] = cpu_baseline(sc, prm, args.cpu_steps, nthreads)

@@ -242,19 +242,21 @@ uint32_t mfl_bvh_trimesh_get_root_node(mfl_bvh_trimesh* b);
 int mfl_bvh_trimesh_get_morton(mfl_bvh_trimesh* b, void** keys, void** vals, size_t* count);
 
 /* clCollideBVH (libCL/phys/oclFluid3D.cl:498-550) with its kernel arguments: forceOut, position,
-   velocity (n caller-order float4 each, device), param (host copy), bvh + bvhRoot, triangle (index
-   buffer) and vertex. The counts bound every index the walk reads: a node, triangle or vertex index
-   outside them ends that particle's walk without a change (the reference reads past the buffers).
+   velocity (n caller-order float4 each, device), param (host copy), the tree of `bvh` (its node
+   buffer and root, getNodeBuffer / getRootNode) and the mesh it was built from: triangle (index
+   buffer, index_count entries) and vertex (vertex_count float4). A triangle or vertex index outside
+   the counts ends that particle's walk without a change (the reference reads past the buffers).
    Breadth-first walk with the reference's 256-entry ring; the first triangle that collides moves the
-   particle, zeroes its velocity and adds the contact force. */
-int mfl_collide_bvh(void* force, void* position, void* velocity, size_t n, const mfl_params* p, const void* nodes,
-                    uint32_t node_count, uint32_t root, const uint32_t* triangle, size_t index_count,
-                    const void* vertex, size_t vertex_count, void* stream);
+   particle, zeroes its velocity and adds the contact force. The walk keeps 16 ring slots per particle
+   in LDS and redoes the rare walks that need more with all 256; the list of those lives in `bvh`, so
+   calls with one BVH on concurrent streams race. Fails if `bvh` has not been built. */
+int mfl_collide_bvh(void* force, void* position, void* velocity, size_t n, const mfl_params* p, mfl_bvh_trimesh* bvh,
+                    const uint32_t* triangle, size_t index_count, const void* vertex, size_t vertex_count,
+                    void* stream);
 /* The same kernel bound to a program's position / velocity / force buffers and Params, as an
    EVT_INTEGRATE handler enqueues it (getKernel("clCollideBVH") + the program's buffers). */
-int mfl_fluid3d_collide_bvh(mfl_fluid3d* f, const void* nodes, uint32_t node_count, uint32_t root,
-                            const uint32_t* triangle, size_t index_count, const void* vertex, size_t vertex_count,
-                            void* stream);
+int mfl_fluid3d_collide_bvh(mfl_fluid3d* f, mfl_bvh_trimesh* bvh, const uint32_t* triangle, size_t index_count,
+                            const void* vertex, size_t vertex_count, void* stream);
 
 /* ---- batched scenes (addition; no counterpart in the reference) --------------------------------- */
 

@@ -168,9 +168,8 @@ def load() -> ctypes.CDLL:
         "mfl_bvh_trimesh_get_root_node": (ctypes.c_uint32, [vp]),
         "mfl_bvh_trimesh_get_morton": (ctypes.c_int, [vp, ctypes.POINTER(vp), ctypes.POINTER(vp),
                                                       ctypes.POINTER(sz)]),
-        "mfl_collide_bvh": (ctypes.c_int, [vp, vp, vp, sz, pp, vp, ctypes.c_uint32, ctypes.c_uint32, vp, sz, vp, sz,
-                                           vp]),
-        "mfl_fluid3d_collide_bvh": (ctypes.c_int, [vp, vp, ctypes.c_uint32, ctypes.c_uint32, vp, sz, vp, sz, vp]),
+        "mfl_collide_bvh": (ctypes.c_int, [vp, vp, vp, sz, pp, vp, vp, sz, vp, sz, vp]),
+        "mfl_fluid3d_collide_bvh": (ctypes.c_int, [vp, vp, vp, sz, vp, sz, vp]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(L, name)

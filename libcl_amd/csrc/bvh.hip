@@ -287,63 +287,144 @@ __device__ inline bool collide_triangle(float4 V0, float4 V1, float4 V2, float4&
     return true;
 }
 
+// The particle's ray as collideAABB uses it (cl:486-496): origin, 1 / normalize(velocity) per axis and
+// the acceptance limit max(radius, fast_length(velocity)). The walk changes neither the position's
+// x, y, z nor the velocity before the collision that ends it, so these are computed once per walk.
+struct Ray {
+    float ox, oy, oz, ix, iy, iz, lim;
+};
+
+__device__ inline Ray make_ray(float4 ro, float4 rd, float radius)
+{
+    const float4 nd = normalize4(rd);
+    return Ray{ro.x, ro.y, ro.z, 1.0f / nd.x, 1.0f / nd.y, 1.0f / nd.z, cl_max(radius, sqrtf(dot4(rd, rd)))};
+}
+
 // intersectBox (cl:464-484) + collideAABB (cl:486-496); only x, y, z reach the result
-__device__ inline bool collide_aabb(const BvhNode* __restrict__ node, float4 ro, float4 rd, float radius)
+__device__ inline bool collide_aabb(const BvhNode* __restrict__ node, const Ray& r)
 {
     const float4 bmin = node->bbMin, bmax = node->bbMax;
-    const float4 nd = normalize4(rd);
-    const float ix = 1.0f / nd.x, iy = 1.0f / nd.y, iz = 1.0f / nd.z;
-    const float bx = ix * (bmin.x - ro.x), by = iy * (bmin.y - ro.y), bz = iz * (bmin.z - ro.z);
-    const float tx = ix * (bmax.x - ro.x), ty = iy * (bmax.y - ro.y), tz = iz * (bmax.z - ro.z);
+    const float bx = r.ix * (bmin.x - r.ox), by = r.iy * (bmin.y - r.oy), bz = r.iz * (bmin.z - r.oz);
+    const float tx = r.ix * (bmax.x - r.ox), ty = r.iy * (bmax.y - r.oy), tz = r.iz * (bmax.z - r.oz);
     const float mnx = cl_min(tx, bx), mny = cl_min(ty, by), mnz = cl_min(tz, bz);
     const float mxx = cl_max(tx, bx), mxy = cl_max(ty, by), mxz = cl_max(tz, bz);
     const float largest_tmin = cl_max(cl_max(mnx, mny), cl_max(mnx, mnz));
     const float smallest_tmax = cl_min(cl_min(mxx, mxy), cl_min(mxx, mxz));
-    if (!(smallest_tmax > largest_tmin)) return false;
-    return largest_tmin < cl_max(radius, sqrtf(dot4(rd, rd)));
+    return smallest_tmax > largest_tmin && largest_tmin < r.lim;
 }
 
-// One lane per particle, breadth-first walk with the reference's 256-entry ring (a full ring makes
-// head == tail and ends the walk, as in the reference); the first triangle that collides ends it.
-__global__ __launch_bounds__(kBlock) void k_collide_bvh(float4* __restrict__ force, float4* __restrict__ pos,
-                                                       float4* __restrict__ vel, uint32_t n, mfl_params p,
-                                                       const BvhNode* __restrict__ nodes, uint32_t nnodes, uint32_t root,
-                                                       const uint32_t* __restrict__ tri, uint32_t ntri,
-                                                       const float4* __restrict__ vertex, uint32_t nv)
+// The walk's FIFO: positions head / tail count modulo 256 exactly as the reference's ring (a push
+// that makes tail == head ends the walk, cl:515-548). LdsRing keeps kRingLds slots per lane in LDS
+// (slot s of a lane at ring[s * kBlock + lane]: every lane in its own bank whatever s); a walk whose
+// pending entries would exceed them stops without side effects and is redone with PrivateRing, the
+// full 256 slots in private (scratch) memory. A walk has no side effect before the collision that ends
+// it, so the redo from the start is the same walk.
+constexpr uint32_t kRingLds = 32;  // divides 256, so slot = position % 32 survives the wrap
+
+// Slot is uint16_t for trees below 65,536 nodes (4 KiB of LDS per wave) and uint32_t above.
+template <class Slot>
+struct LdsRing {
+    static constexpr uint32_t cap = kRingLds;
+    Slot* base;
+    __device__ uint32_t get(uint32_t pos) const { return base[(pos % kRingLds) * kBlock]; }
+    __device__ void put(uint32_t pos, uint32_t v) { base[(pos % kRingLds) * kBlock] = (Slot)v; }
+};
+
+struct PrivateRing {
+    static constexpr uint32_t cap = kQueue;
+    uint32_t q[kQueue];
+    __device__ uint32_t get(uint32_t pos) const { return q[pos]; }
+    __device__ void put(uint32_t pos, uint32_t v) { q[pos] = v; }
+};
+
+struct Mesh {
+    const BvhNode* nodes;
+    uint32_t nnodes, root;
+    const uint32_t* tri;
+    uint32_t ntri;
+    const float4* vertex;
+    uint32_t nv;
+};
+
+enum : int { kWalkMiss = 0, kWalkHit = 1, kWalkSpill = -1 };
+
+// clCollideBVH's loop (cl:515-548) for one particle. An index outside the mesh ends the walk.
+template <class Ring>
+__device__ inline int walk(Ring& ring, const Mesh& m, const mfl_params& p, float4& P, float4& V, float4& F)
 {
-    const uint32_t q = blockIdx.x * kBlock + threadIdx.x;
-    if (q >= n) return;
-    float4 P = pos[q], V = vel[q], F = force[q];
-    uint32_t queue[kQueue];
+    const Ray ray = make_ray(P, V, p.particleRadius);
     uint32_t head = 0, tail = 1;
-    queue[0] = root;
+    ring.put(0, m.root);
     while (head != tail) {
-        const uint32_t it = queue[head];
+        const uint32_t it = ring.get(head);
         head = (head + 1u) % kQueue;
-        if (it >= nnodes) break;
-        const int4 link = *reinterpret_cast<const int4*>(&nodes[it].left);
+        if (it >= m.nnodes) return kWalkMiss;
+        const int4 link = *reinterpret_cast<const int4*>(&m.nodes[it].left);
         const uint32_t left = (uint32_t)link.x, right = (uint32_t)link.y;
         if (left == right) {
             const uint32_t t = (uint32_t)link.z;
-            if (t >= ntri) break;
+            if (t >= m.ntri) return kWalkMiss;
             float4 v0, v1, v2;
-            if (!triangle_vertices(tri, vertex, nv, t, v0, v1, v2)) break;
-            if (collide_triangle(v0, v1, v2, P, V, F, p)) {
-                pos[q] = P;
-                vel[q] = V;
-                force[q] = F;
-                break;
-            }
+            if (!triangle_vertices(m.tri, m.vertex, m.nv, t, v0, v1, v2)) return kWalkMiss;
+            if (collide_triangle(v0, v1, v2, P, V, F, p)) return kWalkHit;
         } else {
-            if (left >= nnodes || right >= nnodes) break;
-            if (collide_aabb(&nodes[left], P, V, p.particleRadius)) {
-                queue[tail] = left;
+            if (left >= m.nnodes || right >= m.nnodes) return kWalkMiss;
+            if (collide_aabb(&m.nodes[left], ray)) {
+                ring.put(tail, left);
                 tail = (tail + 1u) % kQueue;
+                if (Ring::cap < kQueue && ((tail - head) % kQueue) > Ring::cap) return kWalkSpill;
             }
-            if (collide_aabb(&nodes[right], P, V, p.particleRadius)) {
-                queue[tail] = right;
+            if (collide_aabb(&m.nodes[right], ray)) {
+                ring.put(tail, right);
                 tail = (tail + 1u) % kQueue;
+                if (Ring::cap < kQueue && ((tail - head) % kQueue) > Ring::cap) return kWalkSpill;
             }
+        }
+    }
+    return kWalkMiss;
+}
+
+__global__ void k_zero_count(uint32_t* __restrict__ spill)
+{
+    if (threadIdx.x == 0) spill[threadIdx.x] = 0u;
+}
+
+// One lane per particle with the LDS ring; walks that outgrow it are listed in spill[1 + k]
+// (count in spill[0]) for k_collide_spill.
+template <class Slot>
+__global__ __launch_bounds__(kBlock) void k_collide_lds(float4* __restrict__ force, float4* __restrict__ pos,
+                                                       float4* __restrict__ vel, uint32_t n, mfl_params p, Mesh m,
+                                                       uint32_t* __restrict__ spill)
+{
+    __shared__ Slot ring[kRingLds * kBlock];
+    const uint32_t q = blockIdx.x * kBlock + threadIdx.x;
+    if (q >= n) return;
+    float4 P = pos[q], V = vel[q], F = force[q];
+    LdsRing<Slot> r{ring + threadIdx.x};
+    const int res = walk(r, m, p, P, V, F);
+    if (res == kWalkHit) {
+        pos[q] = P;
+        vel[q] = V;
+        force[q] = F;
+    } else if (res == kWalkSpill) {
+        spill[1u + atomicAdd(spill, 1u)] = q;
+    }
+}
+
+// The listed particles again, each with the full 256-slot ring; a grid-stride loop over the list.
+__global__ __launch_bounds__(kBlock) void k_collide_spill(float4* __restrict__ force, float4* __restrict__ pos,
+                                                         float4* __restrict__ vel, mfl_params p, Mesh m,
+                                                         const uint32_t* __restrict__ spill)
+{
+    const uint32_t count = spill[0];
+    for (uint32_t k = blockIdx.x * kBlock + threadIdx.x; k < count; k += gridDim.x * kBlock) {
+        const uint32_t q = spill[1u + k];
+        float4 P = pos[q], V = vel[q], F = force[q];
+        PrivateRing r;
+        if (walk(r, m, p, P, V, F) == kWalkHit) {
+            pos[q] = P;
+            vel[q] = V;
+            force[q] = F;
         }
     }
 }
@@ -419,26 +500,35 @@ int BvhTrimesh::compute(const float4* vertex, size_t vertex_count, const uint32_
     return 1;
 }
 
-int collide_bvh(float4* force, float4* pos, float4* vel, size_t n, const mfl_params& p, const BvhNode* nodes,
-                uint32_t node_count, uint32_t root, const uint32_t* triangle, size_t index_count, const float4* vertex,
-                size_t vertex_count, hipStream_t s)
+int BvhTrimesh::collide(float4* force, float4* pos, float4* vel, size_t n, const mfl_params& p,
+                        const uint32_t* triangle, size_t index_count, const float4* vertex, size_t vertex_count,
+                        hipStream_t s)
 {
     if (n == 0) return 1;
-    if (!force || !pos || !vel || !nodes || !triangle || !vertex) {
+    if (count_ == 0) {
+        set_error("clCollideBVH: the BVH has not been built");
+        return 0;
+    }
+    if (!force || !pos || !vel || !triangle || !vertex) {
         set_error("clCollideBVH: null buffer");
         return 0;
     }
-    if (root >= node_count) {
-        set_error("clCollideBVH: root " + std::to_string(root) + " is not below the node count " +
-                  std::to_string(node_count));
-        return 0;
-    }
-    if (n > 0xFFFFFFFFull || vertex_count > 0xFFFFFFFFull || index_count / 3 > 0xFFFFFFFFull) {
+    if (n > 0xFFFFFFFEull || vertex_count > 0xFFFFFFFFull || index_count / 3 > 0xFFFFFFFFull) {
         set_error("clCollideBVH: sizes beyond 32-bit indices");
         return 0;
     }
-    hipLaunchKernelGGL(k_collide_bvh, dim3(blocks_for(n)), dim3(kBlock), 0, s, force, pos, vel, (uint32_t)n, p, nodes,
-                       node_count, root, triangle, (uint32_t)(index_count / 3), vertex, (uint32_t)vertex_count);
+    if (!spill_.fit((n + 1) * sizeof(uint32_t))) return 0;
+    const Mesh m{nodes(), count_, root_, triangle, (uint32_t)(index_count / 3), vertex, (uint32_t)vertex_count};
+    uint32_t* spill = spill_.as<uint32_t>();
+    hipLaunchKernelGGL(k_zero_count, dim3(1), dim3(64), 0, s, spill);
+    if (count_ <= 0xFFFFu)
+        hipLaunchKernelGGL(k_collide_lds<uint16_t>, dim3(blocks_for(n)), dim3(kBlock), 0, s, force, pos, vel,
+                           (uint32_t)n, p, m, spill);
+    else
+        hipLaunchKernelGGL(k_collide_lds<uint32_t>, dim3(blocks_for(n)), dim3(kBlock), 0, s, force, pos, vel,
+                           (uint32_t)n, p, m, spill);
+    hipLaunchKernelGGL(k_collide_spill, dim3(std::min<uint32_t>(blocks_for(n), 1024u)), dim3(kBlock), 0, s, force,
+                       pos, vel, p, m, spill);
     MFL_HIP(hipGetLastError());
     return 1;
 }

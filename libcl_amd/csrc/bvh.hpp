@@ -33,17 +33,18 @@ public:
     uint32_t* morton_keys() const { return keys_.as<uint32_t>(); }
     uint32_t* morton_vals() const { return vals_.as<uint32_t>(); }
 
+    // clCollideBVH (libCL/phys/oclFluid3D.cl:498-550) with this tree (nodes, root) over particles
+    // [0, n) of caller-order position / velocity / force buffers and the mesh it was built from. A
+    // triangle or vertex index outside the given counts ends that particle's walk without a change.
+    // Uses this object's spill list: calls with one BVH on concurrent streams race.
+    int collide(float4* force, float4* pos, float4* vel, size_t n, const mfl_params& p, const uint32_t* triangle,
+                size_t index_count, const float4* vertex, size_t vertex_count, hipStream_t s);
+
 private:
     bool compiled_ = false;
     RadixSort sort_;
-    DeviceBuffer part_, keys_, vals_, nodes_, split_, bound_, info_;
+    DeviceBuffer part_, keys_, vals_, nodes_, split_, bound_, info_, spill_;
     uint32_t count_ = 0, root_ = 0;
 };
-
-// clCollideBVH over particles [0, n) of caller-order position / velocity / force buffers. A node,
-// triangle or vertex index outside the given counts ends that particle's walk without a change.
-int collide_bvh(float4* force, float4* pos, float4* vel, size_t n, const mfl_params& p, const BvhNode* nodes,
-                uint32_t node_count, uint32_t root, const uint32_t* triangle, size_t index_count, const float4* vertex,
-                size_t vertex_count, hipStream_t s);
 
 }  // namespace mfl

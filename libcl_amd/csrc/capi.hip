@@ -545,36 +545,34 @@ int mfl_bvh_trimesh_get_morton(mfl_bvh_trimesh* b, void** keys, void** vals, siz
     return 1;
 }
 
-int mfl_collide_bvh(void* force, void* position, void* velocity, size_t n, const mfl_params* p, const void* nodes,
-                    uint32_t node_count, uint32_t root, const uint32_t* triangle, size_t index_count,
-                    const void* vertex, size_t vertex_count, void* stream)
+int mfl_collide_bvh(void* force, void* position, void* velocity, size_t n, const mfl_params* p, mfl_bvh_trimesh* bvh,
+                    const uint32_t* triangle, size_t index_count, const void* vertex, size_t vertex_count,
+                    void* stream)
 {
+    if (!need(bvh, "BVH")) return 0;
     if (!p) {
         mfl::set_error("null params");
         return 0;
     }
     return guarded([&] {
-        return mfl::collide_bvh(static_cast<float4*>(force), static_cast<float4*>(position),
-                                static_cast<float4*>(velocity), n, *p, static_cast<const mfl::BvhNode*>(nodes),
-                                node_count, root, triangle, index_count, static_cast<const float4*>(vertex),
-                                vertex_count, (hipStream_t)stream);
+        return bvh->impl.collide(static_cast<float4*>(force), static_cast<float4*>(position),
+                                 static_cast<float4*>(velocity), n, *p, triangle, index_count,
+                                 static_cast<const float4*>(vertex), vertex_count, (hipStream_t)stream);
     });
 }
 
-int mfl_fluid3d_collide_bvh(mfl_fluid3d* f, const void* nodes, uint32_t node_count, uint32_t root,
-                            const uint32_t* triangle, size_t index_count, const void* vertex, size_t vertex_count,
-                            void* stream)
+int mfl_fluid3d_collide_bvh(mfl_fluid3d* f, mfl_bvh_trimesh* bvh, const uint32_t* triangle, size_t index_count,
+                            const void* vertex, size_t vertex_count, void* stream)
 {
-    if (!need(f, "fluid")) return 0;
+    if (!need(f, "fluid") || !need(bvh, "BVH")) return 0;
     return guarded([&] {
         void *pos = nullptr, *vel = nullptr, *force = nullptr;
         if (!f->impl.get_buffer(MFL_POSITION, &pos, nullptr) || !f->impl.get_buffer(MFL_VELOCITY, &vel, nullptr) ||
             !f->impl.get_buffer(MFL_FORCE, &force, nullptr))
             return 0;
-        return mfl::collide_bvh(static_cast<float4*>(force), static_cast<float4*>(pos), static_cast<float4*>(vel),
-                                f->impl.particle_count(), f->impl.params, static_cast<const mfl::BvhNode*>(nodes),
-                                node_count, root, triangle, index_count, static_cast<const float4*>(vertex),
-                                vertex_count, (hipStream_t)stream);
+        return bvh->impl.collide(static_cast<float4*>(force), static_cast<float4*>(pos), static_cast<float4*>(vel),
+                                 f->impl.particle_count(), f->impl.params, triangle, index_count,
+                                 static_cast<const float4*>(vertex), vertex_count, (hipStream_t)stream);
     });
 }
 

@@ -283,10 +283,8 @@ class oclFluid3D:
         against `bvh` built over (triangle indices, vertices): what an EVT_INTEGRATE handler enqueues."""
         _f4_tensor(vertex, "vertex")
         _u32_tensor(triangle, "triangle")
-        ptr, _ = bvh.getNodeBuffer()
-        return int(N.load().mfl_fluid3d_collide_bvh(self._h, ptr, bvh.getNodeCount(), bvh.getRootNode(),
-                                                    triangle.data_ptr(), triangle.numel(), vertex.data_ptr(),
-                                                    vertex.numel() // 4, _stream_ptr(stream)))
+        return int(N.load().mfl_fluid3d_collide_bvh(self._h, bvh._h, triangle.data_ptr(), triangle.numel(),
+                                                    vertex.data_ptr(), vertex.numel() // 4, _stream_ptr(stream)))
 
     # ---- host helpers ----
     def upload(self, pos: np.ndarray, vel: np.ndarray, force: np.ndarray) -> int:
@@ -399,11 +397,9 @@ def collide_bvh(force, position, velocity, params: N.Params, bvh: oclBvhTrimesh,
     n = position.numel() // 4
     if force.numel() // 4 < n or velocity.numel() // 4 < n:
         raise ValueError("force / velocity hold fewer particles than position")
-    ptr, _ = bvh.getNodeBuffer()
     return int(N.load().mfl_collide_bvh(force.data_ptr(), position.data_ptr(), velocity.data_ptr(), n,
-                                        ctypes.byref(params), ptr, bvh.getNodeCount(), bvh.getRootNode(),
-                                        triangle.data_ptr(), triangle.numel(), vertex.data_ptr(), vertex.numel() // 4,
-                                        _stream_ptr(stream)))
+                                        ctypes.byref(params), bvh._h, triangle.data_ptr(), triangle.numel(),
+                                        vertex.data_ptr(), vertex.numel() // 4, _stream_ptr(stream)))
 
 
 class FluidBatch:

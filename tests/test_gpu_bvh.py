@@ -129,6 +129,10 @@ def _collide_case(name):
     elif name == "at_rest":
         m = scenes.floor_mesh(32, 32, (0.0, 0.0), (1.0, 1.0), 0.0, 0.003, 9)
         pos, vel = _particles_near(m, 50000, 34, speed=0.0, drift=0.0)
+    elif name == "floor65k_u32":
+        # more than 65,535 nodes: the 32-bit ring slots
+        m = scenes.floor_mesh(256, 128, (0.0, 0.0), (1.28, 0.64), 0.6, 0.002, 4)
+        pos, vel = _particles_near(m, 200000, 35, speed=10.0)
     elif name == "collide_1m":
         sc, m = scenes.config_collide_1m()
         pos, vel = sc.pos, sc.vel
@@ -139,7 +143,7 @@ def _collide_case(name):
     return m, pos, vel, force
 
 
-@pytest.mark.parametrize("name", ["floor", "sphere", "random_ring", "at_rest", "collide_1m"])
+@pytest.mark.parametrize("name", ["floor", "sphere", "random_ring", "at_rest", "floor65k_u32", "collide_1m"])
 def test_collide_matches_oracle(oracle, name):
     import libcl_amd as L
     m, pos, vel, force = _collide_case(name)
@@ -164,19 +168,14 @@ def test_collide_matches_oracle(oracle, name):
     np.testing.assert_array_equal(F.cpu().numpy().view(np.uint32), ef.view(np.uint32))
 
 
-def test_collide_rejects_a_root_outside_the_nodes():
+def test_collide_rejects_an_unbuilt_bvh():
     import libcl_amd as L
-    from libcl_amd import _native as N
-    import ctypes
+    b = L.oclBvhTrimesh()
+    assert b.compile() == 1
     m = scenes.random_triangles(30, seed=6)
-    b, vtx, idx = _gpu_build(m)
     P = torch.zeros(4, 4, device="cuda")
-    ptr, _ = b.getNodeBuffer()
-    prm = L.default_params()
-    ok = N.load().mfl_collide_bvh(P.data_ptr(), P.data_ptr(), P.data_ptr(), 1, ctypes.byref(prm), ptr,
-                                  b.getNodeCount(), b.getNodeCount(), idx.data_ptr(), idx.numel(), vtx.data_ptr(),
-                                  vtx.numel() // 4, None)
-    assert ok == 0 and "root" in L.last_error()
+    assert L.collide_bvh(P, P, P, L.default_params(), b, _dev(m.index.view(np.int32)), _dev(m.vertex)) == 0
+    assert "not been built" in L.last_error()
 
 
 def test_integrate_handler_with_collision(oracle):
