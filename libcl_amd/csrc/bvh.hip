@@ -300,17 +300,42 @@ __device__ inline Ray make_ray(float4 ro, float4 rd, float radius)
     return Ray{ro.x, ro.y, ro.z, 1.0f / nd.x, 1.0f / nd.y, 1.0f / nd.z, cl_max(radius, sqrtf(dot4(rd, rd)))};
 }
 
-// intersectBox (cl:464-484) + collideAABB (cl:486-496); only x, y, z reach the result
-__device__ inline bool collide_aabb(const BvhNode* __restrict__ node, const Ray& r)
+// intersectBox (cl:464-484) + collideAABB (cl:486-496) against the box (mn, mx); only x, y, z reach the
+// result
+__device__ inline bool collide_aabb(const Ray& r, float mnx_, float mny_, float mnz_, float mxx_, float mxy_,
+                                    float mxz_)
 {
-    const float4 bmin = node->bbMin, bmax = node->bbMax;
-    const float bx = r.ix * (bmin.x - r.ox), by = r.iy * (bmin.y - r.oy), bz = r.iz * (bmin.z - r.oz);
-    const float tx = r.ix * (bmax.x - r.ox), ty = r.iy * (bmax.y - r.oy), tz = r.iz * (bmax.z - r.oz);
+    const float bx = r.ix * (mnx_ - r.ox), by = r.iy * (mny_ - r.oy), bz = r.iz * (mnz_ - r.oz);
+    const float tx = r.ix * (mxx_ - r.ox), ty = r.iy * (mxy_ - r.oy), tz = r.iz * (mxz_ - r.oz);
     const float mnx = cl_min(tx, bx), mny = cl_min(ty, by), mnz = cl_min(tz, bz);
     const float mxx = cl_max(tx, bx), mxy = cl_max(ty, by), mxz = cl_max(tz, bz);
     const float largest_tmin = cl_max(cl_max(mnx, mny), cl_max(mnx, mnz));
     const float smallest_tmax = cl_min(cl_min(mxx, mxy), cl_min(mxx, mxz));
     return smallest_tmax > largest_tmin && largest_tmin < r.lim;
+}
+
+// Walk copy of the tree, written after each build: one 64-byte record per internal node with both
+// children's boxes (x, y, z of bbMin and bbMax, the values of the nodes) and their walk entries: an
+// internal node's index, or leaf_tag | triangle for a leaf (the node's bit). One cache line per
+// visited node instead of the node plus both children.
+__global__ __launch_bounds__(kBlock) void k_wide(const BvhNode* __restrict__ nodes, uint32_t T, uint32_t leaf_tag,
+                                                float4* __restrict__ wide)
+{
+    const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
+    if (i + 1 >= T) return;
+    const int32_t l = nodes[i].left, r = nodes[i].right;
+    float4* w = wide + 4 * (size_t)i;
+    if (l < 0 || r < 0 || (uint32_t)l > 2u * (T - 1u) || (uint32_t)r > 2u * (T - 1u)) {  // incomplete tree
+        w[3] = make_float4(__uint_as_float(0xFFFFFFFFu), __uint_as_float(0xFFFFFFFFu), 0.f, 0.f);
+        return;
+    }
+    const float4 lmn = nodes[l].bbMin, lmx = nodes[l].bbMax, rmn = nodes[r].bbMin, rmx = nodes[r].bbMax;
+    const uint32_t el = (uint32_t)l >= T - 1u ? (leaf_tag | nodes[l].bit) : (uint32_t)l;
+    const uint32_t er = (uint32_t)r >= T - 1u ? (leaf_tag | nodes[r].bit) : (uint32_t)r;
+    w[0] = make_float4(lmn.x, lmn.y, lmn.z, lmx.x);
+    w[1] = make_float4(lmx.y, lmx.z, rmn.x, rmn.y);
+    w[2] = make_float4(rmn.z, rmx.x, rmx.y, rmx.z);
+    w[3] = make_float4(__uint_as_float(el), __uint_as_float(er), 0.f, 0.f);
 }
 
 // The walk's FIFO: positions head / tail count modulo 256 exactly as the reference's ring (a push
@@ -338,8 +363,8 @@ struct PrivateRing {
 };
 
 struct Mesh {
-    const BvhNode* nodes;
-    uint32_t nnodes, root;
+    const float4* wide;   // k_wide records
+    uint32_t ninternal, root, leaf_tag;
     const uint32_t* tri;
     uint32_t ntri;
     const float4* vertex;
@@ -356,26 +381,28 @@ __device__ inline int walk(Ring& ring, const Mesh& m, const mfl_params& p, float
     uint32_t head = 0, tail = 1;
     ring.put(0, m.root);
     while (head != tail) {
-        const uint32_t it = ring.get(head);
+        const uint32_t e = ring.get(head);
         head = (head + 1u) % kQueue;
-        if (it >= m.nnodes) return kWalkMiss;
-        const int4 link = *reinterpret_cast<const int4*>(&m.nodes[it].left);
-        const uint32_t left = (uint32_t)link.x, right = (uint32_t)link.y;
-        if (left == right) {
-            const uint32_t t = (uint32_t)link.z;
+        if (e & m.leaf_tag) {
+            // a leaf: collideTriangle(bvh[iter].bit * 3, ...) (cl:521-531)
+            const uint32_t t = e & ~m.leaf_tag;
             if (t >= m.ntri) return kWalkMiss;
             float4 v0, v1, v2;
             if (!triangle_vertices(m.tri, m.vertex, m.nv, t, v0, v1, v2)) return kWalkMiss;
             if (collide_triangle(v0, v1, v2, P, V, F, p)) return kWalkHit;
         } else {
-            if (left >= m.nnodes || right >= m.nnodes) return kWalkMiss;
-            if (collide_aabb(&m.nodes[left], ray)) {
-                ring.put(tail, left);
+            // an internal node: collideAABB on both children, left first (cl:533-546)
+            if (e >= m.ninternal) return kWalkMiss;
+            const float4* w = m.wide + 4 * (size_t)e;
+            const float4 a = w[0], b = w[1], c = w[2], d = w[3];
+            const uint32_t el = __float_as_uint(d.x), er = __float_as_uint(d.y);
+            if (collide_aabb(ray, a.x, a.y, a.z, a.w, b.x, b.y)) {
+                ring.put(tail, el);
                 tail = (tail + 1u) % kQueue;
                 if (Ring::cap < kQueue && ((tail - head) % kQueue) > Ring::cap) return kWalkSpill;
             }
-            if (collide_aabb(&m.nodes[right], ray)) {
-                ring.put(tail, right);
+            if (collide_aabb(ray, b.z, b.w, c.x, c.y, c.z, c.w)) {
+                ring.put(tail, er);
                 tail = (tail + 1u) % kQueue;
                 if (Ring::cap < kQueue && ((tail - head) % kQueue) > Ring::cap) return kWalkSpill;
             }
@@ -461,9 +488,11 @@ int BvhTrimesh::compute(const float4* vertex, size_t vertex_count, const uint32_
         return 0;
     }
     const uint32_t t = (uint32_t)T, nv = (uint32_t)vertex_count;
+    count_ = 0;  // not built until this build succeeds
+    root_ = 0;
     if (!part_.fit((2 * kBoxBlocks + 2) * sizeof(float4)) || !keys_.fit(T * 4) || !vals_.fit(T * 4) ||
         !nodes_.fit((2 * T - 1) * sizeof(BvhNode)) || !split_.fit(T * 4) || !bound_.fit(T * 4) ||
-        !info_.fit(2 * sizeof(uint32_t)) || !sort_.reserve(T, 32))
+        !info_.fit(2 * sizeof(uint32_t)) || !wide_.fit((T - 1) * 4 * sizeof(float4)) || !sort_.reserve(T, 32))
         return 0;
     float4* part = part_.as<float4>();
     uint32_t* info = info_.as<uint32_t>();
@@ -479,6 +508,8 @@ int BvhTrimesh::compute(const float4* vertex, size_t vertex_count, const uint32_
                        index, vertex, nv, t, nodes(), split_.as<uint32_t>(), bound_.as<uint32_t>());
     hipLaunchKernelGGL(k_link_up, dim3(blocks_for(T)), dim3(kBlock), 0, s, nodes(), split_.as<uint32_t>(),
                        bound_.as<uint32_t>(), t, info);
+    hipLaunchKernelGGL(k_wide, dim3(blocks_for(T)), dim3(kBlock), 0, s, nodes(), t, 2 * T - 1 <= 0xFFFFu ? 0x8000u : 0x80000000u,
+                       wide_.as<float4>());
     MFL_HIP(hipGetLastError());
     uint32_t host[2] = {0, 0};
     MFL_HIP(hipMemcpyAsync(host, info, sizeof(host), hipMemcpyDeviceToHost, s));
@@ -518,7 +549,8 @@ int BvhTrimesh::collide(float4* force, float4* pos, float4* vel, size_t n, const
         return 0;
     }
     if (!spill_.fit((n + 1) * sizeof(uint32_t))) return 0;
-    const Mesh m{nodes(), count_, root_, triangle, (uint32_t)(index_count / 3), vertex, (uint32_t)vertex_count};
+    const Mesh m{wide_.as<float4>(), (count_ + 1u) / 2u - 1u, root_, leaf_tag(), triangle, (uint32_t)(index_count / 3),
+                 vertex, (uint32_t)vertex_count};
     uint32_t* spill = spill_.as<uint32_t>();
     hipLaunchKernelGGL(k_zero_count, dim3(1), dim3(64), 0, s, spill);
     if (count_ <= 0xFFFFu)

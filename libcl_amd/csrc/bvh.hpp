@@ -43,7 +43,10 @@ public:
 private:
     bool compiled_ = false;
     RadixSort sort_;
-    DeviceBuffer part_, keys_, vals_, nodes_, split_, bound_, info_, spill_;
+    // leaf entries of the walk: tag | triangle (bit 15 while the tree fits 16-bit ring slots)
+    uint32_t leaf_tag() const { return count_ <= 0xFFFFu ? 0x8000u : 0x80000000u; }
+
+    DeviceBuffer part_, keys_, vals_, nodes_, split_, bound_, info_, spill_, wide_;
     uint32_t count_ = 0, root_ = 0;
 };
 
