@@ -2,15 +2,19 @@
 // and particle-vs-mesh collision (clCollideBVH, libCL/phys/oclFluid3D.cl:365-550) for gfx950.
 //
 // Build: global box (two-level min/max reduction) -> Morton key per triangle -> 32-bit stable radix sort
-// -> node init (split bit of neighbouring keys, leaf boxes) -> one bottom-up pass that links the tree
-// and unites the boxes. The reference links each internal node by scanning left and right for the
-// nearest larger split bit (O(T) per node) and then walks every leaf from the root to unite the boxes;
-// here every leaf climbs once: a node covering leaves [l, r] is the child of split l-1 or split r,
-// whichever has the smaller split bit (ties: r), the first of the two children to arrive stops and the
-// second unites the boxes and climbs on. The tree is the one the reference's scans define (the
-// Cartesian tree of the split bits); where split bits tie, which only duplicate keys produce and where
-// the reference's parallel writes race, the left neighbour counts as the larger, which keeps the tree
-// well formed.
+// -> node init (split bit of neighbouring keys, leaf boxes) -> links -> boxes, with no atomics and no
+// cross-workgroup waits. The reference links each internal node by scanning left and right for the
+// nearest larger split bit (O(T) per node) and walks every leaf from the root to unite the boxes. The
+// tree it defines is the Cartesian tree of the split bits; where split bits tie, which only duplicate
+// keys produce and where the reference's parallel writes race, the left neighbour counts as the larger,
+// which keeps the tree well formed. Here each node finds its leaf range directly: for split bit b > 0
+// the keys sharing key[i]'s bits above b form one contiguous run of the sorted keys (galloping search
+// from i), for b = 0 the range starts at i itself (the tie rule); a node covering leaves [lo, hi] is the
+// child of split lo-1 or split hi, whichever has the smaller split bit (ties: hi), and writes itself
+// there. Boxes: a segment tree over the leaf boxes in leaf order, then every internal node unites its
+// leaf range with an order-preserving query. The reference's box of a node, min(left, right) nested over
+// its subtree with min(x, y) = y < x ? y : x, is the leftmost least value of the range (that min is
+// associative for non-NaN values), which the query returns as well: the boxes are bit-identical.
 //
 // Definitions the OpenCL sources leave to the implementation (the oracle uses the same ones):
 // min(x, y) = y < x ? y : x, max(x, y) = x < y ? y : x; dot(a, b) = ((a.x b.x + a.y b.y) + a.z b.z) +
@@ -20,7 +24,9 @@
 
 #include <algorithm>
 #include <cfloat>
+#include <cstring>
 #include <string>
+#include <vector>
 
 namespace mfl {
 namespace {
@@ -96,7 +102,7 @@ __global__ __launch_bounds__(kBlock) void k_box_partial(const float4* __restrict
                                                        float4* __restrict__ part, uint32_t* __restrict__ info)
 {
     __shared__ float4 smin[kBlock], smax[kBlock];
-    if (blockIdx.x == 0 && threadIdx.x == 0) info[1] = 0u;  // error word of this build
+    if (blockIdx.x == 0 && threadIdx.x < 2) info[threadIdx.x] = threadIdx.x == 0 ? 0xFFFFFFFFu : 0u;  // root, error
     float4 mn = empty_min(), mx = empty_max();
     for (uint32_t i = blockIdx.x * kBlock + threadIdx.x; i < nv; i += gridDim.x * kBlock) {
         const float4 v = vertex[i];
@@ -167,7 +173,7 @@ __device__ inline uint32_t split_bit(uint32_t k0, uint32_t k1)
 __global__ __launch_bounds__(kBlock) void k_nodes(const uint32_t* __restrict__ key, const uint32_t* __restrict__ val,
                                                  const uint32_t* __restrict__ index, const float4* __restrict__ vertex,
                                                  uint32_t nv, uint32_t T, BvhNode* __restrict__ nodes,
-                                                 uint32_t* __restrict__ split, uint32_t* __restrict__ bound)
+                                                 uint32_t* __restrict__ split, float4* __restrict__ seg0)
 {
     const uint32_t g = blockIdx.x * kBlock + threadIdx.x;
     if (g >= T) return;
@@ -182,7 +188,6 @@ __global__ __launch_bounds__(kBlock) void k_nodes(const uint32_t* __restrict__ k
         d.trav = 0u;
         nodes[g] = d;
         split[g] = b;
-        bound[g] = kNoBound;
     }
     const uint32_t tri = val[g];
     float4 v0, v1, v2;
@@ -195,48 +200,143 @@ __global__ __launch_bounds__(kBlock) void k_nodes(const uint32_t* __restrict__ k
     leaf.bit = tri;
     leaf.trav = 0u;
     nodes[T - 1 + g] = leaf;
+    seg0[2 * (size_t)g] = leaf.bbMin;  // level 0 of the box segment tree, leaf order
+    seg0[2 * (size_t)g + 1] = leaf.bbMax;
 }
 
-// ---- bottom-up link + boxes (clLinkNodes cl:144-196, clComputeAABBs cl:239-297) ------------------------
+// ---- links (clLinkNodes cl:144-196, clCreateLeaves cl:198-237) ------------------------------------------
 
-__global__ __launch_bounds__(kBlock) void k_link_up(BvhNode* __restrict__ nodes, const uint32_t* __restrict__ split,
-                                                   uint32_t* __restrict__ bound, uint32_t T, uint32_t* __restrict__ info)
+// first index of the run of sorted keys around `at` whose key >> shift equals key[at] >> shift (dir = -1),
+// or its last index (dir = +1): galloping, then bisection
+__device__ inline int32_t run_end(const uint32_t* __restrict__ key, int32_t T, int32_t at, uint32_t shift, int dir)
 {
-    const uint32_t g = blockIdx.x * kBlock + threadIdx.x;
-    if (g >= T) return;
-    uint32_t cur = T - 1 + g, l = g, r = g;
+    const uint32_t want = key[at] >> shift;
+    int32_t good = at, step = 1, bad;
     for (;;) {
-        // parent: split r (this node is its left child) or split l-1 (right child); the smaller split
-        // bit is the lower node, ties go to r
-        const bool to_r = l == 0u || (r != T - 1u && !(split[l - 1] < split[r]));
-        const uint32_t p = to_r ? r : l - 1u;
-        if (to_r)
-            nodes[p].left = (int32_t)cur;
-        else
-            nodes[p].right = (int32_t)cur;
-        // publish this child (its box and the pointer above) and its outer bound; the second arrival
-        // gets the sibling's bound back and owns the parent
-        const uint32_t other =
-            __hip_atomic_exchange(&bound[p], to_r ? l : r, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
-        if (other == kNoBound) return;
-        if (to_r)
-            r = other;
-        else
-            l = other;
-        const int32_t lc = nodes[p].left, rc = nodes[p].right;
-        if (lc < 0 || rc < 0 || (uint32_t)lc > 2u * (T - 1u) || (uint32_t)rc > 2u * (T - 1u)) {
-            info[1] = 2u;  // a child link is missing: report instead of reading outside the nodes
-            return;
+        const int32_t probe = good + dir * step;
+        if (probe < 0 || probe >= T) {
+            bad = dir < 0 ? -1 : T;
+            break;
         }
-        nodes[p].bbMin = min4(nodes[lc].bbMin, nodes[rc].bbMin);
-        nodes[p].bbMax = max4(nodes[lc].bbMax, nodes[rc].bbMax);
-        nodes[p].trav = 2u;  // cl:282-286: the first of the two arrivals leaves 2
-        cur = p;
-        if (l == 0u && r == T - 1u) {
-            info[0] = p;
-            return;
+        if ((key[probe] >> shift) != want) {
+            bad = probe;
+            break;
         }
+        good = probe;
+        step <<= 1;
     }
+    while ((bad - good) * dir > 1) {
+        const int32_t mid = good + (bad - good) / 2;
+        if ((key[mid] >> shift) == want)
+            good = mid;
+        else
+            bad = mid;
+    }
+    return good;
+}
+
+// One thread per node (internal i < T-1, leaf T-1+g): leaf range, parent, and the child pointer in the
+// parent; internal nodes keep their range for the box pass, the root reports itself in info[0].
+__global__ __launch_bounds__(kBlock) void k_link(const uint32_t* __restrict__ key, const uint32_t* __restrict__ split,
+                                                uint32_t T, BvhNode* __restrict__ nodes, uint2* __restrict__ range,
+                                                uint32_t* __restrict__ info)
+{
+    const uint32_t t = blockIdx.x * kBlock + threadIdx.x;
+    if (t >= 2 * T - 1) return;
+    uint32_t lo, hi;
+    if (t < T - 1) {
+        const uint32_t b = split[t];
+        lo = b == 0u ? t : (uint32_t)run_end(key, (int32_t)T, (int32_t)t, b + 1u, -1);
+        hi = (uint32_t)run_end(key, (int32_t)T, (int32_t)t + 1, b + 1u, +1);
+        range[t] = make_uint2(lo, hi);
+        if (lo == 0u && hi == T - 1u) {
+            info[0] = t;
+            return;
+        }
+    } else {
+        lo = hi = t - (T - 1u);
+    }
+    const bool to_hi = lo == 0u || (hi != T - 1u && !(split[lo - 1u] < split[hi]));
+    if (to_hi)
+        nodes[hi].left = (int32_t)t;
+    else
+        nodes[lo - 1u].right = (int32_t)t;
+}
+
+// ---- boxes (clComputeAABBs cl:239-297) --------------------------------------------------------------
+
+// Levels [k+1, k+8] of the segment tree from level k (n entries): block b reduces entries
+// [256 b, 256 b + 256) in LDS; level k+j entry e unites entries 2e and 2e+1 of level k+j-1 (left first).
+__global__ __launch_bounds__(kBlock) void k_seg_up(float4* __restrict__ seg, const uint64_t* __restrict__ offs,
+                                                  const uint32_t* __restrict__ sizes, uint32_t k, uint32_t levels)
+{
+    __shared__ float4 smin[kBlock], smax[kBlock];
+    const uint32_t t = threadIdx.x, e = blockIdx.x * kBlock + t;
+    const uint32_t n = sizes[k];
+    if (e < n) {
+        smin[t] = seg[2 * (offs[k] + e)];
+        smax[t] = seg[2 * (offs[k] + e) + 1];
+    }
+    __syncthreads();
+    uint32_t width = kBlock;
+    for (uint32_t j = 1; j <= levels; j++) {
+        width >>= 1;
+        const uint32_t nj = sizes[k + j], nprev = sizes[k + j - 1];
+        const uint32_t ej = blockIdx.x * width + t;  // entry of level k+j
+        float4 mn, mx;
+        const bool live = t < width && ej < nj;
+        if (live) {
+            mn = smin[2 * t];
+            mx = smax[2 * t];
+            if (2 * ej + 1 < nprev) {
+                mn = min4(mn, smin[2 * t + 1]);
+                mx = max4(mx, smax[2 * t + 1]);
+            }
+        }
+        __syncthreads();
+        if (live) {
+            smin[t] = mn;
+            smax[t] = mx;
+            seg[2 * (offs[k + j] + ej)] = mn;
+            seg[2 * (offs[k + j] + ej) + 1] = mx;
+        }
+        __syncthreads();
+    }
+}
+
+__global__ __launch_bounds__(kBlock) void k_node_boxes(const float4* __restrict__ seg, const uint64_t* __restrict__ offs,
+                                                      const uint2* __restrict__ range, uint32_t T,
+                                                      BvhNode* __restrict__ nodes)
+{
+    const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
+    if (i + 1 >= T) return;
+    const uint2 rg = range[i];
+    uint32_t l = rg.x, r = rg.y + 1u, k = 0;
+    float4 lmn = empty_min(), lmx = empty_max(), rmn = empty_min(), rmx = empty_max();
+    bool hl = false, hr = false;
+    while (l < r) {
+        const float4* lv = seg + 2 * offs[k];
+        if (l & 1u) {
+            const float4 a = lv[2 * l], b = lv[2 * l + 1];
+            lmn = hl ? min4(lmn, a) : a;
+            lmx = hl ? max4(lmx, b) : b;
+            hl = true;
+            l++;
+        }
+        if (r & 1u) {
+            r--;
+            const float4 a = lv[2 * r], b = lv[2 * r + 1];
+            rmn = hr ? min4(a, rmn) : a;
+            rmx = hr ? max4(b, rmx) : b;
+            hr = true;
+        }
+        l >>= 1;
+        r >>= 1;
+        k++;
+    }
+    nodes[i].bbMin = hl && hr ? min4(lmn, rmn) : (hl ? lmn : rmn);
+    nodes[i].bbMax = hl && hr ? max4(lmx, rmx) : (hl ? lmx : rmx);
+    nodes[i].trav = 2u;  // cl:282-286: the first of the two arrivals leaves 2
 }
 
 // ---- collision (clCollideBVH, libCL/phys/oclFluid3D.cl:365-550) ----------------------------------------
@@ -491,7 +591,7 @@ int BvhTrimesh::compute(const float4* vertex, size_t vertex_count, const uint32_
     count_ = 0;  // not built until this build succeeds
     root_ = 0;
     if (!part_.fit((2 * kBoxBlocks + 2) * sizeof(float4)) || !keys_.fit(T * 4) || !vals_.fit(T * 4) ||
-        !nodes_.fit((2 * T - 1) * sizeof(BvhNode)) || !split_.fit(T * 4) || !bound_.fit(T * 4) ||
+        !nodes_.fit((2 * T - 1) * sizeof(BvhNode)) || !split_.fit(T * 4) || !range_.fit(T * sizeof(uint2)) ||
         !info_.fit(2 * sizeof(uint32_t)) || !wide_.fit((T - 1) * 4 * sizeof(float4)) || !sort_.reserve(T, 32))
         return 0;
     float4* part = part_.as<float4>();
@@ -504,10 +604,36 @@ int BvhTrimesh::compute(const float4* vertex, size_t vertex_count, const uint32_
     MFL_HIP(hipGetLastError());
     // cpp:108-112: stable sort of all 32 key bits; the values start as 0..T-1
     if (!sort_.compute(keys_.as<uint32_t>(), vals_.as<uint32_t>(), T, 0, 32, s, true)) return 0;
+    // segment tree levels: sizes n_0 = T, n_{j+1} = ceil(n_j / 2) down to 1, entries of 2 float4
+    std::vector<uint64_t> offs{0};
+    std::vector<uint32_t> sizes{t};
+    while (sizes.back() > 1u) {
+        offs.push_back(offs.back() + sizes.back());
+        sizes.push_back((sizes.back() + 1u) / 2u);
+    }
+    const uint32_t levels = (uint32_t)sizes.size();
+    const uint64_t entries = offs.back() + sizes.back();
+    if (!seg_.fit(entries * 2 * sizeof(float4)) || !seg_meta_.fit(levels * (sizeof(uint64_t) + sizeof(uint32_t))))
+        return 0;
+    uint64_t* d_offs = seg_meta_.as<uint64_t>();
+    uint32_t* d_sizes = reinterpret_cast<uint32_t*>(d_offs + levels);
+    seg_host_.resize(levels * 3);
+    std::memcpy(seg_host_.data(), offs.data(), levels * sizeof(uint64_t));
+    std::memcpy(reinterpret_cast<char*>(seg_host_.data()) + levels * sizeof(uint64_t), sizes.data(),
+                levels * sizeof(uint32_t));
+    MFL_HIP(hipMemcpyAsync(d_offs, seg_host_.data(), levels * (sizeof(uint64_t) + sizeof(uint32_t)),
+                           hipMemcpyHostToDevice, s));
+    float4* seg = seg_.as<float4>();
     hipLaunchKernelGGL(k_nodes, dim3(blocks_for(T)), dim3(kBlock), 0, s, keys_.as<uint32_t>(), vals_.as<uint32_t>(),
-                       index, vertex, nv, t, nodes(), split_.as<uint32_t>(), bound_.as<uint32_t>());
-    hipLaunchKernelGGL(k_link_up, dim3(blocks_for(T)), dim3(kBlock), 0, s, nodes(), split_.as<uint32_t>(),
-                       bound_.as<uint32_t>(), t, info);
+                       index, vertex, nv, t, nodes(), split_.as<uint32_t>(), seg);
+    hipLaunchKernelGGL(k_link, dim3(blocks_for(2 * T - 1)), dim3(kBlock), 0, s, keys_.as<uint32_t>(),
+                       split_.as<uint32_t>(), t, nodes(), range_.as<uint2>(), info);
+    for (uint32_t k = 0; k + 1 < levels; k += 8) {
+        const uint32_t up = std::min<uint32_t>(8u, levels - 1u - k);
+        hipLaunchKernelGGL(k_seg_up, dim3(blocks_for(sizes[k])), dim3(kBlock), 0, s, seg, d_offs, d_sizes, k, up);
+    }
+    hipLaunchKernelGGL(k_node_boxes, dim3(blocks_for(T)), dim3(kBlock), 0, s, seg, d_offs, range_.as<uint2>(), t,
+                       nodes());
     hipLaunchKernelGGL(k_wide, dim3(blocks_for(T)), dim3(kBlock), 0, s, nodes(), t, 2 * T - 1 <= 0xFFFFu ? 0x8000u : 0x80000000u,
                        wide_.as<float4>());
     MFL_HIP(hipGetLastError());

@@ -2,6 +2,8 @@
 // versus mesh collision kernel clCollideBVH (libCL/phys/oclFluid3D.cl:365-550) on gfx950.
 #pragma once
 
+#include <vector>
+
 #include "mfl_common.hpp"
 #include "radix_sort.hpp"
 
@@ -46,7 +48,8 @@ private:
     // leaf entries of the walk: tag | triangle (bit 15 while the tree fits 16-bit ring slots)
     uint32_t leaf_tag() const { return count_ <= 0xFFFFu ? 0x8000u : 0x80000000u; }
 
-    DeviceBuffer part_, keys_, vals_, nodes_, split_, bound_, info_, spill_, wide_;
+    DeviceBuffer part_, keys_, vals_, nodes_, split_, range_, info_, spill_, wide_, seg_, seg_meta_;
+    std::vector<uint64_t> seg_host_;  // staging of the segment-tree level offsets and sizes
     uint32_t count_ = 0, root_ = 0;
 };
 
