@@ -5,8 +5,8 @@ python bench.py [--gpus 1] [--steps K] [--warmup W] [--config 1m|small] [--no-cp
 Workload: the 1,048,576-particle dam break (128x64x128 jittered lattice at 0.5h, h = 0.02, box
 [0,4]^3, fp32), inputs resident in HBM. A step is one oclFluid3D::compute(): hash + bucket clear,
 sort, reorder + bounds, density, forces + clip box + integration. Prints one JSON line; its
-secondary legs are the batched scenes, the adjoint and the particle-vs-mesh collision (BVH build +
-clCollideBVH on the same 1M particles).
+secondary legs are the batched scenes, the adjoint, the particle-vs-mesh collision (BVH build +
+clCollideBVH on the same 1M particles) and the 2D sibling solver oclFluid3Dnext at 1M particles.
 """
 from __future__ import annotations
 
@@ -267,6 +267,46 @@ def collide_leg(launches: int = 20, cpu: bool = True) -> dict:
     return out
 
 
+def next_leg(steps: int = 50, cpu: bool = True) -> dict:
+    """oclFluid3Dnext (the 2D dense-grid sibling solver) on 1,048,576 particles (config_next_1m): steps
+    timed with HIP events on one stream, inputs resident; the oracle times one step on the host."""
+    import torch
+
+    import libcl_amd as L
+    from libcl_amd import scenes
+
+    st = scenes.config_next_1m()
+    n = st.shape[0]
+    f = L.oclFluid3Dnext()
+    assert f.compile() == 1 and f.setParticleCount(n) == 1, L.last_error()
+    assert f.upload(st) == 1
+    stream = torch.cuda.Stream()
+    for _ in range(3):
+        assert f.compute(stream) == 1, L.last_error()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record(stream)
+    for _ in range(steps):
+        f.compute(stream)
+    e1.record(stream)
+    torch.cuda.synchronize()
+    ms = e0.elapsed_time(e1) / steps
+    out = {"workload": "next_1m_block", "particles": n, "steps": steps, "ms_per_step": ms,
+           "particle_steps_per_s": n / (ms * 1e-3)}
+    if cpu:
+        from oracle import oracle as O
+        O.build()
+        threads = O.set_threads(min(os.cpu_count() or 1, 16))
+        fo = O.Next(st, O.next_default_params())
+        fo.step()
+        t0 = time.perf_counter()
+        fo.step()
+        dt = time.perf_counter() - t0
+        out["cpu_baseline"] = {"value": n / dt, "unit": "particle-steps/s", "cores": threads, "kind": "port",
+                               "sample": f"one step of the same {n}-particle state, oracle/next_oracle.c with "
+                                         f"{threads} OpenMP threads in its neighbour passes ({dt:.2f} s)"}
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -279,7 +319,8 @@ def main():
     ap.add_argument("--no-batched", action="store_true")
     ap.add_argument("--no-adjoint", action="store_true")
     ap.add_argument("--no-collide", action="store_true")
-    ap.add_argument("--only", default=None, choices=["batched", "adjoint", "collide"],
+    ap.add_argument("--no-next", action="store_true")
+    ap.add_argument("--only", default=None, choices=["batched", "adjoint", "collide", "next"],
                     help="run one secondary leg alone and print its JSON (profiling)")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="also run a plain device copy of MIB MiB three times (counter calibration)")
@@ -314,6 +355,9 @@ def main():
         return
     if args.only == "collide":
         print(json.dumps(collide_leg(cpu=not args.no_cpu)), flush=True)
+        return
+    if args.only == "next":
+        print(json.dumps(next_leg(cpu=not args.no_cpu)), flush=True)
         return
 
     sc = scenes.config_1m() if args.config == "1m" else scenes.config_small()
@@ -432,6 +476,8 @@ def main():
         out["adjoint"] = adjoint_leg()
     if not args.no_collide:
         out["collide"] = collide_leg(cpu=not args.no_cpu)
+    if not args.no_next:
+        out["next"] = next_leg(cpu=not args.no_cpu)
     if not args.no_cpu:
         nthreads = min(os.cpu_count() or 1, 16)
         out["cpu_baseline"] = cpu_baseline(sc, prm, args.cpu_steps, nthreads)

@@ -258,6 +258,61 @@ int mfl_collide_bvh(void* force, void* position, void* velocity, size_t n, const
 int mfl_fluid3d_collide_bvh(mfl_fluid3d* f, mfl_bvh_trimesh* bvh, const uint32_t* triangle, size_t index_count,
                             const void* vertex, size_t vertex_count, void* stream);
 
+/* ---- oclFluid3Dnext (libCL/phys/oclFluid3Dnext.{h,cpp,cl}) ---------------------------------------- */
+
+/* System of libCL/phys/oclFluid3Dnext.cl:15-30 (Params, oclFluid3Dnext.h:46-61, names field 2
+   kernelRadius): 9 fp32 fields, then the grid size as two int32. */
+typedef struct mfl_next_params {
+    float deltaTime;
+    float particleRadius;
+    float density;
+    float stiffness0;
+    float stiffness1;
+    float viscosity0;
+    float viscosity1;
+    float radius;
+    float cellSize;
+    int32_t cellCountX;
+    int32_t cellCountY;
+} mfl_next_params;
+/* clInitFluid (oclFluid3Dnext.cl:95-113): dt 0.0333333333, density 10000, stiffness 0.004 / 0.01,
+   viscosity 1 / 1, radius 12, cellSize 2, grid 512 x 384, particleRadius 0.87 (1/density)^(1/3) */
+void mfl_next_params_default(mfl_next_params* p);
+/* buffers: the state (x, y, vx, vy as float4, caller order) is the program's "position" buffer
+   (setPositionBuffer, cpp:113-123); pressure and force are float2 per sorted particle */
+enum mfl_next_buffer {
+    MFL_NEXT_POSITION = 0,
+    MFL_NEXT_SORTED_STATE = 1,
+    MFL_NEXT_INDEX = 2,
+    MFL_NEXT_CELL = 3,
+    MFL_NEXT_CELL_START = 4,
+    MFL_NEXT_CELL_END = 5,
+    MFL_NEXT_PRESSURE = 6,
+    MFL_NEXT_FORCE = 7,
+    MFL_NEXT_PARAMS = 8
+};
+typedef struct mfl_fluid3dnext mfl_fluid3dnext;
+/* oclFluid3Dnext::oclFluid3Dnext / ~oclFluid3Dnext (cpp:19-82); 16,384 particles initially (cpp:53) */
+mfl_fluid3dnext* mfl_fluid3dnext_create(void);
+void mfl_fluid3dnext_destroy(mfl_fluid3dnext* f);
+/* compile (cpp:231-255): clInitFluid defaults, cell tables of cellCountX * cellCountY entries */
+int mfl_fluid3dnext_compile(mfl_fluid3dnext* f);
+/* setParticleCount / getParticleCount (cpp:85-111) */
+int mfl_fluid3dnext_set_particle_count(mfl_fluid3dnext* f, size_t n);
+size_t mfl_fluid3dnext_get_particle_count(mfl_fluid3dnext* f);
+/* getParameters (cpp:150-153): live host copy, applied at the next compute (the grid tables grow when
+   cellCountX * cellCountY does; the reference keeps its compile-time tables) */
+mfl_next_params* mfl_fluid3dnext_params(mfl_fluid3dnext* f);
+int mfl_fluid3dnext_set_buffer(mfl_fluid3dnext* f, int kind, void* ptr, size_t bytes);
+int mfl_fluid3dnext_get_buffer(mfl_fluid3dnext* f, int kind, void** ptr, size_t* bytes);
+int mfl_fluid3dnext_write_buffer(mfl_fluid3dnext* f, int kind, const void* host, size_t bytes);
+int mfl_fluid3dnext_read_buffer(mfl_fluid3dnext* f, int kind, void* host, size_t bytes);
+/* oclFluid3Dnext::compute (cpp:264-317): clHash, sort of bits [0, 24), clReorder, clInitBounds,
+   clFindBounds, clComputePressure, clComputeForces, clIntegrateForce. Fails for a particle count that
+   is not a non-zero multiple of 256 (cLocalSize). A cell index outside the grid is not recorded in
+   the bounds (the reference writes past its tables). */
+int mfl_fluid3dnext_compute(mfl_fluid3dnext* f, void* stream);
+
 /* ---- batched scenes (addition; no counterpart in the reference) --------------------------------- */
 
 /* Many independent oclFluid3D systems advanced together: one set of launches per step for all scenes

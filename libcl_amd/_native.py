@@ -41,11 +41,28 @@ EXPORTS = (
     "mfl_bvh_trimesh_create", "mfl_bvh_trimesh_destroy", "mfl_bvh_trimesh_compile", "mfl_bvh_trimesh_compute",
     "mfl_bvh_trimesh_get_node_buffer", "mfl_bvh_trimesh_get_node_count", "mfl_bvh_trimesh_get_root_node",
     "mfl_bvh_trimesh_get_morton", "mfl_collide_bvh", "mfl_fluid3d_collide_bvh",
+    "mfl_next_params_default", "mfl_fluid3dnext_create", "mfl_fluid3dnext_destroy", "mfl_fluid3dnext_compile",
+    "mfl_fluid3dnext_set_particle_count", "mfl_fluid3dnext_get_particle_count", "mfl_fluid3dnext_params",
+    "mfl_fluid3dnext_set_buffer", "mfl_fluid3dnext_get_buffer", "mfl_fluid3dnext_write_buffer",
+    "mfl_fluid3dnext_read_buffer", "mfl_fluid3dnext_compute",
 )
+MFL_NEXT_POSITION, MFL_NEXT_SORTED_STATE, MFL_NEXT_INDEX, MFL_NEXT_CELL = 0, 1, 2, 3
+MFL_NEXT_CELL_START, MFL_NEXT_CELL_END, MFL_NEXT_PRESSURE, MFL_NEXT_FORCE, MFL_NEXT_PARAMS = 4, 5, 6, 7, 8
+NEXT_FIELDS = ("deltaTime", "particleRadius", "density", "stiffness0", "stiffness1", "viscosity0", "viscosity1",
+               "radius", "cellSize", "cellCountX", "cellCountY")
 MFL_ADJ_POSITION_GRAD, MFL_ADJ_VELOCITY_GRAD = 0, 1
 
 STAGES = ("hash", "sort", "reorder", "bounds", "density", "forces", "order")
 TRAVERSALS = ("sorted", "caller", "morton")
+
+
+class NextParams(ctypes.Structure):
+    """mfl_next_params == System of libCL/phys/oclFluid3Dnext.cl:15-30 (9 x fp32, 2 x int32)."""
+    _fields_ = [(name, ctypes.c_float) for name in NEXT_FIELDS[:9]] + [("cellCountX", ctypes.c_int32),
+                                                                        ("cellCountY", ctypes.c_int32)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k in NEXT_FIELDS}
 
 
 class Params(ctypes.Structure):
@@ -170,6 +187,18 @@ def load() -> ctypes.CDLL:
                                                       ctypes.POINTER(sz)]),
         "mfl_collide_bvh": (ctypes.c_int, [vp, vp, vp, sz, pp, vp, vp, sz, vp, sz, vp]),
         "mfl_fluid3d_collide_bvh": (ctypes.c_int, [vp, vp, vp, sz, vp, sz, vp]),
+        "mfl_next_params_default": (None, [ctypes.POINTER(NextParams)]),
+        "mfl_fluid3dnext_create": (vp, []),
+        "mfl_fluid3dnext_destroy": (None, [vp]),
+        "mfl_fluid3dnext_compile": (ctypes.c_int, [vp]),
+        "mfl_fluid3dnext_set_particle_count": (ctypes.c_int, [vp, sz]),
+        "mfl_fluid3dnext_get_particle_count": (sz, [vp]),
+        "mfl_fluid3dnext_params": (ctypes.POINTER(NextParams), [vp]),
+        "mfl_fluid3dnext_set_buffer": (ctypes.c_int, [vp, ctypes.c_int, vp, sz]),
+        "mfl_fluid3dnext_get_buffer": (ctypes.c_int, [vp, ctypes.c_int, ctypes.POINTER(vp), ctypes.POINTER(sz)]),
+        "mfl_fluid3dnext_write_buffer": (ctypes.c_int, [vp, ctypes.c_int, vp, sz]),
+        "mfl_fluid3dnext_read_buffer": (ctypes.c_int, [vp, ctypes.c_int, vp, sz]),
+        "mfl_fluid3dnext_compute": (ctypes.c_int, [vp, vp]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(L, name)

@@ -13,7 +13,8 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 BUILD_DIR = os.path.join(HERE, "_build")
 LIB_PATH = os.path.join(HERE, "libmfl.so")
-SOURCES = ["runtime.hip", "radix_sort.hip", "fluid3d.hip", "batch.hip", "adjoint.hip", "bvh.hip", "capi.hip"]
+SOURCES = ["runtime.hip", "radix_sort.hip", "fluid3d.hip", "batch.hip", "adjoint.hip", "bvh.hip", "fluid3dnext.hip",
+           "capi.hip"]
 ARCH = "gfx950"
 
 # Packed FP32 (v_pk_*_f32) generation is disabled for all device code: hand-written kernels here use

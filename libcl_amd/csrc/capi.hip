@@ -6,6 +6,7 @@
 #include "adjoint.hpp"
 #include "batch.hpp"
 #include "bvh.hpp"
+#include "fluid3dnext.hpp"
 #include "fluid3d.hpp"
 #include "radix_sort.hpp"
 
@@ -24,6 +25,10 @@ struct mfl_radixsort {
 
 struct mfl_bvh_trimesh {
     mfl::BvhTrimesh impl;
+};
+
+struct mfl_fluid3dnext {
+    mfl::Fluid3DNext impl;
 };
 
 struct mfl_adjoint {
@@ -574,6 +579,88 @@ int mfl_fluid3d_collide_bvh(mfl_fluid3d* f, mfl_bvh_trimesh* bvh, const uint32_t
                                  f->impl.particle_count(), f->impl.params, triangle, index_count,
                                  static_cast<const float4*>(vertex), vertex_count, (hipStream_t)stream);
     });
+}
+
+}  // extern "C"
+
+// ---- oclFluid3Dnext --------------------------------------------------------------------------------
+
+extern "C" {
+
+void mfl_next_params_default(mfl_next_params* p)
+{
+    if (!p) return;
+    // clInitFluid, libCL/phys/oclFluid3Dnext.cl:95-113 (double literals rounded into the float fields)
+    std::memset(p, 0, sizeof(*p));
+    p->deltaTime = (float)0.0333333333;
+    p->density = 10000.0f;
+    p->stiffness0 = (float)0.004;
+    p->stiffness1 = (float)0.01;
+    p->viscosity0 = 1.0f;
+    p->viscosity1 = 1.0f;
+    p->radius = 12.0f;
+    p->cellSize = 2.0f;
+    p->cellCountX = (int32_t)(1024 / p->cellSize);
+    p->cellCountY = (int32_t)(768 / p->cellSize);
+    p->particleRadius = (float)(0.87 * std::pow(1.0 / (double)p->density, (double)(1 / 3.0f)));
+}
+
+mfl_fluid3dnext* mfl_fluid3dnext_create(void)
+{
+    try {
+        return new mfl_fluid3dnext();
+    } catch (...) {
+        mfl::set_error("out of host memory");
+        return nullptr;
+    }
+}
+
+void mfl_fluid3dnext_destroy(mfl_fluid3dnext* f) { delete f; }
+
+int mfl_fluid3dnext_compile(mfl_fluid3dnext* f)
+{
+    if (!need(f, "oclFluid3Dnext")) return 0;
+    return guarded([&] { return f->impl.compile(); });
+}
+
+int mfl_fluid3dnext_set_particle_count(mfl_fluid3dnext* f, size_t n)
+{
+    if (!need(f, "oclFluid3Dnext")) return 0;
+    return guarded([&] { return f->impl.set_particle_count(n); });
+}
+
+size_t mfl_fluid3dnext_get_particle_count(mfl_fluid3dnext* f) { return f ? f->impl.particle_count() : 0; }
+
+mfl_next_params* mfl_fluid3dnext_params(mfl_fluid3dnext* f) { return f ? &f->impl.params : nullptr; }
+
+int mfl_fluid3dnext_set_buffer(mfl_fluid3dnext* f, int kind, void* ptr, size_t bytes)
+{
+    if (!need(f, "oclFluid3Dnext")) return 0;
+    return guarded([&] { return f->impl.set_buffer(kind, ptr, bytes); });
+}
+
+int mfl_fluid3dnext_get_buffer(mfl_fluid3dnext* f, int kind, void** ptr, size_t* bytes)
+{
+    if (!need(f, "oclFluid3Dnext")) return 0;
+    return guarded([&] { return f->impl.get_buffer(kind, ptr, bytes); });
+}
+
+int mfl_fluid3dnext_write_buffer(mfl_fluid3dnext* f, int kind, const void* host, size_t bytes)
+{
+    if (!need(f, "oclFluid3Dnext")) return 0;
+    return guarded([&] { return f->impl.write_buffer(kind, host, bytes); });
+}
+
+int mfl_fluid3dnext_read_buffer(mfl_fluid3dnext* f, int kind, void* host, size_t bytes)
+{
+    if (!need(f, "oclFluid3Dnext")) return 0;
+    return guarded([&] { return f->impl.read_buffer(kind, host, bytes); });
+}
+
+int mfl_fluid3dnext_compute(mfl_fluid3dnext* f, void* stream)
+{
+    if (!need(f, "oclFluid3Dnext")) return 0;
+    return guarded([&] { return f->impl.compute((hipStream_t)stream); });
 }
 
 }  // extern "C"

@@ -402,6 +402,77 @@ def collide_bvh(force, position, velocity, params: N.Params, bvh: oclBvhTrimesh,
                                         vertex.data_ptr(), vertex.numel() // 4, _stream_ptr(stream)))
 
 
+class oclFluid3Dnext:
+    """libCL/phys/oclFluid3Dnext.h:20-130 over libmfl's gfx950 kernels: a 2D particle state (x, y, vx, vy)
+    per float4 on a dense cellCountX x cellCountY grid."""
+
+    cLocalSize = 256
+
+    def __init__(self):
+        L = N.load()
+        self._h = L.mfl_fluid3dnext_create()
+        if not self._h:
+            raise RuntimeError(N.last_error())
+        self._user = None
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h and N is not None and N._lib is not None:
+            N._lib.mfl_fluid3dnext_destroy(h)
+            self._h = None
+
+    def compile(self) -> int:
+        return int(N.load().mfl_fluid3dnext_compile(self._h))
+
+    def compute(self, stream=None) -> int:
+        return int(N.load().mfl_fluid3dnext_compute(self._h, _stream_ptr(stream)))
+
+    def setParticleCount(self, n: int) -> int:
+        return int(N.load().mfl_fluid3dnext_set_particle_count(self._h, n))
+
+    def getParticleCount(self) -> int:
+        return int(N.load().mfl_fluid3dnext_get_particle_count(self._h))
+
+    def getParameters(self) -> N.NextParams:
+        """Live host Params (getParameters, cpp:150-153); edits apply at the next compute."""
+        return N.load().mfl_fluid3dnext_params(self._h).contents
+
+    def setPositionBuffer(self, tensor) -> int:
+        _f4_tensor(tensor, "state")
+        ok = int(N.load().mfl_fluid3dnext_set_buffer(self._h, N.MFL_NEXT_POSITION, tensor.data_ptr(),
+                                                     tensor.numel() * 4))
+        if ok:
+            self._user = tensor
+        return ok
+
+    def _read(self, kind: int, dtype, cols: int):
+        ptr, nbytes = ctypes.c_void_p(), ctypes.c_size_t()
+        if not N.load().mfl_fluid3dnext_get_buffer(self._h, kind, ctypes.byref(ptr), ctypes.byref(nbytes)):
+            raise RuntimeError(N.last_error())
+        out = np.empty(nbytes.value // np.dtype(dtype).itemsize, dtype)
+        if not N.load().mfl_fluid3dnext_read_buffer(self._h, kind, out.ctypes.data, out.nbytes):
+            raise RuntimeError(N.last_error())
+        return out.reshape(-1, cols) if cols > 1 else out
+
+    def upload(self, state: np.ndarray) -> int:
+        state = np.ascontiguousarray(state, np.float32)
+        return int(N.load().mfl_fluid3dnext_write_buffer(self._h, N.MFL_NEXT_POSITION, state.ctypes.data,
+                                                         state.nbytes))
+
+    def state(self) -> dict:
+        n = self.getParticleCount()
+        return dict(state=self._read(N.MFL_NEXT_POSITION, np.float32, 4)[:n])
+
+    def internals(self) -> dict:
+        n = self.getParticleCount()
+        return dict(cell=self._read(N.MFL_NEXT_CELL, np.uint32, 1)[:n], index=self._read(N.MFL_NEXT_INDEX, np.uint32, 1)[:n],
+                    sorted=self._read(N.MFL_NEXT_SORTED_STATE, np.float32, 4)[:n],
+                    cell_start=self._read(N.MFL_NEXT_CELL_START, np.uint32, 1),
+                    cell_end=self._read(N.MFL_NEXT_CELL_END, np.uint32, 1),
+                    pressure=self._read(N.MFL_NEXT_PRESSURE, np.float32, 2)[:n],
+                    force=self._read(N.MFL_NEXT_FORCE, np.float32, 2)[:n])
+
+
 class FluidBatch:
     """Many independent oclFluid3D scenes advanced by one set of launches per step (an addition with no
     reference counterpart; each scene behaves as its own oclFluid3D, libCL/phys/oclFluid3D.cpp:293-342)."""

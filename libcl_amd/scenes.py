@@ -247,3 +247,25 @@ def config_collide_1m() -> tuple[Scene, Mesh]:
     sc.name = "collide_1m_floor9k"
     mesh = floor_mesh(96, 48, (0.0, 0.0), (1.28, 0.64), 0.6, 0.002, 4, name="floor9k")
     return sc, mesh
+
+
+# ---- 2D states for oclFluid3Dnext ----------------------------------------------------------------------
+
+def next_block(nx: int, ny: int, spacing: float = 0.7, origin=(20.0, 20.0), jitter: float = 0.1, seed: int = 0,
+               vel=(0.0, 0.0)) -> np.ndarray:
+    """float32 [nx * ny, 4] states (x, y, vx, vy) of a jittered nx x ny block (x fastest) inside the
+    reference's 1024 x 768 domain (clamp of libCL/phys/oclFluid3Dnext.cl:258)."""
+    iy, ix = np.meshgrid(np.arange(ny), np.arange(nx), indexing="ij")
+    n = nx * ny
+    u = uniform(seed, 2 * n).reshape(n, 2)
+    st = np.zeros((n, 4), np.float32)
+    st[:, 0] = (origin[0] + ix.ravel() * spacing + (2 * u[:, 0] - 1) * jitter).astype(np.float32)
+    st[:, 1] = (origin[1] + iy.ravel() * spacing + (2 * u[:, 1] - 1) * jitter).astype(np.float32)
+    st[:, 2] = vel[0]
+    st[:, 3] = vel[1]
+    return st
+
+
+def config_next_1m() -> np.ndarray:
+    """1,048,576 particles: a 1024 x 1024 block at spacing 0.7 from (20, 20), jitter 0.1 (seed 8), at rest."""
+    return next_block(1024, 1024, 0.7, (20.0, 20.0), 0.1, 8)

@@ -36,6 +36,19 @@ class Params(ctypes.Structure):
         return {k: float(getattr(self, k)) for k in PARAM_FIELDS}
 
 
+NEXT_FIELDS = ("deltaTime", "particleRadius", "density", "stiffness0", "stiffness1", "viscosity0", "viscosity1",
+               "radius", "cellSize", "cellCountX", "cellCountY")
+
+
+class NextParams(ctypes.Structure):
+    """System of libCL/phys/oclFluid3Dnext.cl:15-30 (9 x fp32, 2 x int32)."""
+    _fields_ = [(name, ctypes.c_float) for name in NEXT_FIELDS[:9]] + [("cellCountX", ctypes.c_int32),
+                                                                        ("cellCountY", ctypes.c_int32)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k in NEXT_FIELDS}
+
+
 _f32p = ctypes.POINTER(ctypes.c_float)
 _u32p = ctypes.POINTER(ctypes.c_uint32)
 
@@ -99,6 +112,17 @@ def lib():
         L.orc_steps.restype = ctypes.c_int
         L.orc_set_threads.argtypes = [ctypes.c_int]
         L.orc_set_threads.restype = ctypes.c_int
+        NP = ctypes.POINTER(NextParams)
+        L.orc_next_init_params.argtypes = [NP]
+        L.orc_next_w.argtypes = [ctypes.c_float, ctypes.c_float]
+        L.orc_next_w.restype = ctypes.c_float
+        L.orc_next_hash.argtypes = [_f32p, ctypes.c_uint32, NP, _u32p, _u32p]
+        L.orc_next_reorder.argtypes = [_u32p, _f32p, _f32p, ctypes.c_uint32]
+        L.orc_next_bounds.argtypes = [_u32p, ctypes.c_uint32, ctypes.c_uint32, _u32p, _u32p]
+        L.orc_next_pressure.argtypes = [_f32p, _f32p, _u32p, _u32p, ctypes.c_uint32, NP]
+        L.orc_next_forces.argtypes = [_f32p, _f32p, _f32p, _u32p, _u32p, ctypes.c_uint32, NP]
+        L.orc_next_integrate.argtypes = [_f32p, _f32p, _u32p, ctypes.c_uint32, NP]
+        L.orc_next_step.argtypes = [_f32p, ctypes.c_uint32, NP, _u32p, _u32p, _f32p, _u32p, _u32p, _f32p, _f32p]
         L.orc_morton_code.argtypes = [ctypes.c_uint32]
         L.orc_morton_code.restype = ctypes.c_uint32
         L.orc_bvh_aabb.argtypes = [_f32p, ctypes.c_uint32, _f32p, _f32p]
@@ -330,3 +354,52 @@ def collide_bvh(pos, vel, force, params: Params, nodes, root, index, vertex, fil
     lib().orc_collide_bvh(_f(force), _f(pos), _f(vel), pos.shape[0], ctypes.byref(params), nodes.ctypes.data, root,
                           _u(index), _f(vertex), _u(hit), None if filled is None else _u(filled))
     return pos, vel, force, hit
+
+
+# ---- oclFluid3Dnext (oracle/next_oracle.c) -------------------------------------------------------------
+
+def next_default_params() -> NextParams:
+    p = NextParams()
+    lib().orc_next_init_params(ctypes.byref(p))
+    return p
+
+
+def next_w(h: float, q: float) -> float:
+    return float(lib().orc_next_w(h, q))
+
+
+class Next:
+    """CPU state of one oclFluid3Dnext program: state (x, y, vx, vy) in caller order plus its scratch."""
+
+    def __init__(self, state, params: NextParams):
+        n = state.shape[0]
+        self.params = params
+        self.state = np.ascontiguousarray(state, np.float32).copy()
+        self.cell = np.zeros(n, np.uint32)
+        self.index = np.zeros(n, np.uint32)
+        self.sorted = np.zeros((n, 4), np.float32)
+        nc = int(params.cellCountX) * int(params.cellCountY)
+        self.start = np.zeros(nc, np.uint32)
+        self.end = np.zeros(nc, np.uint32)
+        self.pressure = np.zeros((n, 2), np.float32)
+        self.force = np.zeros((n, 2), np.float32)
+
+    def step(self):
+        lib().orc_next_step(_f(self.state), self.state.shape[0], ctypes.byref(self.params), _u(self.cell),
+                            _u(self.index), _f(self.sorted), _u(self.start), _u(self.end), _f(self.pressure),
+                            _f(self.force))
+
+    def hash(self):
+        lib().orc_next_hash(_f(self.state), self.state.shape[0], ctypes.byref(self.params), _u(self.cell),
+                            _u(self.index))
+
+    def bounds(self):
+        lib().orc_next_bounds(_u(self.cell), self.cell.shape[0], self.start.shape[0], _u(self.start), _u(self.end))
+
+    def pressure_pass(self):
+        lib().orc_next_pressure(_f(self.sorted), _f(self.pressure), _u(self.start), _u(self.end),
+                                self.sorted.shape[0], ctypes.byref(self.params))
+
+    def force_pass(self):
+        lib().orc_next_forces(_f(self.sorted), _f(self.pressure), _f(self.force), _u(self.start), _u(self.end),
+                              self.sorted.shape[0], ctypes.byref(self.params))
