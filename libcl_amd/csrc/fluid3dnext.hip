@@ -14,6 +14,7 @@
 // clamp = fmin(fmax(x, lo), hi); cell indices outside the grid are not recorded in the bounds.
 #include "fluid3dnext.hpp"
 
+#include <cstring>
 #include <string>
 
 namespace mfl {
@@ -205,6 +206,8 @@ int Fluid3DNext::compile()
     mfl_next_params_default(&params);  // clInitFluid, cl:95-113
     if (!fit_tables() || !allocate_internal()) return 0;
     MFL_HIP(hipMemcpy(params_buf_.ptr, &params, sizeof(params), hipMemcpyHostToDevice));
+    uploaded_ = params;
+    params_synced_ = true;
     compiled_ = true;
     return 1;
 }
@@ -317,7 +320,11 @@ int Fluid3DNext::compute(hipStream_t s)
     auto* index = index_.as<uint32_t>();
     auto* start = start_.as<uint32_t>();
     auto* end = end_.as<uint32_t>();
-    MFL_HIP(hipMemcpyAsync(params_buf_.ptr, &params, sizeof(params), hipMemcpyHostToDevice, s));
+    if (!params_synced_ || std::memcmp(&params, &uploaded_, sizeof(params)) != 0) {  // bfParams, on change only
+        uploaded_ = params;
+        MFL_HIP(hipMemcpyAsync(params_buf_.ptr, &uploaded_, sizeof(params), hipMemcpyHostToDevice, s));
+        params_synced_ = true;
+    }
     hipLaunchKernelGGL(k_next_hash, dim3(blocks_for(n)), dim3(kBlock), 0, s, state, n, params, cell, index, start, end,
                        ncells_);
     MFL_HIP(hipGetLastError());

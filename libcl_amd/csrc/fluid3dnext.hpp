@@ -34,6 +34,8 @@ private:
     size_t n_ = 16384;        // 128 * 128, the reference's initial count (cpp:53)
     uint32_t ncells_ = 256;   // mCellCount before compile (cpp:56)
     bool compiled_ = false;
+    mfl_next_params uploaded_{};  // the Params last copied to params_buf_
+    bool params_synced_ = false;
 };
 
 }  // namespace mfl
