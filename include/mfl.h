@@ -338,9 +338,10 @@ int mfl_batch_write_buffer(mfl_fluid_batch* b, int kind, const void* host, size_
 int mfl_batch_read_buffer(mfl_fluid_batch* b, int kind, void* host, size_t bytes);
 int mfl_batch_compute(mfl_fluid_batch* b, void* stream);
 /* Launch shape of the batched step, as mfl_fluid3d_*_tuning: MFL_PASS_LIST (group traversal in sorted
-   order, density_lanes 8) or MFL_PASS_TILE_LIST (the tile density pass in a Morton order of each scene's
-   cells, refreshed every 16 steps; density_lanes 4 or 8), recorded-visit forces either way. The
-   autotuner times the candidates from the batch's current state and leaves that state unchanged. */
+   order, density_lanes 8, recorded-visit forces), MFL_PASS_TILE_LIST or MFL_PASS_TILE_KEPT (the tile
+   density pass in a Morton order of each scene's cells, refreshed every 16 steps; density_lanes 4 or 8;
+   recorded-visit or kept-entry forces). The autotuner times the candidates from the batch's current
+   state and leaves that state unchanged. */
 int mfl_batch_get_tuning(mfl_fluid_batch* b, mfl_tuning* out);
 int mfl_batch_set_tuning(mfl_fluid_batch* b, const mfl_tuning* t);
 int mfl_batch_autotune(mfl_fluid_batch* b, void* stream, uint32_t steps, mfl_tuning* best, double* ms_per_step);

@@ -189,7 +189,7 @@ __device__ __noinline__ uint32_t b_density_walk(int cx, int cy, int cz, float4 p
             if (j != il && accept_r2(r2, h, h2_lo, h2_hi)) {
                 const float w = h2 - r2;
                 acc = fmaf(w * w, w, acc);
-                if (count < kNbrCap) nbr[nbr_at(ig, count, N)] = j;
+                if (nbr && count < kNbrCap) nbr[nbr_at(ig, count, N)] = j;
                 count++;
             }
         }
@@ -198,11 +198,13 @@ __device__ __noinline__ uint32_t b_density_walk(int cx, int cy, int cz, float4 p
     return count;
 }
 
-// The tile density pass of the single-scene program (k_density_tile, TILE_LIST form) over all scenes:
-// traversal slot t in the batch's Morton order, each 8-particle segment inside one scene (scene counts
-// are multiples of 256 and the order keeps scenes in place), that scene's Params, tables and
-// scene-local indices; the visits are recorded by sorted slot for k_b_forces_step.
-template <int W>
+// The tile density pass of the single-scene program (k_density_tile) over all scenes: traversal slot t
+// in the batch's Morton order, each 8-particle segment inside one scene (scene counts are multiples of
+// 256 and the order keeps scenes in place), that scene's Params, tables and scene-local indices.
+// KEPT = false records the visits by sorted slot for k_b_forces_step (TILE_LIST); KEPT = true writes
+// each cell's kept entries, the lanes' acceptance bits and the per-particle meta for k_b_forces_kept
+// (TILE_KEPT), with the same layout as the single-scene pass.
+template <int W, bool KEPT>
 __global__ __launch_bounds__(kThreads) void k_b_density_tile(const float4* __restrict__ pos, float4* __restrict__ spos,
                                                              float4* __restrict__ svel,
                                                              const uint32_t* __restrict__ cs_all,
@@ -212,7 +214,9 @@ __global__ __launch_bounds__(kThreads) void k_b_density_tile(const float4* __res
                                                              const mfl_params* __restrict__ params, uint32_t N,
                                                              uint32_t depth, const uint32_t* __restrict__ porder,
                                                              const uint32_t* __restrict__ inv,
-                                                             uint32_t* __restrict__ nbr, uint32_t* __restrict__ ncount)
+                                                             uint32_t* __restrict__ nbr, uint32_t* __restrict__ ncount,
+                                                             uint32_t* __restrict__ kept_out,
+                                                             uint64_t* __restrict__ hitmask, uint32_t* __restrict__ meta)
 {
     constexpr uint32_t P = W / 8;
     constexpr uint32_t SEGS = kWave / W;
@@ -250,6 +254,8 @@ __global__ __launch_bounds__(kThreads) void k_b_density_tile(const float4* __res
     const int gx = grid_coord(pi.x, h), gy = grid_coord(pi.y, h), gz = grid_coord(pi.z, h);
     float acc = 0.f;
     uint32_t count = 0;
+    uint32_t used = 0;  // KEPT: entries of this segment's kept buffer already written
+    const uint32_t gseg = wave * SEGS + L.seg;
     uint64_t todo = L.bits(__ballot(live));
     while (__ballot(todo != 0) != 0) {
         const SegCell c = next_cell_seg<W>(L, todo, live, gx, gy, gz, pi);
@@ -291,7 +297,23 @@ __global__ __launch_bounds__(kThreads) void k_b_density_tile(const float4* __res
                     hits = (hits << 1) + (ok ? 1ull : 0ull);
                 }
             }
-            if (c.member) {
+            if constexpr (KEPT) {
+                if (c.member) {
+                    acc += a;
+                    count += (uint32_t)__popcll(hits);
+                }
+                if (used + kept <= kSegCap) {
+                    for (uint32_t q = L.ls; q < kept; q += W)
+                        kept_out[(size_t)gseg * kSegCap + used + q] = __float_as_uint(tile[q].w);
+                    if (c.member) {
+                        hitmask[(size_t)t * P + slot] = hits;
+                        if (slot == 0) meta[t] = used | (kept << 16);
+                    }
+                    used += kept;
+                } else if (c.member && slot == 0) {
+                    meta[t] = kMetaWalk;
+                }
+            } else if (c.member) {
                 acc += a;
                 const uint32_t mine = (uint32_t)__popcll(hits);
                 count += mine;
@@ -310,8 +332,9 @@ __global__ __launch_bounds__(kThreads) void k_b_density_tile(const float4* __res
                 }
             }
         } else if (c.member && slot == 0) {
+            if (KEPT) meta[t] = kMetaWalk;
             float a = 0.f;
-            count += b_density_walk(c.cx, c.cy, c.cz, pi, il, i, sp, cs, ce, n_s, h, N, nbr, &a);
+            count += b_density_walk(c.cx, c.cy, c.cz, pi, il, i, sp, cs, ce, n_s, h, N, KEPT ? nullptr : nbr, &a);
             acc += a;
         }
         wave_sync();
@@ -329,7 +352,138 @@ __global__ __launch_bounds__(kThreads) void k_b_density_tile(const float4* __res
     }
     spos[i].w = rho;
     svel[i].w = prs;
-    ncount[i] = count;
+    if (!KEPT) ncount[i] = count;
+}
+
+__device__ __noinline__ void b_force_fallback(const float4* __restrict__ sp, const float4* __restrict__ sv,
+                                             const uint32_t* __restrict__ cs, const uint32_t* __restrict__ ce,
+                                             uint32_t il, uint32_t n_s, float4 pi, float4 vi, float h, float3* P,
+                                             float3* V)
+{
+    sph_force_sum(sp, sv, cs, ce, il, n_s, pi, vi, h, *P, *V);
+}
+
+// The kept-entry force pass of the single-scene program (k_forces_kept) over all scenes, with the same
+// waves and segments as k_b_density_tile<W, true>: each segment stages its cell's kept entries (scene-
+// local indices) 64 at a time in LDS and every lane sums the pair terms its acceptance bits mark,
+// earliest trip first; then the scene's clip box, clamp, gravity and Euler update.
+template <int W>
+__global__ __launch_bounds__(kThreads) void k_b_forces_kept(const float4* __restrict__ spos,
+                                                            const float4* __restrict__ svel, float4* __restrict__ pos,
+                                                            float4* __restrict__ vel, float4* __restrict__ force,
+                                                            const uint32_t* __restrict__ cs_all,
+                                                            const uint32_t* __restrict__ ce_all,
+                                                            const uint32_t* __restrict__ off,
+                                                            const uint32_t* __restrict__ pscene,
+                                                            const mfl_params* __restrict__ params,
+                                                            const float4* __restrict__ boxes, uint32_t N,
+                                                            const uint32_t* __restrict__ porder,
+                                                            const uint32_t* __restrict__ inv,
+                                                            const uint32_t* __restrict__ kept_in,
+                                                            const uint64_t* __restrict__ hitmask,
+                                                            const uint32_t* __restrict__ meta, float4 gravity_force)
+{
+    constexpr uint32_t P = W / 8;
+    constexpr uint32_t SEGS = kWave / W;
+    extern __shared__ float4 lds[];
+    const Lanes<W> L(threadIdx.x & (kWave - 1));
+    const uint32_t sid = (threadIdx.x / kWave) * SEGS + L.seg;
+    float4* tpos = lds + (size_t)sid * 2 * kKeptChunk;
+    float4* tvel = tpos + kKeptChunk;
+    const uint32_t grp = L.ls / P, slot = L.ls % P;
+    const uint32_t wave = (xcd_block() * blockDim.x + threadIdx.x) / kWave;
+    const uint32_t t = wave * (kWave / P) + L.seg * 8 + grp;
+    const uint32_t gseg = wave * SEGS + L.seg;
+    uint32_t i = N, o = N, sc = 0;
+    if (t < N) {
+        o = porder[t];
+        if (o < N) {
+            i = inv[o];
+            sc = pscene[o];
+        }
+    }
+    const bool live = i < N;
+    const uint32_t s = L.from(sc, 0u);
+    const uint32_t lo = off[s], n_s = off[s + 1] - lo;
+    const float4* sp = spos + lo;
+    const float4* sv = svel + lo;
+    const mfl_params p = params[s];
+    const float h = p.cellSize;
+    const float4 pi = live ? spos[i] : make_float4(0.f, 0.f, 0.f, 0.f);
+    const float4 vi = live ? svel[i] : make_float4(0.f, 0.f, 0.f, 0.f);
+    const uint32_t mt = live ? meta[t] : kMetaWalk;
+    const bool staged = live && mt != kMetaWalk;
+    const uint64_t hm = staged ? hitmask[(size_t)t * P + slot] : 0ull;
+    float3 Pa = make_float3(0.f, 0.f, 0.f), Va = make_float3(0.f, 0.f, 0.f);
+    uint64_t todo = L.bits(__ballot(staged));
+    while (__ballot(todo != 0) != 0) {
+        const bool active = todo != 0;
+        const uint32_t src = active ? (uint32_t)(__ffsll((unsigned long long)todo) - 1) : 0u;
+        const uint32_t gm = L.from(mt, src);
+        const bool member = active && staged && mt == gm;
+        todo &= ~L.bits(__ballot(member));
+        const uint32_t offk = active ? (gm & 0xFFFFu) : 0u, kept = active ? (gm >> 16) : 0u;
+        const uint32_t* kj = kept_in + (size_t)gseg * kSegCap + offk;
+        uint32_t kmax = kept;
+        if constexpr (W < kWave) kmax = max(kmax, (uint32_t)__shfl_xor((int)kmax, W, kWave));
+        kmax = __builtin_amdgcn_readfirstlane(kmax);
+        const uint32_t trips = (kept + P - 1) / P;
+        for (uint32_t c0 = 0; c0 < kmax; c0 += kKeptChunk) {
+            for (uint32_t q0 = 0; q0 < kKeptChunk; q0 += 2 * W) {
+                const uint32_t qa = q0 + L.ls, qb = qa + W;
+                const bool ha = c0 + qa < kept, hb = qb < kKeptChunk && c0 + qb < kept;
+                const uint32_t ja = ha ? kj[c0 + qa] : 0u, jb = hb ? kj[c0 + qb] : 0u;
+                float4 pa, va, pb, vb;
+                if (ha) {
+                    pa = sp[ja];
+                    va = sv[ja];
+                }
+                if (hb) {
+                    pb = sp[jb];
+                    vb = sv[jb];
+                }
+                if (ha) {
+                    tpos[qa] = pa;
+                    tvel[qa] = va;
+                }
+                if (hb) {
+                    tpos[qb] = pb;
+                    tvel[qb] = vb;
+                }
+            }
+            wave_sync();
+            if (member) {
+                const uint32_t mlo = c0 / P, mhi = min((c0 + kKeptChunk) / P, trips);
+                uint64_t bits = 0;
+                if (mlo < mhi) {
+                    const uint32_t blo = trips - mhi, bhi = trips - mlo;
+                    const uint64_t upto = bhi >= 64 ? ~0ull : (1ull << bhi) - 1ull;
+                    bits = hm & upto & ~((1ull << blo) - 1ull);
+                }
+                while (bits) {
+                    const uint32_t b = 63u - (uint32_t)__builtin_clzll(bits);  // earliest trip first
+                    bits &= ~(1ull << b);
+                    const uint32_t q = (trips - 1u - b) * P + slot - c0;
+                    const float4 pj = tpos[q], vj = tvel[q];
+                    const float dx = pi.x - pj.x, dy = pi.y - pj.y, dz = pi.z - pj.z;
+                    pair_force(pi, vi, pj, vj, dx, dy, dz, dist2(dx, dy, dz), h, Pa, Va);
+                }
+            }
+            wave_sync();
+        }
+    }
+    if (live && !staged && slot == 0)
+        b_force_fallback(sp, sv, cs_all + (size_t)s * kBuckets, ce_all + (size_t)s * kBuckets, i - lo, n_s, pi, vi, h,
+                         &Pa, &Va);
+    Pa = make_float3(particle_sum<W>(Pa.x), particle_sum<W>(Pa.y), particle_sum<W>(Pa.z));
+    Va = make_float3(particle_sum<W>(Va.x), particle_sum<W>(Va.y), particle_sum<W>(Va.z));
+    if (!live || slot != 0) return;
+    float4 f = add_sph_force(force[o], Pa, Va, p);
+    f = clip_box(f, pi, vi, p.particleRadius, boxes[2 * s], boxes[2 * s + 1]);
+    const float4 v = integrate_force(vel[o], f, p);
+    force[o] = gravity_force;
+    vel[o] = v;
+    pos[o] = f4_add(pos[o], f4_scale(v, p.deltaTime));
 }
 
 template <int CH>
@@ -566,19 +720,27 @@ int FluidBatch::enqueue_step(hipStream_t s)
         hipLaunchKernelGGL(k_b_clear, dim3(blocks), b, 0, s, cs_all, ce_all, S_, clear);
     }
     hipLaunchKernelGGL(k_b_find_bounds, g, b, 0, s, cell, off, N_, cs_all, ce_all);
-    if (neighbour_pass == MFL_PASS_TILE_LIST && porder_valid_) {
-        const uint32_t lanes = (uint32_t)density_lanes, ppw = kWave / lanes;
-        const size_t waves = ((size_t)N_ + ppw - 1) / ppw;
-        size_t blocks = (waves + block_waves - 1) / block_waves;
-        blocks = (blocks + kXcds - 1) / kXcds * kXcds;
+    const bool tiles = (neighbour_pass == MFL_PASS_TILE_LIST || neighbour_pass == MFL_PASS_TILE_KEPT) && porder_valid_;
+    const bool kept = tiles && neighbour_pass == MFL_PASS_TILE_KEPT;
+    const uint32_t lanes = (uint32_t)density_lanes, ppw = kWave / lanes;
+    const size_t waves = ((size_t)N_ + ppw - 1) / ppw;
+    size_t blocks = (waves + block_waves - 1) / block_waves;
+    blocks = (blocks + kXcds - 1) / kXcds * kXcds;
+    if (tiles) {
         const size_t lds = (size_t)block_waves * (8 / lanes) * ((tile_depth + kGroupLanes) * sizeof(float4) +
                                                                 sizeof(StageAux));
-#define MFL_B_TILE(W)                                                                                              \
-    hipLaunchKernelGGL(k_b_density_tile<W>, dim3((unsigned)blocks), dim3(block_waves * kWave), lds, s, pos, spos,  \
+#define MFL_B_TILE(W, K)                                                                                           \
+    hipLaunchKernelGGL((k_b_density_tile<W, K>), dim3((unsigned)blocks), dim3(block_waves * kWave), lds, s, pos, spos, \
                        svel, cs_all, ce_all, off, pscene_.as<uint32_t>(), params, N_, tile_depth,                     \
-                       porder_.as<uint32_t>(), inv_.as<uint32_t>(), nbr_.as<uint32_t>(), ncount_.as<uint32_t>())
-        if (lanes == 4) MFL_B_TILE(32);
-        else MFL_B_TILE(64);
+                       porder_.as<uint32_t>(), inv_.as<uint32_t>(), nbr_.as<uint32_t>(), ncount_.as<uint32_t>(),     \
+                       kept_.as<uint32_t>(), hitmask_.as<uint64_t>(), meta_.as<uint32_t>())
+        if (lanes == 4) {
+            if (kept) MFL_B_TILE(32, true);
+            else MFL_B_TILE(32, false);
+        } else {
+            if (kept) MFL_B_TILE(64, true);
+            else MFL_B_TILE(64, false);
+        }
 #undef MFL_B_TILE
     } else {
         hipLaunchKernelGGL(k_b_density, grid_for((size_t)N_ * kGroupLanes), b, 0, s, spos, svel, cs_all, ce_all, cell,
@@ -588,8 +750,21 @@ int FluidBatch::enqueue_step(hipStream_t s)
     hipLaunchKernelGGL(k_b_forces_step<CH>, g, b, 0, s, spos, svel, pos, vel, force, index, cell, cs_all, ce_all, off, \
                        params, boxes_.as<float4>(), nbr_.as<uint32_t>(), ncount_.as<uint32_t>(), N_,              \
                        gravity_force_of(gravity))
-    if (force_chunks == 2) MFL_B_FORCES(2);
-    else MFL_B_FORCES(1);
+    if (kept) {
+        const size_t lds_k = (size_t)block_waves * (8 / lanes) * kKeptChunk * 2 * sizeof(float4);
+#define MFL_B_KEPT(W)                                                                                              \
+    hipLaunchKernelGGL(k_b_forces_kept<W>, dim3((unsigned)blocks), dim3(block_waves * kWave), lds_k, s, spos, svel,  \
+                       pos, vel, force, cs_all, ce_all, off, pscene_.as<uint32_t>(), params, boxes_.as<float4>(), N_, \
+                       porder_.as<uint32_t>(), inv_.as<uint32_t>(), kept_.as<uint32_t>(), hitmask_.as<uint64_t>(),  \
+                       meta_.as<uint32_t>(), gravity_force_of(gravity))
+        if (lanes == 4) MFL_B_KEPT(32);
+        else MFL_B_KEPT(64);
+#undef MFL_B_KEPT
+    } else if (force_chunks == 2) {
+        MFL_B_FORCES(2);
+    } else {
+        MFL_B_FORCES(1);
+    }
 #undef MFL_B_FORCES
     MFL_HIP(hipGetLastError());
     return 1;
@@ -646,7 +821,7 @@ int FluidBatch::compute(hipStream_t s)
         MFL_HIP(hipMemcpy(boxes_.ptr, boxes.data(), boxes.size() * 16, hipMemcpyHostToDevice));
         params_dirty_ = false;
     }
-    if (neighbour_pass == MFL_PASS_TILE_LIST &&
+    if (neighbour_pass != MFL_PASS_LIST &&
         (!porder_valid_ || (refresh_interval > 0 && steps_since_refresh_ >= refresh_interval))) {
         if (!refresh_order(s)) return 0;
         steps_since_refresh_ = 0;
@@ -685,13 +860,14 @@ int FluidBatch::get_tuning(mfl_tuning* out) const
 
 int FluidBatch::set_tuning(const mfl_tuning& t)
 {
-    if (t.neighbour_pass != MFL_PASS_LIST && t.neighbour_pass != MFL_PASS_TILE_LIST) {
-        set_error("batched scenes take MFL_PASS_LIST or MFL_PASS_TILE_LIST");
+    if (t.neighbour_pass != MFL_PASS_LIST && t.neighbour_pass != MFL_PASS_TILE_LIST &&
+        t.neighbour_pass != MFL_PASS_TILE_KEPT) {
+        set_error("batched scenes take MFL_PASS_LIST, MFL_PASS_TILE_LIST or MFL_PASS_TILE_KEPT");
         return 0;
     }
-    const bool tile = t.neighbour_pass == MFL_PASS_TILE_LIST;
+    const bool tile = t.neighbour_pass != MFL_PASS_LIST;
     if (tile ? (t.density_lanes != 4 && t.density_lanes != 8) : t.density_lanes != 8) {
-        set_error(tile ? "density_lanes must be 4 or 8 with MFL_PASS_TILE_LIST"
+        set_error(tile ? "density_lanes must be 4 or 8 with the tile passes"
                        : "density_lanes must be 8 with MFL_PASS_LIST for batched scenes");
         return 0;
     }
@@ -716,6 +892,13 @@ int FluidBatch::set_tuning(const mfl_tuning& t)
     if (t.force_chunks != 1 && t.force_chunks != 2) {
         set_error("force_chunks must be 1 or 2");
         return 0;
+    }
+    if (t.neighbour_pass == MFL_PASS_TILE_KEPT && !kept_.ptr) {
+        // kept entries of every 8-particle segment, acceptance bits of every lane, meta per particle
+        const size_t segs = ((size_t)N_ + 7) / 8;
+        if (!kept_.allocate(segs * kSegCap * 4) || !hitmask_.allocate((size_t)N_ * 8 * sizeof(uint64_t)) ||
+            !meta_.allocate((size_t)N_ * 4))
+            return 0;
     }
     neighbour_pass = t.neighbour_pass;
     density_lanes = t.density_lanes;
@@ -763,12 +946,14 @@ int FluidBatch::autotune(hipStream_t s, uint32_t steps, mfl_tuning* best, double
     for (uint32_t w : {1u, 2u, 4u})
         for (uint32_t d : {128u, 192u, 256u, 384u})
             for (int lanes : {4, 8})
-                for (uint32_t ch : {1u, 2u}) {
-                    const mfl_tuning t{MFL_PASS_TILE_LIST, lanes, d, w, start.sort_items, ch};
-                    if (d <= 64u * (uint32_t)lanes &&
-                        (size_t)w * (8 / lanes) * ((d + kGroupLanes) * sizeof(float4) + sizeof(StageAux)) <= 65536)
-                        cand.push_back(t);
-                }
+                for (int pass : {MFL_PASS_TILE_LIST, MFL_PASS_TILE_KEPT})
+                    for (uint32_t ch : {1u, 2u}) {
+                        if (pass == MFL_PASS_TILE_KEPT && ch == 2u) continue;  // chunks only shape the list pass
+                        const mfl_tuning t{pass, lanes, d, w, start.sort_items, ch};
+                        if (d <= 64u * (uint32_t)lanes &&
+                            (size_t)w * (8 / lanes) * ((d + kGroupLanes) * sizeof(float4) + sizeof(StageAux)) <= 65536)
+                            cand.push_back(t);
+                    }
     const bool graph_was = use_graph;
     const uint32_t since = steps_since_refresh_;
     use_graph = false;

@@ -35,7 +35,7 @@ public:
 
     // launch shape (mfl_tuning): LIST (group traversal in sorted order) or TILE_LIST (the single-scene
     // tile density pass in a per-scene Morton order); recorded-visit forces either way
-    int neighbour_pass = MFL_PASS_TILE_LIST;
+    int neighbour_pass = MFL_PASS_TILE_KEPT;
     int density_lanes = 4;
     uint32_t tile_depth = 192;
     uint32_t block_waves = 2;
@@ -54,6 +54,7 @@ private:
     bool porder_valid_ = false;
     uint32_t steps_since_refresh_ = 0;
     DeviceBuffer inv_, porder_, mkey_;
+    DeviceBuffer kept_, hitmask_, meta_;  // TILE_KEPT: per-segment kept entries, acceptance bits, meta
 
     bool ok_ = false;
     uint32_t S_ = 0, N_ = 0, scene_bits_ = 0;

@@ -499,12 +499,8 @@ __device__ __noinline__ VisitSums density_walk(int cx, int cy, int cz, float4 pi
 }
 
 // Outputs of the tile density pass: none, recorded visit lists, or each cell's kept entries plus the
-// lanes' acceptance bits (kept buffer of kSegCap entries per 8-particle segment; meta[t] = offset | kept
-// << 16 of the particle's cell, kMetaWalk when the force pass must walk the ranges itself).
+// lanes' acceptance bits (kSegCap, kMetaWalk and kKeptChunk in tile.hpp).
 enum : int { kOutNone = 0, kOutList = 1, kOutKept = 2 };
-constexpr uint32_t kSegCap = 512;
-constexpr uint32_t kMetaWalk = 0xFFFFFFFFu;
-constexpr uint32_t kKeptChunk = 64;  // kept entries per segment staged at once by the force pass (multiple of 8)
 
 // Density pass over LDS-staged neighbourhoods (cl:150-200). The wave is split into 64 / W segments of
 // 8 particles (W / 8 lanes each); each segment stages its cell's neighbourhood in its own `depth + 8`

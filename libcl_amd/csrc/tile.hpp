@@ -20,6 +20,13 @@ namespace sph {
 constexpr float kKeepMargin = 1.0001f;      // h^2 (1 + 1e-4): squared-distance margin of the box prune
 constexpr uint32_t kStageRounds = 8;        // visits of one neighbourhood laid end to end: at most 8 x 64
 constexpr uint32_t kTileMinDepth = 64, kTileMaxDepth = kStageRounds * 64;  // kept <= staged <= 512
+// Kept-entry passes: the density pass writes each 8-particle segment's kept entries (at most kSegCap per
+// segment over its cells) and, per particle, meta = offset | kept << 16 of its cell's entries, or
+// kMetaWalk when the force pass must walk the bucket ranges itself; the force pass stages kKeptChunk
+// entries per segment at a time (a multiple of 8).
+constexpr uint32_t kSegCap = 512;
+constexpr uint32_t kMetaWalk = 0xFFFFFFFFu;
+constexpr uint32_t kKeptChunk = 64;
 
 // Per-wave LDS bookkeeping of a staging: bit p % 64 of starts[p / 64] marks position p where a
 // non-empty bucket range begins, and delta[c] maps a position of the c-th non-empty range to its

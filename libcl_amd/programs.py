@@ -524,7 +524,7 @@ class FluidBatch:
         return t.as_dict()
 
     def setTuning(self, **kw) -> int:
-        """As oclFluid3D.setTuning; batched scenes take the "list" and "tile_list" passes."""
+        """As oclFluid3D.setTuning; batched scenes take the "list", "tile_list" and "tile_kept" passes."""
         cur = self.getTuning()
         if "density_lanes" not in kw and kw.get("neighbour_pass") == "list":
             kw = dict(kw, density_lanes=8)

@@ -15,7 +15,7 @@ def _plain(sc, prm, steps, shape="list"):
     if shape == "list":
         assert f.setTuning(neighbour_pass="list") == 1
     else:
-        assert f.setTuning(neighbour_pass="tile_list", density_lanes=4, tile_depth=192, block_waves=2,
+        assert f.setTuning(neighbour_pass=shape, density_lanes=4, tile_depth=192, block_waves=2,
                            force_chunks=2) == 1
     assert f.setParticleCount(sc.n) == 1
     f.setParameters(prm)
@@ -28,11 +28,11 @@ def _plain(sc, prm, steps, shape="list"):
 
 
 @pytest.mark.parametrize("graph", [False, True])
-@pytest.mark.parametrize("shape", ["list", "tile_list"])
+@pytest.mark.parametrize("shape", ["list", "tile_list", "tile_kept"])
 def test_batch_matches_single_scene_runs(graph, shape):
     """Every scene of the batch equals the same scene run alone by a single-scene program with the same
-    launch shape (LIST: group traversal; TILE_LIST: 4-lane tiles in a Morton order refreshed every 16
-    steps), bit for bit."""
+    launch shape (LIST: group traversal; TILE_LIST / TILE_KEPT: 4-lane tiles in a Morton order refreshed
+    every 16 steps, recorded-visit or kept-entry forces), bit for bit."""
     import torch
     import libcl_amd as L
     from libcl_amd import _native as N
