@@ -80,6 +80,11 @@ int mfl_memcpy(void* dst, const void* src, size_t bytes);
 
 /* Defaults of clInitFluid (libCL/phys/oclFluid3D.cl:130-148), evaluated on the host. */
 void mfl_params_default(mfl_params* p);
+/* The least float T with sqrtf(T) >= h: the neighbour passes accept a visit at squared distance r2
+   when r2 < T, which for IEEE (correctly rounded, non-decreasing) sqrt is exactly sqrt(r2) < h, the
+   library's form of the reference's fast_length(d) < h (libCL/phys/oclFluid3D.cl:177-178). Host only;
+   no counterpart in the reference. */
+float mfl_sqrt_cut(float h);
 /* kernelConstant, viscosityConstant, pressureConstant from cellSize and viscosity, with the
    expressions of libCL/phys/oclFluid3D.cl:124-128,145-147. Never applied implicitly. */
 void mfl_params_derive(mfl_params* p);

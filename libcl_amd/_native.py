@@ -22,7 +22,7 @@ PARAM_FIELDS = (
 # every symbol include/mfl.h declares
 EXPORTS = (
     "mfl_last_error", "mfl_clear_error", "mfl_abi_version", "mfl_set_device", "mfl_device_finish", "mfl_memcpy",
-    "mfl_params_default", "mfl_params_derive",
+    "mfl_params_default", "mfl_params_derive", "mfl_sqrt_cut",
     "mfl_radixsort_create", "mfl_radixsort_destroy", "mfl_radixsort_compile", "mfl_radixsort_compute",
     "mfl_fluid3d_create", "mfl_fluid3d_destroy", "mfl_fluid3d_compile", "mfl_fluid3d_compute",
     "mfl_fluid3d_set_particle_count", "mfl_fluid3d_get_particle_count", "mfl_fluid3d_set_buffer",
@@ -122,6 +122,7 @@ def load() -> ctypes.CDLL:
         "mfl_device_finish": (ctypes.c_int, []),
         "mfl_memcpy": (ctypes.c_int, [vp, vp, sz]),
         "mfl_params_default": (None, [pp]),
+        "mfl_sqrt_cut": (ctypes.c_float, [ctypes.c_float]),
         "mfl_params_derive": (None, [pp]),
         "mfl_radixsort_create": (vp, []),
         "mfl_radixsort_destroy": (None, [vp]),

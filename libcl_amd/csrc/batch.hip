@@ -216,7 +216,8 @@ __global__ __launch_bounds__(kThreads) void k_b_density_tile(const float4* __res
                                                              const uint32_t* __restrict__ inv,
                                                              uint32_t* __restrict__ nbr, uint32_t* __restrict__ ncount,
                                                              uint32_t* __restrict__ kept_out,
-                                                             uint64_t* __restrict__ hitmask, uint32_t* __restrict__ meta)
+                                                             uint64_t* __restrict__ hitmask, uint32_t* __restrict__ meta,
+                                                             const float* __restrict__ cuts)
 {
     constexpr uint32_t P = W / 8;
     constexpr uint32_t SEGS = kWave / W;
@@ -249,7 +250,7 @@ __global__ __launch_bounds__(kThreads) void k_b_density_tile(const float4* __res
     const uint32_t il = live ? i - lo : n_s;
     const float h = p.cellSize;
     const float h2 = h * h;
-    const float h2_lo = h2 * 0.9999996f, h2_hi = h2 * 1.0000005f;
+    const float cut = cuts[s];  // sqrt(r2) < h exactly when r2 < cut
     const float inf = __builtin_inff();
     const int gx = grid_coord(pi.x, h), gy = grid_coord(pi.y, h), gz = grid_coord(pi.z, h);
     float acc = 0.f;
@@ -271,32 +272,16 @@ __global__ __launch_bounds__(kThreads) void k_b_density_tile(const float4* __res
             const uint32_t qmax = kept + slot;
             float a = 0.f;
             uint64_t hits = 0;
-            bool band = false;
             for (uint32_t m = 0; m < wtrips; m++) {
                 const float4 e = tile[min(m * P + slot, qmax)];
                 const float dx = pi.x - e.x, dy = pi.y - e.y, dz = pi.z - e.z;
                 const float r2 = dist2(dx, dy, dz);
-                const bool other = __float_as_uint(e.w) != il;
-                const bool ok = other && r2 < h2_lo;
-                band |= other & (r2 >= h2_lo) & (r2 <= h2_hi);
+                const bool ok = __float_as_uint(e.w) != il && r2 < cut;
                 const float w = h2 - r2;
                 a = ok ? fmaf(w * w, w, a) : a;
                 hits = (hits << 1) + (ok ? 1ull : 0ull);
             }
             hits >>= wtrips - trips;
-            if (__ballot(band && c.member) != 0) {
-                a = 0.f;
-                hits = 0;
-                for (uint32_t m = 0; m < trips; m++) {
-                    const float4 e = tile[m * P + slot];
-                    const float dx = pi.x - e.x, dy = pi.y - e.y, dz = pi.z - e.z;
-                    const float r2 = dist2(dx, dy, dz);
-                    const bool ok = __float_as_uint(e.w) != il && accept_r2(r2, h, h2_lo, h2_hi);
-                    const float w = h2 - r2;
-                    a = ok ? fmaf(w * w, w, a) : a;
-                    hits = (hits << 1) + (ok ? 1ull : 0ull);
-                }
-            }
             if constexpr (KEPT) {
                 if (c.member) {
                     acc += a;
@@ -567,6 +552,7 @@ FluidBatch::FluidBatch(uint32_t num_scenes, const uint32_t* counts)
         for (uint32_t g = off_host_[s]; g < off_host_[s + 1]; g++) pscene[g] = s;
     const size_t tables = (size_t)S_ * kBuckets * 4;
     if (!off_.allocate((S_ + 1) * 4) || !pscene_.allocate(u1) || !params_.allocate(S_ * sizeof(mfl_params)) ||
+        !cuts_.allocate(S_ * sizeof(float)) ||
         !boxes_.allocate((size_t)S_ * 32) || !pos_.allocate(v4) || !vel_.allocate(v4) || !force_.allocate(v4) ||
         !cell_.allocate(u1) || !index_.allocate(u1) || !spos_.allocate(v4) || !svel_.allocate(v4) ||
         !tables_.allocate(2 * tables) || !nbr_.allocate((size_t)kNbrCap * u1) || !ncount_.allocate(u1) ||
@@ -722,6 +708,10 @@ int FluidBatch::enqueue_step(hipStream_t s)
     hipLaunchKernelGGL(k_b_find_bounds, g, b, 0, s, cell, off, N_, cs_all, ce_all);
     const bool tiles = (neighbour_pass == MFL_PASS_TILE_LIST || neighbour_pass == MFL_PASS_TILE_KEPT) && porder_valid_;
     const bool kept = tiles && neighbour_pass == MFL_PASS_TILE_KEPT;
+    if (kept && !(kept_.ptr && hitmask_.ptr && meta_.ptr)) {
+        set_error("kept-entry buffers are not allocated");
+        return 0;
+    }
     const uint32_t lanes = (uint32_t)density_lanes, ppw = kWave / lanes;
     const size_t waves = ((size_t)N_ + ppw - 1) / ppw;
     size_t blocks = (waves + block_waves - 1) / block_waves;
@@ -733,7 +723,7 @@ int FluidBatch::enqueue_step(hipStream_t s)
     hipLaunchKernelGGL((k_b_density_tile<W, K>), dim3((unsigned)blocks), dim3(block_waves * kWave), lds, s, pos, spos, \
                        svel, cs_all, ce_all, off, pscene_.as<uint32_t>(), params, N_, tile_depth,                     \
                        porder_.as<uint32_t>(), inv_.as<uint32_t>(), nbr_.as<uint32_t>(), ncount_.as<uint32_t>(),     \
-                       kept_.as<uint32_t>(), hitmask_.as<uint64_t>(), meta_.as<uint32_t>())
+                       kept_.as<uint32_t>(), hitmask_.as<uint64_t>(), meta_.as<uint32_t>(), cuts_.as<float>())
         if (lanes == 4) {
             if (kept) MFL_B_TILE(32, true);
             else MFL_B_TILE(32, false);
@@ -805,12 +795,23 @@ int FluidBatch::launch_graph(hipStream_t s)
     return 1;
 }
 
+bool FluidBatch::fit_kept()
+{
+    if (kept_.ptr && hitmask_.ptr && meta_.ptr) return true;
+    // kept entries of every 8-particle segment, acceptance bits of every lane, meta per particle
+    const size_t segs = ((size_t)N_ + 7) / 8;
+    return kept_.allocate(segs * kSegCap * 4) && hitmask_.allocate((size_t)N_ * 8 * sizeof(uint64_t)) &&
+           meta_.allocate((size_t)N_ * 4);
+}
+
 int FluidBatch::compute(hipStream_t s)
 {
     if (!ok_) {
         set_error("batch was not created");
         return 0;
     }
+    // the kept-entry pass (the default) writes these; allocate before any launch or graph capture
+    if (neighbour_pass == MFL_PASS_TILE_KEPT && !fit_kept()) return 0;
     if (params_dirty_) {
         // the scene Params and boxes are read from device arrays, so their values are not baked into a
         // captured graph; refreshing them needs no re-capture
@@ -818,6 +819,9 @@ int FluidBatch::compute(hipStream_t s)
         std::memcpy(boxes.data(), box_host_.data(), box_host_.size() * 4);
         MFL_HIP(hipStreamSynchronize(s));  // steps in flight may still read the previous values
         MFL_HIP(hipMemcpy(params_.ptr, params_host_.data(), S_ * sizeof(mfl_params), hipMemcpyHostToDevice));
+        std::vector<float> cuts(S_);
+        for (uint32_t q = 0; q < S_; q++) cuts[q] = sqrt_cut(params_host_[q].cellSize);
+        MFL_HIP(hipMemcpy(cuts_.ptr, cuts.data(), S_ * sizeof(float), hipMemcpyHostToDevice));
         MFL_HIP(hipMemcpy(boxes_.ptr, boxes.data(), boxes.size() * 16, hipMemcpyHostToDevice));
         params_dirty_ = false;
     }
@@ -893,13 +897,7 @@ int FluidBatch::set_tuning(const mfl_tuning& t)
         set_error("force_chunks must be 1 or 2");
         return 0;
     }
-    if (t.neighbour_pass == MFL_PASS_TILE_KEPT && !kept_.ptr) {
-        // kept entries of every 8-particle segment, acceptance bits of every lane, meta per particle
-        const size_t segs = ((size_t)N_ + 7) / 8;
-        if (!kept_.allocate(segs * kSegCap * 4) || !hitmask_.allocate((size_t)N_ * 8 * sizeof(uint64_t)) ||
-            !meta_.allocate((size_t)N_ * 4))
-            return 0;
-    }
+    if (t.neighbour_pass == MFL_PASS_TILE_KEPT && !fit_kept()) return 0;
     neighbour_pass = t.neighbour_pass;
     density_lanes = t.density_lanes;
     tile_depth = t.tile_depth;

@@ -55,6 +55,7 @@ private:
     uint32_t steps_since_refresh_ = 0;
     DeviceBuffer inv_, porder_, mkey_;
     DeviceBuffer kept_, hitmask_, meta_;  // TILE_KEPT: per-segment kept entries, acceptance bits, meta
+    bool fit_kept();                      // allocates them on first use
 
     bool ok_ = false;
     uint32_t S_ = 0, N_ = 0, scene_bits_ = 0;
@@ -62,7 +63,7 @@ private:
     std::vector<mfl_params> params_host_;
     std::vector<float> box_host_;  // per scene: min xyzw, max xyzw
     bool params_dirty_ = true;
-    DeviceBuffer off_, pscene_, params_, boxes_;
+    DeviceBuffer off_, pscene_, params_, boxes_, cuts_;  // cuts_: sqrt_cut(cellSize) per scene
     DeviceBuffer pos_, vel_, force_, cell_, index_, spos_, svel_, tables_, nbr_, ncount_;
     hipGraph_t graph_ = nullptr;
     hipGraphExec_t graph_exec_ = nullptr;

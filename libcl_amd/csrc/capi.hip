@@ -113,6 +113,8 @@ void mfl_params_derive(mfl_params* p)
     p->pressureConstant = (float)(-c);                           // cl:147
 }
 
+float mfl_sqrt_cut(float h) { return mfl::sqrt_cut(h); }
+
 void mfl_params_default(mfl_params* p)
 {
     if (!p) return;

@@ -516,7 +516,8 @@ __global__ __launch_bounds__(kThreads) void k_density_tile(const float4* __restr
                                                            const uint32_t* __restrict__ porder,
                                                            const uint32_t* __restrict__ inv, uint32_t* __restrict__ nbr,
                                                            uint32_t* __restrict__ ncount, uint32_t* __restrict__ kept_out,
-                                                           uint64_t* __restrict__ hitmask, uint32_t* __restrict__ meta)
+                                                           uint64_t* __restrict__ hitmask, uint32_t* __restrict__ meta,
+                                                           float cut)
 {
     constexpr bool LIST = OUT == kOutList;
     constexpr uint32_t P = W / 8;  // lanes per particle
@@ -534,7 +535,6 @@ __global__ __launch_bounds__(kThreads) void k_density_tile(const float4* __restr
     const bool live = i < n;
     const float h = p.cellSize;
     const float h2 = h * h;
-    const float h2_lo = h2 * 0.9999996f, h2_hi = h2 * 1.0000005f;
     const float inf = __builtin_inff();
     const int gx = grid_coord(pi.x, h), gy = grid_coord(pi.y, h), gz = grid_coord(pi.z, h);
     float acc = 0.f;
@@ -560,34 +560,17 @@ __global__ __launch_bounds__(kThreads) void k_density_tile(const float4* __restr
             const uint32_t qmax = kept + slot;
             float a = 0.f;
             uint64_t hits = 0;
-            bool band = false;
             for (uint32_t m = 0; m < wtrips; m++) {
                 const float4 e = tile[min(m * P + slot, qmax)];
                 const float dx = pi.x - e.x, dy = pi.y - e.y, dz = pi.z - e.z;
                 const float r2 = dist2(dx, dy, dz);
-                const bool other = __float_as_uint(e.w) != i;
-                const bool ok = other && r2 < h2_lo;
-                band |= other & (r2 >= h2_lo) & (r2 <= h2_hi);
+                const bool ok = __float_as_uint(e.w) != i && r2 < cut;  // sqrt(r2) < h, exactly
                 const float w = h2 - r2;  // h^2 - vlen^2 of cl:123
                 a = ok ? fmaf(w * w, w, a) : a;
                 hits = (hits << 1) + (ok ? 1ull : 0ull);
             }
             // entries past a segment's end were end markers (never accepted): drop their bits
             hits >>= wtrips - trips;
-            if (__ballot(band && c.member) != 0) {
-                // a squared distance within rounding of h^2: redo the cell with the exact test
-                a = 0.f;
-                hits = 0;
-                for (uint32_t m = 0; m < trips; m++) {
-                    const float4 e = tile[m * P + slot];
-                    const float dx = pi.x - e.x, dy = pi.y - e.y, dz = pi.z - e.z;
-                    const float r2 = dist2(dx, dy, dz);
-                    const bool ok = __float_as_uint(e.w) != i && accept_r2(r2, h, h2_lo, h2_hi);
-                    const float w = h2 - r2;
-                    a = ok ? fmaf(w * w, w, a) : a;
-                    hits = (hits << 1) + (ok ? 1ull : 0ull);
-                }
-            }
             if (c.member) {
                 acc += a;
                 count += (uint32_t)__popcll(hits);
@@ -1316,6 +1299,7 @@ int Fluid3D::enqueue_tile_passes(hipStream_t s, int order)
     const size_t lds_d = tile_lds_bytes(false, block_waves, tile_depth, lanes);
     const size_t lds_f = tile_lds_bytes(true, block_waves, tile_depth);
     const uint32_t depth = tile_depth;
+    const float cut = sqrt_cut(params.cellSize);
     timer.begin(MFL_STAGE_DENSITY, s);
     auto* nbr = nbr_.as<uint32_t>();
     auto* ncount = ncount_.as<uint32_t>();
@@ -1326,7 +1310,7 @@ int Fluid3D::enqueue_tile_passes(hipStream_t s, int order)
                     : neighbour_pass == MFL_PASS_TILE_KEPT ? kOutKept : kOutNone;
 #define MFL_DENSITY_TILE(O, OUT, W)                                                                              \
     hipLaunchKernelGGL((k_density_tile<O, OUT, W>), gd, b, lds_d, s, pos, spos, svel, cs, ce, n, params, depth,   \
-                       porder, inv, nbr, ncount, kept, hitmask, meta)
+                       porder, inv, nbr, ncount, kept, hitmask, meta, cut)
 #define MFL_DENSITY_TILE_W(O, OUT)                 \
     if (lanes == 4) MFL_DENSITY_TILE(O, OUT, 32); \
     else MFL_DENSITY_TILE(O, OUT, 64)

@@ -3,6 +3,7 @@
 
 #include <hip/hip_runtime.h>
 
+#include <cmath>
 #include <cstddef>
 #include <cstdint>
 #include <string>
@@ -47,6 +48,18 @@ __host__ __device__ inline uint32_t cell_hash(int gx, int gy, int gz)
 {
     const uint32_t n = (73856093u * (uint32_t)gx) ^ (19349663u * (uint32_t)gy) ^ (83492791u * (uint32_t)gz);
     return n & (kBuckets - 1u);
+}
+
+// The least float T with sqrtf(T) >= h. IEEE square root is correctly rounded and non-decreasing, so
+// for every r2 >= 0: sqrtf(r2) < h exactly when r2 < T, the reference's fast_length(d) < h test in the
+// exact form the library defines it, as one comparison (host and device sqrt round alike).
+inline float sqrt_cut(float h)
+{
+    if (!(h > 0.f)) return h > 0.f ? h : (h == h ? 0.f : h);  // h <= 0: never accepted; NaN: never
+    float x = h * h;
+    while (x > 0.f && std::sqrt(x) >= h) x = std::nextafter(x, 0.f);
+    while (std::sqrt(x) < h) x = std::nextafter(x, __builtin_inff());
+    return x;
 }
 
 // |d|^2 as fma(dz, dz, fma(dy, dy, dx * dx)). Every kernel forms squared distances with it (the

@@ -30,13 +30,29 @@ def test_library_exports_every_declared_symbol():
     assert set(declared) <= exported
 
 
-def test_header_compiles_as_c():
+def test_header_compiles_as_c(tmp_path):
     src = '#include "mfl.h"\nint main(void){ mfl_params p; mfl_params_default(&p); return (int)sizeof(p) - 52; }\n'
-    exe = os.path.join(ROOT, "_work", "hdr_check")
-    os.makedirs(os.path.dirname(exe), exist_ok=True)
     r = subprocess.run(["gcc", "-std=c99", "-Wall", "-Werror", "-x", "c", "-", "-I", os.path.join(ROOT, "include"),
-                        "-c", "-o", exe + ".o"], input=src, capture_output=True, text=True)
+                        "-c", "-o", str(tmp_path / "hdr_check.o")], input=src, capture_output=True, text=True)
     assert r.returncode == 0, r.stderr
+
+
+def test_sqrt_cut_is_the_exact_acceptance_threshold():
+    """mfl_sqrt_cut(h) is the least float T with sqrtf(T) >= h, so r2 < T decides sqrtf(r2) < h for every
+    float r2 >= 0 (numpy's float32 sqrt is IEEE, correctly rounded)."""
+    from libcl_amd import _native as N
+    from libcl_amd import scenes
+    lib = N.load()
+    hs = np.concatenate([np.float32([0.02, 0.0316, 0.04, 1.0, 2.0, 3.0]),
+                         (1e-3 + 10 * scenes.uniform(17, 400)).astype(np.float32)])
+    for h in hs:
+        t = np.float32(lib.mfl_sqrt_cut(float(h)))
+        below = np.nextafter(t, np.float32(0))
+        assert np.sqrt(t) >= h and np.sqrt(below) < h, (h, t)
+        # a window of floats around the threshold, decided both ways
+        r2 = t + (np.arange(-64, 65, dtype=np.float32) * np.spacing(t)).astype(np.float32)
+        np.testing.assert_array_equal(r2 < t, np.sqrt(r2) < h)
+    assert lib.mfl_sqrt_cut(0.0) == 0.0 and lib.mfl_sqrt_cut(-1.0) == 0.0
 
 
 def test_params_match_oracle_and_fixtures(oracle):

@@ -28,7 +28,7 @@ def _plain(sc, prm, steps, shape="list"):
 
 
 @pytest.mark.parametrize("graph", [False, True])
-@pytest.mark.parametrize("shape", ["list", "tile_list", "tile_kept"])
+@pytest.mark.parametrize("shape", ["list", "tile_list", "tile_kept", "default"])
 def test_batch_matches_single_scene_runs(graph, shape):
     """Every scene of the batch equals the same scene run alone by a single-scene program with the same
     launch shape (LIST: group traversal; TILE_LIST / TILE_KEPT: 4-lane tiles in a Morton order refreshed
@@ -40,7 +40,10 @@ def test_batch_matches_single_scene_runs(graph, shape):
     prm = L.params_from(scs[0].params)
     counts = [s.n for s in scs]
     b = L.FluidBatch(counts)
-    assert b.setTuning(neighbour_pass=shape) == 1, L.last_error()
+    if shape == "default":  # the batch's own default launch shape, never set (kept-entry passes)
+        shape = "tile_kept"
+    else:
+        assert b.setTuning(neighbour_pass=shape) == 1, L.last_error()
     assert b.setParameters(prm) == 1
     assert b.setBox((0, 0, 0, 0), (1, 1, 1, 0)) == 1
     assert b.upload(np.concatenate([s.pos for s in scs]), np.concatenate([s.vel for s in scs]),
