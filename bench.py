@@ -314,7 +314,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=10)
     ap.add_argument("--config", default="1m", choices=["1m", "small"])
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--cpu-steps", type=int, default=20)
+    ap.add_argument("--cpu-steps", type=int, default=80, help="oracle steps timed for the CPU baseline (~10 s)")
     ap.add_argument("--order", default="morton", choices=["sorted", "caller", "morton"])
     ap.add_argument("--no-batched", action="store_true")
     ap.add_argument("--no-adjoint", action="store_true")
