@@ -251,7 +251,6 @@ __global__ __launch_bounds__(kThreads) void k_b_density_tile(const float4* __res
     const float h = p.cellSize;
     const float h2 = h * h;
     const float cut = cuts[s];  // sqrt(r2) < h exactly when r2 < cut
-    const float inf = __builtin_inff();
     const int gx = grid_coord(pi.x, h), gy = grid_coord(pi.y, h), gz = grid_coord(pi.z, h);
     float acc = 0.f;
     uint32_t count = 0;
@@ -262,26 +261,15 @@ __global__ __launch_bounds__(kThreads) void k_b_density_tile(const float4* __res
         const SegCell c = next_cell_seg<W>(L, todo, live, gx, gy, gz, pi);
         todo &= ~L.bits(__ballot(c.member));
         const uint32_t kept = stage_seg<W>(L, c, h2 * kKeepMargin, sp, cs, ce, n_s, depth, tile, aux);
+        // trips of the wave: the longest of its segments that test from LDS (taken while every lane is active)
+        const uint32_t wtrips = wave_trips<W>(kept <= depth ? kept : 0u);
         if (kept <= depth) {
-            if (L.ls < kGroupLanes) tile[kept + L.ls] = make_float4(inf, inf, inf, __uint_as_float(kEmpty));
-            wave_sync();
             const uint32_t trips = (kept + P - 1) / P;
-            uint32_t wtrips = trips;
-            if constexpr (W < kWave) wtrips = max(wtrips, (uint32_t)__shfl_xor((int)wtrips, W, kWave));
-            wtrips = __builtin_amdgcn_readfirstlane(wtrips);
-            const uint32_t qmax = kept + slot;
+            pad_tile<W>(L, tile, kept, wtrips * P);
+            wave_sync();
             float a = 0.f;
-            uint64_t hits = 0;
-            for (uint32_t m = 0; m < wtrips; m++) {
-                const float4 e = tile[min(m * P + slot, qmax)];
-                const float dx = pi.x - e.x, dy = pi.y - e.y, dz = pi.z - e.z;
-                const float r2 = dist2(dx, dy, dz);
-                const bool ok = __float_as_uint(e.w) != il && r2 < cut;
-                const float w = h2 - r2;
-                a = ok ? fmaf(w * w, w, a) : a;
-                hits = (hits << 1) + (ok ? 1ull : 0ull);
-            }
-            hits >>= wtrips - trips;
+            uint64_t hits = accept_sweep<P>(tile + slot, wtrips, pi, il, h2, cut, a);
+            hits >>= min(wtrips - trips, 63u);  // trips = 0 leaves no bit set
             if constexpr (KEPT) {
                 if (c.member) {
                     acc += a;

@@ -535,7 +535,6 @@ __global__ __launch_bounds__(kThreads) void k_density_tile(const float4* __restr
     const bool live = i < n;
     const float h = p.cellSize;
     const float h2 = h * h;
-    const float inf = __builtin_inff();
     const int gx = grid_coord(pi.x, h), gy = grid_coord(pi.y, h), gz = grid_coord(pi.z, h);
     float acc = 0.f;
     uint32_t count = 0;  // this lane's accepted visits (summed over the particle's lanes at the end)
@@ -546,31 +545,19 @@ __global__ __launch_bounds__(kThreads) void k_density_tile(const float4* __restr
         const SegCell c = next_cell_seg<W>(L, todo, live, gx, gy, gz, pi);
         todo &= ~L.bits(__ballot(c.member));
         const uint32_t kept = stage_seg<W>(L, c, h2 * kKeepMargin, spos, cell_start, cell_end, n, depth, tile, aux);
+        // trips of the wave: the longest of its segments that test from LDS (taken while every lane is active)
+        const uint32_t wtrips = wave_trips<W>(kept <= depth ? kept : 0u);
         if (kept <= depth) {
             // entry P m + slot is tested by this lane in trip m (of `trips`); bit trips - 1 - m of `hits`
-            // records its acceptance (kept <= 64 P, so trips <= 64). Eight entries past the end hold
-            // positions at infinity, so the last trip needs no bound test.
-            if (L.ls < kGroupLanes) tile[kept + L.ls] = make_float4(inf, inf, inf, __uint_as_float(kEmpty));
-            wave_sync();
+            // records its acceptance (kept <= 64 P, so trips <= 64). The wave runs the longest segment's
+            // trip count; past its own end a lane reads an end marker.
             const uint32_t trips = (kept + P - 1) / P;
-            // the wave runs the longest segment's trip count; past its own end a lane reads an end marker
-            uint32_t wtrips = trips;
-            if constexpr (W < kWave) wtrips = max(wtrips, (uint32_t)__shfl_xor((int)wtrips, W, kWave));
-            wtrips = __builtin_amdgcn_readfirstlane(wtrips);
-            const uint32_t qmax = kept + slot;
+            pad_tile<W>(L, tile, kept, wtrips * P);
+            wave_sync();
             float a = 0.f;
-            uint64_t hits = 0;
-            for (uint32_t m = 0; m < wtrips; m++) {
-                const float4 e = tile[min(m * P + slot, qmax)];
-                const float dx = pi.x - e.x, dy = pi.y - e.y, dz = pi.z - e.z;
-                const float r2 = dist2(dx, dy, dz);
-                const bool ok = __float_as_uint(e.w) != i && r2 < cut;  // sqrt(r2) < h, exactly
-                const float w = h2 - r2;  // h^2 - vlen^2 of cl:123
-                a = ok ? fmaf(w * w, w, a) : a;
-                hits = (hits << 1) + (ok ? 1ull : 0ull);
-            }
+            uint64_t hits = accept_sweep<P>(tile + slot, wtrips, pi, i, h2, cut, a);
             // entries past a segment's end were end markers (never accepted): drop their bits
-            hits >>= wtrips - trips;
+            hits >>= min(wtrips - trips, 63u);  // trips = 0 leaves no bit set
             if (c.member) {
                 acc += a;
                 count += (uint32_t)__popcll(hits);

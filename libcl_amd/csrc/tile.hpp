@@ -401,5 +401,49 @@ __device__ inline uint32_t stage_seg(const Lanes<W>& L, const SegCell& c, float 
     return over ? kStageOverflow : kept;
 }
 
+// Trip count of a staged neighbourhood of `kept` entries for the wave: a lane tests entries P m + slot
+// in trips m = 0, 1, ...; a wave runs the longest of its segments' trip counts.
+template <int W>
+__device__ inline uint32_t wave_trips(uint32_t kept)
+{
+    constexpr uint32_t P = W / 8;
+    uint32_t wtrips = (kept + P - 1) / P;
+    if constexpr (W < kWave) wtrips = max(wtrips, (uint32_t)__shfl_xor((int)wtrips, W, kWave));
+    return __builtin_amdgcn_readfirstlane(wtrips);
+}
+
+// End markers (positions at infinity, index kEmpty) in entries [kept, end) of a segment's tile, so that
+// every entry a lane reads past its segment's end is rejected without a bound test. With end = wtrips P
+// taken over segments whose kept <= depth, end <= depth + P - 1 < depth + 8, the tile's size.
+template <int W>
+__device__ inline void pad_tile(const Lanes<W>& L, float4* tile, uint32_t kept, uint32_t end)
+{
+    const float inf = __builtin_inff();
+    for (uint32_t q = kept + L.ls; q < end; q += W) tile[q] = make_float4(inf, inf, inf, __uint_as_float(kEmpty));
+}
+
+// One lane's acceptance sweep over a staged neighbourhood: trip m (of the wave's wtrips <= 64) tests
+// entry P m of `row` (= tile + slot). An entry is accepted when it is not the particle itself and its
+// squared distance is below `cut` (sqrt(r2) < h, exactly). The accepted (h^2 - r2)^3 terms are added to
+// `a` in trip order and bit wtrips - 1 - m of the result records trip m. The padded tile lets the
+// reads use one base address and immediate offsets.
+template <uint32_t P>
+__device__ inline uint64_t accept_sweep(const float4* __restrict__ row, uint32_t wtrips, const float4& pi,
+                                        uint32_t self, float h2, float cut, float& a)
+{
+    auto test = [&](uint32_t m) -> uint32_t {
+        const float4 e = row[m * P];
+        const float dx = pi.x - e.x, dy = pi.y - e.y, dz = pi.z - e.z;
+        const float r2 = dist2(dx, dy, dz);
+        const bool ok = __float_as_uint(e.w) != self && r2 < cut;
+        const float w = h2 - r2;  // h^2 - vlen^2 of cl:123
+        a = ok ? fmaf(w * w, w, a) : a;
+        return ok ? 1u : 0u;
+    };
+    uint64_t hits = 0;
+    for (uint32_t m = 0; m < wtrips; m++) hits = (hits << 1) + test(m);
+    return hits;
+}
+
 }  // namespace sph
 }  // namespace mfl
