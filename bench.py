@@ -6,7 +6,8 @@ Workload: the 1,048,576-particle dam break (128x64x128 jittered lattice at 0.5h,
 [0,4]^3, fp32), inputs resident in HBM. A step is one oclFluid3D::compute(): hash + bucket clear,
 sort, reorder + bounds, density, forces + clip box + integration. Prints one JSON line; its
 secondary legs are the batched scenes, the adjoint, the particle-vs-mesh collision (BVH build +
-clCollideBVH on the same 1M particles) and the 2D sibling solver oclFluid3Dnext at 1M particles.
+clCollideBVH on the same 1M particles), the 2D sibling solver oclFluid3Dnext at 1M particles and the
+standalone key/value sort of 16M cell keys.
 """
 from __future__ import annotations
 
@@ -307,6 +308,72 @@ def next_leg(steps: int = 50, cpu: bool = True) -> dict:
     return out
 
 
+SORT_BYTES_PER_PAIR = 52  # 3 passes x (R 8 + W 8) + one 4-byte histogram read per key
+
+
+def sort_leg(reps: int = 10, cpu: bool = True) -> dict:
+    """oclRadixSort on 16,777,216 (key, value) pairs by bits [0, 24): keys over 2^24 cells, uniform
+    (seed 3) and skewed (half the keys in a hot 10% of the cells, chosen with seed 4), values 0..N-1.
+    Every timed sort starts from the same unsorted input, restored between sorts outside the HIP events
+    that time the sort; the sorted output is checked for order and for the value permutation."""
+    import numpy as np
+    import torch
+
+    import libcl_amd as L
+    from libcl_amd import scenes
+
+    n = 1 << 24
+    s = L.oclRadixSort()
+    assert s.compile() == 1, L.last_error()
+    out = {"workload": "sort_16m_cells24", "pairs": n, "bits": [0, 24], "passes": 3,
+           "alg_bytes_per_pair": SORT_BYTES_PER_PAIR, "reps": reps}
+    keys_uniform = None
+    for name, skew in (("uniform", False), ("skew", True)):
+        keys = scenes.sort_keys(n, seed=3, skew=skew, bits=24)
+        if not skew:
+            keys_uniform = keys
+        k0 = torch.from_numpy(keys.view(np.int32).copy()).cuda()
+        v0 = torch.arange(n, dtype=torch.int32, device="cuda")
+        k, v = k0.clone(), v0.clone()
+        times = []
+        for _ in range(reps + 2):
+            k.copy_(k0)
+            v.copy_(v0)
+            e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+            e0.record()
+            assert s.compute(k, v, 0, 24) == 1, L.last_error()
+            e1.record()
+            times.append((e0, e1))
+        torch.cuda.synchronize()
+        us = [a.elapsed_time(z) * 1e3 for a, z in times[2:]]
+        avg = sum(us) / len(us)
+        ku = k.to(torch.int64) & 0xFFFFFFFF  # the keys as unsigned values
+        ordered = bool((ku[1:] >= ku[:-1]).all())
+        perm = bool((torch.sort(v.to(torch.int64)).values == torch.arange(n, device="cuda")).all())
+        gbs = SORT_BYTES_PER_PAIR * n / (avg * 1e-6) / 1e9
+        out[name] = {"us_per_sort": avg, "pairs_per_s": n / (avg * 1e-6), "achieved_GBs": gbs,
+                     "hbm_frac": gbs / HBM_PEAK_GBS, "ordered": ordered and perm}
+    if cpu:
+        from oracle import oracle as O
+        O.build()
+        ks, vs = keys_uniform.copy(), np.arange(n, dtype=np.uint32)
+        t0 = time.perf_counter()
+        O.std_stable_sort(ks, vs, 0, 24)
+        dt = time.perf_counter() - t0
+        out["cpu_baseline"] = {"value": n / dt, "unit": "pairs/s", "cores": 1, "kind": "port",
+                               "sample": f"std::stable_sort of the {n} uniform pairs by bits [0, 24) "
+                                         f"(oracle/std_stable_sort.cpp), one core ({dt:.2f} s)"}
+        m = 4194300  # the reference's own sort takes fewer than 4,194,304 keys
+        ks, vs = keys_uniform[:m].copy(), np.arange(m, dtype=np.uint32)
+        t0 = time.perf_counter()
+        assert O.radix_sort(ks, vs, 0, 24)
+        dt = time.perf_counter() - t0
+        out["cpu_reference_restated"] = {"value": m / dt, "unit": "pairs/s", "cores": 1, "kind": "port",
+                                         "sample": f"the restated 4-bit reference sort (oracle/sph_oracle.c "
+                                                   f"orc_radix_sort) on the first {m} uniform pairs ({dt:.2f} s)"}
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -320,7 +387,8 @@ def main():
     ap.add_argument("--no-adjoint", action="store_true")
     ap.add_argument("--no-collide", action="store_true")
     ap.add_argument("--no-next", action="store_true")
-    ap.add_argument("--only", default=None, choices=["batched", "adjoint", "collide", "next"],
+    ap.add_argument("--no-sort", action="store_true")
+    ap.add_argument("--only", default=None, choices=["batched", "adjoint", "collide", "next", "sort"],
                     help="run one secondary leg alone and print its JSON (profiling)")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="also run a plain device copy of MIB MiB three times (counter calibration)")
@@ -358,6 +426,9 @@ def main():
         return
     if args.only == "next":
         print(json.dumps(next_leg(cpu=not args.no_cpu)), flush=True)
+        return
+    if args.only == "sort":
+        print(json.dumps(sort_leg(cpu=not args.no_cpu)), flush=True)
         return
 
     sc = scenes.config_1m() if args.config == "1m" else scenes.config_small()
@@ -478,6 +549,8 @@ def main():
         out["collide"] = collide_leg(cpu=not args.no_cpu)
     if not args.no_next:
         out["next"] = next_leg(cpu=not args.no_cpu)
+    if not args.no_sort:
+        out["sort"] = sort_leg(cpu=not args.no_cpu)
     if not args.no_cpu:
         nthreads = min(os.cpu_count() or 1, 16)
         out["cpu_baseline"] = cpu_baseline(sc, prm, args.cpu_steps, nthreads)

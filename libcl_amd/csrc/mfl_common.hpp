@@ -17,6 +17,7 @@ constexpr uint32_t kEmpty = 0xFFFFFFFFu;       // libCL/phys/oclFluid3D.cpp:309
 constexpr uint32_t kReferenceClear = 65536u;   // 262,144-byte fill of uint entries, cpp:310-312
 constexpr uint32_t kGroup = 256u;              // cLocalSize, cpp:16
 constexpr int kWave = 64;
+constexpr uint32_t kXcds = 8;                  // XCDs of an MI355X, each with its own L2
 
 // ---- error reporting (mfl_last_error) ---------------------------------------------------------
 void set_error(const std::string& msg);

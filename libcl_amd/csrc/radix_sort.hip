@@ -44,6 +44,17 @@ __device__ inline uint32_t block_exclusive_scan(uint32_t v, uint32_t* s_wave, ui
     return before + x - v;
 }
 
+// Tile of this block. The hardware deals a grid's blocks to the 8 XCDs round-robin (block b runs on
+// XCD b % 8); the tiles are dealt so that every XCD takes one contiguous range of them. Neighbouring
+// tiles write neighbouring words of the count table and adjoining pieces of each digit's run, so
+// their partial cache lines then meet in one L2 instead of being written back from eight.
+__device__ inline uint32_t xcd_tile()
+{
+    const uint32_t nb = gridDim.x, b = blockIdx.x;
+    const uint32_t q = nb / kXcds, r = nb % kXcds, x = b % kXcds;
+    return x * q + min(x, r) + b / kXcds;
+}
+
 template <int ITEMS>
 __global__ __launch_bounds__(kThreads) void k_tile_counts(const uint32_t* __restrict__ keys, uint32_t n, int shift,
                                                           int bits, uint32_t* __restrict__ counts, uint32_t ntiles)
@@ -52,14 +63,15 @@ __global__ __launch_bounds__(kThreads) void k_tile_counts(const uint32_t* __rest
     const uint32_t radix = 1u << bits, mask = radix - 1u;
     for (uint32_t d = threadIdx.x; d < radix; d += kThreads) hist[d] = 0;
     __syncthreads();
-    const size_t base = (size_t)blockIdx.x * (kThreads * ITEMS);
+    const uint32_t tile = xcd_tile();
+    const size_t base = (size_t)tile * (kThreads * ITEMS);
 #pragma unroll
     for (int k = 0; k < ITEMS; k++) {
         const size_t i = base + (size_t)k * kThreads + threadIdx.x;
         if (i < n) atomicAdd(&hist[(keys[i] >> shift) & mask], 1u);
     }
     __syncthreads();
-    for (uint32_t d = threadIdx.x; d < radix; d += kThreads) counts[(size_t)d * ntiles + blockIdx.x] = hist[d];
+    for (uint32_t d = threadIdx.x; d < radix; d += kThreads) counts[(size_t)d * ntiles + tile] = hist[d];
 }
 
 // One block per digit: row d of the digit-major [radix x ntiles] table becomes its exclusive scan.
@@ -111,7 +123,7 @@ __global__ __launch_bounds__(kThreads) void k_scatter(const uint32_t* __restrict
     const int lane = t & (kWave - 1);
     const int wave = t / kWave;
     const uint32_t radix = 1u << bits, mask = radix - 1u;
-    const uint32_t tile = blockIdx.x;
+    const uint32_t tile = xcd_tile();
     const size_t tile_base = (size_t)tile * TILE;
 
     // thread t owns digits 2t and 2t + 1 in the per-digit loops

@@ -13,7 +13,6 @@ constexpr int kThreads = 256;
 // b % 8), and each XCD has its own L2. Mapping b to chunk (b % 8) * (grid / 8) + b / 8 hands every XCD
 // one contiguous run of the traversal order, so particles that share neighbours share an L2. Grids
 // launched with it are a multiple of 8 blocks (grid_xcd).
-constexpr uint32_t kXcds = 8;
 
 __device__ inline uint32_t xcd_block()
 {
