@@ -65,10 +65,24 @@ __global__ __launch_bounds__(kThreads) void k_tile_counts(const uint32_t* __rest
     __syncthreads();
     const uint32_t tile = xcd_tile();
     const size_t base = (size_t)tile * (kThreads * ITEMS);
+    if (base + kThreads * ITEMS <= n && (reinterpret_cast<uintptr_t>(keys) & 15u) == 0) {
+        // a whole tile of a 16-byte aligned array, read 16 bytes per load (a histogram does not depend
+        // on the order its keys are counted in)
+        const uint4* k4 = reinterpret_cast<const uint4*>(keys + base);
 #pragma unroll
-    for (int k = 0; k < ITEMS; k++) {
-        const size_t i = base + (size_t)k * kThreads + threadIdx.x;
-        if (i < n) atomicAdd(&hist[(keys[i] >> shift) & mask], 1u);
+        for (int k = 0; k < ITEMS / 4; k++) {
+            const uint4 q = k4[k * kThreads + threadIdx.x];
+            atomicAdd(&hist[(q.x >> shift) & mask], 1u);
+            atomicAdd(&hist[(q.y >> shift) & mask], 1u);
+            atomicAdd(&hist[(q.z >> shift) & mask], 1u);
+            atomicAdd(&hist[(q.w >> shift) & mask], 1u);
+        }
+    } else {
+#pragma unroll
+        for (int k = 0; k < ITEMS; k++) {
+            const size_t i = base + (size_t)k * kThreads + threadIdx.x;
+            if (i < n) atomicAdd(&hist[(keys[i] >> shift) & mask], 1u);
+        }
     }
     __syncthreads();
     for (uint32_t d = threadIdx.x; d < radix; d += kThreads) counts[(size_t)d * ntiles + tile] = hist[d];
@@ -104,13 +118,15 @@ __global__ __launch_bounds__(kThreads) void k_scan_digits(uint32_t* __restrict__
 // Stable scatter of one tile. Keys of wave w, round r, lane l sit at tile offset (w*ITEMS + r)*64 + l,
 // so (wave, round, lane) order is input order; a key's rank is
 //   [same digit in earlier waves] + [same digit in earlier rounds of its wave] + [lower lanes, same round].
-template <int ITEMS, bool IDENTITY_VALS>
+// BITS > 0 fixes the digit width at compile time (the common 7, 8 and 9); BITS = 0 takes `bits_rt`.
+template <int ITEMS, bool IDENTITY_VALS, int BITS>
 __global__ __launch_bounds__(kThreads) void k_scatter(const uint32_t* __restrict__ kin, const uint32_t* __restrict__ vin,
                                                       uint32_t* __restrict__ kout, uint32_t* __restrict__ vout,
-                                                      uint32_t n, int shift, int bits,
+                                                      uint32_t n, int shift, int bits_rt,
                                                       const uint32_t* __restrict__ counts,
                                                       const uint32_t* __restrict__ totals, uint32_t ntiles)
 {
+    const int bits = BITS > 0 ? BITS : bits_rt;
     constexpr int TILE = kThreads * ITEMS;
     __shared__ uint32_t s_keys[TILE];
     __shared__ uint32_t s_vals[TILE];
@@ -138,30 +154,48 @@ __global__ __launch_bounds__(kThreads) void k_scatter(const uint32_t* __restrict
     }
 
     uint32_t key[ITEMS], val[ITEMS], rank[ITEMS];
+    const uint32_t first = (uint32_t)(wave * ITEMS) * kWave + (uint32_t)lane;  // tile offset of round 0
+    if (tile_base + TILE <= n) {  // a whole tile: no bound tests
+        const uint32_t* kt = kin + tile_base + first;
+        const uint32_t* vt = vin + tile_base + first;
 #pragma unroll
-    for (int r = 0; r < ITEMS; r++) {
-        const size_t idx = tile_base + ((size_t)(wave * ITEMS + r) << 6) + lane;
-        const bool valid = idx < n;
-        key[r] = valid ? kin[idx] : 0u;
-        if (IDENTITY_VALS)
-            val[r] = (uint32_t)idx;
-        else
-            val[r] = valid ? vin[idx] : 0u;
+        for (int r = 0; r < ITEMS; r++) {
+            key[r] = kt[r * kWave];
+            val[r] = IDENTITY_VALS ? (uint32_t)(tile_base + first + r * kWave) : vt[r * kWave];
+        }
+    } else {
+#pragma unroll
+        for (int r = 0; r < ITEMS; r++) {
+            const size_t idx = tile_base + first + (size_t)r * kWave;
+            const bool valid = idx < n;
+            key[r] = valid ? kin[idx] : 0u;
+            if (IDENTITY_VALS)
+                val[r] = (uint32_t)idx;
+            else
+                val[r] = valid ? vin[idx] : 0u;
+        }
     }
     __syncthreads();
 
     const uint64_t lower_lanes = (1ull << lane) - 1ull;
 #pragma unroll
     for (int r = 0; r < ITEMS; r++) {
-        const size_t idx = tile_base + ((size_t)(wave * ITEMS + r) << 6) + lane;
+        const size_t idx = tile_base + first + (size_t)r * kWave;
         const bool valid = idx < n;
         const uint32_t d = (key[r] >> shift) & mask;
-        uint64_t same = __ballot(valid);
-        for (int b = 0; b < bits; b++) {
-            const bool bit = (d >> b) & 1u;
-            const uint64_t bal = __ballot(bit);
-            same &= bit ? bal : ~bal;
+        // lanes whose digit differs from this lane's in some bit (or that hold no key): per bit b, the
+        // ballot of bit b XOR this lane's bit b replicated over 64 bits
+        const uint64_t live = __ballot(valid);
+        uint32_t differ_lo = ~(uint32_t)live, differ_hi = ~(uint32_t)(live >> 32);
+#pragma unroll
+        for (int b = 0; b < (BITS > 0 ? BITS : kMaxDigitBits); b++) {
+            if (BITS == 0 && b >= bits) break;
+            const uint32_t rep = (uint32_t)((int)(d << (31 - b)) >> 31);  // 0 or all ones
+            const uint64_t bal = __ballot(rep != 0u);
+            differ_lo |= (uint32_t)bal ^ rep;
+            differ_hi |= (uint32_t)(bal >> 32) ^ rep;
         }
+        const uint64_t same = ~((uint64_t)differ_hi << 32 | differ_lo);
         const int leader = __ffsll((unsigned long long)same) - 1;
         uint32_t base = 0;
         if (valid && lane == leader) {
@@ -236,12 +270,23 @@ bool run_pass(const uint32_t* kin, const uint32_t* vin, uint32_t* kout, uint32_t
     const uint32_t radix = 1u << bits;
     hipLaunchKernelGGL(k_tile_counts<ITEMS>, dim3(ntiles), dim3(kThreads), 0, s, kin, n, shift, bits, counts, ntiles);
     hipLaunchKernelGGL(k_scan_digits, dim3(radix), dim3(kThreads), 0, s, counts, ntiles, totals);
-    if (identity)
-        hipLaunchKernelGGL((k_scatter<ITEMS, true>), dim3(ntiles), dim3(kThreads), 0, s, kin, vin, kout, vout, n,
-                           shift, bits, counts, totals, ntiles);
-    else
-        hipLaunchKernelGGL((k_scatter<ITEMS, false>), dim3(ntiles), dim3(kThreads), 0, s, kin, vin, kout, vout, n,
-                           shift, bits, counts, totals, ntiles);
+#define MFL_SCATTER(ID, B)                                                                                      \
+    hipLaunchKernelGGL((k_scatter<ITEMS, ID, B>), dim3(ntiles), dim3(kThreads), 0, s, kin, vin, kout, vout, n, shift, \
+                       bits, counts, totals, ntiles)
+#define MFL_SCATTER_BITS(ID)                                                                                    \
+    switch (bits) {                                                                                             \
+    case 9: MFL_SCATTER(ID, 9); break;                                                                          \
+    case 8: MFL_SCATTER(ID, 8); break;                                                                          \
+    case 7: MFL_SCATTER(ID, 7); break;                                                                          \
+    default: MFL_SCATTER(ID, 0); break;                                                                         \
+    }
+    if (identity) {
+        MFL_SCATTER_BITS(true)
+    } else {
+        MFL_SCATTER_BITS(false)
+    }
+#undef MFL_SCATTER_BITS
+#undef MFL_SCATTER
     return hip_ok(hipGetLastError(), "radix sort pass launch");
 }
 
