@@ -128,11 +128,12 @@ __global__ __launch_bounds__(kThreads) void k_scatter(const uint32_t* __restrict
 {
     const int bits = BITS > 0 ? BITS : bits_rt;
     constexpr int TILE = kThreads * ITEMS;
+    constexpr uint32_t R = BITS > 0 ? 1u << BITS : kMaxRadix;  // digit table size (LDS per block)
     __shared__ uint32_t s_keys[TILE];
     __shared__ uint32_t s_vals[TILE];
-    __shared__ uint32_t s_cnt[kWaves * kMaxRadix];
-    __shared__ uint32_t s_start[kMaxRadix];
-    __shared__ uint32_t s_base[kMaxRadix];
+    __shared__ uint32_t s_cnt[kWaves * R];
+    __shared__ uint32_t s_start[R];
+    __shared__ uint32_t s_base[R];
     __shared__ uint32_t s_wave[kWaves];
 
     const int t = threadIdx.x;
@@ -144,7 +145,7 @@ __global__ __launch_bounds__(kThreads) void k_scatter(const uint32_t* __restrict
 
     // thread t owns digits 2t and 2t + 1 in the per-digit loops
     const uint32_t d0 = 2u * (uint32_t)t, d1 = d0 + 1u;
-    for (uint32_t i = t; i < kWaves * radix; i += kThreads) s_cnt[(i / radix) * kMaxRadix + i % radix] = 0;
+    for (uint32_t i = t; i < kWaves * radix; i += kThreads) s_cnt[(i / radix) * R + i % radix] = 0;
     {
         const uint32_t t0 = d0 < radix ? totals[d0] : 0u, t1 = d1 < radix ? totals[d1] : 0u;
         uint32_t all;
@@ -199,8 +200,8 @@ __global__ __launch_bounds__(kThreads) void k_scatter(const uint32_t* __restrict
         const int leader = __ffsll((unsigned long long)same) - 1;
         uint32_t base = 0;
         if (valid && lane == leader) {
-            base = s_cnt[wave * kMaxRadix + d];
-            s_cnt[wave * kMaxRadix + d] = base + (uint32_t)__popcll(same);
+            base = s_cnt[wave * R + d];
+            s_cnt[wave * R + d] = base + (uint32_t)__popcll(same);
         }
         base = __shfl(base, leader & (kWave - 1), kWave);
         rank[r] = base + (uint32_t)__popcll(same & lower_lanes);
@@ -212,8 +213,8 @@ __global__ __launch_bounds__(kThreads) void k_scatter(const uint32_t* __restrict
         uint32_t run = 0;
 #pragma unroll
         for (int w = 0; w < kWaves; w++) {
-            const uint32_t c = s_cnt[w * kMaxRadix + d0];
-            s_cnt[w * kMaxRadix + d0] = run;
+            const uint32_t c = s_cnt[w * R + d0];
+            s_cnt[w * R + d0] = run;
             run += c;
         }
         tile_cnt0 = run;
@@ -222,8 +223,8 @@ __global__ __launch_bounds__(kThreads) void k_scatter(const uint32_t* __restrict
         uint32_t run = 0;
 #pragma unroll
         for (int w = 0; w < kWaves; w++) {
-            const uint32_t c = s_cnt[w * kMaxRadix + d1];
-            s_cnt[w * kMaxRadix + d1] = run;
+            const uint32_t c = s_cnt[w * R + d1];
+            s_cnt[w * R + d1] = run;
             run += c;
         }
         tile_cnt1 = run;
@@ -241,7 +242,7 @@ __global__ __launch_bounds__(kThreads) void k_scatter(const uint32_t* __restrict
         const size_t idx = tile_base + ((size_t)(wave * ITEMS + r) << 6) + lane;
         if (idx < n) {
             const uint32_t d = (key[r] >> shift) & mask;
-            const uint32_t pos = s_start[d] + s_cnt[wave * kMaxRadix + d] + rank[r];
+            const uint32_t pos = s_start[d] + s_cnt[wave * R + d] + rank[r];
             s_keys[pos] = key[r];
             s_vals[pos] = val[r];
         }
