@@ -780,20 +780,13 @@ __global__ __launch_bounds__(kThreads) void k_forces_kept(const float4* __restri
             }
             wave_sync();
             if (member) {
-                // trips [c0 / P, (c0 + chunk) / P) test entries of this chunk: their bits are
-                // [trips - hi, trips - lo) of the acceptance mask (bit trips - 1 - m is trip m)
-                const uint32_t mlo = c0 / P, mhi = min((c0 + kKeptChunk) / P, trips);
-                uint64_t bits = 0;
-                if (mlo < mhi) {
-                    const uint32_t blo = trips - mhi, bhi = trips - mlo;  // bhi <= 64
-                    const uint64_t upto = bhi >= 64 ? ~0ull : (1ull << bhi) - 1ull;
-                    bits = hm & upto & ~((1ull << blo) - 1ull);
-                }
-                while (bits) {
-                    const uint32_t b = 63u - (uint32_t)__builtin_clzll(bits);  // earliest trip first
-                    bits &= ~(1ull << b);
-                    const uint32_t q = (trips - 1u - b) * P + slot - c0;
-                    const float4 pj = tpos[q], vj = tvel[q];
+                uint32_t w = chunk_bits<P>(hm, trips, c0);
+                const float4* rp = tpos + slot;  // this lane's entries: k P + slot of the chunk
+                const float4* rv = tvel + slot;
+                while (w) {
+                    const uint32_t k = (uint32_t)__builtin_clz(w);  // earliest trip first
+                    w ^= 0x80000000u >> k;
+                    const float4 pj = rp[k * P], vj = rv[k * P];
                     const float dx = pi.x - pj.x, dy = pi.y - pj.y, dz = pi.z - pj.z;
                     pair_force(pi, vi, pj, vj, dx, dy, dz, dist2(dx, dy, dz), h, Pa, Va);
                 }

@@ -401,6 +401,20 @@ __device__ inline uint32_t stage_seg(const Lanes<W>& L, const SegCell& c, float 
     return over ? kStageOverflow : kept;
 }
 
+// Acceptance bits of one chunk [c0, c0 + kKeptChunk) of a kept list (c0 a multiple of kKeptChunk), from
+// a lane's mask hm over `trips` trips (bit trips - 1 - m is trip m): trip c0 / P + k of the chunk
+// becomes bit 31 - k of the result, so counting leading zeros visits the accepted entries of the chunk
+// earliest trip first, entry k P + slot of the chunk for k leading zeros.
+template <uint32_t P>
+__device__ inline uint32_t chunk_bits(uint64_t hm, uint32_t trips, uint32_t c0)
+{
+    static_assert(kKeptChunk % P == 0 && kKeptChunk / P <= 32, "a chunk's trips fit one word");
+    const uint32_t mlo = c0 / P, mhi = min((c0 + kKeptChunk) / P, trips);
+    if (mlo >= mhi) return 0u;
+    const uint32_t blo = trips - mhi, nb = mhi - mlo;
+    return (uint32_t)((hm >> blo) << (32u - nb));  // bits of earlier chunks leave the word
+}
+
 // Trip count of a staged neighbourhood of `kept` entries for the wave: a lane tests entries P m + slot
 // in trips m = 0, 1, ...; a wave runs the longest of its segments' trip counts.
 template <int W>
