@@ -168,10 +168,16 @@ __global__ __launch_bounds__(kThreads) void k_b_density(float4* __restrict__ spo
 
 // One lane walks particle i's 27 bucket ranges of its scene from global memory and records the visits
 // (the overflow path of the batched tile pass; scene-local indices throughout).
-__device__ __noinline__ uint32_t b_density_walk(int cx, int cy, int cz, float4 pi, uint32_t il, uint32_t ig,
-                                               const float4* __restrict__ spos, const uint32_t* __restrict__ cs,
-                                               const uint32_t* __restrict__ ce, uint32_t n_s, float h, uint32_t N,
-                                               uint32_t* __restrict__ nbr, float* acc_out)
+// Sums returned by value: a pointer to the caller's accumulators would keep them in scratch memory.
+struct BVisitSums {
+    float acc;
+    uint32_t count;
+};
+
+__device__ __noinline__ BVisitSums b_density_walk(int cx, int cy, int cz, float4 pi, uint32_t il, uint32_t ig,
+                                                 const float4* __restrict__ spos, const uint32_t* __restrict__ cs,
+                                                 const uint32_t* __restrict__ ce, uint32_t n_s, float h, uint32_t N,
+                                                 uint32_t* __restrict__ nbr)
 {
     const float h2 = h * h;
     const float h2_lo = h2 * 0.9999996f, h2_hi = h2 * 1.0000005f;
@@ -194,8 +200,7 @@ __device__ __noinline__ uint32_t b_density_walk(int cx, int cy, int cz, float4 p
             }
         }
     }
-    *acc_out = acc;
-    return count;
+    return {acc, count};
 }
 
 // The tile density pass of the single-scene program (k_density_tile) over all scenes: traversal slot t
@@ -306,9 +311,9 @@ __global__ __launch_bounds__(kThreads) void k_b_density_tile(const float4* __res
             }
         } else if (c.member && slot == 0) {
             if (KEPT) meta[t] = kMetaWalk;
-            float a = 0.f;
-            count += b_density_walk(c.cx, c.cy, c.cz, pi, il, i, sp, cs, ce, n_s, h, N, KEPT ? nullptr : nbr, &a);
-            acc += a;
+            const BVisitSums f = b_density_walk(c.cx, c.cy, c.cz, pi, il, i, sp, cs, ce, n_s, h, N, KEPT ? nullptr : nbr);
+            count += f.count;
+            acc += f.acc;
         }
         wave_sync();
     }
@@ -328,12 +333,17 @@ __global__ __launch_bounds__(kThreads) void k_b_density_tile(const float4* __res
     if (!KEPT) ncount[i] = count;
 }
 
-__device__ __noinline__ void b_force_fallback(const float4* __restrict__ sp, const float4* __restrict__ sv,
-                                             const uint32_t* __restrict__ cs, const uint32_t* __restrict__ ce,
-                                             uint32_t il, uint32_t n_s, float4 pi, float4 vi, float h, float3* P,
-                                             float3* V)
+struct BPairSums {
+    float3 P, V;
+};
+
+__device__ __noinline__ BPairSums b_force_fallback(const float4* __restrict__ sp, const float4* __restrict__ sv,
+                                                  const uint32_t* __restrict__ cs, const uint32_t* __restrict__ ce,
+                                                  uint32_t il, uint32_t n_s, float4 pi, float4 vi, float h)
 {
-    sph_force_sum(sp, sv, cs, ce, il, n_s, pi, vi, h, *P, *V);
+    BPairSums r;
+    sph_force_sum(sp, sv, cs, ce, il, n_s, pi, vi, h, r.P, r.V);
+    return r;
 }
 
 // The kept-entry force pass of the single-scene program (k_forces_kept) over all scenes, with the same
@@ -440,9 +450,12 @@ __global__ __launch_bounds__(kThreads) void k_b_forces_kept(const float4* __rest
             wave_sync();
         }
     }
-    if (live && !staged && slot == 0)
-        b_force_fallback(sp, sv, cs_all + (size_t)s * kBuckets, ce_all + (size_t)s * kBuckets, i - lo, n_s, pi, vi, h,
-                         &Pa, &Va);
+    if (live && !staged && slot == 0) {
+        const BPairSums f = b_force_fallback(sp, sv, cs_all + (size_t)s * kBuckets, ce_all + (size_t)s * kBuckets,
+                                             i - lo, n_s, pi, vi, h);
+        Pa = f.P;
+        Va = f.V;
+    }
     Pa = make_float3(particle_sum<W>(Pa.x), particle_sum<W>(Pa.y), particle_sum<W>(Pa.z));
     Va = make_float3(particle_sum<W>(Va.x), particle_sum<W>(Va.y), particle_sum<W>(Va.z));
     if (!live || slot != 0) return;
