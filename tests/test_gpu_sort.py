@@ -34,7 +34,9 @@ def _run(sorter, keys, vals, start, end):
 
 
 @pytest.mark.parametrize("n", [1, 7, 64, 255, 256, 257, 1000, 1024, 4099, 65536, 100003, 1 << 20])
-@pytest.mark.parametrize("bits", [(0, 24), (0, 32), (4, 12)])
+# digit widths per pass: (0, 24) 8+8+8, (0, 32) 4 x 8, (4, 12) one 8-bit pass, (0, 20) 7+7+6,
+# (8, 28) 7+7+6 above the low byte, (0, 12) 6+6 (widths 7-9 are compiled in, the others run generic)
+@pytest.mark.parametrize("bits", [(0, 24), (0, 32), (4, 12), (0, 20), (8, 28), (0, 12)])
 def test_sort_matches_stable_argsort(sorter, n, bits):
     print(f"sort n={n} bits={bits}", flush=True)
     keys = scenes.sort_keys(n, seed=100 + n, bits=24) ^ (scenes.splitmix64(7, n) << np.uint64(24)).astype(np.uint32)
@@ -81,6 +83,27 @@ def test_sort_16m_cell_keys(sorter, skew):
     order = np.argsort(keys, kind="stable")
     np.testing.assert_array_equal(k, keys[order])
     np.testing.assert_array_equal(v, order.astype(np.uint32))
+
+
+@pytest.mark.parametrize("n", [4096, 1 << 20, (1 << 23) + 5])
+def test_sort_unaligned_arrays(sorter, n):
+    """Key and value arrays that are 4-byte but not 16-byte aligned (views one element into their
+    buffers): whole tiles of aligned arrays are counted with 16-byte loads, these take 4-byte loads."""
+    import torch
+    print(f"sort unaligned n={n}", flush=True)
+    keys = scenes.sort_keys(n, seed=31, bits=24)
+    vals = scenes.splitmix64(32, n).astype(np.uint32)
+    kb = torch.empty(n + 1, dtype=torch.int32, device="cuda")
+    vb = torch.empty(n + 1, dtype=torch.int32, device="cuda")
+    k, v = kb[1:], vb[1:]
+    assert k.data_ptr() % 16 != 0 and k.is_contiguous()
+    k.copy_(torch.from_numpy(keys.view(np.int32)))
+    v.copy_(torch.from_numpy(vals.view(np.int32)))
+    assert sorter.compute(k, v, 0, 24) == 1
+    torch.cuda.synchronize()
+    ek, ev = _expected(keys, vals, 0, 24)
+    np.testing.assert_array_equal(k.cpu().numpy().view(np.uint32), ek)
+    np.testing.assert_array_equal(v.cpu().numpy().view(np.uint32), ev)
 
 
 def test_sort_reference_smoke(sorter):
