@@ -75,6 +75,48 @@ def test_batch_matches_single_scene_runs(graph, shape):
         np.testing.assert_array_equal(ce, it["cell_end"])
 
 
+def test_batch_per_scene_params_and_boxes():
+    """Scenes with their own Params (cell size, viscosity, stiffness, time step) and clip boxes: each
+    equals the scene run alone with those Params and that box, bit for bit (default launch shape)."""
+    import torch
+    import libcl_amd as L
+    scs = scenes.batched_scenes(num=6, seed=9, steps=5)
+    prms, boxes = [], []
+    for s, sc in enumerate(scs):
+        h = 0.02 * (1.0 + 0.15 * (s % 3))
+        v = dict(scenes.sph_param_values(h), viscosity=4.0 + s, stiffness=1.0 + 0.5 * s, deltaTime=1e-4 * (1 + s % 2))
+        prms.append(L.params_from(v))
+        boxes.append(((0.0, 0.0, 0.0, 0.0), (0.6 + 0.1 * s, 1.0, 0.8, 0.0)))
+    b = L.FluidBatch([sc.n for sc in scs])
+    for s in range(len(scs)):
+        assert b.setParameters(prms[s], scene=s) == 1, L.last_error()
+        assert b.setBox(*boxes[s], scene=s) == 1, L.last_error()
+    assert b.upload(np.concatenate([sc.pos for sc in scs]), np.concatenate([sc.vel for sc in scs]),
+                    np.concatenate([sc.force for sc in scs])) == 1
+    stream = torch.cuda.Stream()
+    steps = 5
+    for _ in range(steps):
+        assert b.compute(stream) == 1, L.last_error()
+    stream.synchronize()
+    st = b.state()
+    for s, sc in enumerate(scs):
+        print(f"scene {s} n={sc.n} h={prms[s].cellSize}", flush=True)
+        f = L.oclFluid3D()
+        assert f.compile() == 1 and f.setParticleCount(sc.n) == 1
+        assert f.setTuning(neighbour_pass="tile_kept", density_lanes=4, tile_depth=192, block_waves=2,
+                           force_chunks=2) == 1  # the batch's default shape
+        f.setParameters(prms[s])
+        assert f.setBox(*boxes[s]) == 1
+        assert f.upload(sc.pos, sc.vel, sc.force) == 1
+        for _ in range(steps):
+            assert f.compute() == 1, L.last_error()
+        L.load().mfl_device_finish()
+        ref = f.state()
+        lo = int(b.offsets[s])
+        for k in ("pos", "vel", "force"):
+            np.testing.assert_array_equal(st[k][lo:lo + sc.n], ref[k])
+
+
 def test_batch_rejects_bad_counts():
     import libcl_amd as L
     with pytest.raises(RuntimeError, match="multiple of 256"):
