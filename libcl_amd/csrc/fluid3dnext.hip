@@ -14,6 +14,7 @@
 // clamp = fmin(fmax(x, lo), hi); cell indices outside the grid are not recorded in the bounds.
 #include "fluid3dnext.hpp"
 
+#include <cmath>
 #include <cstring>
 #include <string>
 
@@ -25,11 +26,13 @@ constexpr uint32_t kNoCell = 0xFFFFFFFFu;
 
 inline uint32_t blocks_for(size_t n) { return (uint32_t)((n + kBlock - 1) / kBlock); }
 
-// W, cl:116-133
-__device__ inline float next_w(float h, float q)
+// W, cl:116-133. POW2: h is a power of two, so q * (1 / h) equals q / h exactly and the pair loop
+// multiplies instead of dividing.
+template <bool POW2 = false>
+__device__ inline float next_w(float h, float q, float inv_h = 0.f)
 {
     float value = 0.f;
-    q /= h;
+    q = POW2 ? q * inv_h : q / h;
     if (q < 1.f) {
         value = 1.0f - 1.5f * q * q + 0.75f * q * q * q;
     } else if (q < 2.f) {
@@ -37,6 +40,13 @@ __device__ inline float next_w(float h, float q)
         value = 0.25f * q * q * q;
     }
     return 2.0f / (3.0f * h) * value;
+}
+
+// h a power of two whose reciprocal is a normal float: q * (1 / h) == q / h for every q
+inline bool reciprocal_exact(float h)
+{
+    int e = 0;
+    return h > 0.f && std::isfinite(h) && std::frexp(h, &e) == 0.5f && e > -125 && e < 126;
 }
 
 struct Window {
@@ -93,6 +103,7 @@ __global__ __launch_bounds__(kBlock) void k_next_reorder_bounds(const uint32_t* 
 }
 
 // clComputePressure, cl:135-181
+template <bool POW2>
 __global__ __launch_bounds__(kBlock) void k_next_pressure(const float4* __restrict__ sorted,
                                                          float2* __restrict__ pressure,
                                                          const uint32_t* __restrict__ start,
@@ -103,6 +114,7 @@ __global__ __launch_bounds__(kBlock) void k_next_pressure(const float4* __restri
     if (i >= n) return;
     const float4 si = sorted[i];
     const Window w = window_of(si.x, si.y, p);
+    const float inv_h = 1.0f / p.cellSize;
     float density = 0.f;
     for (int x = w.x0; x <= w.x1; x++)
         for (int y = w.y0; y <= w.y1; y++) {
@@ -114,7 +126,7 @@ __global__ __launch_bounds__(kBlock) void k_next_pressure(const float4* __restri
                 if (j == i) continue;
                 const float4 sj = sorted[j];
                 const float dx = sj.x - si.x, dy = sj.y - si.y;
-                density += next_w(p.cellSize, sqrtf(dx * dx + dy * dy));
+                density += next_w<POW2>(p.cellSize, sqrtf(dx * dx + dy * dy), inv_h);
             }
         }
     float pr = 0.f;
@@ -331,8 +343,12 @@ int Fluid3DNext::compute(hipStream_t s)
     if (!sort_.compute(cell, index, n, 0, 24, s, true)) return 0;  // cpp:272
     hipLaunchKernelGGL(k_next_reorder_bounds, dim3(blocks_for(n)), dim3(kBlock), 0, s, index, state, sorted, cell, n,
                        start, end, ncells_);
-    hipLaunchKernelGGL(k_next_pressure, dim3(blocks_for(n)), dim3(kBlock), 0, s, sorted, pressure_.as<float2>(), start,
-                       end, n, params);
+    if (reciprocal_exact(params.cellSize))
+        hipLaunchKernelGGL(k_next_pressure<true>, dim3(blocks_for(n)), dim3(kBlock), 0, s, sorted, pressure_.as<float2>(),
+                           start, end, n, params);
+    else
+        hipLaunchKernelGGL(k_next_pressure<false>, dim3(blocks_for(n)), dim3(kBlock), 0, s, sorted, pressure_.as<float2>(),
+                           start, end, n, params);
     hipLaunchKernelGGL(k_next_forces_integrate, dim3(blocks_for(n)), dim3(kBlock), 0, s, sorted,
                        pressure_.as<float2>(), force_.as<float2>(), index, state, start, end, n, params);
     MFL_HIP(hipGetLastError());
