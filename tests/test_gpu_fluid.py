@@ -115,32 +115,64 @@ def _oracle_compare(f, fo, check_fields=True):
     return it
 
 
+_TRAJECTORIES = {}
+
+
+def _oracle_snapshots(oracle, sc, prm, clear, targets):
+    """States of the oracle's own trajectory at the given steps (positions, velocities, forces and
+    bucket tables), computed once per scene and clear count."""
+    key = (sc.name, clear, targets)
+    if key not in _TRAJECTORIES:
+        op = oracle.Params(*prm.values())
+        fo = oracle.Fluid(sc.pos, sc.vel, sc.force, op, box_min=sc.box_min, box_max=sc.box_max, clear_count=clear)
+        snaps, done = [], 0
+        for target in targets:
+            fo.steps(target - done)
+            done = target
+            snaps.append((target, fo.pos.copy(), fo.vel.copy(), fo.force.copy(), fo.cell_start.copy(),
+                          fo.cell_end.copy()))
+        _TRAJECTORIES[key] = snaps
+    return _TRAJECTORIES[key]
+
+
+def _step_from_snapshot(oracle, snap, prm, sc, clear, shape):
+    """One step (forces only) from an oracle snapshot on both sides: keys, permutation and tables
+    bit-exact, density/pressure and forces within TOL."""
+    target, pos, vel, force, cs, ce = snap
+    op = oracle.Params(*prm.values())
+    f = make_fluid(pos, vel, force, prm, sc.box_min, sc.box_max, clear, tables=(cs, ce), shape=shape)
+    run_forces_only(f)
+    fo2 = oracle.Fluid(pos, vel, force, op, box_min=sc.box_min, box_max=sc.box_max, clear_count=clear,
+                       cell_start=cs, cell_end=ce)
+    fo2.step(integrate=False)
+    _oracle_compare(f, fo2)
+    nw, per = force_errors(f.state()["force"], fo2.force)
+    assert nw <= TOL and per <= TOL, (target, nw, per)
+
+
 @pytest.mark.parametrize("shape", list(SHAPES))
 @pytest.mark.parametrize("clear", [65536, 262144])
 def test_small_dam_break_vs_oracle(oracle, clear, shape):
-    """16,384-particle dam break: snapshots along the oracle trajectory (steps 0, 1, 10, 60) are
-    stepped once on both sides, bucket tables included (stale ranges of the reference's partial
-    clear are part of the state)."""
+    """16,384-particle dam break over its 1000 steps: snapshots along the oracle trajectory (steps 0, 1,
+    10, 100, 1000) are stepped once on both sides, bucket tables included (stale ranges of the
+    reference's partial clear are part of the state)."""
     sc = scenes.config_small()
     import libcl_amd as L
     prm = L.params_from(sc.params)
-    op = oracle.Params(*prm.values())
-    fo = oracle.Fluid(sc.pos, sc.vel, sc.force, op, box_min=sc.box_min, box_max=sc.box_max, clear_count=clear)
-    done = 0
-    for target in (0, 1, 10, 60):
-        fo.steps(target - done)
-        done = target
-        print(f"16k snapshot step {target} clear {clear} {shape}", flush=True)
-        snap = (fo.pos.copy(), fo.vel.copy(), fo.force.copy(), fo.cell_start.copy(), fo.cell_end.copy())
-        f = make_fluid(snap[0], snap[1], snap[2], prm, sc.box_min, sc.box_max, clear, tables=(snap[3], snap[4]),
-                       shape=shape)
-        run_forces_only(f)
-        fo2 = oracle.Fluid(*snap[:3], op, box_min=sc.box_min, box_max=sc.box_max, clear_count=clear,
-                           cell_start=snap[3], cell_end=snap[4])
-        fo2.step(integrate=False)
-        _oracle_compare(f, fo2)
-        nw, per = force_errors(f.state()["force"], fo2.force)
-        assert nw <= TOL and per <= TOL, (target, nw, per)
+    for snap in _oracle_snapshots(oracle, sc, prm, clear, (0, 1, 10, 100, 1000)):
+        print(f"16k snapshot step {snap[0]} clear {clear} {shape}", flush=True)
+        _step_from_snapshot(oracle, snap, prm, sc, clear, shape)
+
+
+def test_1m_trajectory_snapshots_vs_oracle(oracle):
+    """1,048,576 particles over their 200 steps: snapshots of the oracle trajectory at steps 1, 10, 100
+    and 200 (the block falling and spreading on the floor) stepped once by the default program."""
+    sc = scenes.config_1m()
+    import libcl_amd as L
+    prm = L.params_from(sc.params)
+    for snap in _oracle_snapshots(oracle, sc, prm, 65536, (1, 10, 100, 200)):
+        print(f"1M snapshot step {snap[0]}", flush=True)
+        _step_from_snapshot(oracle, snap, prm, sc, 65536, "tile_kept")
 
 
 @pytest.mark.parametrize("shape", ["tile_list", "tile_list8", "tile", "tile_kept", "list"])
