@@ -245,15 +245,16 @@ __global__ __launch_bounds__(kThreads) void k_b_density_tile(const float4* __res
         }
     }
     const bool live = i < N;
-    // the segment's scene (N is a multiple of 256, so a segment is all live or all past the end)
-    const uint32_t s = L.from(sc, 0u);
+    // The wave's scene, in scalar registers: a wave holds 16 (or 8) consecutive traversal slots from a
+    // multiple of 16, scenes hold multiples of 256 slots in the traversal order, so a wave never
+    // straddles two scenes (and is all live or all past N, whose lanes carry scene 0).
+    const uint32_t s = __builtin_amdgcn_readfirstlane(sc);
     const uint32_t lo = off[s], n_s = off[s + 1] - lo;
     const float4* sp = spos + lo;
     const uint32_t* cs = cs_all + (size_t)s * kBuckets;
     const uint32_t* ce = ce_all + (size_t)s * kBuckets;
-    const mfl_params p = params[s];
     const uint32_t il = live ? i - lo : n_s;
-    const float h = p.cellSize;
+    const float h = params[s].cellSize;  // the other Params are read after the loop (fewer live VGPRs)
     const float h2 = h * h;
     const float cut = cuts[s];  // sqrt(r2) < h exactly when r2 < cut
     const int gx = grid_coord(pi.x, h), gy = grid_coord(pi.y, h), gz = grid_coord(pi.z, h);
@@ -320,6 +321,7 @@ __global__ __launch_bounds__(kThreads) void k_b_density_tile(const float4* __res
     acc = particle_sum<W>(acc);
     count = particle_sum_u<W>(count);
     if (!live || slot != 0) return;
+    const mfl_params p = params[s];
     float rho, prs;
     if (count > 2) {
         rho = fmaxf(1.0f, p.mass * p.kernelConstant * acc);
@@ -386,12 +388,11 @@ __global__ __launch_bounds__(kThreads) void k_b_forces_kept(const float4* __rest
         }
     }
     const bool live = i < N;
-    const uint32_t s = L.from(sc, 0u);
+    const uint32_t s = __builtin_amdgcn_readfirstlane(sc);  // the wave's scene (k_b_density_tile)
     const uint32_t lo = off[s], n_s = off[s + 1] - lo;
     const float4* sp = spos + lo;
     const float4* sv = svel + lo;
-    const mfl_params p = params[s];
-    const float h = p.cellSize;
+    const float h = params[s].cellSize;  // the other Params are read after the loop (fewer live VGPRs)
     const float4 pi = live ? spos[i] : make_float4(0.f, 0.f, 0.f, 0.f);
     const float4 vi = live ? svel[i] : make_float4(0.f, 0.f, 0.f, 0.f);
     const uint32_t mt = live ? meta[t] : kMetaWalk;
@@ -459,6 +460,7 @@ __global__ __launch_bounds__(kThreads) void k_b_forces_kept(const float4* __rest
     Pa = make_float3(particle_sum<W>(Pa.x), particle_sum<W>(Pa.y), particle_sum<W>(Pa.z));
     Va = make_float3(particle_sum<W>(Va.x), particle_sum<W>(Va.y), particle_sum<W>(Va.z));
     if (!live || slot != 0) return;
+    const mfl_params p = params[s];
     float4 f = add_sph_force(force[o], Pa, Va, p);
     f = clip_box(f, pi, vi, p.particleRadius, boxes[2 * s], boxes[2 * s + 1]);
     const float4 v = integrate_force(vel[o], f, p);
