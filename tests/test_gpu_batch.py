@@ -8,15 +8,29 @@ from libcl_amd import scenes
 pytestmark = pytest.mark.gpu
 
 
-def _plain(sc, prm, steps, shape="list"):
+# batch launch shapes under test, and the single-scene shape each is compared with
+BATCH_SHAPES = {
+    "list": ({"neighbour_pass": "list"}, {"neighbour_pass": "list"}),
+    "tile_list": ({"neighbour_pass": "tile_list"},
+                  {"neighbour_pass": "tile_list", "density_lanes": 4, "tile_depth": 192, "block_waves": 2,
+                   "force_chunks": 2}),
+    "tile_kept": ({"neighbour_pass": "tile_kept"},
+                  {"neighbour_pass": "tile_kept", "density_lanes": 4, "tile_depth": 192, "block_waves": 2,
+                   "force_chunks": 2}),
+    # 8 lanes per particle: one 64-lane segment of 8 particles per wave
+    "tile_kept8": ({"neighbour_pass": "tile_kept", "density_lanes": 8, "tile_depth": 256},
+                   {"neighbour_pass": "tile_kept", "density_lanes": 8, "tile_depth": 256, "block_waves": 2,
+                    "force_chunks": 2}),
+    "default": (None, {"neighbour_pass": "tile_kept", "density_lanes": 4, "tile_depth": 192, "block_waves": 2,
+                       "force_chunks": 2}),
+}
+
+
+def _plain(sc, prm, steps, tuning):
     import libcl_amd as L
     f = L.oclFluid3D()
     assert f.compile() == 1
-    if shape == "list":
-        assert f.setTuning(neighbour_pass="list") == 1
-    else:
-        assert f.setTuning(neighbour_pass=shape, density_lanes=4, tile_depth=192, block_waves=2,
-                           force_chunks=2) == 1
+    assert f.setTuning(**tuning) == 1, L.last_error()
     assert f.setParticleCount(sc.n) == 1
     f.setParameters(prm)
     f.setBox(sc.box_min, sc.box_max)
@@ -28,7 +42,7 @@ def _plain(sc, prm, steps, shape="list"):
 
 
 @pytest.mark.parametrize("graph", [False, True])
-@pytest.mark.parametrize("shape", ["list", "tile_list", "tile_kept", "default"])
+@pytest.mark.parametrize("shape", list(BATCH_SHAPES))
 def test_batch_matches_single_scene_runs(graph, shape):
     """Every scene of the batch equals the same scene run alone by a single-scene program with the same
     launch shape (LIST: group traversal; TILE_LIST / TILE_KEPT: 4-lane tiles in a Morton order refreshed
@@ -40,10 +54,9 @@ def test_batch_matches_single_scene_runs(graph, shape):
     prm = L.params_from(scs[0].params)
     counts = [s.n for s in scs]
     b = L.FluidBatch(counts)
-    if shape == "default":  # the batch's own default launch shape, never set (kept-entry passes)
-        shape = "tile_kept"
-    else:
-        assert b.setTuning(neighbour_pass=shape) == 1, L.last_error()
+    batch_tuning, single_tuning = BATCH_SHAPES[shape]
+    if batch_tuning is not None:  # None: the batch's own default launch shape, never set
+        assert b.setTuning(**batch_tuning) == 1, L.last_error()
     assert b.setParameters(prm) == 1
     assert b.setBox((0, 0, 0, 0), (1, 1, 1, 0)) == 1
     assert b.upload(np.concatenate([s.pos for s in scs]), np.concatenate([s.vel for s in scs]),
@@ -60,7 +73,7 @@ def test_batch_matches_single_scene_runs(graph, shape):
     spos = b.read(N.MFL_SORTED_POSITION, np.float32, (-1, 4))
     for s, sc in enumerate(scs):
         print(f"scene {s} n={sc.n} {shape}", flush=True)
-        f = _plain(sc, prm, steps, shape)
+        f = _plain(sc, prm, steps, single_tuning)
         ref = f.state()
         it = f.internals()
         lo, n = int(b.offsets[s]), sc.n
