@@ -6,8 +6,8 @@ Workload: the 1,048,576-particle dam break (128x64x128 jittered lattice at 0.5h,
 [0,4]^3, fp32), inputs resident in HBM. A step is one oclFluid3D::compute(): hash + bucket clear,
 sort, reorder + bounds, density, forces + clip box + integration. Prints one JSON line; its
 secondary legs are the batched scenes, the adjoint, the particle-vs-mesh collision (BVH build +
-clCollideBVH on the same 1M particles), the 2D sibling solver oclFluid3Dnext at 1M particles and the
-standalone key/value sort of 16M cell keys.
+clCollideBVH on the same 1M particles), the 2D sibling solver oclFluid3Dnext at 1M particles, the
+standalone key/value sort of 16M cell keys and the 16k dam break (eager against hipGraph replay).
 """
 from __future__ import annotations
 
@@ -308,6 +308,60 @@ def next_leg(steps: int = 50, cpu: bool = True) -> dict:
     return out
 
 
+def small_leg(steps: int = 1000, reps: int = 3, cpu: bool = True) -> dict:
+    """The 16,384-particle dam break in the unit box (sph_16k_unit_box), its 1000 steps from the initial
+    state, as eager launches and as hipGraph replays on one stream, alternating, `reps` times each (the
+    best run of each counts): at this size a step is launch-bound, so the figure is particle-steps/s and
+    the graph gain."""
+    import numpy as np
+    import torch
+
+    import libcl_amd as L
+    from libcl_amd import scenes
+
+    sc = scenes.config_small()
+    prm = L.params_from(sc.params)
+    stream = torch.cuda.Stream()
+    progs = {}
+    for mode in ("eager", "graph"):
+        f = L.oclFluid3D()
+        assert f.compile() == 1 and f.setParticleCount(sc.n) == 1, L.last_error()
+        f.setParameters(prm)
+        f.setBox(sc.box_min, sc.box_max)
+        assert f.setUseGraph(mode == "graph") == 1
+        progs[mode] = f
+    empty = np.full(262144, 0xFFFFFFFF, np.uint32)
+    best = {"eager": float("inf"), "graph": float("inf")}
+    for _ in range(reps):
+        for mode, f in progs.items():
+            assert f.upload(sc.pos, sc.vel, sc.force) == 1
+            assert f.bfCellStart.write(empty) == 1 and f.bfCellEnd.write(empty) == 1
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            for _ in range(steps):
+                f.compute(stream)
+            torch.cuda.synchronize()
+            best[mode] = min(best[mode], (time.perf_counter() - t0) * 1e3 / steps)
+    out = {"workload": sc.name, "particles": sc.n, "steps": steps, "reps": reps}
+    for mode, ms in best.items():
+        out[f"{mode}_ms_per_step"] = ms
+        out[f"{mode}_particle_steps_per_s"] = sc.n / (ms * 1e-3)
+    out["graph_gain"] = best["eager"] / best["graph"]
+    if cpu:
+        from oracle import oracle as O
+        O.build()
+        threads = O.set_threads(min(os.cpu_count() or 1, 16))
+        op = O.Params(*prm.values())
+        fo = O.Fluid(sc.pos, sc.vel, sc.force, op, box_min=sc.box_min, box_max=sc.box_max)
+        t0 = time.perf_counter()
+        fo.steps(steps)
+        dt = time.perf_counter() - t0
+        out["cpu_baseline"] = {"value": sc.n * steps / dt, "unit": "particle-steps/s", "cores": threads,
+                               "kind": "port", "sample": f"the same {steps} steps, oracle/sph_oracle.c with "
+                                                         f"{threads} OpenMP threads ({dt:.1f} s)"}
+    return out
+
+
 SORT_BYTES_PER_PAIR = 52  # 3 passes x (R 8 + W 8) + one 4-byte histogram read per key
 
 
@@ -388,7 +442,8 @@ def main():
     ap.add_argument("--no-collide", action="store_true")
     ap.add_argument("--no-next", action="store_true")
     ap.add_argument("--no-sort", action="store_true")
-    ap.add_argument("--only", default=None, choices=["batched", "adjoint", "collide", "next", "sort"],
+    ap.add_argument("--no-small", action="store_true")
+    ap.add_argument("--only", default=None, choices=["batched", "adjoint", "collide", "next", "sort", "small"],
                     help="run one secondary leg alone and print its JSON (profiling)")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="also run a plain device copy of MIB MiB three times (counter calibration)")
@@ -429,6 +484,9 @@ def main():
         return
     if args.only == "sort":
         print(json.dumps(sort_leg(cpu=not args.no_cpu)), flush=True)
+        return
+    if args.only == "small":
+        print(json.dumps(small_leg(cpu=not args.no_cpu)), flush=True)
         return
 
     sc = scenes.config_1m() if args.config == "1m" else scenes.config_small()
@@ -551,6 +609,8 @@ def main():
         out["next"] = next_leg(cpu=not args.no_cpu)
     if not args.no_sort:
         out["sort"] = sort_leg(cpu=not args.no_cpu)
+    if not args.no_small:
+        out["small"] = small_leg(cpu=not args.no_cpu)
     if not args.no_cpu:
         nthreads = min(os.cpu_count() or 1, 16)
         out["cpu_baseline"] = cpu_baseline(sc, prm, args.cpu_steps, nthreads)
