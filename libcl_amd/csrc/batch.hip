@@ -310,11 +310,17 @@ __global__ __launch_bounds__(kThreads) void k_b_density_tile(const float4* __res
                     at++;
                 }
             }
-        } else if (c.member && slot == 0) {
-            if (KEPT) meta[t] = kMetaWalk;
-            const BVisitSums f = b_density_walk(c.cx, c.cy, c.cz, pi, il, i, sp, cs, ce, n_s, h, N, KEPT ? nullptr : nbr);
-            count += f.count;
-            acc += f.acc;
+        } else if (c.member) {
+            if constexpr (KEPT) {  // the particle's P lanes split the walk (density_walk_split, as alone)
+                if (slot == 0) meta[t] = kMetaWalk;
+                const WalkSums f = density_walk_split(c.cx, c.cy, c.cz, pi, il, sp, cs, ce, n_s, h, slot, P);
+                count += f.count;
+                acc += f.acc;
+            } else if (slot == 0) {
+                const BVisitSums f = b_density_walk(c.cx, c.cy, c.cz, pi, il, i, sp, cs, ce, n_s, h, N, nbr);
+                count += f.count;
+                acc += f.acc;
+            }
         }
         wave_sync();
     }
@@ -335,18 +341,6 @@ __global__ __launch_bounds__(kThreads) void k_b_density_tile(const float4* __res
     if (!KEPT) ncount[i] = count;
 }
 
-struct BPairSums {
-    float3 P, V;
-};
-
-__device__ __noinline__ BPairSums b_force_fallback(const float4* __restrict__ sp, const float4* __restrict__ sv,
-                                                  const uint32_t* __restrict__ cs, const uint32_t* __restrict__ ce,
-                                                  uint32_t il, uint32_t n_s, float4 pi, float4 vi, float h)
-{
-    BPairSums r;
-    sph_force_sum(sp, sv, cs, ce, il, n_s, pi, vi, h, r.P, r.V);
-    return r;
-}
 
 // The kept-entry force pass of the single-scene program (k_forces_kept) over all scenes, with the same
 // waves and segments as k_b_density_tile<W, true>: each segment stages its cell's kept entries (scene-
@@ -451,9 +445,9 @@ __global__ __launch_bounds__(kThreads) void k_b_forces_kept(const float4* __rest
             wave_sync();
         }
     }
-    if (live && !staged && slot == 0) {
-        const BPairSums f = b_force_fallback(sp, sv, cs_all + (size_t)s * kBuckets, ce_all + (size_t)s * kBuckets,
-                                             i - lo, n_s, pi, vi, h);
+    if (live && !staged) {  // the particle's P lanes split the walk of a cell that overflowed its tile
+        const WalkForces f = force_walk_split(sp, sv, cs_all + (size_t)s * kBuckets, ce_all + (size_t)s * kBuckets,
+                                              i - lo, n_s, pi, vi, h, slot, P);
         Pa = f.P;
         Va = f.V;
     }

@@ -469,7 +469,8 @@ __device__ inline float4 slot_particle(uint32_t t, uint32_t n, const uint32_t* _
 }
 
 // One lane walks particle i's 27 bucket ranges from global memory (the reference's loop, cl:161-190) and
-// records its visits for traversal slot t when nbr is not null: the overflow path of the 4-lane layout.
+// records its visits for traversal slot t: the overflow path of the 4-lane layout when visits are
+// recorded (without a record the lanes split the walk, density_walk_split).
 __device__ __noinline__ VisitSums density_walk(int cx, int cy, int cz, float4 pi, uint32_t i, uint32_t t,
                                               const float4* __restrict__ spos, const uint32_t* __restrict__ cell_start,
                                               const uint32_t* __restrict__ cell_end, uint32_t n, float h,
@@ -600,9 +601,14 @@ __global__ __launch_bounds__(kThreads) void k_density_tile(const float4* __restr
                                                      n, h, LIST ? nbr : nullptr);
                 acc += f.acc;
                 count += slot == 0 ? f.count : 0u;  // every lane of the group holds the group's count
-            } else if (slot == 0) {
-                const VisitSums f = density_walk(c.cx, c.cy, c.cz, pi, i, t, spos, cell_start, cell_end, n, h,
-                                                 LIST ? nbr : nullptr);
+            } else if constexpr (LIST) {
+                if (slot == 0) {
+                    const VisitSums f = density_walk(c.cx, c.cy, c.cz, pi, i, t, spos, cell_start, cell_end, n, h, nbr);
+                    acc += f.acc;
+                    count += f.count;
+                }
+            } else {  // no visits to record: the particle's P lanes split the walk
+                const WalkSums f = density_walk_split(c.cx, c.cy, c.cz, pi, i, spos, cell_start, cell_end, n, h, slot, P);
                 acc += f.acc;
                 count += f.count;
             }
@@ -794,8 +800,8 @@ __global__ __launch_bounds__(kThreads) void k_forces_kept(const float4* __restri
             wave_sync();  // the next chunk overwrites the tile
         }
     }
-    if (live && !staged && slot == 0) {
-        const PairSums f = force_fallback(spos, svel, cell_start, cell_end, i, n, pi, vi, h);
+    if (live && !staged) {  // the particle's P lanes split the walk of a cell that overflowed its tile
+        const WalkForces f = force_walk_split(spos, svel, cell_start, cell_end, i, n, pi, vi, h, slot, P);
         Pa = f.P;
         Va = f.V;
     }

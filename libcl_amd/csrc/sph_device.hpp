@@ -72,10 +72,12 @@ __device__ inline bool accept_r2(float r2, float h, float h2_lo, float h2_hi)
 
 // Pressure and viscosity sums of cl:213-249 for sorted particle i by the full 27-bucket traversal (the
 // fallback when a particle has more visits than its recorded list holds).
+// `first` and `stride` select the visits first, first + stride, ... of every range (all of them by
+// default), so that the lanes of one particle can split the walk.
 __device__ inline void sph_force_sum(const float4* __restrict__ spos, const float4* __restrict__ svel,
                                      const uint32_t* __restrict__ cell_start, const uint32_t* __restrict__ cell_end,
                                      uint32_t i, uint32_t n, const float4& pi, const float4& vi, float h, float3& P,
-                                     float3& V)
+                                     float3& V, uint32_t first = 0, uint32_t stride = 1)
 {
     const float h2 = h * h;
     const float h2_lo = h2 * 0.9999996f, h2_hi = h2 * 1.0000005f;
@@ -89,7 +91,7 @@ __device__ inline void sph_force_sum(const float4* __restrict__ spos, const floa
                 const uint32_t start = cell_start[c];
                 if (start == kEmpty) continue;
                 const uint32_t end = min(cell_end[c], n);  // stale or written ranges always lie in [0, n]
-                for (uint32_t j = start; j < end; j++) {
+                for (uint32_t j = start + first; j < end; j += stride) {
                     if (j == i) continue;
                     const float4 pj = spos[j];
                     const float dx = pi.x - pj.x, dy = pi.y - pj.y, dz = pi.z - pj.z;
@@ -97,6 +99,60 @@ __device__ inline void sph_force_sum(const float4* __restrict__ spos, const floa
                     if (accept_r2(r2, h, h2_lo, h2_hi)) pair_force(pi, vi, pj, svel[j], dx, dy, dz, r2, h, P, V);
                 }
             }
+}
+
+// Walks of a neighbourhood that overflowed the LDS tile (the reference's loops, cl:161-190 and
+// cl:213-249), split over the `stride` lanes of a particle: lane `first` takes the visits first,
+// first + stride, ... of every bucket range, so each visit is tested once and the caller adds the
+// lanes' sums. Out of line (the rare path must not set the tile kernels' register budget) and returning
+// by value (a pointer to the caller's sums would keep them in scratch memory).
+struct WalkSums {
+    float acc;
+    uint32_t count;
+};
+
+__device__ __noinline__ inline WalkSums density_walk_split(int cx, int cy, int cz, float4 pi, uint32_t i,
+                                                          const float4* __restrict__ spos,
+                                                          const uint32_t* __restrict__ cell_start,
+                                                          const uint32_t* __restrict__ cell_end, uint32_t n, float h,
+                                                          uint32_t first, uint32_t stride)
+{
+    const float h2 = h * h;
+    const float h2_lo = h2 * 0.9999996f, h2_hi = h2 * 1.0000005f;
+    WalkSums r{0.f, 0u};
+    for (int k = 0; k < 27; k++) {
+        const uint32_t b = cell_hash(cx + k % 3 - 1, cy + (k / 3) % 3 - 1, cz + k / 9 - 1);
+        const uint32_t st = cell_start[b];
+        if (st == kEmpty) continue;
+        const uint32_t end = min(cell_end[b], n);
+        for (uint32_t j = st + first; j < end; j += stride) {
+            const float4 pj = spos[j];
+            const float dx = pi.x - pj.x, dy = pi.y - pj.y, dz = pi.z - pj.z;
+            const float r2 = dist2(dx, dy, dz);
+            if (j != i && accept_r2(r2, h, h2_lo, h2_hi)) {
+                const float w = h2 - r2;
+                r.acc = fmaf(w * w, w, r.acc);
+                r.count++;
+            }
+        }
+    }
+    return r;
+}
+
+struct WalkForces {
+    float3 P, V;
+};
+
+__device__ __noinline__ inline WalkForces force_walk_split(const float4* __restrict__ spos,
+                                                          const float4* __restrict__ svel,
+                                                          const uint32_t* __restrict__ cell_start,
+                                                          const uint32_t* __restrict__ cell_end, uint32_t i,
+                                                          uint32_t n, float4 pi, float4 vi, float h, uint32_t first,
+                                                          uint32_t stride)
+{
+    WalkForces r;
+    sph_force_sum(spos, svel, cell_start, cell_end, i, n, pi, vi, h, r.P, r.V, first, stride);
+    return r;
 }
 
 __device__ inline float4 add_sph_force(float4 f, const float3& P, const float3& V, const mfl_params& p)
