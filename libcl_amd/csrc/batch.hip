@@ -64,15 +64,24 @@ __global__ __launch_bounds__(kThreads) void k_b_morton_keys(const float4* __rest
     key[g] = (s << (3 * mbits)) | code;
 }
 
-// the reference's per-step reset of the first `clear` entries of each scene's tables (cpp:309-313)
+// the reference's per-step reset of the first `clear` entries of each scene's tables (cpp:309-313):
+// scene blockIdx.y, four entries per thread (16-byte stores; the tables start on 16-byte boundaries)
 __global__ __launch_bounds__(kThreads) void k_b_clear(uint32_t* __restrict__ cs_all, uint32_t* __restrict__ ce_all,
-                                                      uint32_t S, uint32_t clear)
+                                                      uint32_t clear)
 {
-    const size_t total = (size_t)S * clear;
-    for (size_t e = (size_t)blockIdx.x * kThreads + threadIdx.x; e < total; e += (size_t)gridDim.x * kThreads) {
-        const size_t s = e / clear, b = e % clear;
-        cs_all[s * kBuckets + b] = kEmpty;
-        ce_all[s * kBuckets + b] = kEmpty;
+    const uint32_t b = (blockIdx.x * kThreads + threadIdx.x) * 4u;
+    if (b >= clear) return;
+    uint32_t* cs = cs_all + (size_t)blockIdx.y * kBuckets;
+    uint32_t* ce = ce_all + (size_t)blockIdx.y * kBuckets;
+    if (b + 4u <= clear) {
+        const uint4 e = make_uint4(kEmpty, kEmpty, kEmpty, kEmpty);
+        *reinterpret_cast<uint4*>(cs + b) = e;
+        *reinterpret_cast<uint4*>(ce + b) = e;
+    } else {
+        for (uint32_t k = b; k < clear; k++) {
+            cs[k] = kEmpty;
+            ce[k] = kEmpty;
+        }
     }
 }
 
@@ -693,9 +702,8 @@ int FluidBatch::enqueue_step(hipStream_t s)
     if (!sort.sort_bits(cell, index, N_, 0, 18 + (int)scene_bits_, s, true)) return 0;
     hipLaunchKernelGGL(k_b_reorder, g, b, 0, s, index, pos, vel, spos, svel, N_, inv_.as<uint32_t>());
     if (clear) {
-        const size_t total = (size_t)S_ * clear;
-        const unsigned blocks = (unsigned)std::min<size_t>((total + kThreads - 1) / kThreads, 65536);
-        hipLaunchKernelGGL(k_b_clear, dim3(blocks), b, 0, s, cs_all, ce_all, S_, clear);
+        const unsigned bx = (clear + 4u * kThreads - 1u) / (4u * kThreads);
+        hipLaunchKernelGGL(k_b_clear, dim3(bx, S_), b, 0, s, cs_all, ce_all, clear);
     }
     hipLaunchKernelGGL(k_b_find_bounds, g, b, 0, s, cell, off, N_, cs_all, ce_all);
     const bool tiles = (neighbour_pass == MFL_PASS_TILE_LIST || neighbour_pass == MFL_PASS_TILE_KEPT) && porder_valid_;
