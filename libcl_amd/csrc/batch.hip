@@ -31,18 +31,41 @@ __global__ __launch_bounds__(kThreads) void k_b_hash(const float4* __restrict__ 
     cell[g] = (s << 18) | cell_hash(grid_coord(q.x, h), grid_coord(q.y, h), grid_coord(q.z, h));
 }
 
-__global__ __launch_bounds__(kThreads) void k_b_reorder(const uint32_t* __restrict__ index,
-                                                        const float4* __restrict__ pos, const float4* __restrict__ vel,
-                                                        float4* __restrict__ spos, float4* __restrict__ svel, uint32_t N,
-                                                        uint32_t* __restrict__ inv)
+// Sorted gather (cl:66-72) and clFindBounds (cl:85-113) per scene in scene-local indices, in one pass
+// (the tables were reset before the sort; nothing between reads them).
+__global__ __launch_bounds__(kThreads) void k_b_reorder_bounds(const uint32_t* __restrict__ index,
+                                                               const float4* __restrict__ pos,
+                                                               const float4* __restrict__ vel,
+                                                               float4* __restrict__ spos, float4* __restrict__ svel,
+                                                               uint32_t N, uint32_t* __restrict__ inv,
+                                                               const uint32_t* __restrict__ cell,
+                                                               const uint32_t* __restrict__ off,
+                                                               uint32_t* __restrict__ cs_all,
+                                                               uint32_t* __restrict__ ce_all)
 {
     const uint32_t i = blockIdx.x * kThreads + threadIdx.x;
     if (i >= N) return;
     const uint32_t g = index[i];
-    if (g >= N) return;
-    spos[i] = pos[g];
-    svel[i] = vel[g];
-    inv[g] = i;
+    if (g < N) {  // index is a permutation of [0, N)
+        spos[i] = pos[g];
+        svel[i] = vel[g];
+        inv[g] = i;
+    }
+    const uint32_t key = cell[i];
+    const uint32_t s = key >> 18, h = key & (kBuckets - 1u);
+    const uint32_t lo = off[s], n_s = off[s + 1] - lo, li = i - lo;
+    uint32_t* cs = cs_all + (size_t)s * kBuckets;
+    uint32_t* ce = ce_all + (size_t)s * kBuckets;
+    if (li == 0) {
+        cs[h] = 0;
+        return;
+    }
+    const uint32_t prev = cell[i - 1] & (kBuckets - 1u);
+    if (h != prev) {
+        ce[prev] = li;
+        cs[h] = li;
+    }
+    if (li == n_s - 1) ce[h] = n_s;
 }
 
 // Traversal order of the tile density pass: scenes in order (scene in the high key bits), a Morton
@@ -85,29 +108,6 @@ __global__ __launch_bounds__(kThreads) void k_b_clear(uint32_t* __restrict__ cs_
     }
 }
 
-// clFindBounds (cl:85-113) per scene, in scene-local indices
-__global__ __launch_bounds__(kThreads) void k_b_find_bounds(const uint32_t* __restrict__ cell,
-                                                            const uint32_t* __restrict__ off, uint32_t N,
-                                                            uint32_t* __restrict__ cs_all, uint32_t* __restrict__ ce_all)
-{
-    const uint32_t i = blockIdx.x * kThreads + threadIdx.x;
-    if (i >= N) return;
-    const uint32_t key = cell[i];
-    const uint32_t s = key >> 18, h = key & (kBuckets - 1u);
-    const uint32_t lo = off[s], n_s = off[s + 1] - lo, li = i - lo;
-    uint32_t* cs = cs_all + (size_t)s * kBuckets;
-    uint32_t* ce = ce_all + (size_t)s * kBuckets;
-    if (li == 0) {
-        cs[h] = 0;
-        return;
-    }
-    const uint32_t prev = cell[i - 1] & (kBuckets - 1u);
-    if (h != prev) {
-        ce[prev] = li;
-        cs[h] = li;
-    }
-    if (li == n_s - 1) ce[h] = n_s;
-}
 
 // One group of eight lanes per particle (k_density_grp of the single-scene program) on scene s.
 __device__ inline void batch_density_home(uint32_t s, int gx, int gy, int gz, const float4& pi, uint32_t i, uint32_t slot,
@@ -699,13 +699,15 @@ int FluidBatch::enqueue_step(hipStream_t s)
 
     hipLaunchKernelGGL(k_b_hash, g, b, 0, s, pos, pscene_.as<uint32_t>(), params, N_, cell);
     MFL_HIP(hipGetLastError());
-    if (!sort.sort_bits(cell, index, N_, 0, 18 + (int)scene_bits_, s, true)) return 0;
-    hipLaunchKernelGGL(k_b_reorder, g, b, 0, s, index, pos, vel, spos, svel, N_, inv_.as<uint32_t>());
+    // the reference resets the tables between the reorder and the bounds (cpp:309-313); nothing before
+    // the bounds reads them, so the reset goes first and the bounds come out of the reorder pass
     if (clear) {
         const unsigned bx = (clear + 4u * kThreads - 1u) / (4u * kThreads);
         hipLaunchKernelGGL(k_b_clear, dim3(bx, S_), b, 0, s, cs_all, ce_all, clear);
     }
-    hipLaunchKernelGGL(k_b_find_bounds, g, b, 0, s, cell, off, N_, cs_all, ce_all);
+    if (!sort.sort_bits(cell, index, N_, 0, 18 + (int)scene_bits_, s, true)) return 0;
+    hipLaunchKernelGGL(k_b_reorder_bounds, g, b, 0, s, index, pos, vel, spos, svel, N_, inv_.as<uint32_t>(), cell, off,
+                       cs_all, ce_all);
     const bool tiles = (neighbour_pass == MFL_PASS_TILE_LIST || neighbour_pass == MFL_PASS_TILE_KEPT) && porder_valid_;
     const bool kept = tiles && neighbour_pass == MFL_PASS_TILE_KEPT;
     if (kept && !(kept_.ptr && hitmask_.ptr && meta_.ptr)) {
