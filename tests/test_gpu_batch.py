@@ -130,6 +130,31 @@ def test_batch_per_scene_params_and_boxes():
             np.testing.assert_array_equal(st[k][lo:lo + sc.n], ref[k])
 
 
+@pytest.mark.parametrize("sides", [[(8, 8, 4)], [(8, 8, 4), (16, 8, 4)], [(16, 16, 16)] * 3])
+def test_batch_few_scenes(sides):
+    """One, two and three scenes (256 to 4,096 particles; one scene has no scene bits in its keys):
+    each equals its single-scene run bit for bit (default launch shape)."""
+    import torch
+    import libcl_amd as L
+    scs = [scenes.dam_break(nx, ny, nz, 1.0, seed=40 + k, steps=4) for k, (nx, ny, nz) in enumerate(sides)]
+    prm = L.params_from(scs[0].params)
+    b = L.FluidBatch([sc.n for sc in scs])
+    assert b.setParameters(prm) == 1 and b.setBox((0, 0, 0, 0), (1, 1, 1, 0)) == 1
+    assert b.upload(np.concatenate([sc.pos for sc in scs]), np.concatenate([sc.vel for sc in scs]),
+                    np.concatenate([sc.force for sc in scs])) == 1
+    stream = torch.cuda.Stream()
+    for _ in range(4):
+        assert b.compute(stream) == 1, L.last_error()
+    stream.synchronize()
+    st = b.state()
+    for s, sc in enumerate(scs):
+        print(f"scene {s} of {len(scs)} n={sc.n}", flush=True)
+        ref = _plain(sc, prm, 4, BATCH_SHAPES["default"][1]).state()
+        lo = int(b.offsets[s])
+        for k in ("pos", "vel", "force"):
+            np.testing.assert_array_equal(st[k][lo:lo + sc.n], ref[k])
+
+
 def test_batch_rejects_bad_counts():
     import libcl_amd as L
     with pytest.raises(RuntimeError, match="multiple of 256"):
