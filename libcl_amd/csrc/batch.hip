@@ -12,7 +12,6 @@
 #include "tile.hpp"
 
 #include <cstring>
-#include <sstream>
 
 namespace mfl {
 namespace {
@@ -773,11 +772,18 @@ void FluidBatch::drop_graph()
 
 int FluidBatch::launch_graph(hipStream_t s)
 {
-    std::ostringstream sig;
-    sig << clear_count << ':' << sort.tuning.items << ':' << gravity[0] << ',' << gravity[1] << ',' << gravity[2] << ','
-        << gravity[3] << ':' << neighbour_pass << ':' << density_lanes << ':' << tile_depth << ':' << block_waves << ':'
-        << force_chunks << ':' << porder_valid_;
-    if (!graph_exec_ || sig.str() != graph_sig_) {
+    std::string sig;
+    sig.reserve(64);
+    sig_add(sig, clear_count);
+    sig_add(sig, sort.tuning.items);
+    sig_add(sig, gravity);
+    sig_add(sig, neighbour_pass);
+    sig_add(sig, density_lanes);
+    sig_add(sig, tile_depth);
+    sig_add(sig, block_waves);
+    sig_add(sig, force_chunks);
+    sig_add(sig, porder_valid_);
+    if (!graph_exec_ || sig != graph_sig_) {
         drop_graph();
         if (!sort.reserve(N_, 18 + (int)scene_bits_)) return 0;
         MFL_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeRelaxed));
@@ -791,7 +797,7 @@ int FluidBatch::launch_graph(hipStream_t s)
         }
         graph_ = gr;
         MFL_HIP(hipGraphInstantiate(&graph_exec_, graph_, nullptr, nullptr, 0));
-        graph_sig_ = sig.str();
+        graph_sig_ = sig;
     }
     MFL_HIP(hipGraphLaunch(graph_exec_, s));
     return 1;

@@ -23,7 +23,6 @@
 #include "tile.hpp"
 
 #include <cstring>
-#include <sstream>
 
 namespace mfl {
 namespace {
@@ -1528,18 +1527,26 @@ void Fluid3D::drop_graph()
 
 int Fluid3D::launch_graph(hipStream_t s)
 {
-    std::ostringstream sig;
-    sig << n_ << ':' << pos_.ptr << ':' << vel_.ptr << ':' << force_.ptr << ':' << clear_count << ':'
-        << sort.tuning.items << ':' << traversal << ':' << porder_valid_ << ':' << density_lanes << ':'
-        << neighbour_pass << ':' << tile_depth << ':' << block_waves << ':' << force_chunks << ':';
-    const unsigned char* raw[] = {reinterpret_cast<const unsigned char*>(&params),
-                                  reinterpret_cast<const unsigned char*>(box_min),
-                                  reinterpret_cast<const unsigned char*>(box_max),
-                                  reinterpret_cast<const unsigned char*>(gravity)};
-    const size_t len[] = {sizeof(params), sizeof(box_min), sizeof(box_max), sizeof(gravity)};
-    for (int k = 0; k < 4; k++)
-        for (size_t i = 0; i < len[k]; i++) sig << std::hex << (int)raw[k][i] << ',';
-    if (!graph_exec_ || sig.str() != graph_sig_) {
+    std::string sig;
+    sig.reserve(192);
+    sig_add(sig, n_);
+    sig_add(sig, pos_.ptr);
+    sig_add(sig, vel_.ptr);
+    sig_add(sig, force_.ptr);
+    sig_add(sig, clear_count);
+    sig_add(sig, sort.tuning.items);
+    sig_add(sig, traversal);
+    sig_add(sig, porder_valid_);
+    sig_add(sig, density_lanes);
+    sig_add(sig, neighbour_pass);
+    sig_add(sig, tile_depth);
+    sig_add(sig, block_waves);
+    sig_add(sig, force_chunks);
+    sig_add(sig, params);
+    sig_add(sig, box_min);
+    sig_add(sig, box_max);
+    sig_add(sig, gravity);
+    if (!graph_exec_ || sig != graph_sig_) {
         drop_graph();
         if (!sort.reserve(n_, 18)) return 0;
         MFL_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeRelaxed));
@@ -1553,7 +1560,7 @@ int Fluid3D::launch_graph(hipStream_t s)
         }
         graph_ = g;
         MFL_HIP(hipGraphInstantiate(&graph_exec_, graph_, nullptr, nullptr, 0));
-        graph_sig_ = sig.str();
+        graph_sig_ = sig;
     }
     MFL_HIP(hipGraphLaunch(graph_exec_, s));
     return 1;

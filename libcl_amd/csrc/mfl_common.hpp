@@ -43,6 +43,14 @@ struct DeviceBuffer {
 // ---- cell hash (libCL/phys/oclFluid3D.cl:34-52) ------------------------------------------------
 // (int)floor(p / h) with the IEEE divide (HIP's default fp32 divide is correctly rounded), and the
 // uint-wrapping XOR of three prime products reduced mod 2^18.
+// Appends the bytes of v to a launch signature (the state a captured graph depends on): comparing
+// raw bytes costs far less per step than formatting the values.
+template <class T>
+inline void sig_add(std::string& sig, const T& v)
+{
+    sig.append(reinterpret_cast<const char*>(&v), sizeof(T));
+}
+
 __host__ __device__ inline int grid_coord(float p, float h) { return (int)floorf(p / h); }
 
 __host__ __device__ inline uint32_t cell_hash(int gx, int gy, int gz)

@@ -221,6 +221,38 @@ def test_multistep_tables_and_graph(oracle):
         _oracle_compare(fa, fo, check_fields=(step % 5 == 0))
 
 
+def test_graph_recaptures_on_changes():
+    """hipGraph replay against eager launches, step for step, while Params, box, gravity, bucket clear,
+    launch shape change between steps: every change must reach the replayed
+    graph (bit-identical state after every step)."""
+    import torch
+    import libcl_amd as L
+    sc = scenes.dam_break(16, 16, 16, 1.0, seed=33, steps=12)
+    prm = L.params_from(sc.params)
+    fa = make_fluid(sc.pos, sc.vel, sc.force, prm, sc.box_min, sc.box_max)
+    fb = make_fluid(sc.pos, sc.vel, sc.force, prm, sc.box_min, sc.box_max)
+    assert fb.setUseGraph(True) == 1
+    stream = torch.cuda.Stream()
+    prm2 = L.params_from(dict(sc.params, viscosity=9.0, stiffness=2.0))
+    changes = {
+        2: lambda f: f.setParameters(prm2),
+        4: lambda f: f.setBox((0.0, 0.0, 0.0, 0.0), (0.7, 1.0, 1.0, 0.0)),
+        6: lambda f: f.setGravity((0.0, 0.5, 1.0, 0.0)),
+        8: lambda f: f.setBucketClear(262144),
+        10: lambda f: f.setTuning(neighbour_pass="tile_kept", density_lanes=8, tile_depth=256),
+    }
+    for step in range(12):
+        if step in changes:
+            for f in (fa, fb):
+                assert changes[step](f) in (None, 1), L.last_error()
+        assert fa.compute(stream) == 1, L.last_error()
+        assert fb.compute(stream) == 1, L.last_error()
+        stream.synchronize()
+        sa, sb = fa.state(), fb.state()
+        for k in ("pos", "vel", "force"):
+            np.testing.assert_array_equal(sa[k], sb[k], err_msg=f"step {step} {k}")
+
+
 def test_integrate_callback_path(oracle):
     """libCL/TEST/main.cpp:246-343: an EVT_INTEGRATE handler enqueues clIntegrateForce, clGravity,
     clIntegrateVelocity itself (no clip box)."""
