@@ -48,6 +48,41 @@ __global__ __launch_bounds__(kThreads) void k_hash(const float4* __restrict__ po
     if (WRITE_INDEX) index[i] = i;
 }
 
+// k_hash<false, true> that also fills the sort's first-pass count table (bits [0, bits) of the keys; the
+// layout of RadixSort::first_pass_plan): block b hashes the keys of sort tile xcd_tile(), ITEMS per
+// thread at tile offsets k * 256 + thread, and histograms them in LDS. The grid is one block per tile.
+template <int ITEMS>
+__global__ __launch_bounds__(kThreads) void k_hash_counts(const float4* __restrict__ pos, uint32_t n, float h,
+                                                          uint32_t* __restrict__ cell, mfl_params* __restrict__ params_buf,
+                                                          mfl_params p, uint32_t* __restrict__ cell_start,
+                                                          uint32_t* __restrict__ cell_end, uint32_t clear,
+                                                          uint32_t* __restrict__ counts, uint32_t ntiles, uint32_t bits)
+{
+    __shared__ uint32_t hist[512];
+    const uint32_t radix = 1u << bits, mask = radix - 1u;
+    for (uint32_t d = threadIdx.x; d < radix; d += kThreads) hist[d] = 0;
+    if (blockIdx.x == 0 && threadIdx.x == 0 && params_buf) *params_buf = p;  // getParamBuffer() in step
+    for (uint32_t c = blockIdx.x * kThreads + threadIdx.x; c < clear; c += gridDim.x * kThreads) {
+        cell_start[c] = kEmpty;
+        cell_end[c] = kEmpty;
+    }
+    __syncthreads();
+    const uint32_t tile = xcd_tile();
+    const size_t base = (size_t)tile * (kThreads * ITEMS);
+#pragma unroll
+    for (int k = 0; k < ITEMS; k++) {
+        const size_t i = base + (size_t)k * kThreads + threadIdx.x;
+        if (i < n) {
+            const float4 q = pos[i];
+            const uint32_t key = cell_hash(grid_coord(q.x, h), grid_coord(q.y, h), grid_coord(q.z, h));
+            cell[i] = key;
+            atomicAdd(&hist[key & mask], 1u);
+        }
+    }
+    __syncthreads();
+    for (uint32_t d = threadIdx.x; d < radix; d += kThreads) counts[(size_t)d * ntiles + tile] = hist[d];
+}
+
 // Sorted gather (cl:66-72) over [0, n) and the per-step bucket reset (cpp:309-313) over [0, clear).
 __global__ __launch_bounds__(kThreads) void k_reorder_clear(const uint32_t* __restrict__ index,
                                                             const float4* __restrict__ pos,
@@ -72,7 +107,7 @@ __global__ __launch_bounds__(kThreads) void k_reorder_clear(const uint32_t* __re
     }
 }
 
-// k_reorder_clear without the reset (done by k_hash<..., true>) and k_find_bounds in one pass.
+// k_reorder_clear without the reset (done by the hash kernel) and k_find_bounds in one pass.
 __global__ __launch_bounds__(kThreads) void k_reorder_bounds(const uint32_t* __restrict__ index,
                                                              const float4* __restrict__ pos,
                                                              const float4* __restrict__ vel,
@@ -1117,14 +1152,24 @@ int Fluid3D::enqueue_front(hipStream_t s)
     auto* ce = cell_end_.as<uint32_t>();
     const uint32_t clear = clear_count < kBuckets ? clear_count : kBuckets;
 
+    // the hash kernel also counts the sort's first digit per tile, which saves that pass's count launch
+    SortCountPlan plan;
+    if (!sort.first_pass_plan(n, 0, 18, &plan)) return 0;
     timer.begin(MFL_STAGE_HASH, s);
-    hipLaunchKernelGGL((k_hash<false, true>), grid_for(n > clear ? n : clear), b, 0, s, pos, n, params.cellSize, cell,
-                       nullptr, params_buf_.as<mfl_params>(), params, cs, ce, clear);
+#define MFL_HASH(IT)                                                                                              \
+    hipLaunchKernelGGL(k_hash_counts<IT>, dim3(plan.ntiles), b, 0, s, pos, n, params.cellSize, cell,              \
+                       params_buf_.as<mfl_params>(), params, cs, ce, clear, plan.counts, plan.ntiles, plan.bits)
+    switch (plan.items) {
+    case 16: MFL_HASH(16); break;
+    case 8: MFL_HASH(8); break;
+    default: MFL_HASH(4); break;
+    }
+#undef MFL_HASH
     timer.end(MFL_STAGE_HASH, s);
     MFL_HIP(hipGetLastError());
     // the index buffer starts as the identity (cl:63) and is generated inside the first pass
     timer.begin(MFL_STAGE_SORT, s);
-    if (!sort.sort_bits(cell, index, n, 0, 18, s, true)) return 0;
+    if (!sort.sort_bits(cell, index, n, 0, 18, s, true, true)) return 0;
     timer.end(MFL_STAGE_SORT, s);
     // bucket bounds come out of the reorder pass (the BOUNDS stage stays empty)
     timer.begin(MFL_STAGE_REORDER, s);

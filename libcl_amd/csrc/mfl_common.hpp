@@ -43,6 +43,16 @@ struct DeviceBuffer {
 // ---- cell hash (libCL/phys/oclFluid3D.cl:34-52) ------------------------------------------------
 // (int)floor(p / h) with the IEEE divide (HIP's default fp32 divide is correctly rounded), and the
 // uint-wrapping XOR of three prime products reduced mod 2^18.
+// Tile of this block when consecutive tiles should share an L2: the hardware deals a grid's blocks to
+// the 8 XCDs round-robin (block b runs on XCD b % 8), and the tiles are dealt so that every XCD takes
+// one contiguous range of them.
+__device__ inline uint32_t xcd_tile()
+{
+    const uint32_t nb = gridDim.x, b = blockIdx.x;
+    const uint32_t q = nb / kXcds, r = nb % kXcds, x = b % kXcds;
+    return x * q + min(x, r) + b / kXcds;
+}
+
 // Appends the bytes of v to a launch signature (the state a captured graph depends on): comparing
 // raw bytes costs far less per step than formatting the values.
 template <class T>

@@ -44,17 +44,9 @@ __device__ inline uint32_t block_exclusive_scan(uint32_t v, uint32_t* s_wave, ui
     return before + x - v;
 }
 
-// Tile of this block. The hardware deals a grid's blocks to the 8 XCDs round-robin (block b runs on
-// XCD b % 8); the tiles are dealt so that every XCD takes one contiguous range of them. Neighbouring
-// tiles write neighbouring words of the count table and adjoining pieces of each digit's run, so
-// their partial cache lines then meet in one L2 instead of being written back from eight.
-__device__ inline uint32_t xcd_tile()
-{
-    const uint32_t nb = gridDim.x, b = blockIdx.x;
-    const uint32_t q = nb / kXcds, r = nb % kXcds, x = b % kXcds;
-    return x * q + min(x, r) + b / kXcds;
-}
-
+// Tiles are dealt to the XCDs in contiguous ranges (xcd_tile): neighbouring tiles write neighbouring
+// words of the count table and adjoining pieces of each digit's run, so their partial cache lines meet
+// in one L2 instead of being written back from eight.
 template <int ITEMS>
 __global__ __launch_bounds__(kThreads) void k_tile_counts(const uint32_t* __restrict__ keys, uint32_t n, int shift,
                                                           int bits, uint32_t* __restrict__ counts, uint32_t ntiles)
@@ -264,12 +256,14 @@ __global__ __launch_bounds__(kThreads) void k_scatter(const uint32_t* __restrict
 
 template <int ITEMS>
 bool run_pass(const uint32_t* kin, const uint32_t* vin, uint32_t* kout, uint32_t* vout, uint32_t n, int shift,
-              int bits, uint32_t* counts, uint32_t* totals, bool identity, hipStream_t s)
+              int bits, uint32_t* counts, uint32_t* totals, bool identity, hipStream_t s, bool counted)
 {
     constexpr uint32_t TILE = kThreads * ITEMS;
     const uint32_t ntiles = (uint32_t)(((size_t)n + TILE - 1) / TILE);
     const uint32_t radix = 1u << bits;
-    hipLaunchKernelGGL(k_tile_counts<ITEMS>, dim3(ntiles), dim3(kThreads), 0, s, kin, n, shift, bits, counts, ntiles);
+    if (!counted)  // else the caller filled the count table while producing the keys
+        hipLaunchKernelGGL(k_tile_counts<ITEMS>, dim3(ntiles), dim3(kThreads), 0, s, kin, n, shift, bits, counts,
+                           ntiles);
     hipLaunchKernelGGL(k_scan_digits, dim3(radix), dim3(kThreads), 0, s, counts, ntiles, totals);
 #define MFL_SCATTER(ID, B)                                                                                      \
     hipLaunchKernelGGL((k_scatter<ITEMS, ID, B>), dim3(ntiles), dim3(kThreads), 0, s, kin, vin, kout, vout, n, shift, \
@@ -348,8 +342,25 @@ int RadixSort::reserve(size_t n, int nbits)
     return 1;
 }
 
+int RadixSort::first_pass_plan(size_t n, int start_bit, int end_bit, SortCountPlan* plan)
+{
+    const int nbits = end_bit - start_bit;
+    if (n == 0 || n > 0xFFFFFFFFull || start_bit < 0 || end_bit > 32 || nbits <= 0) {
+        set_error("radix sort: no count plan for this sort");
+        return 0;
+    }
+    if (!reserve(n, nbits)) return 0;
+    const int passes = (nbits + kMaxDigitBits - 1) / kMaxDigitBits;
+    plan->items = pick_items(n, tuning.items);
+    plan->tile_keys = (uint32_t)(kThreads * plan->items);
+    plan->ntiles = (uint32_t)((n + plan->tile_keys - 1) / plan->tile_keys);
+    plan->bits = (uint32_t)(nbits / passes + (nbits % passes ? 1 : 0));
+    plan->counts = counts_.as<uint32_t>();
+    return 1;
+}
+
 int RadixSort::sort_bits(uint32_t* keys, uint32_t* vals, size_t n, int start_bit, int end_bit, hipStream_t stream,
-                         bool identity_vals)
+                         bool identity_vals, bool first_counted)
 {
     if (start_bit < 0 || end_bit > 32) {
         set_error("bit range must lie in [0, 32]");
@@ -379,14 +390,15 @@ int RadixSort::sort_bits(uint32_t* keys, uint32_t* vals, size_t n, int start_bit
         uint32_t* kout = last ? keys : tmp_keys_[p & 1].as<uint32_t>();
         uint32_t* vout = last ? vals : tmp_vals_[p & 1].as<uint32_t>();
         const bool ident = identity_vals && p == 0;
+        const bool counted = first_counted && p == 0;
         bool ok;
         switch (items) {
         case 16: ok = run_pass<16>(kin, vin, kout, vout, (uint32_t)n, shift, bits, counts_.as<uint32_t>(),
-                                   totals_.as<uint32_t>(), ident, stream); break;
+                                   totals_.as<uint32_t>(), ident, stream, counted); break;
         case 8: ok = run_pass<8>(kin, vin, kout, vout, (uint32_t)n, shift, bits, counts_.as<uint32_t>(),
-                                 totals_.as<uint32_t>(), ident, stream); break;
+                                 totals_.as<uint32_t>(), ident, stream, counted); break;
         default: ok = run_pass<4>(kin, vin, kout, vout, (uint32_t)n, shift, bits, counts_.as<uint32_t>(),
-                                  totals_.as<uint32_t>(), ident, stream); break;
+                                  totals_.as<uint32_t>(), ident, stream, counted); break;
         }
         if (!ok) return 0;
         kin = kout;

@@ -10,6 +10,12 @@ struct SortTuning {
     int items = 0;        // 0 = pick by size
 };
 
+struct SortCountPlan {
+    uint32_t* counts;
+    uint32_t ntiles, tile_keys, bits;
+    int items;  // keys per thread of a tile of 256 threads
+};
+
 class RadixSort {
 public:
     RadixSort() = default;
@@ -28,8 +34,15 @@ public:
 
     // Same sort for any bit range in [0, 32] (no multiple-of-4 rule); used by the fluid step, whose
     // keys are below 2^18 so that bits [0, 18) order them exactly as the reference's [0, 24).
+    // `first_counted`: the caller already filled the first pass's count table (first_pass_plan).
     int sort_bits(uint32_t* keys, uint32_t* vals, size_t n, int start_bit, int end_bit, hipStream_t stream,
-                  bool identity_vals);
+                  bool identity_vals, bool first_counted = false);
+
+    // The first pass's count table, for a caller that fills it while producing the keys (the fluid
+    // step's hash kernel): tile t holds keys [t tile_keys, (t + 1) tile_keys) and counts[d ntiles + t]
+    // must be the number of its keys whose bits [start_bit, start_bit + bits) equal d. Reserves the
+    // sort's temporaries.
+    int first_pass_plan(size_t n, int start_bit, int end_bit, SortCountPlan* plan);
 
     // Allocates every temporary a sort of n keys over `nbits` bits needs (so that a later sort
     // enqueues no allocation, e.g. under hipGraph capture).
