@@ -18,16 +18,36 @@ namespace {
 
 using namespace sph;
 
-__global__ __launch_bounds__(kThreads) void k_b_hash(const float4* __restrict__ pos, const uint32_t* __restrict__ pscene,
-                                                     const mfl_params* __restrict__ params, uint32_t N,
-                                                     uint32_t* __restrict__ cell)
+// Keys s << 18 | hash per particle, and the sort's first-pass digit counts (RadixSort::first_pass_plan):
+// block b hashes the keys of sort tile xcd_tile(), ITEMS per thread at tile offsets k * 256 + thread,
+// and histograms their low `bits` bits in LDS. The grid is one block per tile.
+template <int ITEMS>
+__global__ __launch_bounds__(kThreads) void k_b_hash_counts(const float4* __restrict__ pos,
+                                                            const uint32_t* __restrict__ pscene,
+                                                            const mfl_params* __restrict__ params, uint32_t N,
+                                                            uint32_t* __restrict__ cell, uint32_t* __restrict__ counts,
+                                                            uint32_t ntiles, uint32_t bits)
 {
-    const uint32_t g = blockIdx.x * kThreads + threadIdx.x;
-    if (g >= N) return;
-    const uint32_t s = pscene[g];
-    const float h = params[s].cellSize;
-    const float4 q = pos[g];
-    cell[g] = (s << 18) | cell_hash(grid_coord(q.x, h), grid_coord(q.y, h), grid_coord(q.z, h));
+    __shared__ uint32_t hist[512];
+    const uint32_t radix = 1u << bits, mask = radix - 1u;
+    for (uint32_t d = threadIdx.x; d < radix; d += kThreads) hist[d] = 0;
+    __syncthreads();
+    const uint32_t tile = xcd_tile();
+    const size_t base = (size_t)tile * (kThreads * ITEMS);
+#pragma unroll
+    for (int k = 0; k < ITEMS; k++) {
+        const size_t g = base + (size_t)k * kThreads + threadIdx.x;
+        if (g < N) {
+            const uint32_t s = pscene[g];
+            const float h = params[s].cellSize;
+            const float4 q = pos[g];
+            const uint32_t key = (s << 18) | cell_hash(grid_coord(q.x, h), grid_coord(q.y, h), grid_coord(q.z, h));
+            cell[g] = key;
+            atomicAdd(&hist[key & mask], 1u);
+        }
+    }
+    __syncthreads();
+    for (uint32_t d = threadIdx.x; d < radix; d += kThreads) counts[(size_t)d * ntiles + tile] = hist[d];
 }
 
 // Sorted gather (cl:66-72) and clFindBounds (cl:85-113) per scene in scene-local indices, in one pass
@@ -696,7 +716,18 @@ int FluidBatch::enqueue_step(hipStream_t s)
     auto* params = params_.as<mfl_params>();
     const uint32_t clear = clear_count < kBuckets ? clear_count : kBuckets;
 
-    hipLaunchKernelGGL(k_b_hash, g, b, 0, s, pos, pscene_.as<uint32_t>(), params, N_, cell);
+    // the hash kernel also counts the sort's first digit per tile (one count launch fewer)
+    SortCountPlan plan;
+    if (!sort.first_pass_plan(N_, 0, 18 + (int)scene_bits_, &plan)) return 0;
+#define MFL_B_HASH(IT)                                                                                            \
+    hipLaunchKernelGGL(k_b_hash_counts<IT>, dim3(plan.ntiles), b, 0, s, pos, pscene_.as<uint32_t>(), params, N_, cell, \
+                       plan.counts, plan.ntiles, plan.bits)
+    switch (plan.items) {
+    case 16: MFL_B_HASH(16); break;
+    case 8: MFL_B_HASH(8); break;
+    default: MFL_B_HASH(4); break;
+    }
+#undef MFL_B_HASH
     MFL_HIP(hipGetLastError());
     // the reference resets the tables between the reorder and the bounds (cpp:309-313); nothing before
     // the bounds reads them, so the reset goes first and the bounds come out of the reorder pass
@@ -704,7 +735,7 @@ int FluidBatch::enqueue_step(hipStream_t s)
         const unsigned bx = (clear + 4u * kThreads - 1u) / (4u * kThreads);
         hipLaunchKernelGGL(k_b_clear, dim3(bx, S_), b, 0, s, cs_all, ce_all, clear);
     }
-    if (!sort.sort_bits(cell, index, N_, 0, 18 + (int)scene_bits_, s, true)) return 0;
+    if (!sort.sort_bits(cell, index, N_, 0, 18 + (int)scene_bits_, s, true, true)) return 0;
     hipLaunchKernelGGL(k_b_reorder_bounds, g, b, 0, s, index, pos, vel, spos, svel, N_, inv_.as<uint32_t>(), cell, off,
                        cs_all, ce_all);
     const bool tiles = (neighbour_pass == MFL_PASS_TILE_LIST || neighbour_pass == MFL_PASS_TILE_KEPT) && porder_valid_;
