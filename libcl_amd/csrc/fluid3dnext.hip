@@ -60,22 +60,40 @@ __device__ inline Window window_of(float x, float y, const mfl_next_params& p)
     return Window{max(gx - 1, 0), min(gx + 1, p.cellCountX - 1), max(gy - 1, 0), min(gy + 1, p.cellCountY - 1)};
 }
 
-// clHash (cl:36-44) + clInitBounds (cl:53-58) over all cells
-__global__ __launch_bounds__(kBlock) void k_next_hash(const float4* __restrict__ state, uint32_t n, mfl_next_params p,
-                                                     uint32_t* __restrict__ cell, uint32_t* __restrict__ index,
-                                                     uint32_t* __restrict__ start, uint32_t* __restrict__ end,
-                                                     uint32_t ncells)
+// clHash (cl:36-44) + clInitBounds (cl:53-58) over all cells, and the sort's first-pass digit counts
+// (RadixSort::first_pass_plan): block b hashes the keys of sort tile xcd_tile(), ITEMS per thread at
+// tile offsets k * 256 + thread, and histograms their low `bits` bits in LDS. The sort generates the
+// identity index itself (cl:43). The grid is one block per tile.
+template <int ITEMS>
+__global__ __launch_bounds__(kBlock) void k_next_hash_counts(const float4* __restrict__ state, uint32_t n,
+                                                             mfl_next_params p, uint32_t* __restrict__ cell,
+                                                             uint32_t* __restrict__ start, uint32_t* __restrict__ end,
+                                                             uint32_t ncells, uint32_t* __restrict__ counts,
+                                                             uint32_t ntiles, uint32_t bits)
 {
-    const uint32_t g = blockIdx.x * kBlock + threadIdx.x;
-    for (uint32_t c = g; c < ncells; c += gridDim.x * kBlock) {
+    __shared__ uint32_t hist[512];
+    const uint32_t radix = 1u << bits, mask = radix - 1u;
+    for (uint32_t d = threadIdx.x; d < radix; d += kBlock) hist[d] = 0;
+    for (uint32_t c = blockIdx.x * kBlock + threadIdx.x; c < ncells; c += gridDim.x * kBlock) {
         start[c] = kNoCell;
         end[c] = kNoCell;
     }
-    if (g >= n) return;
-    const float4 s = state[g];
-    const int gx = (int)floorf(s.x / p.cellSize), gy = (int)floorf(s.y / p.cellSize);
-    cell[g] = (uint32_t)gy * (uint32_t)p.cellCountX + (uint32_t)gx;
-    index[g] = g;
+    __syncthreads();
+    const uint32_t tile = xcd_tile();
+    const size_t base = (size_t)tile * (kBlock * ITEMS);
+#pragma unroll
+    for (int k = 0; k < ITEMS; k++) {
+        const size_t g = base + (size_t)k * kBlock + threadIdx.x;
+        if (g < n) {
+            const float4 st = state[g];
+            const int gx = (int)floorf(st.x / p.cellSize), gy = (int)floorf(st.y / p.cellSize);
+            const uint32_t key = (uint32_t)gy * (uint32_t)p.cellCountX + (uint32_t)gx;
+            cell[g] = key;
+            atomicAdd(&hist[key & mask], 1u);
+        }
+    }
+    __syncthreads();
+    for (uint32_t d = threadIdx.x; d < radix; d += kBlock) counts[(size_t)d * ntiles + tile] = hist[d];
 }
 
 // clReorder (cl:46-51) + clFindBounds (cl:60-92)
@@ -337,10 +355,20 @@ int Fluid3DNext::compute(hipStream_t s)
         MFL_HIP(hipMemcpyAsync(params_buf_.ptr, &uploaded_, sizeof(params), hipMemcpyHostToDevice, s));
         params_synced_ = true;
     }
-    hipLaunchKernelGGL(k_next_hash, dim3(blocks_for(n)), dim3(kBlock), 0, s, state, n, params, cell, index, start, end,
-                       ncells_);
+    // the hash kernel also counts the sort's first digit per tile (one count launch fewer)
+    SortCountPlan plan;
+    if (!sort_.first_pass_plan(n, 0, 24, &plan)) return 0;
+#define MFL_NEXT_HASH(IT)                                                                                         \
+    hipLaunchKernelGGL(k_next_hash_counts<IT>, dim3(plan.ntiles), dim3(kBlock), 0, s, state, n, params, cell, start, \
+                       end, ncells_, plan.counts, plan.ntiles, plan.bits)
+    switch (plan.items) {
+    case 16: MFL_NEXT_HASH(16); break;
+    case 8: MFL_NEXT_HASH(8); break;
+    default: MFL_NEXT_HASH(4); break;
+    }
+#undef MFL_NEXT_HASH
     MFL_HIP(hipGetLastError());
-    if (!sort_.compute(cell, index, n, 0, 24, s, true)) return 0;  // cpp:272
+    if (!sort_.sort_bits(cell, index, n, 0, 24, s, true, true)) return 0;  // mRadixSort.compute, cpp:272
     hipLaunchKernelGGL(k_next_reorder_bounds, dim3(blocks_for(n)), dim3(kBlock), 0, s, index, state, sorted, cell, n,
                        start, end, ncells_);
     if (reciprocal_exact(params.cellSize))
