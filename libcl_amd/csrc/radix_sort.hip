@@ -147,7 +147,7 @@ __global__ __launch_bounds__(kThreads) void k_scatter(const uint32_t* __restrict
     }
 
     uint32_t key[ITEMS], val[ITEMS], rank[ITEMS];
-    const uint32_t first = (uint32_t)(wave * ITEMS) * kWave + (uint32_t)lane;  // tile offset of round 0
+    const uint32_t first = (uint32_t)(wave * ITEMS) * kWave + (uint32_t)lane;  // tile offset of item 0
     if (tile_base + TILE <= n) {  // a whole tile: no bound tests
         const uint32_t* kt = kin + tile_base + first;
         const uint32_t* vt = vin + tile_base + first;
