@@ -27,9 +27,9 @@ HBM_PEAK_GBS = 8000.0  # MI355X HBM3E, vendor figure
 STAGE_BYTES_PER_PARTICLE = {
     "hash": 16 + 4,                     # R pos, W key
     "sort": 16 + 20,                    # 2 passes of 9 bits: counts R 4, scatter R 4/8 W 8 (index generated)
-    "reorder": 4 + 32 + 32 + 4 + 4 + 1, # R index, gather pos/vel, W sorted pos/vel, W inverse, R key,
+    "reorder": 4 + 32 + 32 + 4 + 4 + 1, # R index, R pos/vel, W sorted pos/vel, W inverse, R key,
                                         # W bucket start/end (one occupied bucket per ~8 particles)
-    "bounds": 0,                        # (done by the reorder kernel)
+    "bounds": 0,                        # (done in the reorder stage)
     "density": 16 + 8,                  # R sorted pos, W density/pressure lanes
     "forces": 16 + 16 + 4 + 48 + 48,    # R sorted pos/vel, index; R+W force/vel/pos in caller order
 }

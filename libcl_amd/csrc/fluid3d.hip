@@ -6,11 +6,12 @@
 // tables (cell start / end) that keep their contents between steps.
 //
 // Step (oclFluid3D::compute, cpp:293-342):
-//   k_hash (CLEAR)  cell key per particle + reset of the first clear_count buckets
-//                   (cl:57-64, cpp:309-313; nothing between the two reads the tables, so the reset
-//                    moves to the start of the step)
+//   k_hash_counts   cell key per particle + reset of the first clear_count buckets + the sort's
+//                   first-pass digit counts (cl:57-64, cpp:309-313; nothing between the two reads
+//                   the tables, so the reset moves to the start of the step)
 //   sort            stable key/index sort over bits [0, 18)                 (cpp:299; keys < 2^18)
-//   k_reorder_bounds gather sorted pos/vel + start/end of each occupied bucket (cl:66-72, 85-113)
+//   k_slots_bounds  inverse permutation + start/end of each occupied bucket (cl:85-113)
+//   k_scatter_sorted sorted pos/vel copy, written from the caller's order   (cl:66-72)
 //   density pass    27-bucket neighbourhood, poly6, pressure                (cl:150-200)
 //   force pass      27-bucket pressure + viscosity forces, clip box, clamp, Euler, gravity
 //                   (cl:202-255, 315-361, 258-271, 294-298, 284-292 fused; each particle's update
@@ -107,23 +108,17 @@ __global__ __launch_bounds__(kThreads) void k_reorder_clear(const uint32_t* __re
     }
 }
 
-// k_reorder_clear without the reset (done by the hash kernel) and k_find_bounds in one pass.
-__global__ __launch_bounds__(kThreads) void k_reorder_bounds(const uint32_t* __restrict__ index,
-                                                             const float4* __restrict__ pos,
-                                                             const float4* __restrict__ vel,
-                                                             float4* __restrict__ spos, float4* __restrict__ svel,
-                                                             uint32_t n, const uint32_t* __restrict__ cell,
-                                                             uint32_t* __restrict__ cell_start,
-                                                             uint32_t* __restrict__ cell_end, uint32_t* __restrict__ inv)
+// Sorted slot of every particle (inv[index[i]] = i) and the start / end of each occupied bucket
+// (cl:85-113), both read in sorted order.
+__global__ __launch_bounds__(kThreads) void k_slots_bounds(const uint32_t* __restrict__ index, uint32_t n,
+                                                           const uint32_t* __restrict__ cell,
+                                                           uint32_t* __restrict__ cell_start,
+                                                           uint32_t* __restrict__ cell_end, uint32_t* __restrict__ inv)
 {
     const uint32_t i = blockIdx.x * kThreads + threadIdx.x;
     if (i >= n) return;
     const uint32_t s = index[i];
-    if (s < n) {  // index is a permutation of [0, n)
-        spos[i] = pos[s];
-        svel[i] = vel[s];
-        inv[s] = i;
-    }
+    if (s < n) inv[s] = i;  // index is a permutation of [0, n)
     const uint32_t h = cell[i] & (kBuckets - 1u);  // keys are < 2^18 by construction
     if (i == 0) {
         cell_start[h] = 0;
@@ -241,7 +236,7 @@ __global__ __launch_bounds__(kThreads) void k_forces(const float4* __restrict__ 
 // computed by one thread in a fixed visit order); it only decides which particles share a wave, an
 // L1 and an XCD's L2. Hash order scatters neighbouring cells over the whole sorted array, so the
 // passes walk particles in caller order or in a periodically refreshed Morton order of their cells
-// and reach their sorted slot through the inverse permutation written by k_reorder_clear.
+// and reach their sorted slot through the inverse permutation written by the reorder stage.
 enum : int { kOrderSorted = 0, kOrderCaller = 1, kOrderListed = 2 };
 
 template <int ORDER>
@@ -483,7 +478,7 @@ __device__ __noinline__ PairSums force_fallback(const float4* __restrict__ spos,
 }
 
 // Particle of traversal slot t: its sorted slot i (n when t is past the end) and its position. The
-// caller-order position pos[o] equals spos[i] (k_reorder_clear copies it), so it is read in parallel
+// caller-order position pos[o] equals spos[i] (the reorder stage copies it), so it is read in parallel
 // with the inverse permutation instead of after it.
 template <int ORDER>
 __device__ inline float4 slot_particle(uint32_t t, uint32_t n, const uint32_t* __restrict__ porder,
@@ -1173,8 +1168,8 @@ int Fluid3D::enqueue_front(hipStream_t s)
     timer.end(MFL_STAGE_SORT, s);
     // bucket bounds come out of the reorder pass (the BOUNDS stage stays empty)
     timer.begin(MFL_STAGE_REORDER, s);
-    hipLaunchKernelGGL(k_reorder_bounds, g, b, 0, s, index, pos, vel, spos, svel, n, cell, cs, ce,
-                       inv_.as<uint32_t>());
+    hipLaunchKernelGGL(k_slots_bounds, g, b, 0, s, index, n, cell, cs, ce, inv_.as<uint32_t>());
+    hipLaunchKernelGGL(k_scatter_sorted, g, b, 0, s, inv_.as<uint32_t>(), pos, vel, spos, svel, n);
     timer.end(MFL_STAGE_REORDER, s);
     MFL_HIP(hipGetLastError());
     return 1;

@@ -315,6 +315,28 @@ __device__ inline uint32_t spread_bits3(uint32_t v)
     return v;
 }
 
+namespace {
+// The sorted copy of the state (cl:66-72: spos[i] = pos[index[i]]) written from the caller's order
+// through the inverse permutation inv (inv[index[i]] = i, every entry < n when index is a permutation):
+// the hash scatters buckets over the sorted order, so gathering by index would fetch a whole line of
+// pos / vel for each bucket's 32-byte runs; here the reads are coalesced and a bucket's particles
+// land in consecutive slots.
+__global__ __launch_bounds__(kThreads) void k_scatter_sorted(const uint32_t* __restrict__ inv,
+                                                             const float4* __restrict__ pos,
+                                                             const float4* __restrict__ vel,
+                                                             float4* __restrict__ spos, float4* __restrict__ svel,
+                                                             uint32_t n)
+{
+    const uint32_t s = blockIdx.x * kThreads + threadIdx.x;
+    if (s >= n) return;
+    const uint32_t i = inv[s];
+    if (i < n) {
+        spos[i] = pos[s];
+        svel[i] = vel[s];
+    }
+}
+}  // namespace
+
 inline dim3 grid_for(size_t n) { return dim3((unsigned)((n + kThreads - 1) / kThreads)); }
 
 inline float4 to_f4(const float* v) { return make_float4(v[0], v[1], v[2], v[3]); }
