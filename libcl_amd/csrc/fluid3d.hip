@@ -108,17 +108,33 @@ __global__ __launch_bounds__(kThreads) void k_reorder_clear(const uint32_t* __re
     }
 }
 
+// Particle count from which the reorder stage copies the state in the caller's order (two launches):
+// 2^18 particles hold 8 MiB of pos + vel, twice an XCD's L2.
+constexpr uint32_t kScatterCopyMin = 1u << 18;
+
 // Sorted slot of every particle (inv[index[i]] = i) and the start / end of each occupied bucket
-// (cl:85-113), both read in sorted order.
+// (cl:85-113), both read in sorted order. GATHER also copies the sorted state here (cl:66-72) by
+// gathering through index: one launch fewer, for steps whose pos / vel sit in L2 anyway; larger
+// steps copy with k_scatter_sorted instead.
+template <bool GATHER>
 __global__ __launch_bounds__(kThreads) void k_slots_bounds(const uint32_t* __restrict__ index, uint32_t n,
                                                            const uint32_t* __restrict__ cell,
                                                            uint32_t* __restrict__ cell_start,
-                                                           uint32_t* __restrict__ cell_end, uint32_t* __restrict__ inv)
+                                                           uint32_t* __restrict__ cell_end, uint32_t* __restrict__ inv,
+                                                           const float4* __restrict__ pos,
+                                                           const float4* __restrict__ vel,
+                                                           float4* __restrict__ spos, float4* __restrict__ svel)
 {
     const uint32_t i = blockIdx.x * kThreads + threadIdx.x;
     if (i >= n) return;
     const uint32_t s = index[i];
-    if (s < n) inv[s] = i;  // index is a permutation of [0, n)
+    if (s < n) {  // index is a permutation of [0, n)
+        inv[s] = i;
+        if (GATHER) {
+            spos[i] = pos[s];
+            svel[i] = vel[s];
+        }
+    }
     const uint32_t h = cell[i] & (kBuckets - 1u);  // keys are < 2^18 by construction
     if (i == 0) {
         cell_start[h] = 0;
@@ -1168,8 +1184,14 @@ int Fluid3D::enqueue_front(hipStream_t s)
     timer.end(MFL_STAGE_SORT, s);
     // bucket bounds come out of the reorder pass (the BOUNDS stage stays empty)
     timer.begin(MFL_STAGE_REORDER, s);
-    hipLaunchKernelGGL(k_slots_bounds, g, b, 0, s, index, n, cell, cs, ce, inv_.as<uint32_t>());
-    hipLaunchKernelGGL(k_scatter_sorted, g, b, 0, s, inv_.as<uint32_t>(), pos, vel, spos, svel, n);
+    if (n < kScatterCopyMin) {
+        hipLaunchKernelGGL(k_slots_bounds<true>, g, b, 0, s, index, n, cell, cs, ce, inv_.as<uint32_t>(), pos, vel,
+                           spos, svel);
+    } else {
+        hipLaunchKernelGGL(k_slots_bounds<false>, g, b, 0, s, index, n, cell, cs, ce, inv_.as<uint32_t>(), pos, vel,
+                           spos, svel);
+        hipLaunchKernelGGL(k_scatter_sorted, g, b, 0, s, inv_.as<uint32_t>(), pos, vel, spos, svel, n);
+    }
     timer.end(MFL_STAGE_REORDER, s);
     MFL_HIP(hipGetLastError());
     return 1;
