@@ -193,6 +193,30 @@ def test_1m_step_vs_oracle(oracle, shape):
     assert nw <= TOL and per <= TOL, (nw, per)
 
 
+@pytest.mark.parametrize("count", [262144 - 256, 262144])  # counts are multiples of cLocalSize = 256
+def test_reorder_copy_forms_at_their_boundary(oracle, count):
+    """The reorder stage gathers the sorted state inside its bounds pass below 2^18 particles and
+    copies it from the caller's order through the inverse permutation from 2^18 up. On both sides of
+    that boundary: keys, permutation and bucket tables bit-exact, sorted positions and velocities
+    (a pure copy) bit-exact in xyz, density and pressure within 1e-5."""
+    print(f"reorder boundary n={count}", flush=True)
+    import libcl_amd as L
+    sc = scenes.dam_break(64, 64, 64, 2.0, 7, 1)
+    rng = np.random.default_rng(count)
+    pos, force = sc.pos[:count].copy(), sc.force[:count].copy()
+    vel = np.zeros_like(pos)
+    vel[:, :3] = rng.uniform(-0.5, 0.5, (count, 3)).astype(np.float32)
+    prm = L.params_from(sc.params)
+    op = oracle.Params(*prm.values())
+    f = make_fluid(pos, vel, force, prm, sc.box_min, sc.box_max)
+    run_forces_only(f)
+    fo = oracle.Fluid(pos, vel, force, op, box_min=sc.box_min, box_max=sc.box_max)
+    fo.step(integrate=False)
+    it = _oracle_compare(f, fo)
+    np.testing.assert_array_equal(it["spos"][:, :3], fo.spos[:, :3])
+    np.testing.assert_array_equal(it["svel"][:, :3], fo.svel[:, :3])
+
+
 def test_multistep_tables_and_graph(oracle):
     """20 chained steps at 4,096 particles: eager and hipGraph replay give bit-identical state; the
     bucket tables (with their stale entries) match the oracle fed the same per-step positions."""
