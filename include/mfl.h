@@ -58,7 +58,11 @@ enum mfl_buffer_kind {
     MFL_CELL = 6,
     MFL_CELL_START = 7,
     MFL_CELL_END = 8,
-    MFL_PARAMS = 9
+    MFL_PARAMS = 9,
+    /* uint32[N] by sorted slot: the number of accepted visits of each particle in the last density pass
+       (the `count` of libCL/phys/oclFluid3D.cl:179-182 that decides count > 2), written only while
+       mfl_fluid3d_set_record_counts is on. Read-only. */
+    MFL_NEIGHBOUR_COUNT = 10
 };
 
 /* ---- runtime ------------------------------------------------------------------------------------ */
@@ -148,6 +152,10 @@ int mfl_fluid3d_set_bucket_clear(mfl_fluid3d* f, uint32_t count);
 /* Addition: replay compute() from a captured hipGraph (re-captured whenever the particle count,
    buffers, params, box, gravity or bucket clear change). Ignored while an integrate callback is set. */
 int mfl_fluid3d_set_use_graph(mfl_fluid3d* f, int enable);
+/* Addition for parity checks: while enabled, every density pass also writes each particle's number of
+   accepted visits (integer, the `count` of libCL/phys/oclFluid3D.cl:179-182) to the
+   MFL_NEIGHBOUR_COUNT buffer. Off by default (4 bytes per particle more per step). */
+int mfl_fluid3d_set_record_counts(mfl_fluid3d* f, int enable);
 
 /* Addition: the order in which the density and force passes visit particles. It changes no result
    (every particle's sums are formed by one thread in the reference's visit order), only cache

@@ -13,6 +13,7 @@ LIB_PATH = os.path.join(HERE, "libmfl.so")
 MFL_POSITION, MFL_VELOCITY, MFL_FORCE = 0, 1, 2
 MFL_SORTED_POSITION, MFL_SORTED_VELOCITY, MFL_INDEX = 3, 4, 5
 MFL_CELL, MFL_CELL_START, MFL_CELL_END, MFL_PARAMS = 6, 7, 8, 9
+MFL_NEIGHBOUR_COUNT = 10
 
 PARAM_FIELDS = (
     "deltaTime", "particleRadius", "cellSize", "mass", "viscosity", "pressure", "density", "spacing",
@@ -28,7 +29,8 @@ EXPORTS = (
     "mfl_fluid3d_set_particle_count", "mfl_fluid3d_get_particle_count", "mfl_fluid3d_set_buffer",
     "mfl_fluid3d_get_buffer", "mfl_fluid3d_write_buffer", "mfl_fluid3d_read_buffer", "mfl_fluid3d_params",
     "mfl_fluid3d_set_box", "mfl_fluid3d_set_gravity", "mfl_fluid3d_set_bucket_clear",
-    "mfl_fluid3d_set_use_graph", "mfl_fluid3d_set_traversal", "mfl_fluid3d_set_profiling",
+    "mfl_fluid3d_set_use_graph", "mfl_fluid3d_set_record_counts", "mfl_fluid3d_set_traversal",
+    "mfl_fluid3d_set_profiling",
     "mfl_fluid3d_get_tuning", "mfl_fluid3d_set_tuning", "mfl_fluid3d_autotune",
     "mfl_fluid3d_stage_time",
     "mfl_fluid3d_set_integrate_callback", "mfl_fluid3d_enqueue_kernel",
@@ -143,6 +145,7 @@ def load() -> ctypes.CDLL:
         "mfl_fluid3d_set_gravity": (ctypes.c_int, [vp, ctypes.POINTER(ctypes.c_float)]),
         "mfl_fluid3d_set_bucket_clear": (ctypes.c_int, [vp, ctypes.c_uint32]),
         "mfl_fluid3d_set_use_graph": (ctypes.c_int, [vp, ctypes.c_int]),
+        "mfl_fluid3d_set_record_counts": (ctypes.c_int, [vp, ctypes.c_int]),
         "mfl_fluid3d_set_traversal": (ctypes.c_int, [vp, ctypes.c_int, ctypes.c_uint32]),
         "mfl_fluid3d_set_profiling": (ctypes.c_int, [vp, ctypes.c_uint32]),
         "mfl_fluid3d_get_tuning": (ctypes.c_int, [vp, ctypes.POINTER(Tuning)]),

@@ -797,8 +797,12 @@ void FluidBatch::drop_graph()
 
 int FluidBatch::launch_graph(hipStream_t s)
 {
+    // the sort's temporaries are reserved first: the signature holds their addresses
+    if (!sort.reserve(N_, 18 + (int)scene_bits_)) return 0;
     std::string sig;
-    sig.reserve(64);
+    sig.reserve(192);
+    sort.sig_append(sig);
+    sig_add(sig, kept_.ptr);
     sig_add(sig, clear_count);
     sig_add(sig, sort.tuning.items);
     sig_add(sig, gravity);
@@ -810,7 +814,6 @@ int FluidBatch::launch_graph(hipStream_t s)
     sig_add(sig, porder_valid_);
     if (!graph_exec_ || sig != graph_sig_) {
         drop_graph();
-        if (!sort.reserve(N_, 18 + (int)scene_bits_)) return 0;
         MFL_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeRelaxed));
         const int ok = enqueue_step(s);
         hipGraph_t gr = nullptr;
@@ -950,6 +953,8 @@ int FluidBatch::autotune(hipStream_t s, uint32_t steps, mfl_tuning* best, double
         set_error("autotune needs at least one step per trial");
         return 0;
     }
+    // the trials re-size the sort's temporaries (other tile sizes): no graph may keep the old ones
+    drop_graph();
     const size_t vb = (size_t)N_ * sizeof(float4), tb = (size_t)S_ * kBuckets * 2 * sizeof(uint32_t);
     DeviceBuffer save;
     if (!save.allocate(3 * vb + tb)) return 0;

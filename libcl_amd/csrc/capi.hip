@@ -271,6 +271,13 @@ int mfl_fluid3d_set_use_graph(mfl_fluid3d* f, int enable)
     return 1;
 }
 
+int mfl_fluid3d_set_record_counts(mfl_fluid3d* f, int enable)
+{
+    if (!need(f, "fluid")) return 0;
+    f->impl.record_counts = enable != 0;
+    return 1;
+}
+
 int mfl_fluid3d_set_traversal(mfl_fluid3d* f, int order, uint32_t refresh_steps)
 {
     if (!need(f, "fluid")) return 0;

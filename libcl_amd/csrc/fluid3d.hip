@@ -182,7 +182,7 @@ __global__ __launch_bounds__(kThreads) void k_find_bounds(const uint32_t* __rest
 __global__ __launch_bounds__(kThreads) void k_density(float4* __restrict__ spos, float4* __restrict__ svel,
                                                       const uint32_t* __restrict__ cell_start,
                                                       const uint32_t* __restrict__ cell_end, uint32_t n,
-                                                      mfl_params p)
+                                                      mfl_params p, uint32_t* __restrict__ counts_out)
 {
     const uint32_t i = blockIdx.x * kThreads + threadIdx.x;
     if (i >= n) return;
@@ -211,6 +211,7 @@ __global__ __launch_bounds__(kThreads) void k_density(float4* __restrict__ spos,
                     }
                 }
             }
+    if (counts_out) counts_out[i] = (uint32_t)count;
     float rho, prs;
     if (count > 2) {
         rho = fmaxf(1.0f, p.mass * p.kernelConstant * acc);
@@ -281,7 +282,8 @@ __global__ __launch_bounds__(kThreads) void k_density_nbr(float4* __restrict__ s
                                                           mfl_params p, uint32_t* __restrict__ nbr,
                                                           uint32_t* __restrict__ ncount,
                                                           const uint32_t* __restrict__ porder,
-                                                          const uint32_t* __restrict__ inv)
+                                                          const uint32_t* __restrict__ inv,
+                                                          uint32_t* __restrict__ counts_out)
 {
     const uint32_t t = xcd_block() * kThreads + threadIdx.x;
     if (t >= n) return;
@@ -331,6 +333,7 @@ __global__ __launch_bounds__(kThreads) void k_density_nbr(float4* __restrict__ s
     spos[i].w = rho;
     svel[i].w = prs;
     ncount[t] = count;
+    if (counts_out) counts_out[i] = count;
 }
 
 template <int ORDER>
@@ -340,7 +343,8 @@ __global__ __launch_bounds__(kThreads) void k_density_grp(float4* __restrict__ s
                                                           mfl_params p, uint32_t* __restrict__ nbr,
                                                           uint32_t* __restrict__ ncount,
                                                           const uint32_t* __restrict__ porder,
-                                                          const uint32_t* __restrict__ inv)
+                                                          const uint32_t* __restrict__ inv,
+                                                          uint32_t* __restrict__ counts_out)
 {
     const uint32_t lane = threadIdx.x & (kWave - 1);
     const uint32_t grp = lane / kGroupLanes, slot = lane % kGroupLanes;
@@ -385,6 +389,7 @@ __global__ __launch_bounds__(kThreads) void k_density_grp(float4* __restrict__ s
     spos[i].w = rho;
     svel[i].w = prs;
     ncount[t] = count;
+    if (counts_out) counts_out[i] = count;
 }
 
 template <int ORDER, int CH>
@@ -563,7 +568,7 @@ __global__ __launch_bounds__(kThreads) void k_density_tile(const float4* __restr
                                                            const uint32_t* __restrict__ inv, uint32_t* __restrict__ nbr,
                                                            uint32_t* __restrict__ ncount, uint32_t* __restrict__ kept_out,
                                                            uint64_t* __restrict__ hitmask, uint32_t* __restrict__ meta,
-                                                           float cut)
+                                                           float cut, uint32_t* __restrict__ counts_out)
 {
     constexpr bool LIST = OUT == kOutList;
     constexpr uint32_t P = W / 8;  // lanes per particle
@@ -674,6 +679,7 @@ __global__ __launch_bounds__(kThreads) void k_density_tile(const float4* __restr
     spos[i].w = rho;
     svel[i].w = prs;
     if (LIST) ncount[t] = count;
+    if (counts_out) counts_out[i] = count;
 }
 
 // Force pass over LDS-staged neighbourhoods (cl:202-255). STEP = true fuses clip box, integrate force,
@@ -1008,9 +1014,6 @@ Fluid3D::Fluid3D()
 Fluid3D::~Fluid3D()
 {
     drop_graph();
-    for (DeviceBuffer* b : {&pos_, &vel_, &force_, &cell_, &index_, &spos_, &svel_, &cell_start_, &cell_end_,
-                            &params_buf_, &nbr_, &ncount_, &kept_, &hitmask_, &meta_, &inv_, &porder_, &mkey_})
-        b->release();
 }
 
 bool Fluid3D::allocate_internal()
@@ -1024,7 +1027,7 @@ bool Fluid3D::allocate_internal()
     if (!kept_.allocate((u1 / 4 + 7) / 8 * kSegCap * 4) || !hitmask_.allocate(u1 / 4 * kGroupLanes * 8) ||
         !meta_.allocate(u1))
         return false;
-    if (!inv_.allocate(u1) || !porder_.allocate(u1) || !mkey_.allocate(u1)) return false;
+    if (!inv_.allocate(u1) || !porder_.allocate(u1) || !mkey_.allocate(u1) || !counts_.allocate(u1)) return false;
     porder_valid_ = false;
     steps_since_refresh_ = 0;
     // the reference leaves the tables uninitialised (cpp:53-54); here they start empty
@@ -1072,6 +1075,7 @@ DeviceBuffer* Fluid3D::select(int kind)
     case MFL_CELL_START: return &cell_start_;
     case MFL_CELL_END: return &cell_end_;
     case MFL_PARAMS: return &params_buf_;
+    case MFL_NEIGHBOUR_COUNT: return &counts_;
     default: set_error("unknown buffer kind " + std::to_string(kind)); return nullptr;
     }
 }
@@ -1212,6 +1216,7 @@ int Fluid3D::enqueue_step(hipStream_t s)
     auto* ce = cell_end_.as<uint32_t>();
     if (!enqueue_front(s)) return 0;
     const dim3 gx = grid_xcd(n);
+    uint32_t* counts_out = record_counts ? counts_.as<uint32_t>() : nullptr;
     auto* nbr = nbr_.as<uint32_t>();
     auto* ncount = ncount_.as<uint32_t>();
     auto* inv = inv_.as<uint32_t>();
@@ -1223,30 +1228,30 @@ int Fluid3D::enqueue_step(hipStream_t s)
         switch (order) {
         case MFL_TRAVERSAL_SORTED:
             hipLaunchKernelGGL(k_density_nbr<kOrderSorted>, gx, b, 0, s, spos, svel, cs, ce, n, params, nbr, ncount,
-                               porder, inv);
+                               porder, inv, counts_out);
             break;
         case MFL_TRAVERSAL_CALLER:
             hipLaunchKernelGGL(k_density_nbr<kOrderCaller>, gx, b, 0, s, spos, svel, cs, ce, n, params, nbr, ncount,
-                               porder, inv);
+                               porder, inv, counts_out);
             break;
         default:
             hipLaunchKernelGGL(k_density_nbr<kOrderListed>, gx, b, 0, s, spos, svel, cs, ce, n, params, nbr, ncount,
-                               porder, inv);
+                               porder, inv, counts_out);
         }
     } else {
         const dim3 gg = grid_xcd((size_t)n * kGroupLanes);
         switch (order) {
         case MFL_TRAVERSAL_SORTED:
             hipLaunchKernelGGL(k_density_grp<kOrderSorted>, gg, b, 0, s, spos, svel, cs, ce, n, params, nbr, ncount,
-                               porder, inv);
+                               porder, inv, counts_out);
             break;
         case MFL_TRAVERSAL_CALLER:
             hipLaunchKernelGGL(k_density_grp<kOrderCaller>, gg, b, 0, s, spos, svel, cs, ce, n, params, nbr, ncount,
-                               porder, inv);
+                               porder, inv, counts_out);
             break;
         default:
             hipLaunchKernelGGL(k_density_grp<kOrderListed>, gg, b, 0, s, spos, svel, cs, ce, n, params, nbr, ncount,
-                               porder, inv);
+                               porder, inv, counts_out);
         }
     }
     timer.end(MFL_STAGE_DENSITY, s);
@@ -1347,6 +1352,7 @@ int Fluid3D::enqueue_tile_passes(hipStream_t s, int order)
     const size_t lds_f = tile_lds_bytes(true, block_waves, tile_depth);
     const uint32_t depth = tile_depth;
     const float cut = sqrt_cut(params.cellSize);
+    uint32_t* counts_out = record_counts ? counts_.as<uint32_t>() : nullptr;
     timer.begin(MFL_STAGE_DENSITY, s);
     auto* nbr = nbr_.as<uint32_t>();
     auto* ncount = ncount_.as<uint32_t>();
@@ -1357,7 +1363,7 @@ int Fluid3D::enqueue_tile_passes(hipStream_t s, int order)
                     : neighbour_pass == MFL_PASS_TILE_KEPT ? kOutKept : kOutNone;
 #define MFL_DENSITY_TILE(O, OUT, W)                                                                              \
     hipLaunchKernelGGL((k_density_tile<O, OUT, W>), gd, b, lds_d, s, pos, spos, svel, cs, ce, n, params, depth,   \
-                       porder, inv, nbr, ncount, kept, hitmask, meta, cut)
+                       porder, inv, nbr, ncount, kept, hitmask, meta, cut, counts_out)
 #define MFL_DENSITY_TILE_W(O, OUT)                 \
     if (lanes == 4) MFL_DENSITY_TILE(O, OUT, 32); \
     else MFL_DENSITY_TILE(O, OUT, 64)
@@ -1498,6 +1504,8 @@ int Fluid3D::autotune(hipStream_t s, uint32_t steps, mfl_tuning* best, double* m
         set_error("autotune needs at least one step per trial");
         return 0;
     }
+    // the trials re-size the sort's temporaries (other tile sizes): no graph may keep the old ones
+    drop_graph();
     // snapshot of everything a step reads that a step also writes
     const size_t vb = n_ * sizeof(float4), tb = (size_t)kBuckets * sizeof(uint32_t);
     DeviceBuffer save;
@@ -1589,8 +1597,11 @@ void Fluid3D::drop_graph()
 
 int Fluid3D::launch_graph(hipStream_t s)
 {
+    // the sort's temporaries are reserved first: the signature holds their addresses
+    if (!sort.reserve(n_, 18)) return 0;
     std::string sig;
-    sig.reserve(192);
+    sig.reserve(256);
+    sort.sig_append(sig);
     sig_add(sig, n_);
     sig_add(sig, pos_.ptr);
     sig_add(sig, vel_.ptr);
@@ -1604,13 +1615,13 @@ int Fluid3D::launch_graph(hipStream_t s)
     sig_add(sig, tile_depth);
     sig_add(sig, block_waves);
     sig_add(sig, force_chunks);
+    sig_add(sig, record_counts);
     sig_add(sig, params);
     sig_add(sig, box_min);
     sig_add(sig, box_max);
     sig_add(sig, gravity);
     if (!graph_exec_ || sig != graph_sig_) {
         drop_graph();
-        if (!sort.reserve(n_, 18)) return 0;
         MFL_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeRelaxed));
         const int ok = enqueue_step(s);
         hipGraph_t g = nullptr;
@@ -1684,7 +1695,8 @@ int Fluid3D::enqueue_kernel(const char* name, hipStream_t s)
     else if (k == "clFindBounds")
         hipLaunchKernelGGL(k_find_bounds, g, b, 0, s, cell, n, cs, ce);
     else if (k == "clCalculateDensity")
-        hipLaunchKernelGGL(k_density, g, b, 0, s, spos, svel, cs, ce, n, params);
+        hipLaunchKernelGGL(k_density, g, b, 0, s, spos, svel, cs, ce, n, params,
+                           record_counts ? counts_.as<uint32_t>() : nullptr);
     else if (k == "clCalculateForces")
         hipLaunchKernelGGL(k_forces, g, b, 0, s, spos, svel, force, index, cs, ce, n, params);
     else if (k == "clClipBox")

@@ -55,6 +55,7 @@ public:
     void* integrate_user = nullptr;
     mfl_fluid3d* handle = nullptr;                   // C handle passed back to the callback
     bool use_graph = false;
+    bool record_counts = false;             // density passes write each particle's accepted visits
     int traversal = MFL_TRAVERSAL_MORTON;   // order in which the neighbour passes visit particles
     uint32_t refresh_interval = 16;         // steps between Morton order refreshes
     // launch shape (mfl_tuning); the defaults are the fastest shape measured on the 1M-particle step
@@ -88,6 +89,7 @@ private:
     DeviceBuffer nbr_, ncount_;  // per-step neighbour visits recorded by the density pass
     DeviceBuffer kept_, hitmask_, meta_;  // per-step kept entries and acceptance bits (TILE_KEPT)
     DeviceBuffer inv_;           // inverse of the sort permutation: inv[index[i]] = i
+    DeviceBuffer counts_;        // accepted visits per sorted particle (written when record_counts)
     DeviceBuffer porder_, mkey_; // Morton traversal order (caller indices) and its sort keys
     bool porder_valid_ = false;
     uint32_t steps_since_refresh_ = 0;

@@ -29,10 +29,35 @@ bool hip_ok(hipError_t e, const char* what);
     } while (0)
 
 // ---- device memory owned by a program ----------------------------------------------------------
+// Move-only: a program's buffers are freed with the program (caller-owned memory is never freed).
 struct DeviceBuffer {
     void* ptr = nullptr;
     size_t bytes = 0;
     bool owned = true;
+
+    DeviceBuffer() = default;
+    ~DeviceBuffer() { release(); }
+    DeviceBuffer(const DeviceBuffer&) = delete;
+    DeviceBuffer& operator=(const DeviceBuffer&) = delete;
+    DeviceBuffer(DeviceBuffer&& o) noexcept : ptr(o.ptr), bytes(o.bytes), owned(o.owned)
+    {
+        o.ptr = nullptr;
+        o.bytes = 0;
+        o.owned = true;
+    }
+    DeviceBuffer& operator=(DeviceBuffer&& o) noexcept
+    {
+        if (this != &o) {
+            release();
+            ptr = o.ptr;
+            bytes = o.bytes;
+            owned = o.owned;
+            o.ptr = nullptr;
+            o.bytes = 0;
+            o.owned = true;
+        }
+        return *this;
+    }
 
     bool allocate(size_t nbytes);   // (re)allocates when owned; contents lost
     bool fit(size_t nbytes);        // grow-only (oclRadixSort::fit, libCL/sort/oclRadixSort.cpp:50-59)

@@ -342,6 +342,14 @@ int RadixSort::reserve(size_t n, int nbits)
     return 1;
 }
 
+void RadixSort::sig_append(std::string& sig) const
+{
+    for (const DeviceBuffer* b : {&tmp_keys_[0], &tmp_keys_[1], &tmp_vals_[0], &tmp_vals_[1], &counts_, &totals_}) {
+        sig_add(sig, b->ptr);
+        sig_add(sig, b->bytes);
+    }
+}
+
 int RadixSort::first_pass_plan(size_t n, int start_bit, int end_bit, SortCountPlan* plan)
 {
     const int nbits = end_bit - start_bit;

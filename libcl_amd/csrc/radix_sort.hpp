@@ -48,6 +48,10 @@ public:
     // enqueues no allocation, e.g. under hipGraph capture).
     int reserve(size_t n, int nbits);
 
+    // Appends the addresses of the sort's temporaries to a launch signature: a captured graph holds
+    // them, so a graph must be captured again once any of them is re-allocated.
+    void sig_append(std::string& sig) const;
+
     SortTuning tuning;
 
 private:

@@ -117,6 +117,7 @@ class oclFluid3D:
         self.bfCellStart = DeviceBuffer(self, N.MFL_CELL_START, "bfCellStart")
         self.bfCellEnd = DeviceBuffer(self, N.MFL_CELL_END, "bfCellEnd")
         self.bfParams = DeviceBuffer(self, N.MFL_PARAMS, "bfParams")
+        self.bfNeighbourCount = DeviceBuffer(self, N.MFL_NEIGHBOUR_COUNT, "bfNeighbourCount")
 
     def __del__(self):
         h = getattr(self, "_h", None)
@@ -201,6 +202,11 @@ class oclFluid3D:
 
     def setUseGraph(self, enable: bool) -> int:
         return int(N.load().mfl_fluid3d_set_use_graph(self._h, 1 if enable else 0))
+
+    def setRecordCounts(self, enable: bool) -> int:
+        """While enabled, each density pass writes every particle's accepted-visit count (by sorted slot)
+        to bfNeighbourCount."""
+        return int(N.load().mfl_fluid3d_set_record_counts(self._h, 1 if enable else 0))
 
     def setTraversal(self, order: str, refresh_steps: int = 16) -> int:
         """Visit order of the neighbour passes: "sorted", "caller" or "morton" (results unchanged)."""

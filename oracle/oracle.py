@@ -60,7 +60,7 @@ class _Fluid(ctypes.Structure):
         ("clear_count", ctypes.c_uint32),
         ("pos", _f32p), ("vel", _f32p), ("force", _f32p),
         ("cell", _u32p), ("index", _u32p), ("spos", _f32p), ("svel", _f32p),
-        ("cell_start", _u32p), ("cell_end", _u32p),
+        ("cell_start", _u32p), ("cell_end", _u32p), ("counts", _u32p),
     ]
 
 
@@ -98,7 +98,7 @@ def lib():
         L.orc_reorder.argtypes = [_u32p, _f32p, _f32p, _f32p, _f32p, ctypes.c_uint32]
         L.orc_clear_buckets.argtypes = [_u32p, _u32p, ctypes.c_uint32]
         L.orc_find_bounds.argtypes = [_u32p, ctypes.c_uint32, _u32p, _u32p]
-        L.orc_density.argtypes = [_f32p, _f32p, _u32p, _u32p, ctypes.c_uint32, ctypes.POINTER(Params)]
+        L.orc_density.argtypes = [_f32p, _f32p, _u32p, _u32p, ctypes.c_uint32, ctypes.POINTER(Params), _u32p]
         L.orc_forces.argtypes = [_f32p, _f32p, _f32p, _u32p, _u32p, _u32p, ctypes.c_uint32,
                                  ctypes.POINTER(Params)]
         L.orc_clip_box.argtypes = [_f32p, _f32p, _f32p, _u32p, ctypes.c_uint32, ctypes.POINTER(Params),
@@ -217,6 +217,7 @@ class Fluid:
                            else np.ascontiguousarray(cell_start, np.uint32).copy())
         self.cell_end = (np.full(BUCKETS, EMPTY, np.uint32) if cell_end is None
                          else np.ascontiguousarray(cell_end, np.uint32).copy())
+        self.counts = np.zeros(n, np.uint32)  # accepted visits per sorted particle (last density pass)
         s = _Fluid()
         s.n = n
         s.params = params
@@ -228,6 +229,7 @@ class Fluid:
         s.cell, s.index = _u(self.cell), _u(self.index)
         s.spos, s.svel = _f(self.spos), _f(self.svel)
         s.cell_start, s.cell_end = _u(self.cell_start), _u(self.cell_end)
+        s.counts = _u(self.counts)
         self._s = s
 
     @property
@@ -267,7 +269,11 @@ def find_bounds(cell, cell_start, cell_end):
 
 
 def density(spos, svel, cell_start, cell_end, params: Params):
-    lib().orc_density(_f(spos), _f(svel), _u(cell_start), _u(cell_end), spos.shape[0], ctypes.byref(params))
+    """Writes density / pressure into the w lanes of spos / svel; returns the accepted visits per particle."""
+    counts = np.zeros(spos.shape[0], np.uint32)
+    lib().orc_density(_f(spos), _f(svel), _u(cell_start), _u(cell_end), spos.shape[0], ctypes.byref(params),
+                      _u(counts))
+    return counts
 
 
 def forces(spos, svel, force, index, cell_start, cell_end, params: Params):

@@ -269,6 +269,12 @@ void orc_stable_sort(uint32_t* key, uint32_t* val, uint32_t n, int start_bit, in
 
 /* ---------------------------------------------------------------- fluid kernels */
 
+/* The squared length inside fast_length(vector) (cl:177, cl:232). The reference leaves the builtin to
+   its OpenCL runtime; this restatement fixes it as fma(z, z, fma(y, y, x * x)), the form an OpenCL
+   compiler contracting `x*x + y*y + z*z` produces (FP_CONTRACT is on by default in OpenCL C) and the
+   one the HIP kernels use, so both sides take the same accept / reject decision for every visit. */
+static inline float dot3(float x, float y, float z) { return fmaf(z, z, fmaf(y, y, x * x)); }
+
 void orc_reorder(const uint32_t* index, const float* pos4, const float* vel4, float* spos4, float* svel4,
                  uint32_t n)
 {
@@ -311,7 +317,7 @@ void orc_find_bounds(const uint32_t* cell, uint32_t n, uint32_t* cell_start, uin
 }
 
 void orc_density(float* spos4, float* svel4, const uint32_t* cell_start, const uint32_t* cell_end, uint32_t n,
-                 const orc_params* p)
+                 const orc_params* p, uint32_t* counts)
 {
     const float h = p->cellSize;
     /* libCL/phys/oclFluid3D.cl:150-200. Reads of position[j].w race with other items' writes in the
@@ -338,13 +344,14 @@ void orc_density(float* spos4, float* svel4, const uint32_t* cell_start, const u
                         if (j == (uint32_t)i) continue;
                         const float* pj = spos4 + 4 * (size_t)j;
                         float vx = pi[0] - pj[0], vy = pi[1] - pj[1], vz = pi[2] - pj[2];
-                        float vlen = sqrtf(vx * vx + vy * vy + vz * vz); /* fast_length, w = 0 */
+                        float vlen = sqrtf(dot3(vx, vy, vz)); /* fast_length, w = 0 */
                         if (vlen < h) {
                             density += powf(h * h - vlen * vlen, 3.0f); /* cl:123 */
                             count++;
                         }
                     }
                 }
+        if (counts) counts[i] = (uint32_t)count; /* accepted visits of particle i (cl:179-182) */
         if (count > 2) {
             float d = fmaxf(1.0f, p->mass * p->kernelConstant * density);
             rho[i] = d;
@@ -388,7 +395,7 @@ void orc_forces(const float* spos4, const float* svel4, float* force4, const uin
                         if (j == (uint32_t)i) continue;
                         const float* pj = spos4 + 4 * (size_t)j;
                         float v[3] = {pi[0] - pj[0], pi[1] - pj[1], pi[2] - pj[2]};
-                        float vlen = sqrtf(v[0] * v[0] + v[1] * v[1] + v[2] * v[2]);
+                        float vlen = sqrtf(dot3(v[0], v[1], v[2])); /* fast_length, w = 0 */
                         if (vlen < h) {
                             const float* vj = svel4 + 4 * (size_t)j;
                             float density2 = pi[3] * pj[3];
@@ -494,7 +501,7 @@ int orc_step(orc_fluid* f, int integrate)
     orc_reorder(f->index, f->pos, f->vel, f->spos, f->svel, n);              /* :304 */
     orc_clear_buckets(f->cell_start, f->cell_end, f->clear_count);           /* :309-313 */
     orc_find_bounds(f->cell, n, f->cell_start, f->cell_end);                 /* :315 */
-    orc_density(f->spos, f->svel, f->cell_start, f->cell_end, n, &f->params); /* :318 */
+    orc_density(f->spos, f->svel, f->cell_start, f->cell_end, n, &f->params, f->counts); /* :318 */
     orc_forces(f->spos, f->svel, f->force, f->index, f->cell_start, f->cell_end, n, &f->params); /* :321 */
     if (!integrate) return 1;                                                /* :325-328 */
     orc_clip_box(f->spos, f->svel, f->force, f->index, n, &f->params, f->box_min, f->box_max); /* :331 */

@@ -69,9 +69,10 @@ void orc_reorder(const uint32_t* index, const float* pos4, const float* vel4, fl
 void orc_clear_buckets(uint32_t* cell_start, uint32_t* cell_end, uint32_t clear_count);
 /* clFindBounds, libCL/phys/oclFluid3D.cl:85-113 */
 void orc_find_bounds(const uint32_t* cell, uint32_t n, uint32_t* cell_start, uint32_t* cell_end);
-/* clCalculateDensity, libCL/phys/oclFluid3D.cl:150-200 */
+/* clCalculateDensity, libCL/phys/oclFluid3D.cl:150-200; counts (may be NULL) receives each sorted
+   particle's number of accepted visits */
 void orc_density(float* spos4, float* svel4, const uint32_t* cell_start, const uint32_t* cell_end, uint32_t n,
-                 const orc_params* p);
+                 const orc_params* p, uint32_t* counts);
 /* clCalculateForces, libCL/phys/oclFluid3D.cl:202-255 */
 void orc_forces(const float* spos4, const float* svel4, float* force4, const uint32_t* index,
                 const uint32_t* cell_start, const uint32_t* cell_end, uint32_t n, const orc_params* p);
@@ -105,6 +106,7 @@ typedef struct {
     float* svel;
     uint32_t* cell_start;
     uint32_t* cell_end;
+    uint32_t* counts; /* may be NULL: accepted visits per sorted particle of the last density pass */
 } orc_fluid;
 
 int orc_step(orc_fluid* f, int integrate);

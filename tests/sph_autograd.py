@@ -87,7 +87,81 @@ _OFFS = np.array([(x, y, z) for z in (-1, 0, 1) for y in (-1, 0, 1) for x in (-1
 
 def _pairs(x32: np.ndarray, h: float):
     """(i, j, multiplicity) of every accepted visit for fresh tables: j within h of i, weighted by
-    the number of i's 27 neighbour buckets equal to j's key."""
+    the number of i's 27 neighbour buckets equal to j's key. All pairs for small n, a cell list above."""
+    if x32.shape[0] > 2048:
+        return _pairs_cells(x32, h)
+    return _pairs_dense(x32, h)
+
+
+def _candidates(x32: np.ndarray, radius: float):
+    """Pairs (i, j), i != j, at distance < radius, by a cell list of cell size `radius`: such a j lies in
+    one of the 27 cells around i's cell."""
+    g = np.floor(x32.astype(np.float64) / radius).astype(np.int64)
+    n = x32.shape[0]
+    lo = g.min(0) - 1
+    dims = g.max(0) - lo + 2
+    lin = ((g[:, 2] - lo[2]) * dims[1] + (g[:, 1] - lo[1])) * dims[0] + (g[:, 0] - lo[0])
+    order = np.argsort(lin, kind="stable")
+    slin = lin[order]
+    x64 = x32.astype(np.float64)
+    ii_all, jj_all = [], []
+    for o in _OFFS:
+        t = lin + (o[2] * dims[1] + o[1]) * dims[0] + o[0]
+        a = np.searchsorted(slin, t, side="left")
+        b = np.searchsorted(slin, t, side="right")
+        cnt = b - a
+        ii = np.repeat(np.arange(n), cnt)
+        first = np.repeat(a - np.concatenate([[0], np.cumsum(cnt)[:-1]]), cnt)
+        jj = order[first + np.arange(ii.shape[0])]
+        d = x64[ii] - x64[jj]
+        keep = (np.sqrt((d * d).sum(1)) < radius) & (ii != jj)
+        ii_all.append(ii[keep])
+        jj_all.append(jj[keep])
+    return np.concatenate(ii_all), np.concatenate(jj_all)
+
+
+def _with_multiplicity(x32: np.ndarray, h: float, ii: np.ndarray, jj: np.ndarray):
+    """Accepted visits among candidate pairs: distance < h (fp64 of the fp32 positions), weighted by the
+    number of i's 27 neighbour buckets equal to j's key."""
+    g = np.floor(x32 / np.float32(h)).astype(np.int64)  # float32 IEEE division, as the kernel
+    key = _hash(g)
+    x64 = x32.astype(np.float64)
+    d = x64[ii] - x64[jj]
+    near = np.sqrt((d * d).sum(1)) < h
+    ii, jj = ii[near], jj[near]
+    nbk = np.stack([_hash(g + o) for o in _OFFS], 1)  # [n, 27]
+    mult = (nbk[ii] == key[jj][:, None]).sum(1)
+    keep = mult > 0
+    return ii[keep], jj[keep], mult[keep].astype(np.float64)
+
+
+def _pairs_cells(x32: np.ndarray, h: float):
+    """_pairs by a cell list (the same pairs as the all-pairs search, in another order)."""
+    ii, jj = _candidates(x32, h)
+    return _with_multiplicity(x32, h, ii, jj)
+
+
+class PairCache:
+    """_pairs over a rollout, with candidates within h + skin kept between steps: while no particle has
+    moved skin / 2 from where the candidates were found, every pair within h is among them, so the
+    accepted pairs are exactly those of a fresh search."""
+
+    def __init__(self, h: float, skin: float = 0.2):
+        self.h, self.skin = h, skin * h
+        self.x0 = None
+
+    def __call__(self, x32: np.ndarray):
+        if x32.shape[0] <= 2048:
+            return _pairs_dense(x32, self.h)
+        moved = 0.0 if self.x0 is None else np.sqrt(((x32.astype(np.float64) - self.x0) ** 2).sum(1)).max()
+        if self.x0 is None or moved >= 0.5 * self.skin:
+            self.x0 = x32.astype(np.float64)
+            self.cand = _candidates(x32, self.h + self.skin)
+        return _with_multiplicity(x32, self.h, *self.cand)
+
+
+def _pairs_dense(x32: np.ndarray, h: float):
+    """All-pairs form of _pairs (n^2 distances)."""
     hf = np.float32(h)
     g = np.floor(x32 / hf).astype(np.int64)
     key = _hash(g)
@@ -117,8 +191,9 @@ def cpu_rollout_fp64(pos0, vel0, force0, prm: L.Params, box_min, box_max, steps,
     grav = torch.tensor([-0.0, -0.0, -9.8], dtype=f64)
     mn, mx = np.asarray(box_min, np.float64), np.asarray(box_max, np.float64)
     n = x.shape[0]
+    pairs = PairCache(h)
     for s in range(steps):
-        ii, jj, mult = _pairs(x.detach().numpy().astype(np.float32), h)
+        ii, jj, mult = pairs(x.detach().numpy().astype(np.float32))
         ii_t, jj_t = torch.from_numpy(ii), torch.from_numpy(jj)
         mu_t = torch.from_numpy(mult)
         d = x[ii_t] - x[jj_t]

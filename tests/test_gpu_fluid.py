@@ -38,6 +38,7 @@ def make_fluid(pos, vel, force, params, box_min, box_max, clear=65536, tables=No
     if SHAPES[shape]:
         assert f.setTuning(**SHAPES[shape]) == 1, L.last_error()
     assert f.setParticleCount(pos.shape[0]) == 1, L.last_error()
+    assert f.setRecordCounts(True) == 1  # accepted visits per particle, compared with the oracle's
     f.setParameters(params)
     assert f.setBox(box_min, box_max) == 1
     assert f.setBucketClear(clear) == 1
@@ -102,11 +103,15 @@ def test_step_matches_golden(name, shape):
 
 
 def _oracle_compare(f, fo, check_fields=True):
+    """Keys, permutation, both bucket tables and every particle's accepted-visit count (the integer
+    behind the count > 2 switch, cl:179-199; both sides form |d|^2 as fma(z, z, fma(y, y, x * x)))
+    bit-exact; density and pressure within TOL."""
     it = f.internals()
     np.testing.assert_array_equal(it["cell"], fo.cell)
     np.testing.assert_array_equal(it["index"], fo.index)
     np.testing.assert_array_equal(it["cell_start"], fo.cell_start)
     np.testing.assert_array_equal(it["cell_end"], fo.cell_end)
+    np.testing.assert_array_equal(f.bfNeighbourCount.read(np.uint32)[:fo.counts.shape[0]], fo.counts)
     if check_fields:
         rel = np.abs(it["spos"][:, 3] - fo.spos[:, 3]) / np.abs(fo.spos[:, 3])
         assert rel.max() <= TOL, rel.max()
@@ -394,6 +399,7 @@ def test_tile_passes_match_list_passes(order, shape):
     for k in ("cell", "index", "cell_start", "cell_end"):
         np.testing.assert_array_equal(b[k], a[k])
     np.testing.assert_allclose(b["spos"][:, 3], a["spos"][:, 3], rtol=1e-6)
+    np.testing.assert_array_equal(f.bfNeighbourCount.read(np.uint32), ref.bfNeighbourCount.read(np.uint32))
     nw, per = force_errors(f.state()["force"], ref.state()["force"])
     assert nw <= 1e-6 and per <= 1e-6, (nw, per)
 
@@ -421,3 +427,34 @@ def test_autotune_keeps_state_and_picks_a_valid_shape():
     L.load().mfl_device_finish()
     assert f.setTuning(tile_depth=100) == 0 and "tile_depth" in L.last_error()
     assert f.setTuning(neighbour_pass="tile", tile_depth=1024, block_waves=1) == 0 and "512" in L.last_error()
+
+
+def test_graph_survives_autotune_at_1m():
+    """hipGraph replay after an autotune at 1,048,576 particles. The autotuner's trials re-size the
+    sort's temporaries (a 4-keys-per-thread trial doubles the tile count at this size), so a graph
+    captured before it must not be replayed on the old buffers: compute, autotune, compute in graph
+    mode equals the same steps launched eagerly, bit for bit (recorded-visit passes, whose results do
+    not depend on the traversal order the autotuner invalidates)."""
+    import torch
+    import libcl_amd as L
+    print("graph + autotune at 1M", flush=True)
+    sc = scenes.config_1m()
+    prm = L.params_from(sc.params)
+    fa = make_fluid(sc.pos, sc.vel, sc.force, prm, sc.box_min, sc.box_max, shape="list")
+    fb = make_fluid(sc.pos, sc.vel, sc.force, prm, sc.box_min, sc.box_max, shape="list")
+    shape = fb.getTuning()
+    assert fb.setUseGraph(True) == 1
+    stream = torch.cuda.Stream()
+    assert fa.compute(stream) == 1 and fb.compute(stream) == 1, L.last_error()
+    stream.synchronize()
+    assert fb.autotune(steps=1, stream=stream) is not None, L.last_error()
+    assert fb.setTuning(**shape) == 1, L.last_error()  # the shape the first graph was captured with
+    for _ in range(2):
+        assert fa.compute(stream) == 1 and fb.compute(stream) == 1, L.last_error()
+    stream.synchronize()
+    sa, sb = fa.state(), fb.state()
+    for k in ("pos", "vel", "force"):
+        np.testing.assert_array_equal(sa[k], sb[k], err_msg=k)
+    ia, ib = fa.internals(), fb.internals()
+    for k in ("cell", "index", "cell_start", "cell_end", "spos", "svel"):
+        np.testing.assert_array_equal(ia[k], ib[k], err_msg=k)

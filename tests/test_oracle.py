@@ -316,3 +316,22 @@ def test_stale_buckets_are_visited(oracle):
         found = True
         break
     assert found, "no configuration produced a stale visit"
+
+
+@pytest.mark.parametrize("name", STEP_CASES)
+def test_oracle_accepted_counts_match_brute_force(oracle, name):
+    """Accepted visits per particle (the integer that drives the count > 2 switch, cl:179-199) against the
+    fp64 all-pairs fixture, which weights each pair within h by its visit multiplicity. Particles with
+    a pair whose distance lies within 1e-6 h of h are left out: there the fp32 decision of the restated
+    fast_length may round either way."""
+    z = load_step_case(name)
+    p = oracle.Params(*[float(x) for x in z["params"]])
+    f = oracle.Fluid(z["pos"], z["vel"], z["force"], p, box_min=z["box_min"], box_max=z["box_max"])
+    f.step(integrate=False)
+    x = f.spos[:, :3].astype(np.float64)
+    h = float(np.float32(p.cellSize))
+    r = np.linalg.norm(x[:, None] - x[None], axis=-1)
+    ambiguous = (np.abs(r - h) <= 1e-6 * h).any(1)
+    assert ambiguous.sum() <= 2
+    np.testing.assert_array_equal(f.counts[~ambiguous], z["count"][~ambiguous].astype(np.uint32))
+    assert f.counts.max() > 0
