@@ -144,3 +144,59 @@ def test_checkpoint_interval_does_not_change_gradients():
         # forward values do not depend on the checkpoint interval, and backward leaves the final state
         np.testing.assert_array_equal(xt, grads[0][3])
         np.testing.assert_array_equal(pos[:, :3], xt)
+
+
+@pytest.mark.parametrize("steps", [20, 40])
+def test_adjoint_at_the_differentiable_workload(steps):
+    """The differentiable workload itself: 65,536 particles (64x32x32 block in the origin corner of the
+    unit box), v0 = 0, checkpoints every 20 steps, full bucket clear (the FP64 restatement assumes fresh
+    tables). The hand-written adjoint of the FP32 GPU rollout agrees with torch autograd of the FP64
+    CPU restatement (a cell-list pair search) to 1e-4: dL/dv0 normwise, each parameter gradient
+    relative. 20 and 40 steps: one and two checkpoint segments."""
+    print(f"C5 65536 particles, {steps} steps", flush=True)
+    sc, target = scenes.config_grad()
+    sim = Simulation(sc, steps, 20, clear=262144)
+    tgt = torch.from_numpy(target)
+    v0 = np.zeros((sc.n, 3), np.float32)
+    g_gpu = _gpu_grads(sim, sc, tgt, v0, 6.0, 1.0, 1000.0)
+    g_cpu = _cpu(sc, sim, tgt, v0, 6.0, 1.0, 1000.0, steps)
+    rel = np.linalg.norm(g_gpu[0] - g_cpu[0]) / np.linalg.norm(g_cpu[0])
+    print("v0 grad rel", rel, "params gpu", g_gpu[1:], "cpu", g_cpu[1:], flush=True)
+    assert rel < 1e-4
+    for a, b in zip(g_gpu[1:], g_cpu[1:]):
+        assert abs(a - b) <= 1e-4 * abs(b), (a, b)
+
+
+def test_adjoint_vs_fp64_differences_on_the_4096_variant():
+    """The finite-difference variant of the differentiable workload: 16x16x16 = 4,096 particles in the
+    origin corner, 200 steps, checkpoints every 20, v0 = 0, full bucket clear. Central differences of
+    the FP64 map (cell-list restatement, FP64 throughout) against the GPU adjoint, to 1e-3, for the
+    direction of dL/dv0, viscosity, stiffness and rest density. The map is only piecewise smooth there
+    (visit multiplicities switch when a particle changes cell next to a coordinate plane, wall forces
+    switch on at d = 1e-5), so the steps are relative 1e-5: at 1e-3 they straddle such switches and
+    the differences move by tens of percent for stiffness and rest density, while at 1e-5 .. 1e-7 they
+    agree with the branch-wise derivative (torch autograd of the same FP64 map) to 1e-7."""
+    steps = 200
+    sc, target = scenes.config_grad((16, 16, 16))
+    sim = Simulation(sc, steps, 20, clear=262144)
+    tgt = torch.from_numpy(target)
+    v0 = np.zeros((sc.n, 3), np.float32)
+    gv, gmu, gk, gr = _gpu_grads(sim, sc, tgt, v0, 6.0, 1.0, 1000.0)
+    z = np.zeros((sc.n, 3), np.float64)
+
+    def L64(v, m_, k_, r_):
+        return _cpu(sc, sim, tgt, v, m_, k_, r_, steps, grad=False)
+
+    u = gv / np.linalg.norm(gv)
+    eps = 1e-5
+    fd_v = (L64(z + eps * u, 6.0, 1.0, 1000.0) - L64(z - eps * u, 6.0, 1.0, 1000.0)) / (2 * eps)
+    print("v0 direction", np.linalg.norm(gv), fd_v, flush=True)
+    assert abs(np.linalg.norm(gv) - fd_v) <= 1e-3 * abs(fd_v)
+    # viscosity by 1e-3 relative: the restatement scales the FP32 viscosityConstant of each perturbed value,
+    # whose rounding (6e-8) would otherwise dominate the difference; no switch lies that close in viscosity
+    for name, plus, minus, step, grad in (("viscosity", (6.006, 1.0, 1000.0), (5.994, 1.0, 1000.0), 0.012, gmu),
+                                          ("stiffness", (6.0, 1.0 + 1e-5, 1000.0), (6.0, 1.0 - 1e-5, 1000.0), 2e-5, gk),
+                                          ("density", (6.0, 1.0, 1000.01), (6.0, 1.0, 999.99), 0.02, gr)):
+        fd = (L64(z, *plus) - L64(z, *minus)) / step
+        print(name, grad, fd, flush=True)
+        assert abs(grad - fd) <= 1e-3 * abs(fd), (name, grad, fd)

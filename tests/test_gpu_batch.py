@@ -190,3 +190,73 @@ def test_batch_autotune_keeps_state():
     assert b.compute(stream) == 1, L.last_error()
     stream.synchronize()
     assert b.setTuning(neighbour_pass="tile") == 0 and "TILE_LIST" in L.last_error()
+
+
+def test_batched_workload_vs_oracle():
+    """All 256 scenes of the batched workload (2,048-65,536 particles each, 5.1 M in all) over their
+    500 steps. From the batch's own state at steps 20 and 500 (positions, velocities, forces and each
+    scene's bucket tables) one batched step and one oracle step per scene: keys, permutation and
+    tables bit-exact, density and pressure within 1e-5, and the step's velocities and positions
+    within the change a 1e-5 force error (floored by the scene's RMS force) makes in one step."""
+    import torch
+    import libcl_amd as L
+    from libcl_amd import _native as N
+    from oracle import oracle as O
+    TOL = 1e-5
+    scs = scenes.batched_scenes(num=256, seed=5, steps=500)
+    prm = L.params_from(scs[0].params)
+    op = O.Params(*prm.values())
+    b = L.FluidBatch([s.n for s in scs])
+    assert b.setParameters(prm) == 1 and b.setBox((0, 0, 0, 0), (1, 1, 1, 0)) == 1
+    assert b.upload(np.concatenate([s.pos for s in scs]), np.concatenate([s.vel for s in scs]),
+                    np.concatenate([s.force for s in scs])) == 1
+    assert b.setUseGraph(True) == 1
+    stream = torch.cuda.Stream()
+    dt = float(prm.deltaTime)
+    done = 0
+    for target in (20, 500):
+        for _ in range(target - done):
+            assert b.compute(stream) == 1, L.last_error()
+        stream.synchronize()
+        done = target
+        before = b.state()
+        cs_all = b.read(N.MFL_CELL_START, np.uint32).reshape(len(scs), -1).copy()
+        ce_all = b.read(N.MFL_CELL_END, np.uint32).reshape(len(scs), -1).copy()
+        assert b.compute(stream) == 1, L.last_error()
+        stream.synchronize()
+        done += 1
+        after = b.state()
+        index = b.read(N.MFL_INDEX, np.uint32)
+        cell = b.read(N.MFL_CELL, np.uint32)
+        spos = b.read(N.MFL_SORTED_POSITION, np.float32, (-1, 4))
+        svel = b.read(N.MFL_SORTED_VELOCITY, np.float32, (-1, 4))
+        worst = 0.0
+        for s, sc in enumerate(scs):
+            lo, n = int(b.offsets[s]), sc.n
+            sl = slice(lo, lo + n)
+            fo = O.Fluid(before["pos"][sl], before["vel"][sl], before["force"][sl], op, box_min=(0, 0, 0, 0),
+                         box_max=(1, 1, 1, 0), cell_start=cs_all[s], cell_end=ce_all[s])
+            fo.step(integrate=False)
+            np.testing.assert_array_equal(cell[sl] & 0x3FFFF, fo.cell, err_msg=f"step {target} scene {s}")
+            np.testing.assert_array_equal(index[sl] - lo, fo.index, err_msg=f"step {target} scene {s}")
+            cs, ce = b.scene_tables(s)
+            np.testing.assert_array_equal(cs, fo.cell_start, err_msg=f"step {target} scene {s}")
+            np.testing.assert_array_equal(ce, fo.cell_end, err_msg=f"step {target} scene {s}")
+            rel = np.abs(spos[sl, 3] - fo.spos[:, 3]) / np.abs(fo.spos[:, 3])
+            p = fo.svel[:, 3].astype(np.float64)
+            prel = np.abs(svel[sl, 3] - p) / (np.abs(p) + np.sqrt(np.mean(p ** 2)))
+            assert rel.max() <= TOL and prel.max() <= TOL, (target, s, rel.max(), prel.max())
+            # the rest of the oracle step: clip box, integrate force, gravity, integrate velocity
+            O.clip_box(fo.spos, fo.svel, fo.force, fo.index, op, (0, 0, 0, 0), (1, 1, 1, 0))
+            fclip = fo.force.copy()
+            O.integrate_force(fo.vel, fo.force, op)
+            O.integrate_velocity(fo.pos, fo.vel, op)
+            fmag = np.linalg.norm(fclip[:, :3].astype(np.float64), axis=1)
+            vtol = TOL * (fmag + np.sqrt(np.mean(fmag ** 2))) * dt + 4e-7 * (np.abs(fo.vel[:, :3]).max(1) + 1e-6)
+            dv = np.linalg.norm(after["vel"][sl, :3] - fo.vel[:, :3], axis=1)
+            assert np.all(dv <= vtol), (target, s, (dv / vtol).max())
+            ptol = vtol * dt + 2.5e-7 * np.abs(fo.pos[:, :3]).max(1)
+            assert np.all(np.linalg.norm(after["pos"][sl, :3] - fo.pos[:, :3], axis=1) <= ptol), (target, s)
+            np.testing.assert_array_equal(after["force"][sl], np.tile(scenes.GRAVITY_FORCE, (n, 1)))
+            worst = max(worst, float(rel.max()), float((dv / vtol).max()) * TOL)
+        print(f"C4 step {target}: 256 scenes match the oracle (worst {worst:.2e})", flush=True)
