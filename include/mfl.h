@@ -108,6 +108,17 @@ int mfl_radixsort_compile(mfl_radixsort* s);
 int mfl_radixsort_compute(mfl_radixsort* s, uint32_t* keys, uint32_t* vals, size_t n, int start_bit,
                           int end_bit, void* stream);
 
+/* No counterpart in the reference. mfl_radixsort_compute that allocates nothing, for callers that
+   cannot allocate device memory (code under HIP graph capture, tight serving loops, harnesses that
+   forbid hipMalloc): the sort's temporaries live in a fixed device array of the library (18 MiB + 2 KiB,
+   held by every GPU the library runs on until the process ends) instead of buffers of `s`. Same result,
+   checks and return convention as mfl_radixsort_compute; n above MFL_SORT_NOALLOC_MAX_PAIRS is rejected.
+   Every call on one GPU shares that array: calls that overlap in time on one GPU (other streams, other
+   threads) race; calls in one stream, or separated by synchronisation, do not. */
+#define MFL_SORT_NOALLOC_MAX_PAIRS 1048576u
+int mfl_radixsort_compute_noalloc(mfl_radixsort* s, uint32_t* keys, uint32_t* vals, size_t n, int start_bit,
+                                  int end_bit, void* stream);
+
 /* ---- oclFluid3D (libCL/phys/oclFluid3D.h:25-74) -------------------------------------------------- */
 
 /* oclFluid3D::oclFluid3D (libCL/phys/oclFluid3D.cpp:21-83): owns position/velocity/force buffers for

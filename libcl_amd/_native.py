@@ -25,6 +25,7 @@ EXPORTS = (
     "mfl_last_error", "mfl_clear_error", "mfl_abi_version", "mfl_set_device", "mfl_device_finish", "mfl_memcpy",
     "mfl_params_default", "mfl_params_derive", "mfl_sqrt_cut",
     "mfl_radixsort_create", "mfl_radixsort_destroy", "mfl_radixsort_compile", "mfl_radixsort_compute",
+    "mfl_radixsort_compute_noalloc",
     "mfl_fluid3d_create", "mfl_fluid3d_destroy", "mfl_fluid3d_compile", "mfl_fluid3d_compute",
     "mfl_fluid3d_set_particle_count", "mfl_fluid3d_get_particle_count", "mfl_fluid3d_set_buffer",
     "mfl_fluid3d_get_buffer", "mfl_fluid3d_write_buffer", "mfl_fluid3d_read_buffer", "mfl_fluid3d_params",
@@ -130,6 +131,7 @@ def load() -> ctypes.CDLL:
         "mfl_radixsort_destroy": (None, [vp]),
         "mfl_radixsort_compile": (ctypes.c_int, [vp]),
         "mfl_radixsort_compute": (ctypes.c_int, [vp, vp, vp, sz, ctypes.c_int, ctypes.c_int, vp]),
+        "mfl_radixsort_compute_noalloc": (ctypes.c_int, [vp, vp, vp, sz, ctypes.c_int, ctypes.c_int, vp]),
         "mfl_fluid3d_create": (vp, []),
         "mfl_fluid3d_destroy": (None, [vp]),
         "mfl_fluid3d_compile": (ctypes.c_int, [vp]),

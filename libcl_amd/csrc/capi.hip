@@ -160,6 +160,13 @@ int mfl_radixsort_compute(mfl_radixsort* s, uint32_t* keys, uint32_t* vals, size
     return guarded([&] { return s->impl.compute(keys, vals, n, start_bit, end_bit, (hipStream_t)stream); });
 }
 
+int mfl_radixsort_compute_noalloc(mfl_radixsort* s, uint32_t* keys, uint32_t* vals, size_t n, int start_bit,
+                                  int end_bit, void* stream)
+{
+    if (!need(s, "radix sort")) return 0;
+    return guarded([&] { return s->impl.compute_noalloc(keys, vals, n, start_bit, end_bit, (hipStream_t)stream); });
+}
+
 // ---- fluid --------------------------------------------------------------------------------------
 
 mfl_fluid3d* mfl_fluid3d_create(void)

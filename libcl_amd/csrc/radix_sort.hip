@@ -11,6 +11,8 @@
 //                   reordered by digit in LDS, then runs written contiguously (reads 8, writes 8 B/key)
 #include "radix_sort.hpp"
 
+#include <mutex>
+
 namespace mfl {
 namespace {
 
@@ -367,14 +369,64 @@ int RadixSort::first_pass_plan(size_t n, int start_bit, int end_bit, SortCountPl
     return 1;
 }
 
-int RadixSort::sort_bits(uint32_t* keys, uint32_t* vals, size_t n, int start_bit, int end_bit, hipStream_t stream,
-                         bool identity_vals, bool first_counted)
+namespace {
+
+// Temporaries of one sort: key and value ping-pong buffers of n words each, the digit-major count
+// table (radix x tiles) and the per-digit totals.
+struct SortBuffers {
+    uint32_t* tk[2];
+    uint32_t* tv[2];
+    uint32_t* counts;
+    uint32_t* totals;
+};
+
+// The passes of a sort of n pairs (1 <= n < 2^32, 0 <= start_bit < end_bit <= 32) over buffers
+// that fit it.
+int sort_passes(const SortBuffers& b, uint32_t* keys, uint32_t* vals, size_t n, int start_bit, int end_bit,
+                hipStream_t stream, bool identity_vals, bool first_counted, int items)
+{
+    const int nbits = end_bit - start_bit;
+    const int passes = (nbits + kMaxDigitBits - 1) / kMaxDigitBits;
+    const size_t bytes = n * sizeof(uint32_t);
+    const uint32_t* kin = keys;
+    const uint32_t* vin = vals;
+    int shift = start_bit;
+    for (int p = 0; p < passes; p++) {
+        const int bits = nbits / passes + (p < nbits % passes ? 1 : 0);
+        const bool last = p == passes - 1 && passes > 1;
+        uint32_t* kout = last ? keys : b.tk[p & 1];
+        uint32_t* vout = last ? vals : b.tv[p & 1];
+        const bool ident = identity_vals && p == 0;
+        const bool counted = first_counted && p == 0;
+        bool ok;
+        switch (items) {
+        case 16: ok = run_pass<16>(kin, vin, kout, vout, (uint32_t)n, shift, bits, b.counts, b.totals, ident, stream,
+                                   counted); break;
+        case 8: ok = run_pass<8>(kin, vin, kout, vout, (uint32_t)n, shift, bits, b.counts, b.totals, ident, stream,
+                                 counted); break;
+        default: ok = run_pass<4>(kin, vin, kout, vout, (uint32_t)n, shift, bits, b.counts, b.totals, ident, stream,
+                                  counted); break;
+        }
+        if (!ok) return 0;
+        kin = kout;
+        vin = vout;
+        shift += bits;
+    }
+    if (passes == 1) {
+        MFL_HIP(hipMemcpyAsync(keys, b.tk[0], bytes, hipMemcpyDeviceToDevice, stream));
+        MFL_HIP(hipMemcpyAsync(vals, b.tv[0], bytes, hipMemcpyDeviceToDevice, stream));
+    }
+    return 1;
+}
+
+// Checks shared by the sort entry points; 1 = go on with the sort, 0 = failed, -1 = nothing to sort.
+int check_sort_args(const uint32_t* keys, const uint32_t* vals, size_t n, int start_bit, int end_bit)
 {
     if (start_bit < 0 || end_bit > 32) {
         set_error("bit range must lie in [0, 32]");
         return 0;
     }
-    if (n == 0 || start_bit >= end_bit) return 1;
+    if (n == 0 || start_bit >= end_bit) return -1;
     if (n > 0xFFFFFFFFull) {
         set_error("maximum sortable array size = 4294967295");
         return 0;
@@ -383,41 +435,90 @@ int RadixSort::sort_bits(uint32_t* keys, uint32_t* vals, size_t n, int start_bit
         set_error("radix sort: null key or value buffer");
         return 0;
     }
-    const int nbits = end_bit - start_bit;
-    const int passes = (nbits + kMaxDigitBits - 1) / kMaxDigitBits;
-    const int items = pick_items(n, tuning.items);
-    const size_t bytes = n * sizeof(uint32_t);
-    if (!reserve(n, nbits)) return 0;
-
-    const uint32_t* kin = keys;
-    const uint32_t* vin = vals;
-    int shift = start_bit;
-    for (int p = 0; p < passes; p++) {
-        const int bits = nbits / passes + (p < nbits % passes ? 1 : 0);
-        const bool last = p == passes - 1 && passes > 1;
-        uint32_t* kout = last ? keys : tmp_keys_[p & 1].as<uint32_t>();
-        uint32_t* vout = last ? vals : tmp_vals_[p & 1].as<uint32_t>();
-        const bool ident = identity_vals && p == 0;
-        const bool counted = first_counted && p == 0;
-        bool ok;
-        switch (items) {
-        case 16: ok = run_pass<16>(kin, vin, kout, vout, (uint32_t)n, shift, bits, counts_.as<uint32_t>(),
-                                   totals_.as<uint32_t>(), ident, stream, counted); break;
-        case 8: ok = run_pass<8>(kin, vin, kout, vout, (uint32_t)n, shift, bits, counts_.as<uint32_t>(),
-                                 totals_.as<uint32_t>(), ident, stream, counted); break;
-        default: ok = run_pass<4>(kin, vin, kout, vout, (uint32_t)n, shift, bits, counts_.as<uint32_t>(),
-                                  totals_.as<uint32_t>(), ident, stream, counted); break;
-        }
-        if (!ok) return 0;
-        kin = kout;
-        vin = vout;
-        shift += bits;
-    }
-    if (passes == 1) {
-        MFL_HIP(hipMemcpyAsync(keys, tmp_keys_[0].ptr, bytes, hipMemcpyDeviceToDevice, stream));
-        MFL_HIP(hipMemcpyAsync(vals, tmp_vals_[0].ptr, bytes, hipMemcpyDeviceToDevice, stream));
-    }
     return 1;
+}
+
+// ---- the sort without device allocation ------------------------------------------------------------
+//
+// compute_noalloc's temporaries live in `scratch`, a file-scope device array: each GPU holds its own copy
+// from the library's load until the process ends. Layout (32-bit words): key ping-pong buffers at 0 and
+// kNoAllocMaxPairs, value buffers at 2 and 3 x kNoAllocMaxPairs, the count table (at most
+// kNoAllocMaxPairs / 1024 tiles of >= 1024 keys x 512 digits) at 4 x kNoAllocMaxPairs, the 512 digit
+// totals after it. Every pass writes the count table and totals before reading them and every buffer
+// before reading it, so nothing has to start cleared. Host code never uses the array's name as an
+// address: it is a host-side stand-in; the device address comes from hipGetSymbolAddress.
+constexpr size_t kNoAllocMaxPairs = MFL_SORT_NOALLOC_MAX_PAIRS;
+constexpr size_t kNoAllocCounts = kNoAllocMaxPairs / (kThreads * 4) * kMaxRadix;
+constexpr size_t kNoAllocWords = 4 * kNoAllocMaxPairs + kNoAllocCounts + kMaxRadix;
+static __device__ __align__(256) uint32_t scratch[kNoAllocWords];
+
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+uint32_t* g_scratch[kMaxDevices] = {};  // device address of `scratch` per device ordinal, once looked up
+
+// `scratch`'s address on the current device, looked up on the first call made with that device current.
+uint32_t* scratch_address()
+{
+    int dev = 0;
+    if (!hip_ok(hipGetDevice(&dev), "hipGetDevice")) return nullptr;
+    if (dev < 0 || dev >= kMaxDevices) {
+        set_error("device ordinal " + std::to_string(dev) + " above the no-allocation sort's device table");
+        return nullptr;
+    }
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (!g_scratch[dev]) {
+        void* p = nullptr;
+        if (!hip_ok(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)), "hipGetSymbolAddress(scratch)")) return nullptr;
+        g_scratch[dev] = static_cast<uint32_t*>(p);
+    }
+    return g_scratch[dev];
+}
+
+}  // namespace
+
+int RadixSort::compute_noalloc(uint32_t* keys, uint32_t* vals, size_t n, int start_bit, int end_bit,
+                               hipStream_t stream)
+{
+    if ((end_bit - start_bit) % 4 != 0) {  // libCL/sort/oclRadixSort.cpp:76-80, as compute()
+        set_error("end bit(" + std::to_string(end_bit) + ") - start bit(" + std::to_string(start_bit) +
+                  ") must be divisible by 4");
+        return 0;
+    }
+    const int go = check_sort_args(keys, vals, n, start_bit, end_bit);
+    if (go <= 0) return go < 0 ? 1 : 0;
+    if (n > kNoAllocMaxPairs) {
+        set_error("the no-allocation sort takes at most " + std::to_string(kNoAllocMaxPairs) + " pairs, not " +
+                  std::to_string(n));
+        return 0;
+    }
+    uint32_t* base = scratch_address();
+    if (!base) return 0;
+    const int items = pick_items(n, tuning.items);  // >= 4 keys per thread: ntiles <= kNoAllocMaxPairs / 1024
+    SortBuffers b;
+    b.tk[0] = base;
+    b.tk[1] = base + kNoAllocMaxPairs;
+    b.tv[0] = base + 2 * kNoAllocMaxPairs;
+    b.tv[1] = base + 3 * kNoAllocMaxPairs;
+    b.counts = base + 4 * kNoAllocMaxPairs;
+    b.totals = b.counts + kNoAllocCounts;
+    return sort_passes(b, keys, vals, n, start_bit, end_bit, stream, false, false, items);
+}
+
+int RadixSort::sort_bits(uint32_t* keys, uint32_t* vals, size_t n, int start_bit, int end_bit, hipStream_t stream,
+                         bool identity_vals, bool first_counted)
+{
+    const int go = check_sort_args(keys, vals, n, start_bit, end_bit);
+    if (go <= 0) return go < 0 ? 1 : 0;
+    if (!reserve(n, end_bit - start_bit)) return 0;
+    SortBuffers b;
+    b.tk[0] = tmp_keys_[0].as<uint32_t>();
+    b.tk[1] = tmp_keys_[1].as<uint32_t>();
+    b.tv[0] = tmp_vals_[0].as<uint32_t>();
+    b.tv[1] = tmp_vals_[1].as<uint32_t>();
+    b.counts = counts_.as<uint32_t>();
+    b.totals = totals_.as<uint32_t>();
+    return sort_passes(b, keys, vals, n, start_bit, end_bit, stream, identity_vals, first_counted,
+                       pick_items(n, tuning.items));
 }
 
 }  // namespace mfl

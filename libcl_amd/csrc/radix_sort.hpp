@@ -32,6 +32,11 @@ public:
     int compute(uint32_t* keys, uint32_t* vals, size_t n, int start_bit, int end_bit, hipStream_t stream,
                 bool identity_vals = false);
 
+    // compute() without any device allocation: the temporaries live in a fixed device array of the
+    // library (one per GPU, MFL_SORT_NOALLOC_MAX_PAIRS pairs at most; larger n is rejected). Calls that
+    // overlap in time on one GPU share that array and race.
+    int compute_noalloc(uint32_t* keys, uint32_t* vals, size_t n, int start_bit, int end_bit, hipStream_t stream);
+
     // Same sort for any bit range in [0, 32] (no multiple-of-4 rule); used by the fluid step, whose
     // keys are below 2^18 so that bits [0, 18) order them exactly as the reference's [0, 24).
     // `first_counted`: the caller already filled the first pass's count table (first_pass_plan).
