@@ -79,6 +79,11 @@ int mfl_device_finish(void);
 /* Blocking copy between any two host / device addresses (oclBuffer::read / write,
    libCL/oclBuffer.cpp:47-83, for buffers that are not program-owned). */
 int mfl_memcpy(void* dst, const void* src, size_t bytes);
+/* Device memory of `bytes` bytes on the current device for caller-owned buffers (oclBuffer::create,
+   libCL/oclBuffer.h:61, without the host mapping), NULL on failure; mfl_device_free releases it
+   (~oclBuffer). */
+void* mfl_device_alloc(size_t bytes);
+void mfl_device_free(void* ptr);
 
 /* ---- params ------------------------------------------------------------------------------------- */
 

@@ -23,6 +23,8 @@ PARAM_FIELDS = (
 # every symbol include/mfl.h declares
 EXPORTS = (
     "mfl_last_error", "mfl_clear_error", "mfl_abi_version", "mfl_set_device", "mfl_device_finish", "mfl_memcpy",
+    "mfl_device_alloc", "mfl_device_free",
+    "mfl_device_alloc", "mfl_device_free",
     "mfl_params_default", "mfl_params_derive", "mfl_sqrt_cut",
     "mfl_radixsort_create", "mfl_radixsort_destroy", "mfl_radixsort_compile", "mfl_radixsort_compute",
     "mfl_radixsort_compute_noalloc",
@@ -124,6 +126,8 @@ def load() -> ctypes.CDLL:
         "mfl_set_device": (ctypes.c_int, [ctypes.c_int]),
         "mfl_device_finish": (ctypes.c_int, []),
         "mfl_memcpy": (ctypes.c_int, [vp, vp, sz]),
+        "mfl_device_alloc": (vp, [sz]),
+        "mfl_device_free": (None, [vp]),
         "mfl_params_default": (None, [pp]),
         "mfl_sqrt_cut": (ctypes.c_float, [ctypes.c_float]),
         "mfl_params_derive": (None, [pp]),

@@ -88,6 +88,19 @@ int mfl_device_finish(void)
     return 1;
 }
 
+void* mfl_device_alloc(size_t bytes)
+{
+    void* p = nullptr;
+    if (bytes == 0) bytes = 1;
+    if (!mfl::hip_ok(hipMalloc(&p, bytes), "hipMalloc")) return nullptr;
+    return p;
+}
+
+void mfl_device_free(void* ptr)
+{
+    if (ptr) (void)hipFree(ptr);
+}
+
 int mfl_memcpy(void* dst, const void* src, size_t bytes)
 {
     if (bytes == 0) return 1;
