@@ -189,11 +189,11 @@ int mfl_fluid3d_set_traversal(mfl_fluid3d* f, int order, uint32_t refresh_steps)
    TILE_LIST: the TILE density pass also records the visits, and the LIST force pass reads them.
    TILE_KEPT: the TILE density pass records each cell's kept entries and every lane's acceptance bits;
    the force pass stages the kept entries in LDS (64 at a time) and sums the accepted ones. The default
-   is TILE_KEPT, 4 lanes, depth 192, 2 waves per workgroup. */
+   is TILE_KEPT, 2 lanes, depth 96, 2 waves per workgroup. */
 enum mfl_neighbour_pass { MFL_PASS_LIST = 0, MFL_PASS_TILE = 1, MFL_PASS_TILE_LIST = 2, MFL_PASS_TILE_KEPT = 3 };
 typedef struct mfl_tuning {
     int32_t neighbour_pass; /* mfl_neighbour_pass */
-    int32_t density_lanes;  /* lanes per particle in the density pass: LIST 1 or 8; the tile passes 4 or 8 */
+    int32_t density_lanes;  /* lanes per particle in the density pass: LIST 1 or 8; the tile passes 2, 4 or 8 */
     uint32_t tile_depth;    /* tile passes: LDS entries per staged cell, multiple of 16 in 64..64 x lanes */
     uint32_t block_waves;   /* tile passes: waves per workgroup, 1, 2 or 4 */
     uint32_t sort_items;    /* keys per thread of each radix sort pass: 4, 8 or 16; 0 = by key count */

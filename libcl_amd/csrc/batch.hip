@@ -424,9 +424,7 @@ __global__ __launch_bounds__(kThreads) void k_b_forces_kept(const float4* __rest
         todo &= ~L.bits(__ballot(member));
         const uint32_t offk = active ? (gm & 0xFFFFu) : 0u, kept = active ? (gm >> 16) : 0u;
         const uint32_t* kj = kept_in + (size_t)gseg * kSegCap + offk;
-        uint32_t kmax = kept;
-        if constexpr (W < kWave) kmax = max(kmax, (uint32_t)__shfl_xor((int)kmax, W, kWave));
-        kmax = __builtin_amdgcn_readfirstlane(kmax);
+        const uint32_t kmax = seg_wave_max<W>(kept);
         const uint32_t trips = (kept + P - 1) / P;
         for (uint32_t c0 = 0; c0 < kmax; c0 += kKeptChunk) {
             for (uint32_t q0 = 0; q0 < kKeptChunk; q0 += 2 * W) {
@@ -750,7 +748,10 @@ int FluidBatch::enqueue_step(hipStream_t s)
                        svel, cs_all, ce_all, off, pscene_.as<uint32_t>(), params, N_, tile_depth,                     \
                        porder_.as<uint32_t>(), inv_.as<uint32_t>(), nbr_.as<uint32_t>(), ncount_.as<uint32_t>(),     \
                        kept_.as<uint32_t>(), hitmask_.as<uint64_t>(), meta_.as<uint32_t>(), cuts_.as<float>())
-        if (lanes == 4) {
+        if (lanes == 2) {
+            if (kept) MFL_B_TILE(16, true);
+            else MFL_B_TILE(16, false);
+        } else if (lanes == 4) {
             if (kept) MFL_B_TILE(32, true);
             else MFL_B_TILE(32, false);
         } else {
@@ -773,7 +774,8 @@ int FluidBatch::enqueue_step(hipStream_t s)
                        pos, vel, force, cs_all, ce_all, off, pscene_.as<uint32_t>(), params, boxes_.as<float4>(), N_, \
                        porder_.as<uint32_t>(), inv_.as<uint32_t>(), kept_.as<uint32_t>(), hitmask_.as<uint64_t>(),  \
                        meta_.as<uint32_t>(), gravity_force_of(gravity))
-        if (lanes == 4) MFL_B_KEPT(32);
+        if (lanes == 2) MFL_B_KEPT(16);
+        else if (lanes == 4) MFL_B_KEPT(32);
         else MFL_B_KEPT(64);
 #undef MFL_B_KEPT
     } else if (force_chunks == 2) {
@@ -906,8 +908,8 @@ int FluidBatch::set_tuning(const mfl_tuning& t)
         return 0;
     }
     const bool tile = t.neighbour_pass != MFL_PASS_LIST;
-    if (tile ? (t.density_lanes != 4 && t.density_lanes != 8) : t.density_lanes != 8) {
-        set_error(tile ? "density_lanes must be 4 or 8 with the tile passes"
+    if (tile ? (t.density_lanes != 2 && t.density_lanes != 4 && t.density_lanes != 8) : t.density_lanes != 8) {
+        set_error(tile ? "density_lanes must be 2, 4 or 8 with the tile passes"
                        : "density_lanes must be 8 with MFL_PASS_LIST for batched scenes");
         return 0;
     }
@@ -980,8 +982,8 @@ int FluidBatch::autotune(hipStream_t s, uint32_t steps, mfl_tuning* best, double
     std::vector<mfl_tuning> cand;
     cand.push_back({MFL_PASS_LIST, 8, 192u, 2u, start.sort_items, 2u});
     for (uint32_t w : {1u, 2u, 4u})
-        for (uint32_t d : {128u, 192u, 256u, 384u})
-            for (int lanes : {4, 8})
+        for (uint32_t d : {96u, 128u, 192u, 256u, 384u})
+            for (int lanes : {2, 4, 8})
                 for (int pass : {MFL_PASS_TILE_LIST, MFL_PASS_TILE_KEPT})
                     for (uint32_t ch : {1u, 2u}) {
                         if (pass == MFL_PASS_TILE_KEPT && ch == 2u) continue;  // chunks only shape the list pass

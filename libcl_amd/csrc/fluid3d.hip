@@ -806,9 +806,7 @@ __global__ __launch_bounds__(kThreads) void k_forces_kept(const float4* __restri
         todo &= ~L.bits(__ballot(member));
         const uint32_t off = active ? (gm & 0xFFFFu) : 0u, kept = active ? (gm >> 16) : 0u;
         const uint32_t* kj = kept_in + (size_t)gseg * kSegCap + off;
-        uint32_t kmax = kept;
-        if constexpr (W < kWave) kmax = max(kmax, (uint32_t)__shfl_xor((int)kmax, W, kWave));
-        kmax = __builtin_amdgcn_readfirstlane(kmax);
+        const uint32_t kmax = seg_wave_max<W>(kept);
         const uint32_t trips = (kept + P - 1) / P;
         // kKeptChunk entries at a time (small LDS per segment keeps more waves resident)
         for (uint32_t c0 = 0; c0 < kmax; c0 += kKeptChunk) {
@@ -1345,7 +1343,7 @@ int Fluid3D::enqueue_tile_passes(hipStream_t s, int order)
     auto* ce = cell_end_.as<uint32_t>();
     auto* inv = inv_.as<uint32_t>();
     auto* porder = porder_.as<uint32_t>();
-    const uint32_t lanes = (uint32_t)density_lanes;  // 4 or 8 (set_tuning checks)
+    const uint32_t lanes = (uint32_t)density_lanes;  // 2, 4 or 8 (set_tuning checks)
     const dim3 b(block_waves * kWave);
     const dim3 gd = tile_grid(n, kWave / lanes, block_waves), g = tile_grid(n, kWave / kGroupLanes, block_waves);
     const size_t lds_d = tile_lds_bytes(false, block_waves, tile_depth, lanes);
@@ -1365,7 +1363,8 @@ int Fluid3D::enqueue_tile_passes(hipStream_t s, int order)
     hipLaunchKernelGGL((k_density_tile<O, OUT, W>), gd, b, lds_d, s, pos, spos, svel, cs, ce, n, params, depth,   \
                        porder, inv, nbr, ncount, kept, hitmask, meta, cut, counts_out)
 #define MFL_DENSITY_TILE_W(O, OUT)                 \
-    if (lanes == 4) MFL_DENSITY_TILE(O, OUT, 32); \
+    if (lanes == 2) MFL_DENSITY_TILE(O, OUT, 16); \
+    else if (lanes == 4) MFL_DENSITY_TILE(O, OUT, 32); \
     else MFL_DENSITY_TILE(O, OUT, 64)
 #define MFL_DENSITY_TILE_OUT(O)                                       \
     if (out == kOutList) { MFL_DENSITY_TILE_W(O, kOutList); }         \
@@ -1388,7 +1387,8 @@ int Fluid3D::enqueue_tile_passes(hipStream_t s, int order)
     hipLaunchKernelGGL((k_forces_kept<O, W, STEP>), gd, b, lds_k, s, spos, svel, pos, vel, force, index, cs, ce, n, \
                        params, depth, mn, mx, gf, porder, inv, kept, hitmask, meta)
 #define MFL_FORCES_KEPT_W(O, STEP)                 \
-    if (lanes == 4) MFL_FORCES_KEPT(O, 32, STEP); \
+    if (lanes == 2) MFL_FORCES_KEPT(O, 16, STEP); \
+    else if (lanes == 4) MFL_FORCES_KEPT(O, 32, STEP); \
     else MFL_FORCES_KEPT(O, 64, STEP)
 #define MFL_FORCES_KEPT_O(STEP)                                             \
     switch (order) {                                                        \
@@ -1457,9 +1457,9 @@ int Fluid3D::set_tuning(const mfl_tuning& t)
         return 0;
     }
     if (t.neighbour_pass == MFL_PASS_LIST ? (t.density_lanes != 1 && t.density_lanes != 8)
-                                          : (t.density_lanes != 4 && t.density_lanes != 8)) {
+                                          : (t.density_lanes != 2 && t.density_lanes != 4 && t.density_lanes != 8)) {
         set_error(t.neighbour_pass == MFL_PASS_LIST ? "density_lanes must be 1 or 8 with MFL_PASS_LIST"
-                                                    : "density_lanes must be 4 or 8 with the tile passes");
+                                                    : "density_lanes must be 2, 4 or 8 with the tile passes");
         return 0;
     }
     if (t.tile_depth < kTileMinDepth || t.tile_depth > kTileMaxDepth || t.tile_depth % 16 != 0) {
@@ -1533,8 +1533,8 @@ int Fluid3D::autotune(hipStream_t s, uint32_t steps, mfl_tuning* best, double* m
     for (uint32_t ch : {1u, 2u})
         for (int lanes : {1, 8}) cand.push_back({MFL_PASS_LIST, lanes, 256u, 4u, start.sort_items, ch});
     for (uint32_t w : {1u, 2u, 4u})
-        for (uint32_t d : {128u, 192u, 256u, 384u, 512u})
-            for (int lanes : {4, 8})
+        for (uint32_t d : {96u, 128u, 192u, 256u, 384u, 512u})
+            for (int lanes : {2, 4, 8})
                 if (tile_lds_bytes(true, w, d) <= 65536 && d <= 64u * (uint32_t)lanes) {
                     cand.push_back({MFL_PASS_TILE, lanes, d, w, start.sort_items, 1u});
                     cand.push_back({MFL_PASS_TILE_KEPT, lanes, d, w, start.sort_items, 1u});

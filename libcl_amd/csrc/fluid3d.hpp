@@ -60,8 +60,8 @@ public:
     uint32_t refresh_interval = 16;         // steps between Morton order refreshes
     // launch shape (mfl_tuning); the defaults are the fastest shape measured on the 1M-particle step
     int neighbour_pass = MFL_PASS_TILE_KEPT;  // LIST, TILE, TILE_LIST or TILE_KEPT neighbour passes
-    int density_lanes = 4;                  // lanes per particle in the density pass
-    uint32_t tile_depth = 192;              // tile passes: LDS entries per staged cell
+    int density_lanes = 2;                  // lanes per particle in the density pass
+    uint32_t tile_depth = 96;               // tile passes: LDS entries per staged cell
     uint32_t block_waves = 2;               // tile passes: waves per workgroup
     uint32_t force_chunks = 2;              // list force pass: 16-byte list chunks per trip
     RadixSort sort;

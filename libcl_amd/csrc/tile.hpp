@@ -347,47 +347,69 @@ __device__ inline void stage_seg_rounds(const Lanes<W>& L, uint32_t r0, uint32_t
     }
 }
 
+// The largest value of v over the wave's segments (v equal within each segment), wave-uniform.
+template <int W>
+__device__ inline uint32_t seg_wave_max(uint32_t v)
+{
+#pragma unroll
+    for (int o = W; o < kWave; o <<= 1) v = max(v, (uint32_t)__shfl_xor((int)v, o, kWave));
+    return __builtin_amdgcn_readfirstlane(v);
+}
+
 // Segmented stage_neighbourhood (density form): every segment stages the cell it was given (a segment
-// with active = false stages nothing). Returns the segment's kept count or kStageOverflow.
+// with active = false stages nothing). Returns the segment's kept count or kStageOverflow. A segment of
+// W lanes holds the 27 bucket ranges in ceil(27 / W) layers: bucket k sits in lane k % W of layer k / W.
 template <int W>
 __device__ inline uint32_t stage_seg(const Lanes<W>& L, const SegCell& c, float h2_keep, const float4* __restrict__ spos,
                                      const uint32_t* __restrict__ cell_start, const uint32_t* __restrict__ cell_end,
                                      uint32_t n, uint32_t depth, float4* tile, StageAux* aux)
 {
-    uint32_t st = 0, len = 0;
-    if (c.active && L.ls < 27) {
-        const int dx = (int)(L.ls % 3) - 1, dy = (int)((L.ls / 3) % 3) - 1, dz = (int)(L.ls / 9) - 1;
-        const uint32_t b = cell_hash(c.cx + dx, c.cy + dy, c.cz + dz);
-        const uint32_t s = cell_start[b];
-        const uint32_t e = min(cell_end[b], n);  // stale or written ranges always lie in [0, n]
-        if (s != kEmpty) {
-            st = s;
-            len = e > s ? e - s : 0u;
-        }
-    }
-    // ranges laid end to end in visit order (scan over lanes 0..31 of each half wave)
-    uint32_t incl = len;
+    constexpr int NB = (27 + W - 1) / W;  // layers
+    constexpr uint32_t SW = W < 32 ? W : 32;  // scan width: the 27 ranges of a wider segment are in its first 32 lanes
+    uint32_t st[NB], len[NB], pre[NB];
+    uint32_t total = 0;
 #pragma unroll
-    for (uint32_t o = 1; o < 32; o <<= 1) {
-        const uint32_t up = (uint32_t)__shfl_up((int)incl, o, 32);
-        if ((L.lane & 31u) >= o) incl += up;
+    for (int b = 0; b < NB; b++) {
+        const uint32_t k = L.ls + (uint32_t)b * W;
+        st[b] = 0;
+        len[b] = 0;
+        if (c.active && k < 27) {
+            const int dx = (int)(k % 3) - 1, dy = (int)((k / 3) % 3) - 1, dz = (int)(k / 9) - 1;
+            const uint32_t bk = cell_hash(c.cx + dx, c.cy + dy, c.cz + dz);
+            const uint32_t s = cell_start[bk];
+            const uint32_t e = min(cell_end[bk], n);  // stale or written ranges always lie in [0, n]
+            if (s != kEmpty) {
+                st[b] = s;
+                len[b] = e > s ? e - s : 0u;
+            }
+        }
+        // ranges laid end to end in visit order: layer b after every range of the earlier layers
+        uint32_t incl = len[b];
+#pragma unroll
+        for (uint32_t o = 1; o < SW; o <<= 1) {
+            const uint32_t up = (uint32_t)__shfl_up((int)incl, o, SW);
+            if ((L.lane & (SW - 1u)) >= o) incl += up;
+        }
+        pre[b] = total + incl - len[b];
+        total += L.from(incl, (uint32_t)min(26 - b * W, W - 1));
     }
-    const uint32_t pre = incl - len;
-    const uint32_t total = L.from(incl, 26u);
     const bool over = total > kStageRounds * kWave;
     if (L.ls < kStageRounds) aux->starts[L.ls] = 0ull;
     wave_sync();
-    const bool mark = len > 0 && !over;
-    const uint64_t nonempty = L.bits(__ballot(mark));
-    if (mark) {
-        aux->delta[L.below(nonempty)] = st - pre;
-        atomicOr(&aux->starts[pre / 64], 1ull << (pre % 64));
+    uint32_t ranks = 0;  // non-empty ranges of the earlier layers
+#pragma unroll
+    for (int b = 0; b < NB; b++) {
+        const bool mark = len[b] > 0 && !over;
+        const uint64_t nonempty = L.bits(__ballot(mark));
+        if (mark) {
+            aux->delta[ranks + L.below(nonempty)] = st[b] - pre[b];
+            atomicOr(&aux->starts[pre[b] / 64], 1ull << (pre[b] % 64));
+        }
+        ranks += (uint32_t)__popcll(nonempty);
     }
     wave_sync();
     const uint32_t tot = over ? 0u : total;
-    uint32_t rounds = (tot + W - 1) / W;
-    if constexpr (W < 64) rounds = max(rounds, (uint32_t)__shfl_xor((int)rounds, W, kWave));
-    rounds = __builtin_amdgcn_readfirstlane(rounds);  // the wave runs the longest segment's rounds
+    const uint32_t rounds = seg_wave_max<W>((tot + W - 1) / W);  // the wave runs the longest segment's rounds
     const uint64_t upto = L.ls == 63 ? ~0ull : (2ull << L.ls) - 1ull;  // bits 0..ls
     uint32_t kept = 0, before = 0, r0 = 0;
     for (; r0 + 4 <= rounds; r0 += 4)
@@ -421,9 +443,7 @@ template <int W>
 __device__ inline uint32_t wave_trips(uint32_t kept)
 {
     constexpr uint32_t P = W / 8;
-    uint32_t wtrips = (kept + P - 1) / P;
-    if constexpr (W < kWave) wtrips = max(wtrips, (uint32_t)__shfl_xor((int)wtrips, W, kWave));
-    return __builtin_amdgcn_readfirstlane(wtrips);
+    return seg_wave_max<W>((kept + P - 1) / P);
 }
 
 // End markers (positions at infinity, index kEmpty) in entries [kept, end) of a segment's tile, so that

@@ -17,6 +17,10 @@ BATCH_SHAPES = {
     "tile_kept": ({"neighbour_pass": "tile_kept"},
                   {"neighbour_pass": "tile_kept", "density_lanes": 4, "tile_depth": 192, "block_waves": 2,
                    "force_chunks": 2}),
+    # 2 lanes per particle: four 16-lane segments of 8 particles per wave
+    "tile_kept2": ({"neighbour_pass": "tile_kept", "density_lanes": 2, "tile_depth": 96},
+                   {"neighbour_pass": "tile_kept", "density_lanes": 2, "tile_depth": 96, "block_waves": 2,
+                    "force_chunks": 2}),
     # 8 lanes per particle: one 64-lane segment of 8 particles per wave
     "tile_kept8": ({"neighbour_pass": "tile_kept", "density_lanes": 8, "tile_depth": 256},
                    {"neighbour_pass": "tile_kept", "density_lanes": 8, "tile_depth": 256, "block_waves": 2,

@@ -26,6 +26,9 @@ SHAPES = {"tile_list": {"neighbour_pass": "tile_list", "density_lanes": 4, "tile
           "tile_list4_64": {"neighbour_pass": "tile_list", "density_lanes": 4, "tile_depth": 64, "block_waves": 1},
           "tile_list4_ch2": {"neighbour_pass": "tile_list", "density_lanes": 4, "force_chunks": 2},
           "tile_kept": {},
+          "tile_kept2": {"neighbour_pass": "tile_kept", "density_lanes": 2, "tile_depth": 96},
+          "tile_kept2_64": {"neighbour_pass": "tile_kept", "density_lanes": 2, "tile_depth": 64, "block_waves": 1},
+          "tile_list2": {"neighbour_pass": "tile_list", "density_lanes": 2, "tile_depth": 96},
           "tile_kept8": {"neighbour_pass": "tile_kept", "density_lanes": 8, "tile_depth": 256},
           "tile_kept64": {"neighbour_pass": "tile_kept", "density_lanes": 4, "tile_depth": 64, "block_waves": 1},
           "list": {"neighbour_pass": "list"}}
@@ -180,7 +183,7 @@ def test_1m_trajectory_snapshots_vs_oracle(oracle):
         _step_from_snapshot(oracle, snap, prm, sc, 65536, "tile_kept")
 
 
-@pytest.mark.parametrize("shape", ["tile_list", "tile_list8", "tile", "tile_kept", "list"])
+@pytest.mark.parametrize("shape", ["tile_list", "tile_list8", "tile", "tile_kept", "tile_kept2", "list"])
 def test_1m_step_vs_oracle(oracle, shape):
     """1,048,576 particles in [0,4]^3: keys, permutation, bucket tables bit-exact; density, pressure
     and forces of one step within 1e-5."""
@@ -381,7 +384,8 @@ def test_traversal_orders_bit_identical():
 
 @pytest.mark.parametrize("order", ["sorted", "caller", "morton"])
 @pytest.mark.parametrize("shape", ["tile_list", "tile_list8", "tile", "tile64", "tile_list64", "tile_list4_64",
-                                   "tile_list4_ch2", "tile_kept", "tile_kept8", "tile_kept64"])
+                                   "tile_list4_ch2", "tile_kept", "tile_kept8", "tile_kept64", "tile_kept2",
+                                   "tile_kept2_64", "tile_list2"])
 def test_tile_passes_match_list_passes(order, shape):
     """The LDS-tile passes take the same visits as the recorded-visit passes; only the FP32 summation
     order differs (a particle's kept entries are split among its lanes by position in the pruned
