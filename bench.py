@@ -21,35 +21,39 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0  # MI355X HBM3E, vendor figure
+# VALU issue peak: 256 CUs x 4 SIMDs, one wave64 instruction per SIMD every 4 cycles at 2.4 GHz
+VALU_PEAK_G = 256 * 4 * 2.4 / 4  # G wave64 VALU instructions/s
 
-# Algorithmic HBM bytes per particle of each stage (reads + writes of the arrays the stage must touch
-# once; neighbour re-reads are cache traffic and not counted), plus per-launch constants.
+# Algorithmic HBM bytes per particle of each stage: the reference dataflow with the minimum
+# materialisation its API exposes (sorted buffers, index and state), each array touched once;
+# neighbour re-reads are cache traffic and not counted. 240 B per particle in all, plus 4 MiB per step
+# (the 65,536-entry clear and one read of both 2^18 bucket tables).
 STAGE_BYTES_PER_PARTICLE = {
-    "hash": 16 + 4,                     # R pos, W key
-    "sort": 16 + 20,                    # 2 passes of 9 bits: counts R 4, scatter R 4/8 W 8 (index generated)
-    "reorder": 4 + 32 + 32 + 4 + 4 + 1, # R index, R pos/vel, W sorted pos/vel, W inverse, R key,
-                                        # W bucket start/end (one occupied bucket per ~8 particles)
-    "bounds": 0,                        # (done in the reorder stage)
-    "density": 16 + 8,                  # R sorted pos, W density/pressure lanes
-    "forces": 16 + 16 + 4 + 48 + 48,    # R sorted pos/vel, index; R+W force/vel/pos in caller order
+    "hash": 16 + 4 + 4,                 # R pos, W key, W index
+    "sort": 3 * (8 + 8) + 4,            # 24-bit keys, 3 passes of 8 bits, one histogram read
+    "reorder": 4 + 32 + 32,             # R index, R pos/vel, W sorted pos/vel
+    "bounds": 4,                        # R key
+    "density": 16 + 8,                  # R sorted pos, W density/pressure
+    "forces": 32 + 4 + 32,              # R sorted pos/vel, R index, W pos/vel
 }
+STEP_BYTES_PER_LAUNCH = 4 << 20
 STAGE_BYTES_PER_LAUNCH = {
-    "hash": 2 * 65536 * 4,              # reference bucket clear
     "density": 2 * 262144 * 4,          # one read of both bucket tables
     "forces": 2 * 262144 * 4,
 }
+TUNING_KEYS = ("neighbour_pass", "density_lanes", "tile_depth", "block_waves", "sort_items", "force_chunks")
 
 
 def _profiled(name: str, workload: str, tuning: dict):
-    """A committed counter summary's table for a workload, when it was measured with the same kernels
-    (neighbour pass and lanes per particle) as `tuning`."""
+    """A committed counter summary's table for a workload, when it was measured at exactly the launch
+    shape `tuning` (all six tuning fields and the traversal order)."""
     try:
         with open(os.path.join(ROOT, "profiles", name)) as fh:
             table = json.load(fh)
     except (OSError, ValueError):
         return None
     shape = table.get("shape", {})
-    if any(shape.get(k) != tuning.get(k) for k in ("neighbour_pass", "density_lanes")):
+    if any(shape.get(k) != tuning.get(k) for k in TUNING_KEYS + ("traversal",)):
         return None
     return table.get(workload)
 
@@ -63,26 +67,52 @@ def measured_traffic(stage: str, workload: str, tuning: dict):
 
 
 def measured_valu(stage: str, workload: str, tuning: dict):
-    """VALU busy fraction of a stage's kernel from the committed rocprofv3 SQ counter summary
-    (profiles/valu.json); None when that kernel and workload were not measured."""
+    """VALU wave64 instructions per launch of a stage's kernel from the committed rocprofv3 SQ counter
+    summary (profiles/valu.json: SQ_INSTS_VALU per dispatch); None when that kernel and workload were not
+    measured at this launch shape."""
     t = _profiled("valu.json", workload, tuning)
-    return t[stage]["valu_busy"] if t and stage in t else None
+    if not t or stage not in t:
+        return None
+    return t[stage]["valu_per_wave"] * t[stage]["waves"]
 
 
-def cpu_baseline(sc, prm, steps: int, threads: int) -> dict:
-    """The oracle (a plain-C OpenMP restatement of the reference kernels) on the host cores."""
+def progress(msg: str) -> None:
+    print(f"bench: {msg}", file=sys.stderr, flush=True)
+
+
+def host_cores() -> tuple:
+    """(cores this process may run on, cores of the host)."""
+    try:
+        mine = len(os.sched_getaffinity(0))
+    except (AttributeError, OSError):
+        mine = os.cpu_count() or 1
+    return mine, os.cpu_count() or mine
+
+
+def cpu_baseline(sc, prm, steps: int, steps_one_core: int) -> dict:
+    """The oracle (a plain-C OpenMP restatement of the reference kernels) on every core this process may
+    run on, then on one core, each over a bounded number of steps from the workload's initial state."""
     from oracle import oracle as O
     O.build()
-    used = O.set_threads(threads)
+    mine, total = host_cores()
     op = O.Params(*prm.values())
-    fo = O.Fluid(sc.pos, sc.vel, sc.force, op, box_min=sc.box_min, box_max=sc.box_max)
-    fo.step()  # warm-up
-    t0 = time.perf_counter()
-    fo.steps(steps)
-    dt = time.perf_counter() - t0
+
+    def run(threads: int, k: int):
+        used = O.set_threads(threads)
+        fo = O.Fluid(sc.pos, sc.vel, sc.force, op, box_min=sc.box_min, box_max=sc.box_max)
+        fo.step()  # warm-up
+        t0 = time.perf_counter()
+        fo.steps(k)
+        return used, time.perf_counter() - t0
+
+    used, dt = run(mine, steps)
+    _, dt1 = run(1, steps_one_core)
     return {"value": sc.n * steps / dt, "unit": "particle-steps/s", "cores": used, "kind": "port",
+            "host_cores": total, "one_core_value": sc.n * steps_one_core / dt1,
             "sample": f"{steps} steps of the same {sc.n}-particle workload after 1 warm-up step, "
-                      f"oracle/sph_oracle.c with {used} OpenMP threads ({dt:.1f} s)"}
+                      f"oracle/sph_oracle.c with {used} OpenMP threads ({dt:.1f} s; the host has {total} cores, "
+                      f"{mine} of them available to this process); one_core_value: {steps_one_core} steps on "
+                      f"1 thread ({dt1:.1f} s)"}
 
 
 def batched_leg(steps: int, seq_steps: int, num_scenes: int = 256) -> dict:
@@ -255,7 +285,7 @@ def collide_leg(launches: int = 20, cpu: bool = True) -> dict:
     if cpu:
         from oracle import oracle as O
         O.build()
-        threads = O.set_threads(min(os.cpu_count() or 1, 16))
+        threads = O.set_threads(host_cores()[0])
         nodes = b.nodes()
         pos, vel, force = sc.pos.copy(), sc.vel.copy(), sc.force.copy()
         op = O.Params(*prm.values())
@@ -297,7 +327,7 @@ def next_leg(steps: int = 50, cpu: bool = True) -> dict:
     if cpu:
         from oracle import oracle as O
         O.build()
-        threads = O.set_threads(min(os.cpu_count() or 1, 16))
+        threads = O.set_threads(host_cores()[0])
         fo = O.Next(st, O.next_default_params())
         fo.step()
         t0 = time.perf_counter()
@@ -359,7 +389,7 @@ def small_leg(steps: int = 1000, reps: int = 3, cpu: bool = True) -> dict:
     if cpu:
         from oracle import oracle as O
         O.build()
-        threads = O.set_threads(min(os.cpu_count() or 1, 16))
+        threads = O.set_threads(host_cores()[0])
         op = O.Params(*prm.values())
         fo = O.Fluid(sc.pos, sc.vel, sc.force, op, box_min=sc.box_min, box_max=sc.box_max)
         t0 = time.perf_counter()
@@ -444,7 +474,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=10)
     ap.add_argument("--config", default="1m", choices=["1m", "small"])
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--cpu-steps", type=int, default=80, help="oracle steps timed for the CPU baseline (~10 s)")
+    ap.add_argument("--cpu-steps", type=int, default=80, help="oracle steps timed on all cores (~10 s)")
+    ap.add_argument("--cpu-steps-one-core", type=int, default=6, help="oracle steps timed on one core (~10 s)")
     ap.add_argument("--order", default="morton", choices=["sorted", "caller", "morton"])
     ap.add_argument("--no-batched", action="store_true")
     ap.add_argument("--no-adjoint", action="store_true")
@@ -459,12 +490,15 @@ def main():
     ap.add_argument("--pass", dest="npass", default="tile_kept", choices=["tile_kept", "tile_list", "tile", "list"],
                     help="neighbour passes: LDS-tile density + kept-entry forces, LDS-tile density + recorded-visit "
                          "forces, LDS tiles for both, or recorded visit lists for both")
-    ap.add_argument("--tile-depth", type=int, default=192)
-    ap.add_argument("--lanes", type=int, default=4, help="lanes per particle in the density pass")
-    ap.add_argument("--force-chunks", type=int, default=2, help="list chunks per trip in the force pass")
-    ap.add_argument("--block-waves", type=int, default=2)
-    ap.add_argument("--no-autotune", action="store_true",
-                    help="keep the given launch shape instead of timing the candidates first")
+    ap.add_argument("--tile-depth", type=int, default=None, help="default: the library's")
+    ap.add_argument("--lanes", type=int, default=None, help="lanes per particle in the tile passes")
+    ap.add_argument("--force-chunks", type=int, default=None, help="list chunks per trip in the force pass")
+    ap.add_argument("--block-waves", type=int, default=None)
+    ap.add_argument("--autotune", action="store_true",
+                    help="time the candidate launch shapes first and keep the fastest (the timed shape then "
+                         "may differ from the one the committed counter profiles were taken at)")
+    ap.add_argument("--trajectory-steps", type=int, default=200,
+                    help="also time the workload's whole trajectory from its initial state (0 = skip)")
     ap.add_argument("--batched-steps", type=int, default=100)
     ap.add_argument("--sequential-steps", type=int, default=5)
     args = ap.parse_args()
@@ -473,7 +507,7 @@ def main():
               file=sys.stderr)
         sys.exit(2)
 
-    import numpy as np  # noqa: F401
+    import numpy as np
     import torch
 
     import libcl_amd as L
@@ -508,20 +542,53 @@ def main():
     f.setBox(sc.box_min, sc.box_max)
     assert f.upload(sc.pos, sc.vel, sc.force) == 1
     assert f.setTraversal(args.order, 16) == 1
-    assert f.setTuning(neighbour_pass=args.npass, tile_depth=args.tile_depth, block_waves=args.block_waves,
-                       density_lanes=args.lanes, force_chunks=args.force_chunks) == 1, \
-        L.last_error()
+    shape = {"neighbour_pass": args.npass}
+    for key, val in (("tile_depth", args.tile_depth), ("block_waves", args.block_waves),
+                     ("density_lanes", args.lanes), ("force_chunks", args.force_chunks)):
+        if val is not None:
+            shape[key] = val
+    assert f.setTuning(**shape) == 1, L.last_error()
     stream = torch.cuda.Stream()
 
     tuned = None
-    if not args.no_autotune:
+    if args.autotune:
         res = f.autotune(steps=3, stream=stream)
         assert res is not None, L.last_error()
         tuned = {"ms_per_step": res[1], "pick": res[0]}
+    tuning = dict(f.getTuning(), traversal=args.order)
+
+    # the whole trajectory from the initial state: HIP events on the step stream at steps 0, 1, 10, 100
+    # and the last (each step's cost grows as the flow develops)
+    trajectory = None
+    empty = np.full(262144, 0xFFFFFFFF, np.uint32)
+    if args.trajectory_steps > 0:
+        # one untimed step first (the program's first step sets up its buffers), then the initial state again
+        assert f.compute(stream) == 1, L.last_error()
+        assert f.upload(sc.pos, sc.vel, sc.force) == 1
+        assert f.bfCellStart.write(empty) == 1 and f.bfCellEnd.write(empty) == 1
+        torch.cuda.synchronize()
+        marks = sorted({0, 1, 10, 100, args.trajectory_steps} & set(range(args.trajectory_steps + 1)))
+        events = {}
+        for k in range(args.trajectory_steps + 1):
+            if k in marks:
+                events[k] = torch.cuda.Event(enable_timing=True)
+                events[k].record(stream)
+            if k < args.trajectory_steps:
+                assert f.compute(stream) == 1, L.last_error()
+        torch.cuda.synchronize()
+        total_ms = events[marks[0]].elapsed_time(events[marks[-1]])
+        trajectory = {"steps": args.trajectory_steps, "ms_per_step": total_ms / args.trajectory_steps,
+                      "particle_steps_per_s": n * args.trajectory_steps / (total_ms / 1e3),
+                      "segments_ms_per_step": {f"{a}-{b}": events[a].elapsed_time(events[b]) / (b - a)
+                                               for a, b in zip(marks[:-1], marks[1:])}}
+        assert f.upload(sc.pos, sc.vel, sc.force) == 1
+        assert f.bfCellStart.write(empty) == 1 and f.bfCellEnd.write(empty) == 1
+        torch.cuda.synchronize()
+
     for _ in range(args.warmup):
         assert f.compute(stream) == 1, L.last_error()
     stream.synchronize()
-    # every timed leg starts from this state (the flow evolves, and later states cost more per step)
+    # every timed leg starts from this state
     snap_state, snap_tables = f.state(), f.internals()
 
     def restore():
@@ -571,8 +638,21 @@ def main():
 
     avg_s = dom_ms / max(dom_cnt, 1) / 1e3
     alg_bytes = STAGE_BYTES_PER_PARTICLE[dominant] * n + STAGE_BYTES_PER_LAUNCH.get(dominant, 0)
-    achieved = alg_bytes / avg_s / 1e9
-    step_bytes = sum(STAGE_BYTES_PER_PARTICLE.values()) * n + sum(STAGE_BYTES_PER_LAUNCH.values())
+    hbm_achieved = alg_bytes / avg_s / 1e9
+    step_bytes = sum(STAGE_BYTES_PER_PARTICLE.values()) * n + STEP_BYTES_PER_LAUNCH
+    traffic = measured_traffic(dominant, sc.name, tuning)
+    valu_instr = measured_valu(dominant, sc.name, tuning)
+    hbm = {"achieved": hbm_achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": hbm_achieved / HBM_PEAK_GBS,
+           "alg_bytes_per_launch": alg_bytes, "traffic": traffic}
+    if valu_instr is not None and valu_instr / avg_s / 1e9 / VALU_PEAK_G > hbm["frac"]:
+        # counters taken at this launch shape: the kernel's binding resource is VALU issue
+        achieved = valu_instr / avg_s / 1e9
+        roofline = {"bound": "valu", "kernel": dominant, "achieved": achieved, "peak": VALU_PEAK_G,
+                    "unit": "G wave64 VALU instr/s", "frac": achieved / VALU_PEAK_G, "traffic": traffic,
+                    "valu_instr_per_launch": valu_instr, "avg_launch_us": avg_s * 1e6, "hbm": hbm}
+    else:
+        roofline = dict(hbm, bound="hbm", kernel=dominant, avg_launch_us=avg_s * 1e6,
+                        valu_instr_per_launch=valu_instr)
 
     out = {
         "metric": "SPH particle-steps/s @1M fp32 + % HBM peak; batched gain; adjoint/fwd time",
@@ -589,17 +669,17 @@ def main():
         "data": "synthetic (seeded jittered lattice, SplitMix64)",
         "config": {"workload": sc.name, "particles": n, "cell_size": 0.02, "box": [0.0, 4.0],
                    "buckets": 262144, "bucket_clear": 65536, "traversal": args.order,
-                   "tuning": f.getTuning(), "autotuned": not args.no_autotune},
-        "roofline": {"bound": "hbm", "kernel": dominant, "achieved": achieved, "peak": HBM_PEAK_GBS,
-                     "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
-                     "traffic": measured_traffic(dominant, sc.name, f.getTuning()),
-                     "valu_busy": measured_valu(dominant, sc.name, f.getTuning()),
-                     "avg_launch_us": avg_s * 1e6, "alg_bytes_per_launch": alg_bytes},
+                   "tuning": f.getTuning(), "autotuned": bool(args.autotune),
+                   "timed_from_step": args.warmup},
+        "roofline": roofline,
         "stage_us": {k: v * 1e3 for k, v in stage_ms.items()},
+        "step_alg_bytes": step_bytes,
         "step_hbm_frac": step_bytes / (ms_per_step / 1e3) / 1e9 / HBM_PEAK_GBS,
         "graph_ms_per_step": graph_ms,
         "graph_particle_steps_per_s": n / (graph_ms / 1e3),
     }
+    if trajectory:
+        out["trajectory"] = trajectory
     if tuned:
         out["autotune"] = tuned
     if args.calibrate_copy:
@@ -609,21 +689,19 @@ def main():
             dst.copy_(src)
         torch.cuda.synchronize()
         out["calibration_copy_bytes"] = src.numel() * 4
-    if not args.no_batched:
-        out["batched"] = batched_leg(args.batched_steps, args.sequential_steps)
-    if not args.no_adjoint:
-        out["adjoint"] = adjoint_leg()
-    if not args.no_collide:
-        out["collide"] = collide_leg(cpu=not args.no_cpu)
-    if not args.no_next:
-        out["next"] = next_leg(cpu=not args.no_cpu)
-    if not args.no_sort:
-        out["sort"] = sort_leg(cpu=not args.no_cpu)
-    if not args.no_small:
-        out["small"] = small_leg(cpu=not args.no_cpu)
-    if not args.no_cpu:
-        nthreads = min(os.cpu_count() or 1, 16)
-        out["cpu_baseline"] = cpu_baseline(sc, prm, args.cpu_steps, nthreads)
+    legs = (("batched", args.no_batched, lambda: batched_leg(args.batched_steps, args.sequential_steps)),
+            ("adjoint", args.no_adjoint, adjoint_leg),
+            ("collide", args.no_collide, lambda: collide_leg(cpu=not args.no_cpu)),
+            ("next", args.no_next, lambda: next_leg(cpu=not args.no_cpu)),
+            ("sort", args.no_sort, lambda: sort_leg(cpu=not args.no_cpu)),
+            ("small", args.no_small, lambda: small_leg(cpu=not args.no_cpu)),
+            ("cpu_baseline", args.no_cpu, lambda: cpu_baseline(sc, prm, args.cpu_steps, args.cpu_steps_one_core)))
+    progress(f"headline: {ms_per_step:.4f} ms per step")
+    for name, skip, leg in legs:
+        if not skip:
+            t0 = time.perf_counter()
+            out[name] = leg()
+            progress(f"{name}: {time.perf_counter() - t0:.1f} s")
     print(json.dumps(out), flush=True)
 
 

@@ -232,7 +232,7 @@ __device__ __noinline__ BVisitSums b_density_walk(int cx, int cy, int cz, float4
 // each cell's kept entries, the lanes' acceptance bits and the per-particle meta for k_b_forces_kept
 // (TILE_KEPT), with the same layout as the single-scene pass.
 template <int W, bool KEPT>
-__global__ __launch_bounds__(kThreads) void k_b_density_tile(const float4* __restrict__ pos, float4* __restrict__ spos,
+__global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(5))) void k_b_density_tile(const float4* __restrict__ pos, float4* __restrict__ spos,
                                                              float4* __restrict__ svel,
                                                              const uint32_t* __restrict__ cs_all,
                                                              const uint32_t* __restrict__ ce_all,
@@ -243,8 +243,8 @@ __global__ __launch_bounds__(kThreads) void k_b_density_tile(const float4* __res
                                                              const uint32_t* __restrict__ inv,
                                                              uint32_t* __restrict__ nbr, uint32_t* __restrict__ ncount,
                                                              uint32_t* __restrict__ kept_out,
-                                                             uint64_t* __restrict__ hitmask, uint32_t* __restrict__ meta,
-                                                             const float* __restrict__ cuts)
+                                                             uint64_t* __restrict__ hitmask, uint64_t* __restrict__ hitx,
+                                                             uint32_t* __restrict__ meta, const float* __restrict__ cuts)
 {
     constexpr uint32_t P = W / 8;
     constexpr uint32_t SEGS = kWave / W;
@@ -288,51 +288,56 @@ __global__ __launch_bounds__(kThreads) void k_b_density_tile(const float4* __res
     while (__ballot(todo != 0) != 0) {
         const SegCell c = next_cell_seg<W>(L, todo, live, gx, gy, gz, pi);
         todo &= ~L.bits(__ballot(c.member));
-        const uint32_t kept = stage_seg<W>(L, c, h2 * kKeepMargin, sp, cs, ce, n_s, depth, tile, aux);
-        // trips of the wave: the longest of its segments that test from LDS (taken while every lane is active)
-        const uint32_t wtrips = wave_trips<W>(kept <= depth ? kept : 0u);
-        if (kept <= depth) {
-            const uint32_t trips = (kept + P - 1) / P;
-            pad_tile<W>(L, tile, kept, wtrips * P);
-            wave_sync();
-            float a = 0.f;
-            uint64_t hits = accept_sweep<P>(tile + slot, wtrips, pi, il, h2, cut, a);
-            hits >>= min(wtrips - trips, 63u);  // trips = 0 leaves no bit set
-            if constexpr (KEPT) {
+        const uint32_t total = stage_seg_ranges<W>(L, c, cs, ce, n_s, aux);
+        const uint32_t kept = total == kStageOverflow ? kStageOverflow
+                                                      : stage_seg_fill<W>(L, total, c.lo, c.hi, h2 * kKeepMargin, sp, 0u, depth, tile, aux);
+        if constexpr (KEPT) {
+            // the cell's kept entries go after the segment's earlier cells when they all fit
+            if (kept != kStageOverflow && used + kept > kSegCap && c.member && slot == 0) meta[t] = kMetaWalk;
+        }
+        uint32_t at_base = 0;  // !KEPT: the particle's visits recorded in earlier windows
+        const bool fits = sweep_windows<W>(
+            L, kept, h2 * kKeepMargin, sp, depth, tile, aux, pi, il, h2, cut,
+            [&](uint32_t w, uint32_t nw, uint32_t trips, uint64_t hits, float a) {
                 if (c.member) {
                     acc += a;
                     count += (uint32_t)__popcll(hits);
                 }
-                if (used + kept <= kSegCap) {
-                    for (uint32_t q = L.ls; q < kept; q += W)
-                        kept_out[(size_t)gseg * kSegCap + used + q] = __float_as_uint(tile[q].w);
-                    if (c.member) {
-                        hitmask[(size_t)t * P + slot] = hits;
-                        if (slot == 0) meta[t] = used | (kept << 16);
+                if constexpr (KEPT) {
+                    if (used + kept <= kSegCap) {
+                        for (uint32_t q = L.ls; q < nw; q += W)
+                            kept_out[(size_t)gseg * kSegCap + used + w * depth + q] = __float_as_uint(tile[q].w);
+                        if (c.member) {
+                            if (w == 0) hitmask[(size_t)t * P + slot] = hits;
+                            else hitx[hitx_at(t, P, slot, w)] = hits;
+                        }
                     }
-                    used += kept;
-                } else if (c.member && slot == 0) {
-                    meta[t] = kMetaWalk;
-                }
-            } else if (c.member) {
-                acc += a;
-                const uint32_t mine = (uint32_t)__popcll(hits);
-                count += mine;
-                uint32_t at = mine;
+                } else if (c.member) {
+                    const uint32_t mine = (uint32_t)__popcll(hits);
+                    uint32_t at = mine;
 #pragma unroll
-                for (uint32_t o = 1; o < P; o <<= 1) {
-                    const uint32_t up = (uint32_t)__shfl_up((int)at, o, P);
-                    if (slot >= o) at += up;
+                    for (uint32_t o = 1; o < P; o <<= 1) {
+                        const uint32_t up = (uint32_t)__shfl_up((int)at, o, P);
+                        if (slot >= o) at += up;
+                    }
+                    const uint32_t all = (uint32_t)__shfl((int)at, (int)(L.lane | (P - 1)), kWave);  // the particle's total
+                    at += at_base - mine;
+                    at_base += all;
+                    while (hits) {
+                        const uint32_t m = trips - 1u - (uint32_t)__builtin_ctzll(hits);
+                        hits &= hits - 1ull;
+                        if (at < kNbrCap) nbr[nbr_at(i, at, N)] = __float_as_uint(tile[m * P + slot].w);
+                        at++;
+                    }
                 }
-                at -= mine;
-                while (hits) {
-                    const uint32_t m = trips - 1u - (uint32_t)__builtin_ctzll(hits);
-                    hits &= hits - 1ull;
-                    if (at < kNbrCap) nbr[nbr_at(i, at, N)] = __float_as_uint(tile[m * P + slot].w);
-                    at++;
-                }
+            });
+        if constexpr (KEPT) {
+            if (fits && used + kept <= kSegCap) {
+                if (c.member && slot == 0) meta[t] = used | (kept << 16);
+                used += kept;
             }
-        } else if (c.member) {
+        }
+        if (!fits && c.member) {
             if constexpr (KEPT) {  // the particle's P lanes split the walk (density_walk_split, as alone)
                 if (slot == 0) meta[t] = kMetaWalk;
                 const WalkSums f = density_walk_split(c.cx, c.cy, c.cz, pi, il, sp, cs, ce, n_s, h, slot, P);
@@ -378,10 +383,11 @@ __global__ __launch_bounds__(kThreads) void k_b_forces_kept(const float4* __rest
                                                             const uint32_t* __restrict__ pscene,
                                                             const mfl_params* __restrict__ params,
                                                             const float4* __restrict__ boxes, uint32_t N,
-                                                            const uint32_t* __restrict__ porder,
+                                                            uint32_t depth, const uint32_t* __restrict__ porder,
                                                             const uint32_t* __restrict__ inv,
                                                             const uint32_t* __restrict__ kept_in,
                                                             const uint64_t* __restrict__ hitmask,
+                                                            const uint64_t* __restrict__ hitx,
                                                             const uint32_t* __restrict__ meta, float4 gravity_force)
 {
     constexpr uint32_t P = W / 8;
@@ -389,8 +395,9 @@ __global__ __launch_bounds__(kThreads) void k_b_forces_kept(const float4* __rest
     extern __shared__ float4 lds[];
     const Lanes<W> L(threadIdx.x & (kWave - 1));
     const uint32_t sid = (threadIdx.x / kWave) * SEGS + L.seg;
-    float4* tpos = lds + (size_t)sid * 2 * kKeptChunk;
-    float4* tvel = tpos + kKeptChunk;
+    constexpr uint32_t chunk = kKeptChunk;  // small LDS per segment keeps more waves resident
+    float4* tpos = lds + (size_t)sid * 2 * chunk;
+    float4* tvel = tpos + chunk;
     const uint32_t grp = L.ls / P, slot = L.ls % P;
     const uint32_t wave = (xcd_block() * blockDim.x + threadIdx.x) / kWave;
     const uint32_t t = wave * (kWave / P) + L.seg * 8 + grp;
@@ -423,46 +430,30 @@ __global__ __launch_bounds__(kThreads) void k_b_forces_kept(const float4* __rest
         const bool member = active && staged && mt == gm;
         todo &= ~L.bits(__ballot(member));
         const uint32_t offk = active ? (gm & 0xFFFFu) : 0u, kept = active ? (gm >> 16) : 0u;
-        const uint32_t* kj = kept_in + (size_t)gseg * kSegCap + offk;
         const uint32_t kmax = seg_wave_max<W>(kept);
-        const uint32_t trips = (kept + P - 1) / P;
-        for (uint32_t c0 = 0; c0 < kmax; c0 += kKeptChunk) {
-            for (uint32_t q0 = 0; q0 < kKeptChunk; q0 += 2 * W) {
-                const uint32_t qa = q0 + L.ls, qb = qa + W;
-                const bool ha = c0 + qa < kept, hb = qb < kKeptChunk && c0 + qb < kept;
-                const uint32_t ja = ha ? kj[c0 + qa] : 0u, jb = hb ? kj[c0 + qb] : 0u;
-                float4 pa, va, pb, vb;
-                if (ha) {
-                    pa = sp[ja];
-                    va = sv[ja];
+        // the density pass's windows of `depth` entries, each with its own word of acceptance bits
+        for (uint32_t w0 = 0, w = 0; w0 < kmax; w0 += depth, w++) {
+            const uint32_t nw = kept > w0 ? min(kept - w0, depth) : 0u, nmax = min(kmax - w0, depth);
+            const uint32_t trips = (nw + P - 1) / P;
+            const uint64_t hw = w == 0 ? hm : (member && nw ? hitx[hitx_at(t, P, slot, w)] : 0ull);
+            const uint32_t* kj = kept_in + (size_t)gseg * kSegCap + offk + w0;
+            for (uint32_t c0 = 0; c0 < nmax; c0 += chunk) {
+                stage_kept_chunk<W>(L, kept_in + (size_t)gseg * kSegCap, kj, nw, c0, sp, sv, tpos, tvel);
+                wave_sync();
+                if (member) {
+                    uint64_t bits = chunk_bits<P>(hw, trips, c0, chunk);
+                    const float4* rp = tpos + slot;  // this lane's entries: k P + slot of the chunk
+                    const float4* rv = tvel + slot;
+                    while (bits) {
+                        const uint32_t k = (uint32_t)__builtin_clzll(bits);  // earliest trip first
+                        bits ^= 0x8000000000000000ull >> k;
+                        const float4 pj = rp[k * P], vj = rv[k * P];
+                        const float dx = pi.x - pj.x, dy = pi.y - pj.y, dz = pi.z - pj.z;
+                        pair_force(pi, vi, pj, vj, dx, dy, dz, dist2(dx, dy, dz), h, Pa, Va);
+                    }
                 }
-                if (hb) {
-                    pb = sp[jb];
-                    vb = sv[jb];
-                }
-                if (ha) {
-                    tpos[qa] = pa;
-                    tvel[qa] = va;
-                }
-                if (hb) {
-                    tpos[qb] = pb;
-                    tvel[qb] = vb;
-                }
+                wave_sync();
             }
-            wave_sync();
-            if (member) {
-                uint32_t w = chunk_bits<P>(hm, trips, c0);
-                const float4* rp = tpos + slot;  // this lane's entries: k P + slot of the chunk
-                const float4* rv = tvel + slot;
-                while (w) {
-                    const uint32_t k = (uint32_t)__builtin_clz(w);  // earliest trip first
-                    w ^= 0x80000000u >> k;
-                    const float4 pj = rp[k * P], vj = rv[k * P];
-                    const float dx = pi.x - pj.x, dy = pi.y - pj.y, dz = pi.z - pj.z;
-                    pair_force(pi, vi, pj, vj, dx, dy, dz, dist2(dx, dy, dz), h, Pa, Va);
-                }
-            }
-            wave_sync();
         }
     }
     if (live && !staged) {  // the particle's P lanes split the walk of a cell that overflowed its tile
@@ -732,7 +723,7 @@ int FluidBatch::enqueue_step(hipStream_t s)
     hipLaunchKernelGGL(k_scatter_sorted, g, b, 0, s, inv_.as<uint32_t>(), pos, vel, spos, svel, N_);
     const bool tiles = (neighbour_pass == MFL_PASS_TILE_LIST || neighbour_pass == MFL_PASS_TILE_KEPT) && porder_valid_;
     const bool kept = tiles && neighbour_pass == MFL_PASS_TILE_KEPT;
-    if (kept && !(kept_.ptr && hitmask_.ptr && meta_.ptr)) {
+    if (kept && !(kept_.ptr && hitmask_.ptr && hitx_.ptr && meta_.ptr)) {
         set_error("kept-entry buffers are not allocated");
         return 0;
     }
@@ -747,7 +738,8 @@ int FluidBatch::enqueue_step(hipStream_t s)
     hipLaunchKernelGGL((k_b_density_tile<W, K>), dim3((unsigned)blocks), dim3(block_waves * kWave), lds, s, pos, spos, \
                        svel, cs_all, ce_all, off, pscene_.as<uint32_t>(), params, N_, tile_depth,                     \
                        porder_.as<uint32_t>(), inv_.as<uint32_t>(), nbr_.as<uint32_t>(), ncount_.as<uint32_t>(),     \
-                       kept_.as<uint32_t>(), hitmask_.as<uint64_t>(), meta_.as<uint32_t>(), cuts_.as<float>())
+                       kept_.as<uint32_t>(), hitmask_.as<uint64_t>(), hitx_.as<uint64_t>(), meta_.as<uint32_t>(),   \
+                       cuts_.as<float>())
         if (lanes == 2) {
             if (kept) MFL_B_TILE(16, true);
             else MFL_B_TILE(16, false);
@@ -772,8 +764,8 @@ int FluidBatch::enqueue_step(hipStream_t s)
 #define MFL_B_KEPT(W)                                                                                              \
     hipLaunchKernelGGL(k_b_forces_kept<W>, dim3((unsigned)blocks), dim3(block_waves * kWave), lds_k, s, spos, svel,  \
                        pos, vel, force, cs_all, ce_all, off, pscene_.as<uint32_t>(), params, boxes_.as<float4>(), N_, \
-                       porder_.as<uint32_t>(), inv_.as<uint32_t>(), kept_.as<uint32_t>(), hitmask_.as<uint64_t>(),  \
-                       meta_.as<uint32_t>(), gravity_force_of(gravity))
+                       tile_depth, porder_.as<uint32_t>(), inv_.as<uint32_t>(), kept_.as<uint32_t>(), hitmask_.as<uint64_t>(),  \
+                       hitx_.as<uint64_t>(), meta_.as<uint32_t>(), gravity_force_of(gravity))
         if (lanes == 2) MFL_B_KEPT(16);
         else if (lanes == 4) MFL_B_KEPT(32);
         else MFL_B_KEPT(64);
@@ -835,11 +827,12 @@ int FluidBatch::launch_graph(hipStream_t s)
 
 bool FluidBatch::fit_kept()
 {
-    if (kept_.ptr && hitmask_.ptr && meta_.ptr) return true;
-    // kept entries of every 8-particle segment, acceptance bits of every lane, meta per particle
-    const size_t segs = ((size_t)N_ + 7) / 8;
+    if (kept_.ptr && hitmask_.ptr && hitx_.ptr && meta_.ptr) return true;
+    // kept entries of every 8-particle segment, acceptance bits of every lane (kMaxWindows words for a
+    // neighbourhood tested in several windows), meta per particle
+    const size_t segs = ((size_t)N_ + 63) / 64 * 8;  // every segment of every wave of up to 64 particles
     return kept_.allocate(segs * kSegCap * 4) && hitmask_.allocate((size_t)N_ * 8 * sizeof(uint64_t)) &&
-           meta_.allocate((size_t)N_ * 4);
+           hitx_.allocate((size_t)N_ * 8 * (kMaxWindows - 1) * sizeof(uint64_t)) && meta_.allocate((size_t)N_ * 4);
 }
 
 int FluidBatch::compute(hipStream_t s)

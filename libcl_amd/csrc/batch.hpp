@@ -54,7 +54,7 @@ private:
     bool porder_valid_ = false;
     uint32_t steps_since_refresh_ = 0;
     DeviceBuffer inv_, porder_, mkey_;
-    DeviceBuffer kept_, hitmask_, meta_;  // TILE_KEPT: per-segment kept entries, acceptance bits, meta
+    DeviceBuffer kept_, hitmask_, hitx_, meta_;  // TILE_KEPT: per-segment kept entries, acceptance bits, meta
     bool fit_kept();                      // allocates them on first use
 
     bool ok_ = false;

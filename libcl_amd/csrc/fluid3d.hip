@@ -550,16 +550,17 @@ __device__ __noinline__ VisitSums density_walk(int cx, int cy, int cz, float4 pi
 }
 
 // Outputs of the tile density pass: none, recorded visit lists, or each cell's kept entries plus the
-// lanes' acceptance bits (kSegCap, kMetaWalk and kKeptChunk in tile.hpp).
+// lanes' acceptance bits (kSegCap, kMetaWalk, kMaxWindows in tile.hpp).
 enum : int { kOutNone = 0, kOutList = 1, kOutKept = 2 };
 
 // Density pass over LDS-staged neighbourhoods (cl:150-200). The wave is split into 64 / W segments of
 // 8 particles (W / 8 lanes each); each segment stages its cell's neighbourhood in its own `depth + 8`
-// float4 of dynamic LDS plus a StageAux. A neighbourhood that keeps more than `depth` entries (or holds
-// more than 512 visits) falls back to walking the ranges from global memory. LIST = true also records
+// float4 of dynamic LDS plus a StageAux. A neighbourhood that keeps more than `depth` entries is tested
+// `depth` kept entries at a time (sweep_windows); one that needs more than kMaxWindows windows, or holds
+// more than 512 visits, falls back to walking the ranges from global memory. LIST = true also records
 // each particle's accepted visits (nbr, ncount by traversal slot) for the recorded-visit force pass.
 template <int ORDER, int OUT, int W>
-__global__ __launch_bounds__(kThreads) void k_density_tile(const float4* __restrict__ pos, float4* __restrict__ spos,
+__global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(5))) void k_density_tile(const float4* __restrict__ pos, float4* __restrict__ spos,
                                                            float4* __restrict__ svel,
                                                            const uint32_t* __restrict__ cell_start,
                                                            const uint32_t* __restrict__ cell_end, uint32_t n,
@@ -567,8 +568,9 @@ __global__ __launch_bounds__(kThreads) void k_density_tile(const float4* __restr
                                                            const uint32_t* __restrict__ porder,
                                                            const uint32_t* __restrict__ inv, uint32_t* __restrict__ nbr,
                                                            uint32_t* __restrict__ ncount, uint32_t* __restrict__ kept_out,
-                                                           uint64_t* __restrict__ hitmask, uint32_t* __restrict__ meta,
-                                                           float cut, uint32_t* __restrict__ counts_out)
+                                                           uint64_t* __restrict__ hitmask, uint64_t* __restrict__ hitx,
+                                                           uint32_t* __restrict__ meta, float cut,
+                                                           uint32_t* __restrict__ counts_out)
 {
     constexpr bool LIST = OUT == kOutList;
     constexpr uint32_t P = W / 8;  // lanes per particle
@@ -595,56 +597,59 @@ __global__ __launch_bounds__(kThreads) void k_density_tile(const float4* __restr
     while (__ballot(todo != 0) != 0) {
         const SegCell c = next_cell_seg<W>(L, todo, live, gx, gy, gz, pi);
         todo &= ~L.bits(__ballot(c.member));
-        const uint32_t kept = stage_seg<W>(L, c, h2 * kKeepMargin, spos, cell_start, cell_end, n, depth, tile, aux);
-        // trips of the wave: the longest of its segments that test from LDS (taken while every lane is active)
-        const uint32_t wtrips = wave_trips<W>(kept <= depth ? kept : 0u);
-        if (kept <= depth) {
-            // entry P m + slot is tested by this lane in trip m (of `trips`); bit trips - 1 - m of `hits`
-            // records its acceptance (kept <= 64 P, so trips <= 64). The wave runs the longest segment's
-            // trip count; past its own end a lane reads an end marker.
-            const uint32_t trips = (kept + P - 1) / P;
-            pad_tile<W>(L, tile, kept, wtrips * P);
-            wave_sync();
-            float a = 0.f;
-            uint64_t hits = accept_sweep<P>(tile + slot, wtrips, pi, i, h2, cut, a);
-            // entries past a segment's end were end markers (never accepted): drop their bits
-            hits >>= min(wtrips - trips, 63u);  // trips = 0 leaves no bit set
-            if (c.member) {
-                acc += a;
-                count += (uint32_t)__popcll(hits);
-            }
-            if constexpr (OUT == kOutKept) {
-                // the cell's kept entries once for the segment, the acceptance bits per lane
-                if (used + kept <= kSegCap) {
-                    for (uint32_t q = L.ls; q < kept; q += W)
-                        kept_out[(size_t)gseg * kSegCap + used + q] = __float_as_uint(tile[q].w);
-                    if (c.member) {
-                        hitmask[(size_t)t * P + slot] = hits;
-                        if (slot == 0) meta[t] = used | (kept << 16);
+        const uint32_t total = stage_seg_ranges<W>(L, c, cell_start, cell_end, n, aux);
+        const uint32_t kept = total == kStageOverflow ? kStageOverflow
+                                                      : stage_seg_fill<W>(L, total, c.lo, c.hi, h2 * kKeepMargin, spos, 0u, depth, tile, aux);
+        if constexpr (OUT == kOutKept) {
+            // the cell's kept entries go after the segment's earlier cells when they all fit
+            if (kept != kStageOverflow && used + kept > kSegCap && c.member && slot == 0) meta[t] = kMetaWalk;
+        }
+        uint32_t at_base = 0;  // LIST: the particle's visits recorded in earlier windows
+        const bool fits = sweep_windows<W>(
+            L, kept, h2 * kKeepMargin, spos, depth, tile, aux, pi, i, h2, cut,
+            [&](uint32_t w, uint32_t nw, uint32_t trips, uint64_t hits, float a) {
+                if (c.member) {
+                    acc += a;
+                    count += (uint32_t)__popcll(hits);
+                }
+                if constexpr (OUT == kOutKept) {
+                    // the window's kept entries once for the segment, the acceptance bits per lane
+                    if (used + kept <= kSegCap) {
+                        for (uint32_t q = L.ls; q < nw; q += W)
+                            kept_out[(size_t)gseg * kSegCap + used + w * depth + q] = __float_as_uint(tile[q].w);
+                        if (c.member) {
+                            if (w == 0) hitmask[(size_t)t * P + slot] = hits;
+                            else hitx[hitx_at(t, P, slot, w)] = hits;
+                        }
                     }
-                    used += kept;
-                } else if (c.member && slot == 0) {
-                    meta[t] = kMetaWalk;
                 }
-            }
-            if (LIST && c.member) {
-                // this lane's accepted visits go after those of the particle's lower lanes
-                const uint32_t mine = (uint32_t)__popcll(hits);
-                uint32_t at = mine;
+                if (LIST && c.member) {
+                    // this lane's accepted visits go after those of the particle's earlier windows and lower lanes
+                    const uint32_t mine = (uint32_t)__popcll(hits);
+                    uint32_t at = mine;
 #pragma unroll
-                for (uint32_t o = 1; o < P; o <<= 1) {
-                    const uint32_t up = (uint32_t)__shfl_up((int)at, o, P);
-                    if (slot >= o) at += up;
+                    for (uint32_t o = 1; o < P; o <<= 1) {
+                        const uint32_t up = (uint32_t)__shfl_up((int)at, o, P);
+                        if (slot >= o) at += up;
+                    }
+                    const uint32_t all = (uint32_t)__shfl((int)at, (int)(L.lane | (P - 1)), kWave);  // the particle's total
+                    at += at_base - mine;
+                    at_base += all;
+                    while (hits) {
+                        const uint32_t m = trips - 1u - (uint32_t)__builtin_ctzll(hits);
+                        hits &= hits - 1ull;
+                        if (at < kNbrCap) nbr[nbr_at(t, at, n)] = __float_as_uint(tile[m * P + slot].w);
+                        at++;
+                    }
                 }
-                at -= mine;
-                while (hits) {
-                    const uint32_t m = trips - 1u - (uint32_t)__builtin_ctzll(hits);
-                    hits &= hits - 1ull;
-                    if (at < kNbrCap) nbr[nbr_at(t, at, n)] = __float_as_uint(tile[m * P + slot].w);
-                    at++;
-                }
+            });
+        if constexpr (OUT == kOutKept) {
+            if (fits && used + kept <= kSegCap) {
+                if (c.member && slot == 0) meta[t] = used | (kept << 16);
+                used += kept;
             }
-        } else if (c.member) {
+        }
+        if (!fits && c.member) {
             if (OUT == kOutKept && slot == 0) meta[t] = kMetaWalk;
             if constexpr (W == kWave) {
                 const VisitSums f = density_fallback(c.cx, c.cy, c.cz, pi, i, t, slot, grp, spos, cell_start, cell_end,
@@ -775,6 +780,7 @@ __global__ __launch_bounds__(kThreads) void k_forces_kept(const float4* __restri
                                                           const uint32_t* __restrict__ inv,
                                                           const uint32_t* __restrict__ kept_in,
                                                           const uint64_t* __restrict__ hitmask,
+                                                          const uint64_t* __restrict__ hitx,
                                                           const uint32_t* __restrict__ meta)
 {
     constexpr uint32_t P = W / 8;
@@ -782,8 +788,9 @@ __global__ __launch_bounds__(kThreads) void k_forces_kept(const float4* __restri
     extern __shared__ float4 lds[];
     const Lanes<W> L(threadIdx.x & (kWave - 1));
     const uint32_t sid = (threadIdx.x / kWave) * SEGS + L.seg;
-    float4* tpos = lds + (size_t)sid * 2 * kKeptChunk;
-    float4* tvel = tpos + kKeptChunk;
+    constexpr uint32_t chunk = kKeptChunk;  // small LDS per segment keeps more waves resident
+    float4* tpos = lds + (size_t)sid * 2 * chunk;
+    float4* tvel = tpos + chunk;
     const uint32_t grp = L.ls / P, slot = L.ls % P;
     const uint32_t wave = (xcd_block() * blockDim.x + threadIdx.x) / kWave;
     const uint32_t t = wave * (kWave / P) + L.seg * 8 + grp;
@@ -805,48 +812,31 @@ __global__ __launch_bounds__(kThreads) void k_forces_kept(const float4* __restri
         const bool member = active && staged && mt == gm;
         todo &= ~L.bits(__ballot(member));
         const uint32_t off = active ? (gm & 0xFFFFu) : 0u, kept = active ? (gm >> 16) : 0u;
-        const uint32_t* kj = kept_in + (size_t)gseg * kSegCap + off;
         const uint32_t kmax = seg_wave_max<W>(kept);
-        const uint32_t trips = (kept + P - 1) / P;
-        // kKeptChunk entries at a time (small LDS per segment keeps more waves resident)
-        for (uint32_t c0 = 0; c0 < kmax; c0 += kKeptChunk) {
-            for (uint32_t q0 = 0; q0 < kKeptChunk; q0 += 2 * W) {
-                // two entries per lane per trip: their indices, then their state, in flight together
-                const uint32_t qa = q0 + L.ls, qb = qa + W;
-                const bool ha = c0 + qa < kept, hb = qb < kKeptChunk && c0 + qb < kept;
-                const uint32_t ja = ha ? kj[c0 + qa] : 0u, jb = hb ? kj[c0 + qb] : 0u;
-                float4 pa, va, pb, vb;
-                if (ha) {
-                    pa = spos[ja];
-                    va = svel[ja];
+        // the density pass's windows of `depth` entries, each with its own word of acceptance bits
+        for (uint32_t w0 = 0, w = 0; w0 < kmax; w0 += depth, w++) {
+            const uint32_t nw = kept > w0 ? min(kept - w0, depth) : 0u, nmax = min(kmax - w0, depth);
+            const uint32_t trips = (nw + P - 1) / P;
+            const uint64_t hw = w == 0 ? hm : (member && nw ? hitx[hitx_at(t, P, slot, w)] : 0ull);
+            const uint32_t* kj = kept_in + (size_t)gseg * kSegCap + off + w0;
+            // kKeptChunk entries at a time
+            for (uint32_t c0 = 0; c0 < nmax; c0 += chunk) {
+                stage_kept_chunk<W>(L, kept_in + (size_t)gseg * kSegCap, kj, nw, c0, spos, svel, tpos, tvel);
+                wave_sync();
+                if (member) {
+                    uint64_t bits = chunk_bits<P>(hw, trips, c0, chunk);
+                    const float4* rp = tpos + slot;  // this lane's entries: k P + slot of the chunk
+                    const float4* rv = tvel + slot;
+                    while (bits) {
+                        const uint32_t k = (uint32_t)__builtin_clzll(bits);  // earliest trip first
+                        bits ^= 0x8000000000000000ull >> k;
+                        const float4 pj = rp[k * P], vj = rv[k * P];
+                        const float dx = pi.x - pj.x, dy = pi.y - pj.y, dz = pi.z - pj.z;
+                        pair_force(pi, vi, pj, vj, dx, dy, dz, dist2(dx, dy, dz), h, Pa, Va);
+                    }
                 }
-                if (hb) {
-                    pb = spos[jb];
-                    vb = svel[jb];
-                }
-                if (ha) {
-                    tpos[qa] = pa;
-                    tvel[qa] = va;
-                }
-                if (hb) {
-                    tpos[qb] = pb;
-                    tvel[qb] = vb;
-                }
+                wave_sync();  // the next chunk overwrites the tile
             }
-            wave_sync();
-            if (member) {
-                uint32_t w = chunk_bits<P>(hm, trips, c0);
-                const float4* rp = tpos + slot;  // this lane's entries: k P + slot of the chunk
-                const float4* rv = tvel + slot;
-                while (w) {
-                    const uint32_t k = (uint32_t)__builtin_clz(w);  // earliest trip first
-                    w ^= 0x80000000u >> k;
-                    const float4 pj = rp[k * P], vj = rv[k * P];
-                    const float dx = pi.x - pj.x, dy = pi.y - pj.y, dz = pi.z - pj.z;
-                    pair_force(pi, vi, pj, vj, dx, dy, dz, dist2(dx, dy, dz), h, Pa, Va);
-                }
-            }
-            wave_sync();  // the next chunk overwrites the tile
         }
     }
     if (live && !staged) {  // the particle's P lanes split the walk of a cell that overflowed its tile
@@ -1022,8 +1012,10 @@ bool Fluid3D::allocate_internal()
     if (!cell_start_.allocate(kBuckets * 4) || !cell_end_.allocate(kBuckets * 4)) return false;
     if (!nbr_.allocate((size_t)kNbrCap * u1) || !ncount_.allocate(u1)) return false;
     // kept entries of the tile density pass: kSegCap per 8-particle segment, 64-bit bits per lane
-    if (!kept_.allocate((u1 / 4 + 7) / 8 * kSegCap * 4) || !hitmask_.allocate(u1 / 4 * kGroupLanes * 8) ||
-        !meta_.allocate(u1))
+    // and kMaxWindows - 1 more words per lane for neighbourhoods tested in several windows
+    // (segments of every wave of up to 64 particles, so that any segment's region exists)
+    if (!kept_.allocate((n_ + 63) / 64 * 8 * kSegCap * 4) || !hitmask_.allocate(u1 / 4 * kGroupLanes * 8) ||
+        !hitx_.allocate(u1 / 4 * kGroupLanes * (kMaxWindows - 1) * 8) || !meta_.allocate(u1))
         return false;
     if (!inv_.allocate(u1) || !porder_.allocate(u1) || !mkey_.allocate(u1) || !counts_.allocate(u1)) return false;
     porder_valid_ = false;
@@ -1356,12 +1348,13 @@ int Fluid3D::enqueue_tile_passes(hipStream_t s, int order)
     auto* ncount = ncount_.as<uint32_t>();
     auto* kept = kept_.as<uint32_t>();
     auto* hitmask = hitmask_.as<uint64_t>();
+    auto* hitx = hitx_.as<uint64_t>();
     auto* meta = meta_.as<uint32_t>();
     const int out = neighbour_pass == MFL_PASS_TILE_LIST ? kOutList
                     : neighbour_pass == MFL_PASS_TILE_KEPT ? kOutKept : kOutNone;
 #define MFL_DENSITY_TILE(O, OUT, W)                                                                              \
     hipLaunchKernelGGL((k_density_tile<O, OUT, W>), gd, b, lds_d, s, pos, spos, svel, cs, ce, n, params, depth,   \
-                       porder, inv, nbr, ncount, kept, hitmask, meta, cut, counts_out)
+                       porder, inv, nbr, ncount, kept, hitmask, hitx, meta, cut, counts_out)
 #define MFL_DENSITY_TILE_W(O, OUT)                 \
     if (lanes == 2) MFL_DENSITY_TILE(O, OUT, 16); \
     else if (lanes == 4) MFL_DENSITY_TILE(O, OUT, 32); \
@@ -1385,7 +1378,7 @@ int Fluid3D::enqueue_tile_passes(hipStream_t s, int order)
         const size_t lds_k = (size_t)block_waves * (8 / lanes) * kKeptChunk * 2 * sizeof(float4);
 #define MFL_FORCES_KEPT(O, W, STEP)                                                                              \
     hipLaunchKernelGGL((k_forces_kept<O, W, STEP>), gd, b, lds_k, s, spos, svel, pos, vel, force, index, cs, ce, n, \
-                       params, depth, mn, mx, gf, porder, inv, kept, hitmask, meta)
+                       params, depth, mn, mx, gf, porder, inv, kept, hitmask, hitx, meta)
 #define MFL_FORCES_KEPT_W(O, STEP)                 \
     if (lanes == 2) MFL_FORCES_KEPT(O, 16, STEP); \
     else if (lanes == 4) MFL_FORCES_KEPT(O, 32, STEP); \

@@ -87,7 +87,7 @@ private:
     DeviceBuffer pos_, vel_, force_;
     DeviceBuffer cell_, index_, spos_, svel_, cell_start_, cell_end_, params_buf_;
     DeviceBuffer nbr_, ncount_;  // per-step neighbour visits recorded by the density pass
-    DeviceBuffer kept_, hitmask_, meta_;  // per-step kept entries and acceptance bits (TILE_KEPT)
+    DeviceBuffer kept_, hitmask_, hitx_, meta_;  // per-step kept entries and acceptance bits (TILE_KEPT)
     DeviceBuffer inv_;           // inverse of the sort permutation: inv[index[i]] = i
     DeviceBuffer counts_;        // accepted visits per sorted particle (written when record_counts)
     DeviceBuffer porder_, mkey_; // Morton traversal order (caller indices) and its sort keys

@@ -30,10 +30,13 @@ constexpr uint32_t kKeptChunk = 64;
 
 // Per-wave LDS bookkeeping of a staging: bit p % 64 of starts[p / 64] marks position p where a
 // non-empty bucket range begins, and delta[c] maps a position of the c-th non-empty range to its
-// sorted index (j = p + delta[c]).
+// sorted index (j = p + delta[c]). The segmented staging also keeps the cell's box (lo xyz, hi xyz) and
+// its number of visits there, for the windows staged after the first.
 struct StageAux {
     unsigned long long starts[kStageRounds];
     uint32_t delta[32];
+    float box[6];
+    uint32_t total;
 };
 
 __device__ inline void wave_sync()
@@ -300,11 +303,13 @@ __device__ inline SegCell next_cell_seg(const Lanes<W>& L, uint64_t todo, bool l
 }
 
 // Rounds r0 .. r0 + NR - 1 of a segment's staging (positions W r + ls): loads, box prune, in-order
-// compaction of the kept entries into the tile as float4(x, y, z, bits of j).
+// compaction of the kept entries into the tile as float4(x, y, z, bits of j). Kept entry k goes to tile
+// slot k - skip when skip <= k < skip + depth (one window of the kept list).
 template <int W, int NR>
 __device__ inline void stage_seg_rounds(const Lanes<W>& L, uint32_t r0, uint32_t total, uint64_t upto, float3 lo,
-                                        float3 hi, float h2_keep, const float4* __restrict__ spos, uint32_t depth,
-                                        float4* tile, const StageAux* aux, uint32_t& kept, uint32_t& before)
+                                        float3 hi, float h2_keep, const float4* __restrict__ spos, uint32_t skip,
+                                        uint32_t depth, float4* tile, const StageAux* aux, uint32_t& kept,
+                                        uint32_t& before)
 {
     float4 v[NR];
     uint32_t jv[NR], cr[NR];
@@ -327,20 +332,19 @@ __device__ inline void stage_seg_rounds(const Lanes<W>& L, uint32_t r0, uint32_t
         const uint32_t q = (r0 + r) * W + L.ls;
         has[r] = q < total;
         jv[r] = q + cr[r];
-        if (has[r]) v[r] = spos[jv[r]];
+        v[r] = spos[has[r] ? jv[r] : 0u];  // lanes past the end load entry 0 (n >= 1) and keep nothing
     }
 #pragma unroll
     for (int r = 0; r < NR; r++) {
-        bool keep = false;
-        if (has[r]) {
-            const float ex = fmaxf(fmaxf(lo.x - v[r].x, v[r].x - hi.x), 0.f);
-            const float ey = fmaxf(fmaxf(lo.y - v[r].y, v[r].y - hi.y), 0.f);
-            const float ez = fmaxf(fmaxf(lo.z - v[r].z, v[r].z - hi.z), 0.f);
-            keep = dist2(ex, ey, ez) < h2_keep;
-        }
+        // offset of the entry from the box (v minus its clamp to [lo, hi]); NaN positions are dropped,
+        // as no particle accepts them
+        const float ex = v[r].x - __builtin_amdgcn_fmed3f(v[r].x, lo.x, hi.x);
+        const float ey = v[r].y - __builtin_amdgcn_fmed3f(v[r].y, lo.y, hi.y);
+        const float ez = v[r].z - __builtin_amdgcn_fmed3f(v[r].z, lo.z, hi.z);
+        const bool keep = (dist2(ex, ey, ez) < h2_keep) & has[r];
         const uint64_t m = L.bits(__ballot(keep));
         if (keep) {
-            const uint32_t at = kept + L.below(m);
+            const uint32_t at = kept + L.below(m) - skip;  // wraps above depth when below the window
             if (at < depth) tile[at] = make_float4(v[r].x, v[r].y, v[r].z, __uint_as_float(jv[r]));
         }
         kept += (uint32_t)__popcll(m);
@@ -356,13 +360,13 @@ __device__ inline uint32_t seg_wave_max(uint32_t v)
     return __builtin_amdgcn_readfirstlane(v);
 }
 
-// Segmented stage_neighbourhood (density form): every segment stages the cell it was given (a segment
-// with active = false stages nothing). Returns the segment's kept count or kStageOverflow. A segment of
-// W lanes holds the 27 bucket ranges in ceil(27 / W) layers: bucket k sits in lane k % W of layer k / W.
+// Segmented stage_neighbourhood (density form), first half: lays the 27 bucket ranges of the segment's
+// cell end to end in `aux` (a segment with active = false lays none) and returns the number of visits,
+// 0 for an inactive segment, or kStageOverflow when they exceed kStageRounds x 64. A segment of W lanes
+// holds the 27 ranges in ceil(27 / W) layers: bucket k sits in lane k % W of layer k / W.
 template <int W>
-__device__ inline uint32_t stage_seg(const Lanes<W>& L, const SegCell& c, float h2_keep, const float4* __restrict__ spos,
-                                     const uint32_t* __restrict__ cell_start, const uint32_t* __restrict__ cell_end,
-                                     uint32_t n, uint32_t depth, float4* tile, StageAux* aux)
+__device__ inline uint32_t stage_seg_ranges(const Lanes<W>& L, const SegCell& c, const uint32_t* __restrict__ cell_start,
+                                            const uint32_t* __restrict__ cell_end, uint32_t n, StageAux* aux)
 {
     constexpr int NB = (27 + W - 1) / W;  // layers
     constexpr uint32_t SW = W < 32 ? W : 32;  // scan width: the 27 ranges of a wider segment are in its first 32 lanes
@@ -407,34 +411,100 @@ __device__ inline uint32_t stage_seg(const Lanes<W>& L, const SegCell& c, float 
         }
         ranks += (uint32_t)__popcll(nonempty);
     }
+    if (L.ls == 0) {
+        aux->box[0] = c.lo.x;
+        aux->box[1] = c.lo.y;
+        aux->box[2] = c.lo.z;
+        aux->box[3] = c.hi.x;
+        aux->box[4] = c.hi.y;
+        aux->box[5] = c.hi.z;
+        aux->total = over ? 0u : total;
+    }
     wave_sync();
-    const uint32_t tot = over ? 0u : total;
-    const uint32_t rounds = seg_wave_max<W>((tot + W - 1) / W);  // the wave runs the longest segment's rounds
+    return over ? kStageOverflow : total;
+}
+
+// Second half: loads the `total` laid-out visits (0 stages nothing), keeps those that can lie within h
+// of the cell's box [c.lo, c.hi] and writes kept entries skip .. skip + depth - 1 to the tile. Returns
+// the segment's kept count (over the whole list, whatever the window). Called by the whole wave.
+template <int W>
+__device__ inline uint32_t stage_seg_fill(const Lanes<W>& L, uint32_t total, float3 lo, float3 hi, float h2_keep,
+                                          const float4* __restrict__ spos, uint32_t skip, uint32_t depth, float4* tile,
+                                          const StageAux* aux)
+{
+    const uint32_t rounds = seg_wave_max<W>((total + W - 1) / W);  // the wave runs the longest segment's rounds
     const uint64_t upto = L.ls == 63 ? ~0ull : (2ull << L.ls) - 1ull;  // bits 0..ls
     uint32_t kept = 0, before = 0, r0 = 0;
     for (; r0 + 4 <= rounds; r0 += 4)
-        stage_seg_rounds<W, 4>(L, r0, tot, upto, c.lo, c.hi, h2_keep, spos, depth, tile, aux, kept, before);
+        stage_seg_rounds<W, 4>(L, r0, total, upto, lo, hi, h2_keep, spos, skip, depth, tile, aux, kept, before);
     if (r0 + 2 <= rounds) {
-        stage_seg_rounds<W, 2>(L, r0, tot, upto, c.lo, c.hi, h2_keep, spos, depth, tile, aux, kept, before);
+        stage_seg_rounds<W, 2>(L, r0, total, upto, lo, hi, h2_keep, spos, skip, depth, tile, aux, kept, before);
         r0 += 2;
     }
-    if (r0 < rounds) stage_seg_rounds<W, 1>(L, r0, tot, upto, c.lo, c.hi, h2_keep, spos, depth, tile, aux, kept, before);
+    if (r0 < rounds) stage_seg_rounds<W, 1>(L, r0, total, upto, lo, hi, h2_keep, spos, skip, depth, tile, aux, kept, before);
     wave_sync();
-    return over ? kStageOverflow : kept;
+    return kept;
 }
 
-// Acceptance bits of one chunk [c0, c0 + kKeptChunk) of a kept list (c0 a multiple of kKeptChunk), from
-// a lane's mask hm over `trips` trips (bit trips - 1 - m is trip m): trip c0 / P + k of the chunk
-// becomes bit 31 - k of the result, so counting leading zeros visits the accepted entries of the chunk
-// earliest trip first, entry k P + slot of the chunk for k leading zeros.
-template <uint32_t P>
-__device__ inline uint32_t chunk_bits(uint64_t hm, uint32_t trips, uint32_t c0)
+// Both halves: the segment's kept count, its first `depth` kept entries in the tile, or kStageOverflow.
+template <int W>
+__device__ inline uint32_t stage_seg(const Lanes<W>& L, const SegCell& c, float h2_keep, const float4* __restrict__ spos,
+                                     const uint32_t* __restrict__ cell_start, const uint32_t* __restrict__ cell_end,
+                                     uint32_t n, uint32_t depth, float4* tile, StageAux* aux)
 {
-    static_assert(kKeptChunk % P == 0 && kKeptChunk / P <= 32, "a chunk's trips fit one word");
-    const uint32_t mlo = c0 / P, mhi = min((c0 + kKeptChunk) / P, trips);
-    if (mlo >= mhi) return 0u;
+    const uint32_t total = stage_seg_ranges<W>(L, c, cell_start, cell_end, n, aux);
+    const uint32_t kept =
+        stage_seg_fill<W>(L, total == kStageOverflow ? 0u : total, c.lo, c.hi, h2_keep, spos, 0u, depth, tile, aux);
+    return total == kStageOverflow ? kStageOverflow : kept;
+}
+
+// Acceptance bits of one chunk [c0, c0 + chunk) of a window's kept entries (c0 and chunk multiples of
+// P, chunk <= 64 P), from a lane's word hm over the window's `trips` trips (bit trips - 1 - m is trip
+// m): trip c0 / P + k of the chunk becomes bit 63 - k of the result, so counting leading zeros visits
+// the accepted entries of the chunk earliest trip first, entry k P + slot of the chunk for k leading
+// zeros.
+template <uint32_t P>
+__device__ inline uint64_t chunk_bits(uint64_t hm, uint32_t trips, uint32_t c0, uint32_t chunk)
+{
+    const uint32_t mlo = c0 / P, mhi = min((c0 + chunk) / P, trips);
+    if (mlo >= mhi) return 0ull;
     const uint32_t blo = trips - mhi, nb = mhi - mlo;
-    return (uint32_t)((hm >> blo) << (32u - nb));  // bits of earlier chunks leave the word
+    return (hm >> blo) << (64u - nb);  // bits of earlier chunks leave the word
+}
+
+// Kept-entry force pass: stages entries c0 .. c0 + kKeptChunk - 1 of a window (those below nw) of a
+// segment's kept list kj in its LDS tiles, every index first and then every state, so that all the
+// loads of a lane are in flight together. Lanes past the window read the segment's first entry (`seg0`)
+// and spos / svel[0] instead, without branching; their tile slots are never used.
+template <int W>
+__device__ inline void stage_kept_chunk(const Lanes<W>& L, const uint32_t* __restrict__ seg0, const uint32_t* kj,
+                                        uint32_t nw, uint32_t c0, const float4* __restrict__ spos,
+                                        const float4* __restrict__ svel, float4* tpos, float4* tvel)
+{
+    constexpr uint32_t E = kKeptChunk / (W < (int)kKeptChunk ? W : kKeptChunk);  // entries per lane
+    constexpr uint32_t S = kKeptChunk / E;                                         // lanes per entry row
+    uint32_t j[E];
+#pragma unroll
+    for (uint32_t e = 0; e < E; e++) {
+        const uint32_t q = e * S + L.ls;
+        j[e] = *(q < kKeptChunk && c0 + q < nw ? kj + c0 + q : seg0);
+    }
+    float4 p[E], v[E];
+#pragma unroll
+    for (uint32_t e = 0; e < E; e++) {
+        const uint32_t q = e * S + L.ls;
+        const uint32_t jj = q < kKeptChunk && c0 + q < nw ? j[e] : 0u;
+        p[e] = spos[jj];
+        v[e] = svel[jj];
+    }
+#pragma unroll
+    for (uint32_t e = 0; e < E; e++) {
+        const uint32_t q = e * S + L.ls;
+        if (q < kKeptChunk) {
+            tpos[q] = p[e];
+            tvel[q] = v[e];
+        }
+    }
 }
 
 // Trip count of a staged neighbourhood of `kept` entries for the wave: a lane tests entries P m + slot
@@ -475,8 +545,56 @@ __device__ inline uint64_t accept_sweep(const float4* __restrict__ row, uint32_t
         return ok ? 1u : 0u;
     };
     uint64_t hits = 0;
+#pragma unroll 4
     for (uint32_t m = 0; m < wtrips; m++) hits = (hits << 1) + test(m);
     return hits;
+}
+
+// Windows of a kept list: a staged cell's kept entries are tested `depth` at a time from LDS, window w
+// holding entries w depth .. w depth + depth - 1, each window one 64-bit word of acceptance bits per
+// lane (depth <= 64 x lanes per particle). Window 0 goes to the per-slot hitmask array, windows 1 ..
+// kMaxWindows - 1 to an array of kMaxWindows - 1 words per slot; a cell with more windows is walked.
+constexpr uint32_t kMaxWindows = 4;
+
+__device__ inline size_t hitx_at(uint32_t t, uint32_t lanes, uint32_t slot, uint32_t w)
+{
+    return ((size_t)t * lanes + slot) * (kMaxWindows - 1) + (w - 1);
+}
+
+// Sweeps the segment's particles over every window of its staged cell: the first window is already in
+// the tile (stage_seg_fill with skip 0 returned `kept`), later ones are staged again from the ranges,
+// box and visit count stage_seg_ranges left in `aux`. For each window of this segment, on(w, nw, trips, hits, a) gets the window's entry count, this
+// lane's trip count, acceptance bits (bit trips - 1 - m for trip m) and sum of accepted terms. Returns
+// whether the kept list fit in kMaxWindows windows (otherwise nothing was called). Called by the whole
+// wave; leaves the tile free for the next staging.
+template <int W, class On>
+__device__ inline bool sweep_windows(const Lanes<W>& L, uint32_t kept, float h2_keep, const float4* __restrict__ spos,
+                                     uint32_t depth, float4* tile, const StageAux* aux, const float4& pi, uint32_t self,
+                                     float h2, float cut, On&& on)
+{
+    constexpr uint32_t P = W / 8;
+    const bool fits = kept != kStageOverflow && kept <= kMaxWindows * depth;
+    const uint32_t nwin = fits ? (kept + depth - 1) / depth : 0u;
+    const uint32_t wwin = seg_wave_max<W>(nwin);
+    for (uint32_t w = 0; w < wwin; w++) {
+        if (w > 0) {  // the cell's box and visits from the first staging
+            const float3 lo = make_float3(aux->box[0], aux->box[1], aux->box[2]);
+            const float3 hi = make_float3(aux->box[3], aux->box[4], aux->box[5]);
+            stage_seg_fill<W>(L, w < nwin ? aux->total : 0u, lo, hi, h2_keep, spos, w * depth, depth, tile, aux);
+        }
+        const uint32_t nw = w < nwin ? min(kept - w * depth, depth) : 0u;
+        const uint32_t wtrips = wave_trips<W>(nw);
+        const uint32_t trips = (nw + P - 1) / P;
+        pad_tile<W>(L, tile, nw, wtrips * P);
+        wave_sync();
+        float a = 0.f;
+        uint64_t hits = accept_sweep<P>(tile + L.ls % P, wtrips, pi, self, h2, cut, a);
+        // entries past a segment's end were end markers (never accepted): drop their bits
+        hits >>= min(wtrips - trips, 63u);  // trips = 0 leaves no bit set
+        if (w < nwin) on(w, nw, trips, hits, a);
+        wave_sync();  // the next window overwrites the tile
+    }
+    return fits;
 }
 
 }  // namespace sph

@@ -226,8 +226,8 @@ class oclFluid3D:
         summation order."""
         cur = self.getTuning()
         if "density_lanes" not in kw and "neighbour_pass" in kw:
-            # keep the lane count valid for the pass being selected (list: 1 or 8, tiles: 4 or 8)
-            ok = (1, 8) if kw["neighbour_pass"] == "list" else (4, 8)
+            # keep the lane count valid for the pass being selected (list: 1 or 8, tiles: 2, 4 or 8)
+            ok = (1, 8) if kw["neighbour_pass"] == "list" else (2, 4, 8)
             if cur["density_lanes"] not in ok:
                 kw = dict(kw, density_lanes=8)
         cur.update(kw)

@@ -21,6 +21,10 @@ BATCH_SHAPES = {
     "tile_kept2": ({"neighbour_pass": "tile_kept", "density_lanes": 2, "tile_depth": 96},
                    {"neighbour_pass": "tile_kept", "density_lanes": 2, "tile_depth": 96, "block_waves": 2,
                     "force_chunks": 2}),
+    # 2 lanes, 64-entry windows (dense cells take several)
+    "tile_kept2_64": ({"neighbour_pass": "tile_kept", "density_lanes": 2, "tile_depth": 64},
+                      {"neighbour_pass": "tile_kept", "density_lanes": 2, "tile_depth": 64, "block_waves": 2,
+                       "force_chunks": 2}),
     # 8 lanes per particle: one 64-lane segment of 8 particles per wave
     "tile_kept8": ({"neighbour_pass": "tile_kept", "density_lanes": 8, "tile_depth": 256},
                    {"neighbour_pass": "tile_kept", "density_lanes": 8, "tile_depth": 256, "block_waves": 2,

@@ -17,7 +17,8 @@ TOL = 1e-5
 
 
 # launch shapes of the neighbour passes: LDS tiles (tile_kept is the default), tiles so shallow that
-# dense neighbourhoods overflow to the global-memory walk, and the recorded-visit-list passes
+# dense neighbourhoods are tested in several windows (or, past four windows, walked from global
+# memory), and the recorded-visit-list passes
 SHAPES = {"tile_list": {"neighbour_pass": "tile_list", "density_lanes": 4, "tile_depth": 192},
           "tile": {"neighbour_pass": "tile", "density_lanes": 8, "tile_depth": 256},
           "tile_list8": {"neighbour_pass": "tile_list", "density_lanes": 8, "tile_depth": 256, "force_chunks": 1},
@@ -181,6 +182,32 @@ def test_1m_trajectory_snapshots_vs_oracle(oracle):
     for snap in _oracle_snapshots(oracle, sc, prm, 65536, (1, 10, 100, 200)):
         print(f"1M snapshot step {snap[0]}", flush=True)
         _step_from_snapshot(oracle, snap, prm, sc, 65536, "tile_kept")
+
+
+@pytest.mark.parametrize("spacing", [0.4, 0.3])
+@pytest.mark.parametrize("shape", ["tile_kept", "tile_kept2", "tile_kept2_64", "tile_kept64",
+                                   "tile_kept8", "tile_list2", "tile_list4_64", "tile64"])
+def test_dense_blocks_vs_oracle(oracle, spacing, shape):
+    """Blocks denser than the dam break's 0.5h lattice. At 0.4h spacing (about 16 particles per cell and
+    190 kept entries per cell) the tile passes test the kept lists in several windows of `depth`
+    entries; at 0.3h (about 37 per cell, more than 512 visits per neighbourhood) every particle walks
+    its bucket ranges from global memory. One step against the oracle: keys, permutation, tables and
+    accepted-visit counts bit-exact, fields within 1e-5."""
+    import libcl_amd as L
+    h = 0.02
+    print(f"dense block spacing {spacing}h {shape}", flush=True)
+    pos = scenes.lattice(32, 16, 16, spacing * h, (0.25 * h,) * 3, 0.01 * h, 43)
+    vel = np.zeros_like(pos)
+    force = np.tile(np.asarray(scenes.GRAVITY_FORCE, np.float32), (pos.shape[0], 1))
+    prm = L.params_from(scenes.sph_param_values(h))
+    box_min, box_max = (0.0, 0.0, 0.0, 0.0), (1.0, 1.0, 1.0, 0.0)
+    f = make_fluid(pos, vel, force, prm, box_min, box_max, shape=shape)
+    run_forces_only(f)
+    fo = oracle.Fluid(pos, vel, force, oracle.Params(*prm.values()), box_min=box_min, box_max=box_max)
+    fo.step(integrate=False)
+    _oracle_compare(f, fo)
+    nw, per = force_errors(f.state()["force"], fo.force)
+    assert nw <= TOL and per <= TOL, (nw, per)
 
 
 @pytest.mark.parametrize("shape", ["tile_list", "tile_list8", "tile", "tile_kept", "tile_kept2", "list"])
