@@ -394,10 +394,12 @@ __global__ __launch_bounds__(kThreads) void k_b_forces_kept(const float4* __rest
     constexpr uint32_t SEGS = kWave / W;
     extern __shared__ float4 lds[];
     const Lanes<W> L(threadIdx.x & (kWave - 1));
-    const uint32_t sid = (threadIdx.x / kWave) * SEGS + L.seg;
     constexpr uint32_t chunk = kKeptChunk;  // small LDS per segment keeps more waves resident
-    float4* tpos = lds + (size_t)sid * 2 * chunk;
-    float4* tvel = tpos + chunk;
+    // the wave's tiles: positions of its segments' chunks, then their velocities
+    float4* wpos = lds + (size_t)(threadIdx.x / kWave) * SEGS * 2 * chunk;
+    float4* wvel = wpos + SEGS * chunk;
+    float4* tpos = wpos + L.seg * chunk;
+    float4* tvel = wvel + L.seg * chunk;
     const uint32_t grp = L.ls / P, slot = L.ls % P;
     const uint32_t wave = (xcd_block() * blockDim.x + threadIdx.x) / kWave;
     const uint32_t t = wave * (kWave / P) + L.seg * 8 + grp;
@@ -436,9 +438,9 @@ __global__ __launch_bounds__(kThreads) void k_b_forces_kept(const float4* __rest
             const uint32_t nw = kept > w0 ? min(kept - w0, depth) : 0u, nmax = min(kmax - w0, depth);
             const uint32_t trips = (nw + P - 1) / P;
             const uint64_t hw = w == 0 ? hm : (member && nw ? hitx[hitx_at(t, P, slot, w)] : 0ull);
-            const uint32_t* kj = kept_in + (size_t)gseg * kSegCap + offk + w0;
+            const size_t kbase = (size_t)gseg * kSegCap + offk + w0;
             for (uint32_t c0 = 0; c0 < nmax; c0 += chunk) {
-                stage_kept_chunk<W>(L, kept_in + (size_t)gseg * kSegCap, kj, nw, c0, sp, sv, tpos, tvel);
+                stage_kept_chunk<W>(kept_in, kbase, nw, c0, L.lane, sp, sv, wpos, wvel);
                 wave_sync();
                 if (member) {
                     uint64_t bits = chunk_bits<P>(hw, trips, c0, chunk);

@@ -472,39 +472,42 @@ __device__ inline uint64_t chunk_bits(uint64_t hm, uint32_t trips, uint32_t c0, 
     return (hm >> blo) << (64u - nb);  // bits of earlier chunks leave the word
 }
 
-// Kept-entry force pass: stages entries c0 .. c0 + kKeptChunk - 1 of a window (those below nw) of a
-// segment's kept list kj in its LDS tiles, every index first and then every state, so that all the
-// loads of a lane are in flight together. Lanes past the window read the segment's first entry (`seg0`)
-// and spos / svel[0] instead, without branching; their tile slots are never used.
+// Kept-entry force pass: stages entries c0 .. c0 + kKeptChunk - 1 of the current window of every
+// segment of the wave into the wave's LDS tiles, segment e's entries at wpos / wvel[e kKeptChunk ..]:
+// for each segment the 64 lanes read 64 consecutive indices of its kept list (the window starts at
+// kept_in[kbase] and holds nw entries, both per segment) and the states they name go straight from
+// global memory to LDS (global_load_lds_dwordx4: lane l fills slot l of the 1 KiB block given), so no
+// state passes through registers. Lanes past a window load spos / svel[0] into slots never used.
 template <int W>
-__device__ inline void stage_kept_chunk(const Lanes<W>& L, const uint32_t* __restrict__ seg0, const uint32_t* kj,
-                                        uint32_t nw, uint32_t c0, const float4* __restrict__ spos,
-                                        const float4* __restrict__ svel, float4* tpos, float4* tvel)
+__device__ inline void stage_kept_chunk(const uint32_t* __restrict__ kept_in, size_t kbase, uint32_t nw, uint32_t c0,
+                                        uint32_t lane, const float4* __restrict__ spos,
+                                        const float4* __restrict__ svel, float4* wpos, float4* wvel)
 {
-    constexpr uint32_t E = kKeptChunk / (W < (int)kKeptChunk ? W : kKeptChunk);  // entries per lane
-    constexpr uint32_t S = kKeptChunk / E;                                         // lanes per entry row
-    uint32_t j[E];
+    static_assert(kKeptChunk == kWave, "a chunk of one segment is one entry per lane");
+    constexpr uint32_t SEGS = kWave / W;
+    uint32_t j[SEGS];
 #pragma unroll
-    for (uint32_t e = 0; e < E; e++) {
-        const uint32_t q = e * S + L.ls;
-        j[e] = *(q < kKeptChunk && c0 + q < nw ? kj + c0 + q : seg0);
+    for (uint32_t e = 0; e < SEGS; e++) {
+        // the segment's window, from its first lane (wave-uniform values)
+        const size_t b = ((size_t)__builtin_amdgcn_readlane((uint32_t)(kbase >> 32), e * W) << 32) |
+                         __builtin_amdgcn_readlane((uint32_t)kbase, e * W);
+        const uint32_t m = __builtin_amdgcn_readlane(nw, e * W);
+        const uint32_t q = c0 + lane;
+        const uint32_t k = kept_in[b + min(q, (m > 0 ? m : 1u) - 1u)];  // inside the segment's list
+        j[e] = q < m ? k : 0u;
     }
-    float4 p[E], v[E];
+    // every index before the first state load (s_waitcnt vmcnt(0) alone: expcnt 7, lgkmcnt 15)
+    __builtin_amdgcn_s_waitcnt(0x0F70);
+#if defined(__HIP_DEVICE_COMPILE__)  // the LDS address space exists in the device pass only
 #pragma unroll
-    for (uint32_t e = 0; e < E; e++) {
-        const uint32_t q = e * S + L.ls;
-        const uint32_t jj = q < kKeptChunk && c0 + q < nw ? j[e] : 0u;
-        p[e] = spos[jj];
-        v[e] = svel[jj];
+    for (uint32_t e = 0; e < SEGS; e++) {
+        __builtin_amdgcn_global_load_lds(spos + j[e], (__attribute__((address_space(3))) void*)(wpos + e * kKeptChunk),
+                                         16, 0, 0);
+        __builtin_amdgcn_global_load_lds(svel + j[e], (__attribute__((address_space(3))) void*)(wvel + e * kKeptChunk),
+                                         16, 0, 0);
     }
-#pragma unroll
-    for (uint32_t e = 0; e < E; e++) {
-        const uint32_t q = e * S + L.ls;
-        if (q < kKeptChunk) {
-            tpos[q] = p[e];
-            tvel[q] = v[e];
-        }
-    }
+#endif
+    __builtin_amdgcn_s_waitcnt(0);  // the states are in LDS
 }
 
 // Trip count of a staged neighbourhood of `kept` entries for the wave: a lane tests entries P m + slot
