@@ -826,12 +826,12 @@ __global__ __launch_bounds__(kThreads) void k_forces_kept(const float4* __restri
                 stage_kept_chunk<W>(kept_in, kbase, nw, c0, L.lane, spos, svel, wpos, wvel);
                 wave_sync();
                 if (member) {
-                    uint64_t bits = chunk_bits<P>(hw, trips, c0, chunk);
+                    uint32_t bits = chunk_bits<P>(hw, trips, c0);
                     const float4* rp = tpos + slot;  // this lane's entries: k P + slot of the chunk
                     const float4* rv = tvel + slot;
                     while (bits) {
-                        const uint32_t k = (uint32_t)__builtin_clzll(bits);  // earliest trip first
-                        bits ^= 0x8000000000000000ull >> k;
+                        const uint32_t k = (uint32_t)__builtin_clz(bits);  // earliest trip first
+                        bits ^= 0x80000000u >> k;
                         const float4 pj = rp[k * P], vj = rv[k * P];
                         const float dx = pi.x - pj.x, dy = pi.y - pj.y, dz = pi.z - pj.z;
                         pair_force(pi, vi, pj, vj, dx, dy, dz, dist2(dx, dy, dz), h, Pa, Va);

@@ -49,8 +49,8 @@ __device__ inline void pair_force(const float4& pi, const float4& vi, const floa
     const float inv_r = __builtin_amdgcn_rsqf(r2);
     const float hr = fmaf(-r2, inv_r, h);
     const float inv_d2 = __builtin_amdgcn_rcpf(pi.w * pj.w);
-    const float s = (vi.w + vj.w) * inv_d2 * hr * hr * inv_r;
     const float w = hr * inv_d2;
+    const float s = (vi.w + vj.w) * w * hr * inv_r;  // (p_i + p_j) (h - r)^2 / (rho_i rho_j r), w reused
     // `use` = false computes the terms and keeps the sums (a select, not a branch)
     P.x = use ? fmaf(s, dx, P.x) : P.x;
     P.y = use ? fmaf(s, dy, P.y) : P.y;

@@ -458,18 +458,18 @@ __device__ inline uint32_t stage_seg(const Lanes<W>& L, const SegCell& c, float 
     return total == kStageOverflow ? kStageOverflow : kept;
 }
 
-// Acceptance bits of one chunk [c0, c0 + chunk) of a window's kept entries (c0 and chunk multiples of
-// P, chunk <= 64 P), from a lane's word hm over the window's `trips` trips (bit trips - 1 - m is trip
-// m): trip c0 / P + k of the chunk becomes bit 63 - k of the result, so counting leading zeros visits
-// the accepted entries of the chunk earliest trip first, entry k P + slot of the chunk for k leading
-// zeros.
+// Acceptance bits of one chunk [c0, c0 + kKeptChunk) of a window's kept entries (c0 a multiple of
+// kKeptChunk), from a lane's word hm over the window's `trips` trips (bit trips - 1 - m is trip m): trip
+// c0 / P + k of the chunk becomes bit 31 - k of the result, so counting leading zeros visits the
+// accepted entries of the chunk earliest trip first, entry k P + slot of the chunk for k leading zeros.
 template <uint32_t P>
-__device__ inline uint64_t chunk_bits(uint64_t hm, uint32_t trips, uint32_t c0, uint32_t chunk)
+__device__ inline uint32_t chunk_bits(uint64_t hm, uint32_t trips, uint32_t c0)
 {
-    const uint32_t mlo = c0 / P, mhi = min((c0 + chunk) / P, trips);
-    if (mlo >= mhi) return 0ull;
+    static_assert(kKeptChunk % P == 0 && kKeptChunk / P <= 32, "a chunk's trips fit one word");
+    const uint32_t mlo = c0 / P, mhi = min((c0 + kKeptChunk) / P, trips);
+    if (mlo >= mhi) return 0u;
     const uint32_t blo = trips - mhi, nb = mhi - mlo;
-    return (hm >> blo) << (64u - nb);  // bits of earlier chunks leave the word
+    return (uint32_t)((hm >> blo) << (32u - nb));  // bits of earlier chunks leave the word
 }
 
 // Kept-entry force pass: stages entries c0 .. c0 + kKeptChunk - 1 of the current window of every
