@@ -7,10 +7,12 @@
 //
 // Step (oclFluid3D::compute, cpp:293-342):
 //   k_hash_counts   cell key per particle + reset of the first clear_count buckets + the sort's
-//                   first-pass digit counts (cl:57-64, cpp:309-313; nothing between the two reads
-//                   the tables, so the reset moves to the start of the step)
-//   sort            stable key/index sort over bits [0, 18)                 (cpp:299; keys < 2^18)
-//   k_slots_bounds  inverse permutation + start/end of each occupied bucket (cl:85-113)
+//                   high-digit counts (cl:57-64, cpp:309-313; nothing between the two reads the
+//                   tables, so the reset moves to the start of the step)
+//   sort            stable key/index sort over bits [0, 18) (cpp:299; keys < 2^18): one counted
+//                   pass by bits [9, 18), then k_bucket_runs orders each high digit's run by bits
+//                   [0, 9) and writes the inverse permutation and the start / end of each occupied
+//                   bucket (cl:85-113) from the same histograms
 //   k_scatter_sorted sorted pos/vel copy, written from the caller's order   (cl:66-72)
 //   density pass    27-bucket neighbourhood, poly6, pressure                (cl:150-200)
 //   force pass      27-bucket pressure + viscosity forces, clip box, clamp, Euler, gravity
@@ -49,15 +51,16 @@ __global__ __launch_bounds__(kThreads) void k_hash(const float4* __restrict__ po
     if (WRITE_INDEX) index[i] = i;
 }
 
-// k_hash<false, true> that also fills the sort's first-pass count table (bits [0, bits) of the keys; the
-// layout of RadixSort::first_pass_plan): block b hashes the keys of sort tile xcd_tile(), ITEMS per
+// k_hash<false, true> that also fills the sort's first-pass count table (bits [shift, shift + bits) of the
+// keys; the layout of RadixSort::first_pass_plan): block b hashes the keys of sort tile xcd_tile(), ITEMS per
 // thread at tile offsets k * 256 + thread, and histograms them in LDS. The grid is one block per tile.
 template <int ITEMS>
 __global__ __launch_bounds__(kThreads) void k_hash_counts(const float4* __restrict__ pos, uint32_t n, float h,
                                                           uint32_t* __restrict__ cell, mfl_params* __restrict__ params_buf,
                                                           mfl_params p, uint32_t* __restrict__ cell_start,
                                                           uint32_t* __restrict__ cell_end, uint32_t clear,
-                                                          uint32_t* __restrict__ counts, uint32_t ntiles, uint32_t bits)
+                                                          uint32_t* __restrict__ counts, uint32_t ntiles, uint32_t shift,
+                                                          uint32_t bits)
 {
     __shared__ uint32_t hist[512];
     const uint32_t radix = 1u << bits, mask = radix - 1u;
@@ -77,7 +80,7 @@ __global__ __launch_bounds__(kThreads) void k_hash_counts(const float4* __restri
             const float4 q = pos[i];
             const uint32_t key = cell_hash(grid_coord(q.x, h), grid_coord(q.y, h), grid_coord(q.z, h));
             cell[i] = key;
-            atomicAdd(&hist[key & mask], 1u);
+            atomicAdd(&hist[(key >> shift) & mask], 1u);
         }
     }
     __syncthreads();
@@ -112,40 +115,133 @@ __global__ __launch_bounds__(kThreads) void k_reorder_clear(const uint32_t* __re
 // 2^18 particles hold 8 MiB of pos + vel, twice an XCD's L2.
 constexpr uint32_t kScatterCopyMin = 1u << 18;
 
-// Sorted slot of every particle (inv[index[i]] = i) and the start / end of each occupied bucket
-// (cl:85-113), both read in sorted order. GATHER also copies the sorted state here (cl:66-72) by
-// gathering through index: one launch fewer, for steps whose pos / vel sit in L2 anyway; larger
-// steps copy with k_scatter_sorted instead.
+// Second half of the step's sort. RadixSort::counted_pass has ordered the keys (< 2^18) by their high
+// digit, bits [9, 18); block `c` (one per high digit, dealt to the XCDs in contiguous ranges) takes the
+// contiguous run of keys with high digit c and orders it stably by bits [0, 9): a digit histogram of the
+// run in LDS, then, chunk by chunk in input order, ranks by wave ballot matching (the scatter of
+// radix_sort.hip) added to each digit's running offset. High digit c then low digit d is bucket
+// c * 512 + d, so the run holds every key of buckets c * 512 .. c * 512 + 511, and the histogram gives
+// their bounds directly (cl:85-113: start and end written for occupied buckets only). The block also
+// writes each particle's sorted slot (inv) and, GATHER, copies the sorted state by gathering through the
+// index (for steps whose pos / vel sit in L2 anyway; larger steps copy with k_scatter_sorted).
+constexpr int kRunItems = 8;  // keys per thread and chunk
+constexpr uint32_t kRunDigits = 512;
+
 template <bool GATHER>
-__global__ __launch_bounds__(kThreads) void k_slots_bounds(const uint32_t* __restrict__ index, uint32_t n,
-                                                           const uint32_t* __restrict__ cell,
-                                                           uint32_t* __restrict__ cell_start,
-                                                           uint32_t* __restrict__ cell_end, uint32_t* __restrict__ inv,
-                                                           const float4* __restrict__ pos,
-                                                           const float4* __restrict__ vel,
-                                                           float4* __restrict__ spos, float4* __restrict__ svel)
+__global__ __launch_bounds__(kThreads) void k_bucket_runs(const uint32_t* __restrict__ kin,
+                                                          const uint32_t* __restrict__ vin,
+                                                          const uint32_t* __restrict__ totals,
+                                                          uint32_t* __restrict__ cell, uint32_t* __restrict__ index,
+                                                          uint32_t* __restrict__ cell_start,
+                                                          uint32_t* __restrict__ cell_end, uint32_t* __restrict__ inv,
+                                                          const float4* __restrict__ pos,
+                                                          const float4* __restrict__ vel, float4* __restrict__ spos,
+                                                          float4* __restrict__ svel)
 {
-    const uint32_t i = blockIdx.x * kThreads + threadIdx.x;
-    if (i >= n) return;
-    const uint32_t s = index[i];
-    if (s < n) {  // index is a permutation of [0, n)
-        inv[s] = i;
-        if (GATHER) {
-            spos[i] = pos[s];
-            svel[i] = vel[s];
+    constexpr int kWaves = kThreads / kWave;
+    __shared__ uint32_t run_off[kRunDigits];          // histogram, then each digit's next slot in the run
+    __shared__ uint32_t s_cnt[kWaves * kRunDigits];   // a chunk's per-wave digit counts
+    __shared__ uint32_t s_wave[kWaves];
+    __shared__ uint32_t s_base;
+    const uint32_t c = xcd_tile();
+    const uint32_t t = threadIdx.x, lane = t & (kWave - 1), wave = t / kWave;
+    const uint32_t d0 = 2u * t, d1 = d0 + 1u;  // this thread's two digits in the per-digit loops
+    {
+        const uint32_t t0 = totals[d0], t1 = totals[d1];
+        uint32_t all;
+        const uint32_t ex = block_scan<kThreads>(t0 + t1, s_wave, &all);
+        if (d0 == c) s_base = ex;
+        if (d1 == c) s_base = ex + t0;
+        run_off[d0] = 0;
+        run_off[d1] = 0;
+    }
+    __syncthreads();
+    const uint32_t base = s_base, cnt = totals[c];
+    if (cnt == 0) return;  // block-uniform
+    const uint32_t* rk = kin + base;
+    const uint32_t* rv = vin + base;
+    for (uint32_t i = t; i < cnt; i += kThreads) atomicAdd(&run_off[rk[i] & (kRunDigits - 1u)], 1u);
+    __syncthreads();
+    {
+        const uint32_t h0 = run_off[d0], h1 = run_off[d1];
+        uint32_t all;
+        const uint32_t st = block_scan<kThreads>(h0 + h1, s_wave, &all);
+        const uint32_t b0 = (c << 9) | d0;
+        if (h0) {
+            cell_start[b0] = base + st;
+            cell_end[b0] = base + st + h0;
         }
+        if (h1) {
+            cell_start[b0 + 1] = base + st + h0;
+            cell_end[b0 + 1] = base + st + h0 + h1;
+        }
+        run_off[d0] = st;
+        run_off[d1] = st + h0;
     }
-    const uint32_t h = cell[i] & (kBuckets - 1u);  // keys are < 2^18 by construction
-    if (i == 0) {
-        cell_start[h] = 0;
-        return;
+    const uint64_t lower_lanes = (1ull << lane) - 1ull;
+    for (uint32_t c0 = 0; c0 < cnt; c0 += kThreads * kRunItems) {
+        for (uint32_t i = t; i < kWaves * kRunDigits; i += kThreads) s_cnt[i] = 0;
+        __syncthreads();
+        // item r of wave w, lane l is key c0 + (w kRunItems + r) 64 + l: (wave, round, lane) order is input order
+        uint32_t key[kRunItems], val[kRunItems], rank[kRunItems];
+#pragma unroll
+        for (int r = 0; r < kRunItems; r++) {
+            const uint32_t q = c0 + (wave * kRunItems + r) * kWave + lane;
+            const bool valid = q < cnt;
+            key[r] = valid ? rk[q] : 0u;
+            val[r] = valid ? rv[q] : 0u;
+            const uint32_t d = key[r] & (kRunDigits - 1u);
+            // lanes whose digit differs in some bit (or that hold no key), per bit b: the ballot of bit b
+            // XOR this lane's bit b replicated
+            const uint64_t live = __ballot(valid);
+            uint32_t differ_lo = ~(uint32_t)live, differ_hi = ~(uint32_t)(live >> 32);
+#pragma unroll
+            for (int b = 0; b < 9; b++) {
+                const uint32_t rep = (uint32_t)((int)(d << (31 - b)) >> 31);
+                const uint64_t bal = __ballot(rep != 0u);
+                differ_lo |= (uint32_t)bal ^ rep;
+                differ_hi |= (uint32_t)(bal >> 32) ^ rep;
+            }
+            const uint64_t same = ~((uint64_t)differ_hi << 32 | differ_lo);
+            const int leader = __ffsll((unsigned long long)same) - 1;
+            uint32_t before = 0;
+            if (valid && (int)lane == leader) {
+                before = s_cnt[wave * kRunDigits + d];
+                s_cnt[wave * kRunDigits + d] = before + (uint32_t)__popcll(same);
+            }
+            before = __shfl(before, leader & (kWave - 1), kWave);
+            rank[r] = before + (uint32_t)__popcll(same & lower_lanes);
+        }
+        __syncthreads();
+        uint32_t chunk0 = 0, chunk1 = 0;  // the chunk's keys of digits d0 and d1
+#pragma unroll
+        for (int w = 0; w < kWaves; w++) {
+            const uint32_t a = s_cnt[w * kRunDigits + d0], b = s_cnt[w * kRunDigits + d1];
+            s_cnt[w * kRunDigits + d0] = chunk0;
+            s_cnt[w * kRunDigits + d1] = chunk1;
+            chunk0 += a;
+            chunk1 += b;
+        }
+        __syncthreads();
+#pragma unroll
+        for (int r = 0; r < kRunItems; r++) {
+            const uint32_t q = c0 + (wave * kRunItems + r) * kWave + lane;
+            if (q < cnt) {
+                const uint32_t d = key[r] & (kRunDigits - 1u);
+                const uint32_t o = base + run_off[d] + s_cnt[wave * kRunDigits + d] + rank[r];
+                cell[o] = key[r];
+                index[o] = val[r];
+                inv[val[r]] = o;
+                if (GATHER) {
+                    spos[o] = pos[val[r]];
+                    svel[o] = vel[val[r]];
+                }
+            }
+        }
+        __syncthreads();
+        run_off[d0] += chunk0;
+        run_off[d1] += chunk1;
     }
-    const uint32_t prev = cell[i - 1] & (kBuckets - 1u);
-    if (h != prev) {
-        cell_end[prev] = i;
-        cell_start[h] = i;
-    }
-    if (i == n - 1) cell_end[h] = n;
 }
 
 __global__ __launch_bounds__(kThreads) void k_fill_buckets(uint32_t* __restrict__ cell_start,
@@ -1159,13 +1255,15 @@ int Fluid3D::enqueue_front(hipStream_t s)
     auto* ce = cell_end_.as<uint32_t>();
     const uint32_t clear = clear_count < kBuckets ? clear_count : kBuckets;
 
-    // the hash kernel also counts the sort's first digit per tile, which saves that pass's count launch
+    // The sort orders the keys (< 2^18) by their high 9 bits in one counted pass, whose digit counts
+    // the hash kernel takes per tile, then k_bucket_runs orders each high digit's run by the low 9 bits
+    // and writes the bucket bounds and the inverse permutation from the same histograms.
     SortCountPlan plan;
-    if (!sort.first_pass_plan(n, 0, 18, &plan)) return 0;
+    if (!sort.first_pass_plan(n, 9, 18, &plan)) return 0;
     timer.begin(MFL_STAGE_HASH, s);
 #define MFL_HASH(IT)                                                                                              \
     hipLaunchKernelGGL(k_hash_counts<IT>, dim3(plan.ntiles), b, 0, s, pos, n, params.cellSize, cell,              \
-                       params_buf_.as<mfl_params>(), params, cs, ce, clear, plan.counts, plan.ntiles, plan.bits)
+                       params_buf_.as<mfl_params>(), params, cs, ce, clear, plan.counts, plan.ntiles, 9u, plan.bits)
     switch (plan.items) {
     case 16: MFL_HASH(16); break;
     case 8: MFL_HASH(8); break;
@@ -1174,20 +1272,23 @@ int Fluid3D::enqueue_front(hipStream_t s)
 #undef MFL_HASH
     timer.end(MFL_STAGE_HASH, s);
     MFL_HIP(hipGetLastError());
-    // the index buffer starts as the identity (cl:63) and is generated inside the first pass
+    // the index buffer starts as the identity (cl:63) and is generated inside the pass
     timer.begin(MFL_STAGE_SORT, s);
-    if (!sort.sort_bits(cell, index, n, 0, 18, s, true, true)) return 0;
+    SortPassOut high;
+    if (!sort.counted_pass(cell, n, 9, 9, s, &high)) return 0;
+    const bool gather = n < kScatterCopyMin;
+    auto* inv = inv_.as<uint32_t>();
+    if (gather)
+        hipLaunchKernelGGL(k_bucket_runs<true>, dim3(kRunDigits), b, 0, s, high.keys, high.vals, high.totals, cell,
+                           index, cs, ce, inv, pos, vel, spos, svel);
+    else
+        hipLaunchKernelGGL(k_bucket_runs<false>, dim3(kRunDigits), b, 0, s, high.keys, high.vals, high.totals, cell,
+                           index, cs, ce, inv, pos, vel, spos, svel);
+    MFL_HIP(hipGetLastError());
     timer.end(MFL_STAGE_SORT, s);
-    // bucket bounds come out of the reorder pass (the BOUNDS stage stays empty)
+    // bucket bounds come out of the sort (the BOUNDS stage stays empty)
     timer.begin(MFL_STAGE_REORDER, s);
-    if (n < kScatterCopyMin) {
-        hipLaunchKernelGGL(k_slots_bounds<true>, g, b, 0, s, index, n, cell, cs, ce, inv_.as<uint32_t>(), pos, vel,
-                           spos, svel);
-    } else {
-        hipLaunchKernelGGL(k_slots_bounds<false>, g, b, 0, s, index, n, cell, cs, ce, inv_.as<uint32_t>(), pos, vel,
-                           spos, svel);
-        hipLaunchKernelGGL(k_scatter_sorted, g, b, 0, s, inv_.as<uint32_t>(), pos, vel, spos, svel, n);
-    }
+    if (!gather) hipLaunchKernelGGL(k_scatter_sorted, g, b, 0, s, inv, pos, vel, spos, svel, n);
     timer.end(MFL_STAGE_REORDER, s);
     MFL_HIP(hipGetLastError());
     return 1;

@@ -78,6 +78,34 @@ __device__ inline uint32_t xcd_tile()
     return x * q + min(x, r) + b / kXcds;
 }
 
+// Exclusive scan of one value per thread over a block of THREADS threads (a multiple of 64; s_wave holds
+// THREADS / 64 words of LDS); *total receives the block's sum. Every thread of the block must call it.
+template <int THREADS>
+__device__ inline uint32_t block_scan(uint32_t v, uint32_t* s_wave, uint32_t* total)
+{
+    constexpr int kWaves = THREADS / kWave;
+    const int lane = threadIdx.x & (kWave - 1);
+    const int wave = threadIdx.x / kWave;
+    uint32_t x = v;
+#pragma unroll
+    for (int d = 1; d < kWave; d <<= 1) {
+        const uint32_t y = __shfl_up(x, d, kWave);
+        if (lane >= d) x += y;
+    }
+    if (lane == kWave - 1) s_wave[wave] = x;
+    __syncthreads();
+    uint32_t before = 0, all = 0;
+#pragma unroll
+    for (int w = 0; w < kWaves; w++) {
+        const uint32_t s = s_wave[w];
+        before += (w < wave) ? s : 0u;
+        all += s;
+    }
+    __syncthreads();
+    *total = all;
+    return before + x - v;
+}
+
 // Appends the bytes of v to a launch signature (the state a captured graph depends on): comparing
 // raw bytes costs far less per step than formatting the values.
 template <class T>

@@ -21,29 +21,9 @@ constexpr int kWaves = kThreads / kWave;
 constexpr int kMaxDigitBits = 9;
 constexpr uint32_t kMaxRadix = 1u << kMaxDigitBits;  // 512 digits: two per thread in the digit loops
 
-// Exclusive scan of one value per thread over a 256-thread block. All threads must call it.
 __device__ inline uint32_t block_exclusive_scan(uint32_t v, uint32_t* s_wave, uint32_t* total)
 {
-    const int lane = threadIdx.x & (kWave - 1);
-    const int wave = threadIdx.x / kWave;
-    uint32_t x = v;
-#pragma unroll
-    for (int d = 1; d < kWave; d <<= 1) {
-        const uint32_t y = __shfl_up(x, d, kWave);
-        if (lane >= d) x += y;
-    }
-    if (lane == kWave - 1) s_wave[wave] = x;
-    __syncthreads();
-    uint32_t before = 0, all = 0;
-#pragma unroll
-    for (int w = 0; w < kWaves; w++) {
-        const uint32_t s = s_wave[w];
-        before += (w < wave) ? s : 0u;
-        all += s;
-    }
-    __syncthreads();
-    *total = all;
-    return before + x - v;
+    return block_scan<kThreads>(v, s_wave, total);
 }
 
 // Tiles are dealt to the XCDs in contiguous ranges (xcd_tile): neighbouring tiles write neighbouring
@@ -502,6 +482,35 @@ int RadixSort::compute_noalloc(uint32_t* keys, uint32_t* vals, size_t n, int sta
     b.counts = base + 4 * kNoAllocMaxPairs;
     b.totals = b.counts + kNoAllocCounts;
     return sort_passes(b, keys, vals, n, start_bit, end_bit, stream, false, false, items);
+}
+
+int RadixSort::counted_pass(const uint32_t* keys, size_t n, int start_bit, int bits, hipStream_t stream,
+                            SortPassOut* out)
+{
+    if (n == 0 || n > 0xFFFFFFFFull || start_bit < 0 || bits < 1 || bits > kMaxDigitBits || start_bit + bits > 32 ||
+        !keys) {
+        set_error("radix sort: bad single pass");
+        return 0;
+    }
+    if (!reserve(n, bits)) return 0;
+    uint32_t* kout = tmp_keys_[0].as<uint32_t>();
+    uint32_t* vout = tmp_vals_[0].as<uint32_t>();
+    uint32_t* counts = counts_.as<uint32_t>();
+    uint32_t* totals = totals_.as<uint32_t>();
+    bool ok;
+    switch (pick_items(n, tuning.items)) {
+    case 16: ok = run_pass<16>(keys, nullptr, kout, vout, (uint32_t)n, start_bit, bits, counts, totals, true, stream,
+                               true); break;
+    case 8: ok = run_pass<8>(keys, nullptr, kout, vout, (uint32_t)n, start_bit, bits, counts, totals, true, stream,
+                             true); break;
+    default: ok = run_pass<4>(keys, nullptr, kout, vout, (uint32_t)n, start_bit, bits, counts, totals, true, stream,
+                              true); break;
+    }
+    if (!ok) return 0;
+    out->keys = kout;
+    out->vals = vout;
+    out->totals = totals;
+    return 1;
 }
 
 int RadixSort::sort_bits(uint32_t* keys, uint32_t* vals, size_t n, int start_bit, int end_bit, hipStream_t stream,

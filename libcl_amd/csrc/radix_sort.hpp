@@ -10,6 +10,14 @@ struct SortTuning {
     int items = 0;        // 0 = pick by size
 };
 
+// Output of RadixSort::counted_pass: the pairs ordered by one digit (in the sort's own buffers, valid
+// until its next call) and the number of keys of each digit value.
+struct SortPassOut {
+    const uint32_t* keys;
+    const uint32_t* vals;
+    const uint32_t* totals;
+};
+
 struct SortCountPlan {
     uint32_t* counts;
     uint32_t ntiles, tile_keys, bits;
@@ -48,6 +56,12 @@ public:
     // must be the number of its keys whose bits [start_bit, start_bit + bits) equal d. Reserves the
     // sort's temporaries.
     int first_pass_plan(size_t n, int start_bit, int end_bit, SortCountPlan* plan);
+
+    // One stable pass by the digit at bits [start_bit, start_bit + bits) (bits <= 9) of n keys with the
+    // values 0 .. n - 1 (the clHash index), whose count table the caller filled (first_pass_plan with the
+    // same bits): the ordered pairs go to the sort's own buffers, for a caller that finishes the order
+    // itself (the fluid step's per-bucket pass).
+    int counted_pass(const uint32_t* keys, size_t n, int start_bit, int bits, hipStream_t stream, SortPassOut* out);
 
     // Allocates every temporary a sort of n keys over `nbits` bits needs (so that a later sort
     // enqueues no allocation, e.g. under hipGraph capture).
