@@ -117,70 +117,73 @@ constexpr uint32_t kScatterCopyMin = 1u << 18;
 
 // Second half of the step's sort. RadixSort::counted_pass has ordered the keys (< 2^18) by their high
 // digit, bits [9, 18); block `c` (one per high digit, dealt to the XCDs in contiguous ranges) takes the
-// contiguous run of keys with high digit c and orders it stably by bits [0, 9): a digit histogram of the
-// run in LDS, then, chunk by chunk in input order, ranks by wave ballot matching (the scatter of
-// radix_sort.hip) added to each digit's running offset. High digit c then low digit d is bucket
+// contiguous run of keys with high digit c and orders it stably by bits [0, 9): chunk by chunk in input
+// order, ranks by wave ballot matching (the scatter of radix_sort.hip), the chunk reordered by digit in
+// LDS and written out in runs at each digit's running offset (a run longer than one chunk takes a digit
+// histogram first; one chunk long, as usual, its histogram is the chunk's). High digit c then low digit d is bucket
 // c * 512 + d, so the run holds every key of buckets c * 512 .. c * 512 + 511, and the histogram gives
 // their bounds directly (cl:85-113: start and end written for occupied buckets only). The block also
 // writes each particle's sorted slot (inv) and, GATHER, copies the sorted state by gathering through the
 // index (for steps whose pos / vel sit in L2 anyway; larger steps copy with k_scatter_sorted).
-constexpr int kRunItems = 8;  // keys per thread and chunk
+constexpr int kRunThreads = 512;     // one thread per low digit
+constexpr int kRunWaves = kRunThreads / kWave;
+constexpr int kRunItems = 4;         // keys per thread and chunk
 constexpr uint32_t kRunDigits = 512;
+constexpr uint32_t kRunChunk = kRunThreads * kRunItems;
 
 template <bool GATHER>
-__global__ __launch_bounds__(kThreads) void k_bucket_runs(const uint32_t* __restrict__ kin,
-                                                          const uint32_t* __restrict__ vin,
-                                                          const uint32_t* __restrict__ totals,
-                                                          uint32_t* __restrict__ cell, uint32_t* __restrict__ index,
-                                                          uint32_t* __restrict__ cell_start,
-                                                          uint32_t* __restrict__ cell_end, uint32_t* __restrict__ inv,
-                                                          const float4* __restrict__ pos,
-                                                          const float4* __restrict__ vel, float4* __restrict__ spos,
-                                                          float4* __restrict__ svel)
+__global__ __launch_bounds__(kRunThreads) void k_bucket_runs(const uint32_t* __restrict__ kin,
+                                                             const uint32_t* __restrict__ vin,
+                                                             const uint32_t* __restrict__ totals,
+                                                             uint32_t* __restrict__ cell, uint32_t* __restrict__ index,
+                                                             uint32_t* __restrict__ cell_start,
+                                                             uint32_t* __restrict__ cell_end, uint32_t* __restrict__ inv,
+                                                             const float4* __restrict__ pos,
+                                                             const float4* __restrict__ vel, float4* __restrict__ spos,
+                                                             float4* __restrict__ svel)
 {
-    constexpr int kWaves = kThreads / kWave;
     __shared__ uint32_t run_off[kRunDigits];          // histogram, then each digit's next slot in the run
-    __shared__ uint32_t s_cnt[kWaves * kRunDigits];   // a chunk's per-wave digit counts
-    __shared__ uint32_t s_wave[kWaves];
+    __shared__ uint32_t chunk_start[kRunDigits];      // each digit's first slot in the chunk's order
+    __shared__ uint32_t s_cnt[kRunWaves * kRunDigits];  // a chunk's per-wave digit counts
+    __shared__ uint32_t s_keys[kRunChunk];            // the chunk ordered by digit
+    __shared__ uint32_t s_vals[kRunChunk];
+    __shared__ uint32_t s_wave[kRunWaves];
     __shared__ uint32_t s_base;
     const uint32_t c = xcd_tile();
-    const uint32_t t = threadIdx.x, lane = t & (kWave - 1), wave = t / kWave;
-    const uint32_t d0 = 2u * t, d1 = d0 + 1u;  // this thread's two digits in the per-digit loops
+    const uint32_t t = threadIdx.x, lane = t & (kWave - 1), wave = t / kWave;  // thread t owns low digit t
     {
-        const uint32_t t0 = totals[d0], t1 = totals[d1];
         uint32_t all;
-        const uint32_t ex = block_scan<kThreads>(t0 + t1, s_wave, &all);
-        if (d0 == c) s_base = ex;
-        if (d1 == c) s_base = ex + t0;
-        run_off[d0] = 0;
-        run_off[d1] = 0;
+        const uint32_t ex = block_scan<kRunThreads>(totals[t], s_wave, &all);
+        if (t == c) s_base = ex;
+        run_off[t] = 0;
     }
     __syncthreads();
     const uint32_t base = s_base, cnt = totals[c];
     if (cnt == 0) return;  // block-uniform
     const uint32_t* rk = kin + base;
     const uint32_t* rv = vin + base;
-    for (uint32_t i = t; i < cnt; i += kThreads) atomicAdd(&run_off[rk[i] & (kRunDigits - 1u)], 1u);
-    __syncthreads();
-    {
-        const uint32_t h0 = run_off[d0], h1 = run_off[d1];
+    // bounds of the run's buckets from its digit histogram, whose exclusive scan starts each digit's slots
+    // (block-uniform call); returns the scan
+    auto bounds = [&](uint32_t h) {
         uint32_t all;
-        const uint32_t st = block_scan<kThreads>(h0 + h1, s_wave, &all);
-        const uint32_t b0 = (c << 9) | d0;
-        if (h0) {
-            cell_start[b0] = base + st;
-            cell_end[b0] = base + st + h0;
+        const uint32_t st = block_scan<kRunThreads>(h, s_wave, &all);
+        if (h) {
+            cell_start[(c << 9) | t] = base + st;
+            cell_end[(c << 9) | t] = base + st + h;
         }
-        if (h1) {
-            cell_start[b0 + 1] = base + st + h0;
-            cell_end[b0 + 1] = base + st + h0 + h1;
-        }
-        run_off[d0] = st;
-        run_off[d1] = st + h0;
+        run_off[t] = st;
+        return st;
+    };
+    // a run of one chunk takes its histogram from the chunk's ranking below; a longer one counts first
+    const bool one = cnt <= kRunChunk;
+    if (!one) {
+        for (uint32_t i = t; i < cnt; i += kRunThreads) atomicAdd(&run_off[rk[i] & (kRunDigits - 1u)], 1u);
+        __syncthreads();
+        bounds(run_off[t]);
     }
     const uint64_t lower_lanes = (1ull << lane) - 1ull;
-    for (uint32_t c0 = 0; c0 < cnt; c0 += kThreads * kRunItems) {
-        for (uint32_t i = t; i < kWaves * kRunDigits; i += kThreads) s_cnt[i] = 0;
+    for (uint32_t c0 = 0; c0 < cnt; c0 += kRunChunk) {
+        for (uint32_t i = t; i < kRunWaves * kRunDigits; i += kRunThreads) s_cnt[i] = 0;
         __syncthreads();
         // item r of wave w, lane l is key c0 + (w kRunItems + r) 64 + l: (wave, round, lane) order is input order
         uint32_t key[kRunItems], val[kRunItems], rank[kRunItems];
@@ -190,6 +193,11 @@ __global__ __launch_bounds__(kThreads) void k_bucket_runs(const uint32_t* __rest
             const bool valid = q < cnt;
             key[r] = valid ? rk[q] : 0u;
             val[r] = valid ? rv[q] : 0u;
+        }
+#pragma unroll
+        for (int r = 0; r < kRunItems; r++) {
+            const uint32_t q = c0 + (wave * kRunItems + r) * kWave + lane;
+            const bool valid = q < cnt;
             const uint32_t d = key[r] & (kRunDigits - 1u);
             // lanes whose digit differs in some bit (or that hold no key), per bit b: the ballot of bit b
             // XOR this lane's bit b replicated
@@ -213,34 +221,47 @@ __global__ __launch_bounds__(kThreads) void k_bucket_runs(const uint32_t* __rest
             rank[r] = before + (uint32_t)__popcll(same & lower_lanes);
         }
         __syncthreads();
-        uint32_t chunk0 = 0, chunk1 = 0;  // the chunk's keys of digits d0 and d1
+        uint32_t mine = 0;  // the chunk's keys of digit t
 #pragma unroll
-        for (int w = 0; w < kWaves; w++) {
-            const uint32_t a = s_cnt[w * kRunDigits + d0], b = s_cnt[w * kRunDigits + d1];
-            s_cnt[w * kRunDigits + d0] = chunk0;
-            s_cnt[w * kRunDigits + d1] = chunk1;
-            chunk0 += a;
-            chunk1 += b;
+        for (int w = 0; w < kRunWaves; w++) {
+            const uint32_t a = s_cnt[w * kRunDigits + t];
+            s_cnt[w * kRunDigits + t] = mine;
+            mine += a;
+        }
+        if (one) {
+            chunk_start[t] = bounds(mine);  // the chunk is the whole run
+        } else {
+            uint32_t all;
+            chunk_start[t] = block_scan<kRunThreads>(mine, s_wave, &all);
         }
         __syncthreads();
+        // the chunk in digit order in LDS, then written out in runs
 #pragma unroll
         for (int r = 0; r < kRunItems; r++) {
             const uint32_t q = c0 + (wave * kRunItems + r) * kWave + lane;
             if (q < cnt) {
                 const uint32_t d = key[r] & (kRunDigits - 1u);
-                const uint32_t o = base + run_off[d] + s_cnt[wave * kRunDigits + d] + rank[r];
-                cell[o] = key[r];
-                index[o] = val[r];
-                inv[val[r]] = o;
-                if (GATHER) {
-                    spos[o] = pos[val[r]];
-                    svel[o] = vel[val[r]];
-                }
+                const uint32_t p = chunk_start[d] + s_cnt[wave * kRunDigits + d] + rank[r];
+                s_keys[p] = key[r];
+                s_vals[p] = val[r];
             }
         }
         __syncthreads();
-        run_off[d0] += chunk0;
-        run_off[d1] += chunk1;
+        const uint32_t chunk_n = min(cnt - c0, kRunChunk);
+        for (uint32_t p = t; p < chunk_n; p += kRunThreads) {
+            const uint32_t k = s_keys[p], v = s_vals[p];
+            const uint32_t d = k & (kRunDigits - 1u);
+            const uint32_t o = base + run_off[d] + (p - chunk_start[d]);
+            cell[o] = k;
+            index[o] = v;
+            inv[v] = o;
+            if (GATHER) {
+                spos[o] = pos[v];
+                svel[o] = vel[v];
+            }
+        }
+        __syncthreads();
+        run_off[t] += mine;
     }
 }
 
@@ -1279,10 +1300,10 @@ int Fluid3D::enqueue_front(hipStream_t s)
     const bool gather = n < kScatterCopyMin;
     auto* inv = inv_.as<uint32_t>();
     if (gather)
-        hipLaunchKernelGGL(k_bucket_runs<true>, dim3(kRunDigits), b, 0, s, high.keys, high.vals, high.totals, cell,
+        hipLaunchKernelGGL(k_bucket_runs<true>, dim3(kRunDigits), dim3(kRunThreads), 0, s, high.keys, high.vals, high.totals, cell,
                            index, cs, ce, inv, pos, vel, spos, svel);
     else
-        hipLaunchKernelGGL(k_bucket_runs<false>, dim3(kRunDigits), b, 0, s, high.keys, high.vals, high.totals, cell,
+        hipLaunchKernelGGL(k_bucket_runs<false>, dim3(kRunDigits), dim3(kRunThreads), 0, s, high.keys, high.vals, high.totals, cell,
                            index, cs, ce, inv, pos, vel, spos, svel);
     MFL_HIP(hipGetLastError());
     timer.end(MFL_STAGE_SORT, s);
