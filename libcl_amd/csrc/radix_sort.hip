@@ -89,10 +89,130 @@ __global__ __launch_bounds__(kThreads) void k_scan_digits(uint32_t* __restrict__
     if (threadIdx.x == 0) totals[blockIdx.x] = carry;
 }
 
-// Stable scatter of one tile. Keys of wave w, round r, lane l sit at tile offset (w*ITEMS + r)*64 + l,
-// so (wave, round, lane) order is input order; a key's rank is
+// ---- one tile of a pass: keys of wave w, round r, lane l sit at tile offset (w*ITEMS + r)*64 + l, so
+// (wave, round, lane) order is input order; a key's rank in the tile is
 //   [same digit in earlier waves] + [same digit in earlier rounds of its wave] + [lower lanes, same round].
-// BITS > 0 fixes the digit width at compile time (the common 7, 8 and 9); BITS = 0 takes `bits_rt`.
+
+template <int ITEMS, bool IDENTITY_VALS>
+__device__ __forceinline__ void load_tile(const uint32_t* __restrict__ kin, const uint32_t* __restrict__ vin, uint32_t n,
+                                 size_t tile_base, uint32_t first, uint32_t* key, uint32_t* val)
+{
+    if (tile_base + kThreads * ITEMS <= n) {  // a whole tile: no bound tests
+        const uint32_t* kt = kin + tile_base + first;
+        const uint32_t* vt = vin + tile_base + first;
+#pragma unroll
+        for (int r = 0; r < ITEMS; r++) {
+            key[r] = kt[r * kWave];
+            val[r] = IDENTITY_VALS ? (uint32_t)(tile_base + first + r * kWave) : vt[r * kWave];
+        }
+    } else {
+#pragma unroll
+        for (int r = 0; r < ITEMS; r++) {
+            const size_t idx = tile_base + first + (size_t)r * kWave;
+            const bool valid = idx < n;
+            key[r] = valid ? kin[idx] : 0u;
+            if (IDENTITY_VALS)
+                val[r] = (uint32_t)idx;
+            else
+                val[r] = valid ? vin[idx] : 0u;
+        }
+    }
+}
+
+// rank[r] = the key's rank among its wave's keys of the same digit (in input order); s_cnt[wave * R + d]
+// (zeroed beforehand) ends as the wave's count of digit d.
+template <int ITEMS, int BITS, uint32_t R>
+__device__ __forceinline__ void rank_tile(const uint32_t* key, uint32_t n, size_t tile_base, uint32_t first, int shift,
+                                 int bits, uint32_t* s_cnt, uint32_t* rank)
+{
+    const int lane = threadIdx.x & (kWave - 1);
+    const int wave = threadIdx.x / kWave;
+    const uint32_t mask = (1u << bits) - 1u;
+    const uint64_t lower_lanes = (1ull << lane) - 1ull;
+#pragma unroll
+    for (int r = 0; r < ITEMS; r++) {
+        const size_t idx = tile_base + first + (size_t)r * kWave;
+        const bool valid = idx < n;
+        const uint32_t d = (key[r] >> shift) & mask;
+        // lanes whose digit differs from this lane's in some bit (or that hold no key): per bit b, the
+        // ballot of bit b XOR this lane's bit b replicated over 64 bits
+        const uint64_t live = __ballot(valid);
+        uint32_t differ_lo = ~(uint32_t)live, differ_hi = ~(uint32_t)(live >> 32);
+#pragma unroll
+        for (int b = 0; b < (BITS > 0 ? BITS : kMaxDigitBits); b++) {
+            if (BITS > 0 || b < bits) {
+                const uint32_t rep = (uint32_t)((int)(d << (31 - b)) >> 31);  // 0 or all ones
+                const uint64_t bal = __ballot(rep != 0u);
+                differ_lo |= (uint32_t)bal ^ rep;
+                differ_hi |= (uint32_t)(bal >> 32) ^ rep;
+            }
+        }
+        const uint64_t same = ~((uint64_t)differ_hi << 32 | differ_lo);
+        const int leader = __ffsll((unsigned long long)same) - 1;
+        uint32_t base = 0;
+        if (valid && lane == leader) {
+            base = s_cnt[wave * R + d];
+            s_cnt[wave * R + d] = base + (uint32_t)__popcll(same);
+        }
+        base = __shfl(base, leader & (kWave - 1), kWave);
+        rank[r] = base + (uint32_t)__popcll(same & lower_lanes);
+    }
+}
+
+// Turns column d of s_cnt (one count per wave) into offsets over the waves; returns the tile's count of
+// digit d (0 for d >= radix).
+template <uint32_t R>
+__device__ __forceinline__ uint32_t wave_offsets(uint32_t d, uint32_t radix, uint32_t* s_cnt)
+{
+    if (d >= radix) return 0u;
+    uint32_t run = 0;
+#pragma unroll
+    for (int w = 0; w < kWaves; w++) {
+        const uint32_t c = s_cnt[w * R + d];
+        s_cnt[w * R + d] = run;
+        run += c;
+    }
+    return run;
+}
+
+// The tile reordered by digit in LDS, then written out in runs: the key at tile position p, digit d,
+// goes to s_base[d] + (p - s_start[d]), s_start[d] being digit d's first position in the tile.
+template <int ITEMS, uint32_t R>
+__device__ __forceinline__ void write_runs(const uint32_t* key, const uint32_t* val, const uint32_t* rank, uint32_t n,
+                                  size_t tile_base, int shift, uint32_t mask, const uint32_t* s_cnt,
+                                  const uint32_t* s_start, const uint32_t* s_base, uint32_t* s_keys,
+                                  uint32_t* s_vals, uint32_t* __restrict__ kout, uint32_t* __restrict__ vout)
+{
+    constexpr int TILE = kThreads * ITEMS;
+    const int t = threadIdx.x;
+    const int lane = t & (kWave - 1);
+    const int wave = t / kWave;
+#pragma unroll
+    for (int r = 0; r < ITEMS; r++) {
+        const size_t idx = tile_base + ((size_t)(wave * ITEMS + r) << 6) + lane;
+        if (idx < n) {
+            const uint32_t d = (key[r] >> shift) & mask;
+            const uint32_t pos = s_start[d] + s_cnt[wave * R + d] + rank[r];
+            s_keys[pos] = key[r];
+            s_vals[pos] = val[r];
+        }
+    }
+    __syncthreads();
+    const size_t remain = n - tile_base;
+    const uint32_t tile_n = remain < (size_t)TILE ? (uint32_t)remain : (uint32_t)TILE;
+    for (uint32_t p = t; p < tile_n; p += kThreads) {
+        const uint32_t k = s_keys[p];
+        const uint32_t d = (k >> shift) & mask;
+        const uint32_t o = s_base[d] + (p - s_start[d]);
+        if (o < n) {  // always true for a consistent count table; keeps a corrupt one in bounds
+            kout[o] = k;
+            vout[o] = s_vals[p];
+        }
+    }
+}
+
+// Stable scatter of one tile by the scanned count table. BITS > 0 fixes the digit width at compile time
+// (the common 7, 8 and 9); BITS = 0 takes `bits_rt`.
 template <int ITEMS, bool IDENTITY_VALS, int BITS>
 __global__ __launch_bounds__(kThreads) void k_scatter(const uint32_t* __restrict__ kin, const uint32_t* __restrict__ vin,
                                                       uint32_t* __restrict__ kout, uint32_t* __restrict__ vout,
@@ -130,79 +250,11 @@ __global__ __launch_bounds__(kThreads) void k_scatter(const uint32_t* __restrict
 
     uint32_t key[ITEMS], val[ITEMS], rank[ITEMS];
     const uint32_t first = (uint32_t)(wave * ITEMS) * kWave + (uint32_t)lane;  // tile offset of item 0
-    if (tile_base + TILE <= n) {  // a whole tile: no bound tests
-        const uint32_t* kt = kin + tile_base + first;
-        const uint32_t* vt = vin + tile_base + first;
-#pragma unroll
-        for (int r = 0; r < ITEMS; r++) {
-            key[r] = kt[r * kWave];
-            val[r] = IDENTITY_VALS ? (uint32_t)(tile_base + first + r * kWave) : vt[r * kWave];
-        }
-    } else {
-#pragma unroll
-        for (int r = 0; r < ITEMS; r++) {
-            const size_t idx = tile_base + first + (size_t)r * kWave;
-            const bool valid = idx < n;
-            key[r] = valid ? kin[idx] : 0u;
-            if (IDENTITY_VALS)
-                val[r] = (uint32_t)idx;
-            else
-                val[r] = valid ? vin[idx] : 0u;
-        }
-    }
+    load_tile<ITEMS, IDENTITY_VALS>(kin, vin, n, tile_base, first, key, val);
     __syncthreads();
-
-    const uint64_t lower_lanes = (1ull << lane) - 1ull;
-#pragma unroll
-    for (int r = 0; r < ITEMS; r++) {
-        const size_t idx = tile_base + first + (size_t)r * kWave;
-        const bool valid = idx < n;
-        const uint32_t d = (key[r] >> shift) & mask;
-        // lanes whose digit differs from this lane's in some bit (or that hold no key): per bit b, the
-        // ballot of bit b XOR this lane's bit b replicated over 64 bits
-        const uint64_t live = __ballot(valid);
-        uint32_t differ_lo = ~(uint32_t)live, differ_hi = ~(uint32_t)(live >> 32);
-#pragma unroll
-        for (int b = 0; b < (BITS > 0 ? BITS : kMaxDigitBits); b++) {
-            if (BITS == 0 && b >= bits) break;
-            const uint32_t rep = (uint32_t)((int)(d << (31 - b)) >> 31);  // 0 or all ones
-            const uint64_t bal = __ballot(rep != 0u);
-            differ_lo |= (uint32_t)bal ^ rep;
-            differ_hi |= (uint32_t)(bal >> 32) ^ rep;
-        }
-        const uint64_t same = ~((uint64_t)differ_hi << 32 | differ_lo);
-        const int leader = __ffsll((unsigned long long)same) - 1;
-        uint32_t base = 0;
-        if (valid && lane == leader) {
-            base = s_cnt[wave * R + d];
-            s_cnt[wave * R + d] = base + (uint32_t)__popcll(same);
-        }
-        base = __shfl(base, leader & (kWave - 1), kWave);
-        rank[r] = base + (uint32_t)__popcll(same & lower_lanes);
-    }
+    rank_tile<ITEMS, BITS, R>(key, n, tile_base, first, shift, bits, s_cnt, rank);
     __syncthreads();
-
-    uint32_t tile_cnt0 = 0, tile_cnt1 = 0;
-    if (d0 < radix) {
-        uint32_t run = 0;
-#pragma unroll
-        for (int w = 0; w < kWaves; w++) {
-            const uint32_t c = s_cnt[w * R + d0];
-            s_cnt[w * R + d0] = run;
-            run += c;
-        }
-        tile_cnt0 = run;
-    }
-    if (d1 < radix) {
-        uint32_t run = 0;
-#pragma unroll
-        for (int w = 0; w < kWaves; w++) {
-            const uint32_t c = s_cnt[w * R + d1];
-            s_cnt[w * R + d1] = run;
-            run += c;
-        }
-        tile_cnt1 = run;
-    }
+    const uint32_t tile_cnt0 = wave_offsets<R>(d0, radix, s_cnt), tile_cnt1 = wave_offsets<R>(d1, radix, s_cnt);
     {
         uint32_t all;
         const uint32_t st = block_exclusive_scan(tile_cnt0 + tile_cnt1, s_wave, &all);
@@ -210,30 +262,8 @@ __global__ __launch_bounds__(kThreads) void k_scatter(const uint32_t* __restrict
         if (d1 < radix) s_start[d1] = st + tile_cnt0;
     }
     __syncthreads();
-
-#pragma unroll
-    for (int r = 0; r < ITEMS; r++) {
-        const size_t idx = tile_base + ((size_t)(wave * ITEMS + r) << 6) + lane;
-        if (idx < n) {
-            const uint32_t d = (key[r] >> shift) & mask;
-            const uint32_t pos = s_start[d] + s_cnt[wave * R + d] + rank[r];
-            s_keys[pos] = key[r];
-            s_vals[pos] = val[r];
-        }
-    }
-    __syncthreads();
-
-    const size_t remain = n - tile_base;
-    const uint32_t tile_n = remain < (size_t)TILE ? (uint32_t)remain : (uint32_t)TILE;
-    for (uint32_t p = t; p < tile_n; p += kThreads) {
-        const uint32_t k = s_keys[p];
-        const uint32_t d = (k >> shift) & mask;
-        const uint32_t o = s_base[d] + (p - s_start[d]);
-        if (o < n) {  // always true for a consistent count table; keeps a corrupt one in bounds
-            kout[o] = k;
-            vout[o] = s_vals[p];
-        }
-    }
+    write_runs<ITEMS, R>(key, val, rank, n, tile_base, shift, mask, s_cnt, s_start, s_base, s_keys, s_vals, kout,
+                         vout);
 }
 
 template <int ITEMS>
@@ -266,6 +296,15 @@ bool run_pass(const uint32_t* kin, const uint32_t* vin, uint32_t* kout, uint32_t
 #undef MFL_SCATTER
     return hip_ok(hipGetLastError(), "radix sort pass launch");
 }
+
+// Temporaries of one sort: key and value ping-pong buffers of n words each, the digit-major count
+// table (radix x tiles) and the per-digit totals.
+struct SortBuffers {
+    uint32_t* tk[2];
+    uint32_t* tv[2];
+    uint32_t* counts;
+    uint32_t* totals;
+};
 
 int pick_items(size_t n, int requested)
 {
@@ -350,15 +389,6 @@ int RadixSort::first_pass_plan(size_t n, int start_bit, int end_bit, SortCountPl
 }
 
 namespace {
-
-// Temporaries of one sort: key and value ping-pong buffers of n words each, the digit-major count
-// table (radix x tiles) and the per-digit totals.
-struct SortBuffers {
-    uint32_t* tk[2];
-    uint32_t* tv[2];
-    uint32_t* counts;
-    uint32_t* totals;
-};
 
 // The passes of a sort of n pairs (1 <= n < 2^32, 0 <= start_bit < end_bit <= 32) over buffers
 // that fit it.
