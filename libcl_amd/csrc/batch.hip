@@ -34,14 +34,24 @@ __global__ __launch_bounds__(kThreads) void k_b_hash_counts(const float4* __rest
     __syncthreads();
     const uint32_t tile = xcd_tile();
     const size_t base = (size_t)tile * (kThreads * ITEMS);
+    // every load issued before the first count (a clamped index keeps the loads unconditional)
+    float4 q[ITEMS];
+    uint32_t sc[ITEMS];
+#pragma unroll
+    for (int k = 0; k < ITEMS; k++) {
+        const size_t g = base + (size_t)k * kThreads + threadIdx.x;
+        const size_t gc = g < N ? g : N - 1;
+        q[k] = pos[gc];
+        sc[k] = pscene[gc];
+    }
 #pragma unroll
     for (int k = 0; k < ITEMS; k++) {
         const size_t g = base + (size_t)k * kThreads + threadIdx.x;
         if (g < N) {
-            const uint32_t s = pscene[g];
+            const uint32_t s = sc[k];
             const float h = params[s].cellSize;
-            const float4 q = pos[g];
-            const uint32_t key = (s << 18) | cell_hash(grid_coord(q.x, h), grid_coord(q.y, h), grid_coord(q.z, h));
+            const uint32_t key =
+                (s << 18) | cell_hash(grid_coord(q[k].x, h), grid_coord(q[k].y, h), grid_coord(q[k].z, h));
             cell[g] = key;
             atomicAdd(&hist[key & mask], 1u);
         }

@@ -73,12 +73,18 @@ __global__ __launch_bounds__(kThreads) void k_hash_counts(const float4* __restri
     __syncthreads();
     const uint32_t tile = xcd_tile();
     const size_t base = (size_t)tile * (kThreads * ITEMS);
+    // every load issued before the first count (a clamped index keeps the loads unconditional)
+    float4 q[ITEMS];
+#pragma unroll
+    for (int k = 0; k < ITEMS; k++) {
+        const size_t i = base + (size_t)k * kThreads + threadIdx.x;
+        q[k] = pos[i < n ? i : n - 1];
+    }
 #pragma unroll
     for (int k = 0; k < ITEMS; k++) {
         const size_t i = base + (size_t)k * kThreads + threadIdx.x;
         if (i < n) {
-            const float4 q = pos[i];
-            const uint32_t key = cell_hash(grid_coord(q.x, h), grid_coord(q.y, h), grid_coord(q.z, h));
+            const uint32_t key = cell_hash(grid_coord(q[k].x, h), grid_coord(q[k].y, h), grid_coord(q[k].z, h));
             cell[i] = key;
             atomicAdd(&hist[(key >> shift) & mask], 1u);
         }
@@ -127,7 +133,7 @@ constexpr uint32_t kScatterCopyMin = 1u << 18;
 // index (for steps whose pos / vel sit in L2 anyway; larger steps copy with k_scatter_sorted).
 constexpr int kRunThreads = 512;     // one thread per low digit
 constexpr int kRunWaves = kRunThreads / kWave;
-constexpr int kRunItems = 4;         // keys per thread and chunk
+constexpr int kRunItems = 8;         // keys per thread and chunk
 constexpr uint32_t kRunDigits = 512;
 constexpr uint32_t kRunChunk = kRunThreads * kRunItems;
 
@@ -177,7 +183,18 @@ __global__ __launch_bounds__(kRunThreads) void k_bucket_runs(const uint32_t* __r
     // a run of one chunk takes its histogram from the chunk's ranking below; a longer one counts first
     const bool one = cnt <= kRunChunk;
     if (!one) {
-        for (uint32_t i = t; i < cnt; i += kRunThreads) atomicAdd(&run_off[rk[i] & (kRunDigits - 1u)], 1u);
+        // four loads in flight per thread (a clamped index keeps them unconditional)
+        for (uint32_t i0 = t; i0 < cnt; i0 += 4 * kRunThreads) {
+            uint32_t k4[4];
+#pragma unroll
+            for (int r = 0; r < 4; r++) {
+                const uint32_t i = i0 + r * kRunThreads;
+                k4[r] = rk[i < cnt ? i : cnt - 1];
+            }
+#pragma unroll
+            for (int r = 0; r < 4; r++)
+                if (i0 + r * kRunThreads < cnt) atomicAdd(&run_off[k4[r] & (kRunDigits - 1u)], 1u);
+        }
         __syncthreads();
         bounds(run_off[t]);
     }

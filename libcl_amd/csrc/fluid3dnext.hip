@@ -81,11 +81,18 @@ __global__ __launch_bounds__(kBlock) void k_next_hash_counts(const float4* __res
     __syncthreads();
     const uint32_t tile = xcd_tile();
     const size_t base = (size_t)tile * (kBlock * ITEMS);
+    // every load issued before the first count (a clamped index keeps the loads unconditional)
+    float4 sv[ITEMS];
+#pragma unroll
+    for (int k = 0; k < ITEMS; k++) {
+        const size_t g = base + (size_t)k * kBlock + threadIdx.x;
+        sv[k] = state[g < n ? g : n - 1];
+    }
 #pragma unroll
     for (int k = 0; k < ITEMS; k++) {
         const size_t g = base + (size_t)k * kBlock + threadIdx.x;
         if (g < n) {
-            const float4 st = state[g];
+            const float4 st = sv[k];
             const int gx = (int)floorf(st.x / p.cellSize), gy = (int)floorf(st.y / p.cellSize);
             const uint32_t key = (uint32_t)gy * (uint32_t)p.cellCountX + (uint32_t)gx;
             cell[g] = key;

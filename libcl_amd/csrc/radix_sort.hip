@@ -42,14 +42,17 @@ __global__ __launch_bounds__(kThreads) void k_tile_counts(const uint32_t* __rest
     if (base + kThreads * ITEMS <= n && (reinterpret_cast<uintptr_t>(keys) & 15u) == 0) {
         // a whole tile of a 16-byte aligned array, read 16 bytes per load (a histogram does not depend
         // on the order its keys are counted in)
+        // every load issued before the first count (else each load waits for the previous one's counts)
         const uint4* k4 = reinterpret_cast<const uint4*>(keys + base);
+        uint4 q[ITEMS / 4];
+#pragma unroll
+        for (int k = 0; k < ITEMS / 4; k++) q[k] = k4[k * kThreads + threadIdx.x];
 #pragma unroll
         for (int k = 0; k < ITEMS / 4; k++) {
-            const uint4 q = k4[k * kThreads + threadIdx.x];
-            atomicAdd(&hist[(q.x >> shift) & mask], 1u);
-            atomicAdd(&hist[(q.y >> shift) & mask], 1u);
-            atomicAdd(&hist[(q.z >> shift) & mask], 1u);
-            atomicAdd(&hist[(q.w >> shift) & mask], 1u);
+            atomicAdd(&hist[(q[k].x >> shift) & mask], 1u);
+            atomicAdd(&hist[(q[k].y >> shift) & mask], 1u);
+            atomicAdd(&hist[(q[k].z >> shift) & mask], 1u);
+            atomicAdd(&hist[(q[k].w >> shift) & mask], 1u);
         }
     } else {
 #pragma unroll
@@ -102,8 +105,9 @@ __device__ __forceinline__ void load_tile(const uint32_t* __restrict__ kin, cons
         const uint32_t* vt = vin + tile_base + first;
 #pragma unroll
         for (int r = 0; r < ITEMS; r++) {
-            key[r] = kt[r * kWave];
-            val[r] = IDENTITY_VALS ? (uint32_t)(tile_base + first + r * kWave) : vt[r * kWave];
+            // read once: non-temporal, so that the partial lines of the tile's output runs stay in L2
+            key[r] = __builtin_nontemporal_load(kt + r * kWave);
+            val[r] = IDENTITY_VALS ? (uint32_t)(tile_base + first + r * kWave) : __builtin_nontemporal_load(vt + r * kWave);
         }
     } else {
 #pragma unroll
