@@ -955,9 +955,14 @@ __global__ __launch_bounds__(kThreads) void k_forces_kept(const float4* __restri
             const uint32_t trips = (nw + P - 1) / P;
             const uint64_t hw = w == 0 ? hm : (member && nw ? hitx[hitx_at(t, P, slot, w)] : 0ull);
             const size_t kbase = (size_t)gseg * kSegCap + off + w0;
-            // kKeptChunk entries at a time
+            // kKeptChunk entries at a time; the next chunk's index loads are in flight during a chunk's sweep
+            uint32_t kidx[SEGS];
+            kept_chunk_index<W>(kept_in, kbase, nw, 0u, L.lane, kidx);
             for (uint32_t c0 = 0; c0 < nmax; c0 += chunk) {
-                stage_kept_chunk<W>(kept_in, kbase, nw, c0, L.lane, spos, svel, wpos, wvel);
+                kept_chunk_load<W>(kidx, nw, c0, L.lane, spos, svel, wpos, wvel);
+                __builtin_amdgcn_s_waitcnt(0x0F70);  // the states are in LDS (vmcnt(0))
+                // issued after that wait, so that no LDS read of the sweep waits for them
+                if (c0 + chunk < nmax) kept_chunk_index<W>(kept_in, kbase, nw, c0 + chunk, L.lane, kidx);
                 wave_sync();
                 if (member) {
                     uint32_t bits = chunk_bits<P>(hw, trips, c0);

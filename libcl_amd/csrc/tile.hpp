@@ -478,26 +478,35 @@ __device__ inline uint32_t chunk_bits(uint64_t hm, uint32_t trips, uint32_t c0)
 // kept_in[kbase] and holds nw entries, both per segment) and the states they name go straight from
 // global memory to LDS (global_load_lds_dwordx4: lane l fills slot l of the 1 KiB block given), so no
 // state passes through registers. Lanes past a window load spos / svel[0] into slots never used.
+// In two halves, so that a chunk's index loads can be in flight while the chunk before it is swept:
+// kept_chunk_index issues the index loads of chunk c0 (raw values, used by kept_chunk_load only),
+// kept_chunk_load turns them into state loads into LDS and returns without waiting for those.
 template <int W>
-__device__ inline void stage_kept_chunk(const uint32_t* __restrict__ kept_in, size_t kbase, uint32_t nw, uint32_t c0,
-                                        uint32_t lane, const float4* __restrict__ spos,
-                                        const float4* __restrict__ svel, float4* wpos, float4* wvel)
+__device__ inline void kept_chunk_index(const uint32_t* __restrict__ kept_in, size_t kbase, uint32_t nw, uint32_t c0,
+                                        uint32_t lane, uint32_t* k)
+{
+    constexpr uint32_t SEGS = kWave / W;
+#pragma unroll
+    for (uint32_t e = 0; e < SEGS; e++) {
+        const size_t b = ((size_t)__builtin_amdgcn_readlane((uint32_t)(kbase >> 32), e * W) << 32) |
+                         __builtin_amdgcn_readlane((uint32_t)kbase, e * W);
+        const uint32_t m = __builtin_amdgcn_readlane(nw, e * W);
+        k[e] = kept_in[b + min(c0 + lane, (m > 0 ? m : 1u) - 1u)];  // inside the segment's list
+    }
+}
+
+template <int W>
+__device__ inline void kept_chunk_load(const uint32_t* k, uint32_t nw, uint32_t c0, uint32_t lane,
+                                       const float4* __restrict__ spos, const float4* __restrict__ svel, float4* wpos,
+                                       float4* wvel)
 {
     static_assert(kKeptChunk == kWave, "a chunk of one segment is one entry per lane");
     constexpr uint32_t SEGS = kWave / W;
     uint32_t j[SEGS];
-#pragma unroll
-    for (uint32_t e = 0; e < SEGS; e++) {
-        // the segment's window, from its first lane (wave-uniform values)
-        const size_t b = ((size_t)__builtin_amdgcn_readlane((uint32_t)(kbase >> 32), e * W) << 32) |
-                         __builtin_amdgcn_readlane((uint32_t)kbase, e * W);
-        const uint32_t m = __builtin_amdgcn_readlane(nw, e * W);
-        const uint32_t q = c0 + lane;
-        const uint32_t k = kept_in[b + min(q, (m > 0 ? m : 1u) - 1u)];  // inside the segment's list
-        j[e] = q < m ? k : 0u;
-    }
     // every index before the first state load (s_waitcnt vmcnt(0) alone: expcnt 7, lgkmcnt 15)
     __builtin_amdgcn_s_waitcnt(0x0F70);
+#pragma unroll
+    for (uint32_t e = 0; e < SEGS; e++) j[e] = c0 + lane < __builtin_amdgcn_readlane(nw, e * W) ? k[e] : 0u;
 #if defined(__HIP_DEVICE_COMPILE__)  // the LDS address space exists in the device pass only
 #pragma unroll
     for (uint32_t e = 0; e < SEGS; e++) {
@@ -507,7 +516,6 @@ __device__ inline void stage_kept_chunk(const uint32_t* __restrict__ kept_in, si
                                          16, 0, 0);
     }
 #endif
-    __builtin_amdgcn_s_waitcnt(0);  // the states are in LDS
 }
 
 // Trip count of a staged neighbourhood of `kept` entries for the wave: a lane tests entries P m + slot
