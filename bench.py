@@ -555,7 +555,7 @@ def main():
         res = f.autotune(steps=3, stream=stream)
         assert res is not None, L.last_error()
         tuned = {"ms_per_step": res[1], "pick": res[0]}
-    tuning = dict(f.getTuning(), traversal=args.order)
+    tuning = f.getTuning()  # with the traversal order (the autotuner may change it)
 
     # the whole trajectory from the initial state: HIP events on the step stream at steps 0, 1, 10, 100
     # and the last (each step's cost grows as the flow develops)
@@ -668,7 +668,7 @@ def main():
         "dtype": "f32",
         "data": "synthetic (seeded jittered lattice, SplitMix64)",
         "config": {"workload": sc.name, "particles": n, "cell_size": 0.02, "box": [0.0, 4.0],
-                   "buckets": 262144, "bucket_clear": 65536, "traversal": args.order,
+                   "buckets": 262144, "bucket_clear": 65536, "traversal": tuning["traversal"],
                    "tuning": f.getTuning(), "autotuned": bool(args.autotune),
                    "timed_from_step": args.warmup},
         "roofline": roofline,

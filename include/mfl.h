@@ -173,12 +173,19 @@ int mfl_fluid3d_set_use_graph(mfl_fluid3d* f, int enable);
    MFL_NEIGHBOUR_COUNT buffer. Off by default (4 bytes per particle more per step). */
 int mfl_fluid3d_set_record_counts(mfl_fluid3d* f, int enable);
 
-/* Addition: the order in which the density and force passes visit particles. It changes no result
-   (every particle's sums are formed by one thread in the reference's visit order), only cache
-   locality: SORTED walks the hash-sorted arrays, CALLER walks the caller's particle order, MORTON
-   walks a Morton order of the particles' cells rebuilt every `refresh_steps` steps (0 = once). */
+/* Addition: the order in which the density and force passes visit particles. Visit sets, counts and
+   acceptance do not depend on it; the LIST pass's sums do not either (one thread per particle, in the
+   reference's visit order), the tile passes' sums only in FP32 summation order (the particles that
+   share a staged neighbourhood set how a particle's visits split over its lanes). It is there for
+   cache locality: SORTED walks the hash-sorted arrays, CALLER walks the caller's particle order, MORTON
+   walks a Morton order of the particles' cells rebuilt every `refresh_steps` steps (0 = once).
+   Until this is called, mfl_fluid3d_set_particle_count picks the order by size: SORTED up to 131,072
+   particles (their state fits one XCD's L2, so the hash order costs no locality, and it never goes
+   stale between refreshes), MORTON with a 16-step refresh above. */
 enum mfl_traversal { MFL_TRAVERSAL_SORTED = 0, MFL_TRAVERSAL_CALLER = 1, MFL_TRAVERSAL_MORTON = 2 };
 int mfl_fluid3d_set_traversal(mfl_fluid3d* f, int order, uint32_t refresh_steps);
+/* The current order and refresh period. */
+int mfl_fluid3d_get_traversal(mfl_fluid3d* f, int* order, uint32_t* refresh_steps);
 
 /* Addition: launch shape of the neighbour passes. Results differ between shapes only in FP32
    summation order (visit sets, counts and acceptance are identical).

@@ -33,6 +33,7 @@ EXPORTS = (
     "mfl_fluid3d_get_buffer", "mfl_fluid3d_write_buffer", "mfl_fluid3d_read_buffer", "mfl_fluid3d_params",
     "mfl_fluid3d_set_box", "mfl_fluid3d_set_gravity", "mfl_fluid3d_set_bucket_clear",
     "mfl_fluid3d_set_use_graph", "mfl_fluid3d_set_record_counts", "mfl_fluid3d_set_traversal",
+    "mfl_fluid3d_get_traversal",
     "mfl_fluid3d_set_profiling",
     "mfl_fluid3d_get_tuning", "mfl_fluid3d_set_tuning", "mfl_fluid3d_autotune",
     "mfl_fluid3d_stage_time",
@@ -153,6 +154,8 @@ def load() -> ctypes.CDLL:
         "mfl_fluid3d_set_use_graph": (ctypes.c_int, [vp, ctypes.c_int]),
         "mfl_fluid3d_set_record_counts": (ctypes.c_int, [vp, ctypes.c_int]),
         "mfl_fluid3d_set_traversal": (ctypes.c_int, [vp, ctypes.c_int, ctypes.c_uint32]),
+        "mfl_fluid3d_get_traversal": (ctypes.c_int, [vp, ctypes.POINTER(ctypes.c_int),
+                                                     ctypes.POINTER(ctypes.c_uint32)]),
         "mfl_fluid3d_set_profiling": (ctypes.c_int, [vp, ctypes.c_uint32]),
         "mfl_fluid3d_get_tuning": (ctypes.c_int, [vp, ctypes.POINTER(Tuning)]),
         "mfl_fluid3d_set_tuning": (ctypes.c_int, [vp, ctypes.POINTER(Tuning)]),

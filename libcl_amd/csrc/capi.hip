@@ -307,6 +307,15 @@ int mfl_fluid3d_set_traversal(mfl_fluid3d* f, int order, uint32_t refresh_steps)
     }
     f->impl.traversal = order;
     f->impl.refresh_interval = refresh_steps;
+    f->impl.traversal_chosen = true;
+    return 1;
+}
+
+int mfl_fluid3d_get_traversal(mfl_fluid3d* f, int* order, uint32_t* refresh_steps)
+{
+    if (!need(f, "fluid")) return 0;
+    if (order) *order = f->impl.traversal;
+    if (refresh_steps) *refresh_steps = f->impl.refresh_interval;
     return 1;
 }
 

@@ -1190,6 +1190,9 @@ int Fluid3D::set_particle_count(size_t n)
         return 0;
     }
     n_ = n;
+    // up to 2^17 particles (32 bytes of position and velocity each: one XCD's L2) the hash-sorted order
+    // loses no locality and, unlike a Morton order, never goes stale between refreshes
+    if (!traversal_chosen) traversal = n <= ((size_t)1 << 17) ? MFL_TRAVERSAL_SORTED : MFL_TRAVERSAL_MORTON;
     return allocate_internal() ? 1 : 0;
 }
 

@@ -58,6 +58,7 @@ public:
     bool record_counts = false;             // density passes write each particle's accepted visits
     int traversal = MFL_TRAVERSAL_MORTON;   // order in which the neighbour passes visit particles
     uint32_t refresh_interval = 16;         // steps between Morton order refreshes
+    bool traversal_chosen = false;          // set by the caller: set_particle_count keeps it
     // launch shape (mfl_tuning); the defaults are the fastest shape measured on the 1M-particle step
     int neighbour_pass = MFL_PASS_TILE_KEPT;  // LIST, TILE, TILE_LIST or TILE_KEPT neighbour passes
     int density_lanes = 2;                  // lanes per particle in the density pass

@@ -69,25 +69,37 @@ __global__ __launch_bounds__(kThreads) void k_tile_counts(const uint32_t* __rest
 __global__ __launch_bounds__(kThreads) void k_scan_digits(uint32_t* __restrict__ counts, uint32_t ntiles,
                                                           uint32_t* __restrict__ totals)
 {
+    // rounds of kThreads x 4 consecutive entries (thread t takes 4 t .. 4 t + 3 of a round: 16-byte
+    // strides across the lanes); the loads of four rounds are in flight together (clamped indices)
+    constexpr uint32_t ROUND = kThreads * 4, ROUNDS = 4;
     __shared__ uint32_t s_wave[kWaves];
     uint32_t* row = counts + (size_t)blockIdx.x * ntiles;
     uint32_t carry = 0;
-    for (uint32_t base = 0; base < ntiles; base += kThreads * 4) {
-        const uint32_t i0 = base + threadIdx.x * 4;
-        uint32_t c[4], sum = 0;
+    for (uint32_t base = 0; base < ntiles; base += ROUND * ROUNDS) {
+        uint32_t c[ROUNDS][4];
 #pragma unroll
-        for (int k = 0; k < 4; k++) {
-            c[k] = (i0 + k < ntiles) ? row[i0 + k] : 0u;
-            sum += c[k];
-        }
-        uint32_t tot;
-        uint32_t run = carry + block_exclusive_scan(sum, s_wave, &tot);
+        for (uint32_t r = 0; r < ROUNDS; r++)
 #pragma unroll
-        for (int k = 0; k < 4; k++) {
-            if (i0 + k < ntiles) row[i0 + k] = run;
-            run += c[k];
+            for (uint32_t k = 0; k < 4; k++) c[r][k] = row[min(base + r * ROUND + threadIdx.x * 4 + k, ntiles - 1)];
+#pragma unroll
+        for (uint32_t r = 0; r < ROUNDS; r++) {
+            const uint32_t i0 = base + r * ROUND + threadIdx.x * 4;
+            if (base + r * ROUND >= ntiles) break;  // block-uniform
+            uint32_t sum = 0;
+#pragma unroll
+            for (uint32_t k = 0; k < 4; k++) {
+                if (i0 + k >= ntiles) c[r][k] = 0u;
+                sum += c[r][k];
+            }
+            uint32_t tot;
+            uint32_t run = carry + block_exclusive_scan(sum, s_wave, &tot);
+#pragma unroll
+            for (uint32_t k = 0; k < 4; k++) {
+                if (i0 + k < ntiles) row[i0 + k] = run;
+                run += c[r][k];
+            }
+            carry += tot;
         }
-        carry += tot;
     }
     if (threadIdx.x == 0) totals[blockIdx.x] = carry;
 }

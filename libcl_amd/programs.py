@@ -212,19 +212,32 @@ class oclFluid3D:
         """Visit order of the neighbour passes: "sorted", "caller" or "morton" (results unchanged)."""
         return int(N.load().mfl_fluid3d_set_traversal(self._h, N.TRAVERSALS.index(order), refresh_steps))
 
+    def getTraversal(self) -> tuple:
+        """(order name, refresh period) of the neighbour passes' visit order."""
+        order, refresh = ctypes.c_int(), ctypes.c_uint32()
+        if not N.load().mfl_fluid3d_get_traversal(self._h, ctypes.byref(order), ctypes.byref(refresh)):
+            raise RuntimeError(N.load().mfl_last_error().decode())
+        return N.TRAVERSALS[order.value], refresh.value
+
     def getTuning(self) -> dict:
-        """Launch shape of the neighbour passes and the sort (mfl_tuning as a dict)."""
+        """Launch shape of the neighbour passes and the sort (mfl_tuning as a dict), and the traversal
+        order."""
         t = N.Tuning()
         if not N.load().mfl_fluid3d_get_tuning(self._h, ctypes.byref(t)):
             raise RuntimeError(N.load().mfl_last_error().decode())
-        return t.as_dict()
+        return dict(t.as_dict(), traversal=self.getTraversal()[0])
 
     def setTuning(self, **kw) -> int:
         """Changes fields of the tuning: neighbour_pass ("list" / "tile" / "tile_list" / "tile_kept"),
         density_lanes,
-        tile_depth, block_waves, sort_items, force_chunks. Results differ between shapes only in FP32
-        summation order."""
+        tile_depth, block_waves, sort_items, force_chunks, and the traversal order (its refresh period
+        kept). Results differ between shapes only in FP32 summation order."""
+        if "traversal" in kw:
+            kw = dict(kw)
+            if not self.setTraversal(kw.pop("traversal"), self.getTraversal()[1]):
+                return 0
         cur = self.getTuning()
+        cur.pop("traversal")
         if "density_lanes" not in kw and "neighbour_pass" in kw:
             # keep the lane count valid for the pass being selected (list: 1 or 8, tiles: 2, 4 or 8)
             ok = (1, 8) if kw["neighbour_pass"] == "list" else (2, 4, 8)
@@ -242,7 +255,7 @@ class oclFluid3D:
         ms = ctypes.c_double()
         if not N.load().mfl_fluid3d_autotune(self._h, _stream_ptr(stream), steps, ctypes.byref(t), ctypes.byref(ms)):
             return None
-        return t.as_dict(), ms.value
+        return dict(t.as_dict(), traversal=self.getTraversal()[0]), ms.value
 
     def setProfiling(self, stages) -> int:
         """Times the named stages (subset of _native.STAGES) with HIP events (oclKernel::profile)."""

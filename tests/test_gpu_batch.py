@@ -39,6 +39,9 @@ def _plain(sc, prm, steps, tuning):
     f = L.oclFluid3D()
     assert f.compile() == 1
     assert f.setTuning(**tuning) == 1, L.last_error()
+    # the batch's traversal (the single-scene default for this size would be the sorted order, which
+    # splits visits over a particle's lanes differently in the tile passes)
+    assert f.setTraversal("morton", 16) == 1
     assert f.setParticleCount(sc.n) == 1
     f.setParameters(prm)
     f.setBox(sc.box_min, sc.box_max)
@@ -123,9 +126,9 @@ def test_batch_per_scene_params_and_boxes():
     for s, sc in enumerate(scs):
         print(f"scene {s} n={sc.n} h={prms[s].cellSize}", flush=True)
         f = L.oclFluid3D()
-        assert f.compile() == 1 and f.setParticleCount(sc.n) == 1
+        assert f.compile() == 1 and f.setTraversal("morton", 16) == 1 and f.setParticleCount(sc.n) == 1
         assert f.setTuning(neighbour_pass="tile_kept", density_lanes=4, tile_depth=192, block_waves=2,
-                           force_chunks=2) == 1  # the batch's default shape
+                           force_chunks=2) == 1  # the batch's default shape and traversal
         f.setParameters(prms[s])
         assert f.setBox(*boxes[s]) == 1
         assert f.upload(sc.pos, sc.vel, sc.force) == 1
