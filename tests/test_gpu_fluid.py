@@ -210,6 +210,33 @@ def test_dense_blocks_vs_oracle(oracle, spacing, shape):
     assert nw <= TOL and per <= TOL, (nw, per)
 
 
+@pytest.mark.parametrize("total", [5120, 262144])
+def test_bucket_run_longer_than_a_chunk(oracle, total):
+    """5,120 particles packed into one cell (one bucket, so one high-digit run longer than the
+    4,096 keys the per-run sort ranks at a time: counted first, then ranked in two chunks), alone and
+    among 257,024 particles of a dam break (the run pass then writes no sorted state and the copy pass
+    does). Keys, permutation, tables and counts bit-exact; density and pressure within 1e-5."""
+    import libcl_amd as L
+    h = 0.02
+    print(f"long bucket run, {total} particles", flush=True)
+    packed = scenes.lattice(16, 16, 20, 0.024 * h, (10.25 * h,) * 3, 0.002 * h, 47)
+    if total > packed.shape[0]:
+        sc = scenes.dam_break(64, 64, 64, 2.0, 7, 1)
+        pos = np.concatenate([sc.pos[:total - packed.shape[0]], packed])
+        box_min, box_max, prm = sc.box_min, sc.box_max, L.params_from(sc.params)
+    else:
+        pos = packed
+        box_min, box_max = (0.0, 0.0, 0.0, 0.0), (1.0, 1.0, 1.0, 0.0)
+        prm = L.params_from(scenes.sph_param_values(h))
+    vel = np.zeros_like(pos)
+    force = np.tile(np.asarray(scenes.GRAVITY_FORCE, np.float32), (pos.shape[0], 1))
+    f = make_fluid(pos, vel, force, prm, box_min, box_max)
+    run_forces_only(f)
+    fo = oracle.Fluid(pos, vel, force, oracle.Params(*prm.values()), box_min=box_min, box_max=box_max)
+    fo.step(integrate=False)
+    _oracle_compare(f, fo)
+
+
 @pytest.mark.parametrize("shape", ["tile_list", "tile_list8", "tile", "tile_kept", "tile_kept2", "list"])
 def test_1m_step_vs_oracle(oracle, shape):
     """1,048,576 particles in [0,4]^3: keys, permutation, bucket tables bit-exact; density, pressure
