@@ -12,7 +12,8 @@
 //   sort            stable key/index sort over bits [0, 18) (cpp:299; keys < 2^18): one counted
 //                   pass by bits [9, 18), then k_bucket_runs orders each high digit's run by bits
 //                   [0, 9) and writes the inverse permutation and the start / end of each occupied
-//                   bucket (cl:85-113) from the same histograms
+//                   bucket (cl:85-113) from the same histograms; up to 2^14 particles there is no
+//                   pass: each k_bucket_runs block selects its high digit's keys from all of them
 //   k_scatter_sorted sorted pos/vel copy, written from the caller's order   (cl:66-72)
 //   density pass    27-bucket neighbourhood, poly6, pressure                (cl:150-200)
 //   force pass      27-bucket pressure + viscosity forces, clip box, clamp, Euler, gravity
@@ -121,6 +122,11 @@ __global__ __launch_bounds__(kThreads) void k_reorder_clear(const uint32_t* __re
 // 2^18 particles hold 8 MiB of pos + vel, twice an XCD's L2.
 constexpr uint32_t kScatterCopyMin = 1u << 18;
 
+// Largest particle count whose step takes no sort pass: each of the run kernel's 512 blocks reads all
+// the keys (64 KiB, from L2; 32 per thread, all loads in flight together) and selects its high digit's
+// own, which costs it less than the pass's two launches (digit scan, scatter) cost the step.
+constexpr uint32_t kSelectMax = 1u << 14;
+
 // Second half of the step's sort. RadixSort::counted_pass has ordered the keys (< 2^18) by their high
 // digit, bits [9, 18); block `c` (one per high digit, dealt to the XCDs in contiguous ranges) takes the
 // contiguous run of keys with high digit c and orders it stably by bits [0, 9): chunk by chunk in input
@@ -137,7 +143,7 @@ constexpr int kRunItems = 8;         // keys per thread and chunk
 constexpr uint32_t kRunDigits = 512;
 constexpr uint32_t kRunChunk = kRunThreads * kRunItems;
 
-template <bool GATHER>
+template <bool GATHER, bool SELECT>
 __global__ __launch_bounds__(kRunThreads) void k_bucket_runs(const uint32_t* __restrict__ kin,
                                                              const uint32_t* __restrict__ vin,
                                                              const uint32_t* __restrict__ totals,
@@ -157,15 +163,121 @@ __global__ __launch_bounds__(kRunThreads) void k_bucket_runs(const uint32_t* __r
     __shared__ uint32_t s_base;
     const uint32_t c = xcd_tile();
     const uint32_t t = threadIdx.x, lane = t & (kWave - 1), wave = t / kWave;  // thread t owns low digit t
-    {
+    const uint64_t lower_lanes = (1ull << lane) - 1ull;
+    // SELECT: the keys are the hash kernel's, in input order (kin[0 .. n), n <= kSelectMax, followed by
+    // 0xFFFFFFFF up to kSelectMax, which no digit matches); every block reads all of them and takes
+    // those of its high digit. Wave w reads the contiguous keys w kSelectPer .. (w + 1) kSelectPer - 1
+    // (round r, lane l: key w kSelectPer + 64 r + l) in batches of 8 rounds, the next batch's loads in
+    // flight while one is counted, so a match's rank in input order is the matches of the earlier waves
+    // plus those of the earlier rounds and lower lanes of its own wave. The count also stages each wave's
+    // first kSelectStage matches, in order, in its own slice of s_keys / s_vals, and builds the run's
+    // low-digit histogram in run_off; after one exchange of the wave totals the staged matches move to
+    // their ranks. A wave with more matches, or a run longer than a chunk, takes sel_store instead: a
+    // second read of the keys that stores the matches of ranks skip .. skip + kRunChunk - 1.
+    constexpr uint32_t kSelectPer = kSelectMax / kRunWaves, kSelectRounds = kSelectPer / kWave;
+    constexpr uint32_t kSelectBatch = 8, kSelectStage = kRunChunk / kRunWaves;
+    __shared__ uint32_t s_sel[2 * kRunWaves];
+    const uint32_t* src = kin + wave * kSelectPer + lane;
+    auto sel_store = [&](uint32_t first_rank, uint32_t skip) {
+        uint32_t rank = first_rank;
+        for (uint32_t b = 0; b < kSelectRounds; b += kSelectBatch) {
+            uint32_t k[kSelectBatch];
+#pragma unroll
+            for (uint32_t r = 0; r < kSelectBatch; r++) k[r] = src[(b + r) * kWave];
+#pragma unroll
+            for (uint32_t r = 0; r < kSelectBatch; r++) {
+                const bool m = (k[r] >> 9) == c;
+                const uint64_t bal = __ballot(m);
+                const uint32_t at = rank + (uint32_t)__popcll(bal & lower_lanes);
+                if (m && at >= skip && at - skip < kRunChunk) {
+                    s_keys[at - skip] = k[r];
+                    s_vals[at - skip] = wave * kSelectPer + (b + r) * kWave + lane;
+                }
+                rank += (uint32_t)__popcll(bal);
+            }
+        }
+    };
+    uint32_t base, cnt, first_rank = 0;
+    bool staged = true;  // every match of the run in s_keys / s_vals after the select (else sel_store)
+    if constexpr (SELECT) {
+        run_off[t] = 0;
+        __syncthreads();
+        uint32_t mine = 0, below = 0;  // the wave's matches so far (uniform); keys below digit c (per lane)
+        uint32_t ka[kSelectBatch], kb[kSelectBatch];
+#pragma unroll
+        for (uint32_t r = 0; r < kSelectBatch; r++) ka[r] = src[r * kWave];
+        for (uint32_t b = 0; b < kSelectRounds; b += kSelectBatch) {
+            if (b + kSelectBatch < kSelectRounds) {
+#pragma unroll
+                for (uint32_t r = 0; r < kSelectBatch; r++) kb[r] = src[(b + kSelectBatch + r) * kWave];
+            }
+#pragma unroll
+            for (uint32_t r = 0; r < kSelectBatch; r++) {
+                const uint32_t hi = ka[r] >> 9;
+                const bool m = hi == c;
+                below += hi < c ? 1u : 0u;
+                const uint64_t bal = __ballot(m);
+                if (m) {
+                    const uint32_t at = mine + (uint32_t)__popcll(bal & lower_lanes);
+                    if (at < kSelectStage) {
+                        s_keys[wave * kSelectStage + at] = ka[r];
+                        s_vals[wave * kSelectStage + at] = wave * kSelectPer + (b + r) * kWave + lane;
+                    }
+                    atomicAdd(&run_off[ka[r] & (kRunDigits - 1u)], 1u);
+                }
+                mine += (uint32_t)__popcll(bal);
+            }
+#pragma unroll
+            for (uint32_t r = 0; r < kSelectBatch; r++) ka[r] = kb[r];
+        }
+#pragma unroll
+        for (int o = 1; o < kWave; o <<= 1) below += (uint32_t)__shfl_xor((int)below, o, kWave);
+        if (lane == 0) {
+            s_sel[wave] = mine;
+            s_sel[kRunWaves + wave] = below;
+        }
+        __syncthreads();
+        base = cnt = 0;
+        uint32_t most = 0;
+#pragma unroll
+        for (uint32_t w = 0; w < kRunWaves; w++) {
+            const uint32_t m = s_sel[w];
+            first_rank += w < wave ? m : 0u;
+            cnt += m;
+            base += s_sel[kRunWaves + w];
+            most = max(most, m);
+        }
+        if (cnt == 0) return;  // block-uniform
+        staged = most <= kSelectStage && cnt <= kRunChunk;
+        if (staged) {  // the staged matches to their ranks (every slice read before any is written)
+            uint32_t mk[kSelectStage / kWave], mv[kSelectStage / kWave];
+#pragma unroll
+            for (uint32_t i = 0; i < kSelectStage / kWave; i++) {
+                const uint32_t j = i * kWave + lane;
+                mk[i] = j < mine ? s_keys[wave * kSelectStage + j] : 0u;
+                mv[i] = j < mine ? s_vals[wave * kSelectStage + j] : 0u;
+            }
+            __syncthreads();
+#pragma unroll
+            for (uint32_t i = 0; i < kSelectStage / kWave; i++) {
+                const uint32_t j = i * kWave + lane;
+                if (j < mine) {
+                    s_keys[first_rank + j] = mk[i];
+                    s_vals[first_rank + j] = mv[i];
+                }
+            }
+        }
+        __syncthreads();  // run_off complete, s_sel read, the staged chunk in place
+    } else {
         uint32_t all;
         const uint32_t ex = block_scan<kRunThreads>(totals[t], s_wave, &all);
         if (t == c) s_base = ex;
         run_off[t] = 0;
+        __syncthreads();
+        base = s_base;
+        cnt = totals[c];
+        if (cnt == 0) return;  // block-uniform
     }
-    __syncthreads();
-    const uint32_t base = s_base, cnt = totals[c];
-    if (cnt == 0) return;  // block-uniform
     const uint32_t* rk = kin + base;
     const uint32_t* rv = vin + base;
     // bounds of the run's buckets from its digit histogram, whose exclusive scan starts each digit's slots
@@ -181,8 +293,11 @@ __global__ __launch_bounds__(kRunThreads) void k_bucket_runs(const uint32_t* __r
         return st;
     };
     // a run of one chunk takes its histogram from the chunk's ranking below; a longer one counts first
-    const bool one = cnt <= kRunChunk;
-    if (!one) {
+    // (SELECT counted it while selecting)
+    const bool one = !SELECT && cnt <= kRunChunk;
+    if (SELECT) {
+        bounds(run_off[t]);
+    } else if (!one) {
         // four loads in flight per thread (a clamped index keeps them unconditional)
         for (uint32_t i0 = t; i0 < cnt; i0 += 4 * kRunThreads) {
             uint32_t k4[4];
@@ -198,9 +313,11 @@ __global__ __launch_bounds__(kRunThreads) void k_bucket_runs(const uint32_t* __r
         __syncthreads();
         bounds(run_off[t]);
     }
-    const uint64_t lower_lanes = (1ull << lane) - 1ull;
     for (uint32_t c0 = 0; c0 < cnt; c0 += kRunChunk) {
         for (uint32_t i = t; i < kRunWaves * kRunDigits; i += kRunThreads) s_cnt[i] = 0;
+        if constexpr (SELECT) {  // this chunk's matches into s_keys / s_vals, unless staged there
+            if (!staged) sel_store(first_rank, c0);
+        }
         __syncthreads();
         // item r of wave w, lane l is key c0 + (w kRunItems + r) 64 + l: (wave, round, lane) order is input order
         uint32_t key[kRunItems], val[kRunItems], rank[kRunItems];
@@ -208,9 +325,15 @@ __global__ __launch_bounds__(kRunThreads) void k_bucket_runs(const uint32_t* __r
         for (int r = 0; r < kRunItems; r++) {
             const uint32_t q = c0 + (wave * kRunItems + r) * kWave + lane;
             const bool valid = q < cnt;
-            key[r] = valid ? rk[q] : 0u;
-            val[r] = valid ? rv[q] : 0u;
+            if constexpr (SELECT) {  // the chunk selected into LDS
+                key[r] = valid ? s_keys[q - c0] : 0u;
+                val[r] = valid ? s_vals[q - c0] : 0u;
+            } else {
+                key[r] = valid ? rk[q] : 0u;
+                val[r] = valid ? rv[q] : 0u;
+            }
         }
+        if (SELECT) __syncthreads();  // read before the reorder below overwrites s_keys / s_vals
 #pragma unroll
         for (int r = 0; r < kRunItems; r++) {
             const uint32_t q = c0 + (wave * kRunItems + r) * kWave + lane;
@@ -1159,6 +1282,12 @@ bool Fluid3D::allocate_internal()
         !hitx_.allocate(u1 / 4 * kGroupLanes * (kMaxWindows - 1) * 8) || !meta_.allocate(u1))
         return false;
     if (!inv_.allocate(u1) || !porder_.allocate(u1) || !mkey_.allocate(u1) || !counts_.allocate(u1)) return false;
+    if (n_ <= kSelectMax) {  // the selecting run pass reads kSelectMax keys: those past n never match
+        if (!skey_.allocate(kSelectMax * 4) || !hip_ok(hipMemset(skey_.ptr, 0xFF, kSelectMax * 4), "hipMemset"))
+            return false;
+    } else {
+        skey_.release();
+    }
     porder_valid_ = false;
     steps_since_refresh_ = 0;
     // the reference leaves the tables uninitialised (cpp:53-54); here they start empty
@@ -1306,9 +1435,12 @@ int Fluid3D::enqueue_front(hipStream_t s)
     // and writes the bucket bounds and the inverse permutation from the same histograms.
     SortCountPlan plan;
     if (!sort.first_pass_plan(n, 9, 18, &plan)) return 0;
+    // small steps: no sort pass; every block of the run kernel selects its high digit's keys itself
+    const bool select = n <= kSelectMax;
+    uint32_t* keys = select ? skey_.as<uint32_t>() : cell;
     timer.begin(MFL_STAGE_HASH, s);
 #define MFL_HASH(IT)                                                                                              \
-    hipLaunchKernelGGL(k_hash_counts<IT>, dim3(plan.ntiles), b, 0, s, pos, n, params.cellSize, cell,              \
+    hipLaunchKernelGGL(k_hash_counts<IT>, dim3(plan.ntiles), b, 0, s, pos, n, params.cellSize, keys,              \
                        params_buf_.as<mfl_params>(), params, cs, ce, clear, plan.counts, plan.ntiles, 9u, plan.bits)
     switch (plan.items) {
     case 16: MFL_HASH(16); break;
@@ -1320,16 +1452,21 @@ int Fluid3D::enqueue_front(hipStream_t s)
     MFL_HIP(hipGetLastError());
     // the index buffer starts as the identity (cl:63) and is generated inside the pass
     timer.begin(MFL_STAGE_SORT, s);
-    SortPassOut high;
-    if (!sort.counted_pass(cell, n, 9, 9, s, &high)) return 0;
     const bool gather = n < kScatterCopyMin;
     auto* inv = inv_.as<uint32_t>();
-    if (gather)
-        hipLaunchKernelGGL(k_bucket_runs<true>, dim3(kRunDigits), dim3(kRunThreads), 0, s, high.keys, high.vals, high.totals, cell,
-                           index, cs, ce, inv, pos, vel, spos, svel);
-    else
-        hipLaunchKernelGGL(k_bucket_runs<false>, dim3(kRunDigits), dim3(kRunThreads), 0, s, high.keys, high.vals, high.totals, cell,
-                           index, cs, ce, inv, pos, vel, spos, svel);
+    if (select) {
+        hipLaunchKernelGGL((k_bucket_runs<true, true>), dim3(kRunDigits), dim3(kRunThreads), 0, s, keys, nullptr,
+                           nullptr, cell, index, cs, ce, inv, pos, vel, spos, svel);
+    } else {
+        SortPassOut high;
+        if (!sort.counted_pass(cell, n, 9, 9, s, &high)) return 0;
+        if (gather)
+            hipLaunchKernelGGL((k_bucket_runs<true, false>), dim3(kRunDigits), dim3(kRunThreads), 0, s, high.keys,
+                               high.vals, high.totals, cell, index, cs, ce, inv, pos, vel, spos, svel);
+        else
+            hipLaunchKernelGGL((k_bucket_runs<false, false>), dim3(kRunDigits), dim3(kRunThreads), 0, s, high.keys,
+                               high.vals, high.totals, cell, index, cs, ce, inv, pos, vel, spos, svel);
+    }
     MFL_HIP(hipGetLastError());
     timer.end(MFL_STAGE_SORT, s);
     // bucket bounds come out of the sort (the BOUNDS stage stays empty)

@@ -92,6 +92,7 @@ private:
     DeviceBuffer inv_;           // inverse of the sort permutation: inv[index[i]] = i
     DeviceBuffer counts_;        // accepted visits per sorted particle (written when record_counts)
     DeviceBuffer porder_, mkey_; // Morton traversal order (caller indices) and its sort keys
+    DeviceBuffer skey_;          // small steps: the cell keys in caller order, for the selecting run pass
     bool porder_valid_ = false;
     uint32_t steps_since_refresh_ = 0;
     int refresh_order(hipStream_t s);
