@@ -404,7 +404,7 @@ __global__ __launch_bounds__(kThreads) void k_b_forces_kept(const float4* __rest
     constexpr uint32_t SEGS = kWave / W;
     extern __shared__ float4 lds[];
     const Lanes<W> L(threadIdx.x & (kWave - 1));
-    constexpr uint32_t chunk = kKeptChunk;  // small LDS per segment keeps more waves resident
+    constexpr uint32_t chunk = kept_chunk<W>();  // small LDS per segment keeps more waves resident
     // the wave's tiles: positions of its segments' chunks, then their velocities
     float4* wpos = lds + (size_t)(threadIdx.x / kWave) * SEGS * 2 * chunk;
     float4* wvel = wpos + SEGS * chunk;
@@ -459,7 +459,7 @@ __global__ __launch_bounds__(kThreads) void k_b_forces_kept(const float4* __rest
                 if (c0 + chunk < nmax) kept_chunk_index<W>(kept_in, kbase, nw, c0 + chunk, L.lane, kidx);
                 wave_sync();
                 if (member) {
-                    uint32_t bits = chunk_bits<P>(hw, trips, c0);
+                    uint32_t bits = chunk_bits<P, chunk>(hw, trips, c0);
                     const float4* rp = tpos + slot;  // this lane's entries: k P + slot of the chunk
                     const float4* rv = tvel + slot;
                     while (bits) {
@@ -778,7 +778,7 @@ int FluidBatch::enqueue_step(hipStream_t s)
                        params, boxes_.as<float4>(), nbr_.as<uint32_t>(), ncount_.as<uint32_t>(), N_,              \
                        gravity_force_of(gravity))
     if (kept) {
-        const size_t lds_k = (size_t)block_waves * (8 / lanes) * kKeptChunk * 2 * sizeof(float4);
+        const size_t lds_k = (size_t)block_waves * (8 / lanes) * (lanes < 8 ? 32 : 64) * 2 * sizeof(float4);
 #define MFL_B_KEPT(W)                                                                                              \
     hipLaunchKernelGGL(k_b_forces_kept<W>, dim3((unsigned)blocks), dim3(block_waves * kWave), lds_k, s, spos, svel,  \
                        pos, vel, force, cs_all, ce_all, off, pscene_.as<uint32_t>(), params, boxes_.as<float4>(), N_, \

@@ -1044,7 +1044,7 @@ __global__ __launch_bounds__(kThreads) void k_forces_kept(const float4* __restri
     constexpr uint32_t SEGS = kWave / W;
     extern __shared__ float4 lds[];
     const Lanes<W> L(threadIdx.x & (kWave - 1));
-    constexpr uint32_t chunk = kKeptChunk;  // small LDS per segment keeps more waves resident
+    constexpr uint32_t chunk = kept_chunk<W>();  // small LDS per segment keeps more waves resident
     // the wave's tiles: positions of its segments' chunks, then their velocities
     float4* wpos = lds + (size_t)(threadIdx.x / kWave) * SEGS * 2 * chunk;
     float4* wvel = wpos + SEGS * chunk;
@@ -1078,7 +1078,7 @@ __global__ __launch_bounds__(kThreads) void k_forces_kept(const float4* __restri
             const uint32_t trips = (nw + P - 1) / P;
             const uint64_t hw = w == 0 ? hm : (member && nw ? hitx[hitx_at(t, P, slot, w)] : 0ull);
             const size_t kbase = (size_t)gseg * kSegCap + off + w0;
-            // kKeptChunk entries at a time; the next chunk's index loads are in flight during a chunk's sweep
+            // `chunk` entries at a time; the next chunk's index loads are in flight during a chunk's sweep
             uint32_t kidx[SEGS];
             kept_chunk_index<W>(kept_in, kbase, nw, 0u, L.lane, kidx);
             for (uint32_t c0 = 0; c0 < nmax; c0 += chunk) {
@@ -1088,7 +1088,7 @@ __global__ __launch_bounds__(kThreads) void k_forces_kept(const float4* __restri
                 if (c0 + chunk < nmax) kept_chunk_index<W>(kept_in, kbase, nw, c0 + chunk, L.lane, kidx);
                 wave_sync();
                 if (member) {
-                    uint32_t bits = chunk_bits<P>(hw, trips, c0);
+                    uint32_t bits = chunk_bits<P, chunk>(hw, trips, c0);
                     const float4* rp = tpos + slot;  // this lane's entries: k P + slot of the chunk
                     const float4* rv = tvel + slot;
                     while (bits) {
@@ -1661,7 +1661,7 @@ int Fluid3D::enqueue_tile_passes(hipStream_t s, int order)
     if (out == kOutList) return enqueue_list_forces(s, order);
     timer.begin(MFL_STAGE_FORCES, s);
     if (out == kOutKept) {
-        const size_t lds_k = (size_t)block_waves * (8 / lanes) * kKeptChunk * 2 * sizeof(float4);
+        const size_t lds_k = (size_t)block_waves * (8 / lanes) * (lanes < 8 ? 32 : 64) * 2 * sizeof(float4);
 #define MFL_FORCES_KEPT(O, W, STEP)                                                                              \
     hipLaunchKernelGGL((k_forces_kept<O, W, STEP>), gd, b, lds_k, s, spos, svel, pos, vel, force, index, cs, ce, n, \
                        params, depth, mn, mx, gf, porder, inv, kept, hitmask, hitx, meta)

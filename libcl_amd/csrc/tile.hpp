@@ -22,11 +22,17 @@ constexpr uint32_t kStageRounds = 8;        // visits of one neighbourhood laid 
 constexpr uint32_t kTileMinDepth = 64, kTileMaxDepth = kStageRounds * 64;  // kept <= staged <= 512
 // Kept-entry passes: the density pass writes each 8-particle segment's kept entries (at most kSegCap per
 // segment over its cells) and, per particle, meta = offset | kept << 16 of its cell's entries, or
-// kMetaWalk when the force pass must walk the bucket ranges itself; the force pass stages kKeptChunk
-// entries per segment at a time (a multiple of 8).
+// kMetaWalk when the force pass must walk the bucket ranges itself; the force pass stages kept_chunk<W>
+// entries per segment at a time: 32 when a wave holds two segments or more (one 64-lane state load then
+// fills two segments' chunks), which halves a wave's LDS and so lets more waves hide the loads' latency;
+// 64 for a one-segment wave.
 constexpr uint32_t kSegCap = 512;
 constexpr uint32_t kMetaWalk = 0xFFFFFFFFu;
-constexpr uint32_t kKeptChunk = 64;
+template <int W>
+constexpr uint32_t kept_chunk()
+{
+    return W < kWave ? 32u : 64u;
+}
 
 // Per-wave LDS bookkeeping of a staging: bit p % 64 of starts[p / 64] marks position p where a
 // non-empty bucket range begins, and delta[c] maps a position of the c-th non-empty range to its
@@ -458,26 +464,27 @@ __device__ inline uint32_t stage_seg(const Lanes<W>& L, const SegCell& c, float 
     return total == kStageOverflow ? kStageOverflow : kept;
 }
 
-// Acceptance bits of one chunk [c0, c0 + kKeptChunk) of a window's kept entries (c0 a multiple of
-// kKeptChunk), from a lane's word hm over the window's `trips` trips (bit trips - 1 - m is trip m): trip
-// c0 / P + k of the chunk becomes bit 31 - k of the result, so counting leading zeros visits the
-// accepted entries of the chunk earliest trip first, entry k P + slot of the chunk for k leading zeros.
-template <uint32_t P>
+// Acceptance bits of one chunk [c0, c0 + CH) of a window's kept entries (c0 a multiple of CH), from a
+// lane's word hm over the window's `trips` trips (bit trips - 1 - m is trip m): trip c0 / P + k of the
+// chunk becomes bit 31 - k of the result, so counting leading zeros visits the accepted entries of the
+// chunk earliest trip first, entry k P + slot of the chunk for k leading zeros.
+template <uint32_t P, uint32_t CH>
 __device__ inline uint32_t chunk_bits(uint64_t hm, uint32_t trips, uint32_t c0)
 {
-    static_assert(kKeptChunk % P == 0 && kKeptChunk / P <= 32, "a chunk's trips fit one word");
-    const uint32_t mlo = c0 / P, mhi = min((c0 + kKeptChunk) / P, trips);
+    static_assert(CH % P == 0 && CH / P <= 32, "a chunk's trips fit one word");
+    const uint32_t mlo = c0 / P, mhi = min((c0 + CH) / P, trips);
     if (mlo >= mhi) return 0u;
     const uint32_t blo = trips - mhi, nb = mhi - mlo;
     return (uint32_t)((hm >> blo) << (32u - nb));  // bits of earlier chunks leave the word
 }
 
-// Kept-entry force pass: stages entries c0 .. c0 + kKeptChunk - 1 of the current window of every
-// segment of the wave into the wave's LDS tiles, segment e's entries at wpos / wvel[e kKeptChunk ..]:
-// for each segment the 64 lanes read 64 consecutive indices of its kept list (the window starts at
-// kept_in[kbase] and holds nw entries, both per segment) and the states they name go straight from
-// global memory to LDS (global_load_lds_dwordx4: lane l fills slot l of the 1 KiB block given), so no
-// state passes through registers. Lanes past a window load spos / svel[0] into slots never used.
+// Kept-entry force pass: stages entries c0 .. c0 + CH - 1 (CH = kept_chunk<W>) of the current window of
+// every segment of the wave into the wave's LDS tiles, segment e's entries at wpos / wvel[e CH ..]: each
+// 64-lane load covers 64 / CH segments (lane l: segment l / CH of its group, entry c0 + l % CH), the
+// lanes read consecutive indices of their segment's kept list (the window starts at kept_in[kbase] and
+// holds nw entries, both per segment) and the states they name go straight from global memory to LDS
+// (global_load_lds_dwordx4: lane l fills slot l of the 1 KiB block given), so no state passes through
+// registers. Lanes past a window load spos / svel[0] into slots never used.
 // In two halves, so that a chunk's index loads can be in flight while the chunk before it is swept:
 // kept_chunk_index issues the index loads of chunk c0 (raw values, used by kept_chunk_load only),
 // kept_chunk_load turns them into state loads into LDS and returns without waiting for those.
@@ -485,13 +492,23 @@ template <int W>
 __device__ inline void kept_chunk_index(const uint32_t* __restrict__ kept_in, size_t kbase, uint32_t nw, uint32_t c0,
                                         uint32_t lane, uint32_t* k)
 {
-    constexpr uint32_t SEGS = kWave / W;
+    constexpr uint32_t SEGS = kWave / W, CH = kept_chunk<W>(), PER = kWave / CH, LOADS = SEGS / PER;
+    static_assert(SEGS % PER == 0, "whole segments per load");
+    const uint32_t half = lane / CH, q = c0 + lane % CH;
 #pragma unroll
-    for (uint32_t e = 0; e < SEGS; e++) {
-        const size_t b = ((size_t)__builtin_amdgcn_readlane((uint32_t)(kbase >> 32), e * W) << 32) |
-                         __builtin_amdgcn_readlane((uint32_t)kbase, e * W);
-        const uint32_t m = __builtin_amdgcn_readlane(nw, e * W);
-        k[e] = kept_in[b + min(c0 + lane, (m > 0 ? m : 1u) - 1u)];  // inside the segment's list
+    for (uint32_t i = 0; i < LOADS; i++) {
+        size_t b = 0;
+        uint32_t m = 0;
+#pragma unroll
+        for (uint32_t h = 0; h < PER; h++) {  // this lane's segment: i PER + half
+            const uint32_t e = i * PER + h;
+            const size_t be = ((size_t)__builtin_amdgcn_readlane((uint32_t)(kbase >> 32), e * W) << 32) |
+                              __builtin_amdgcn_readlane((uint32_t)kbase, e * W);
+            const uint32_t me = __builtin_amdgcn_readlane(nw, e * W);
+            b = h == half ? be : b;
+            m = h == half ? me : m;
+        }
+        k[i] = kept_in[b + min(q, (m > 0 ? m : 1u) - 1u)];  // inside the segment's list
     }
 }
 
@@ -500,19 +517,27 @@ __device__ inline void kept_chunk_load(const uint32_t* k, uint32_t nw, uint32_t 
                                        const float4* __restrict__ spos, const float4* __restrict__ svel, float4* wpos,
                                        float4* wvel)
 {
-    static_assert(kKeptChunk == kWave, "a chunk of one segment is one entry per lane");
-    constexpr uint32_t SEGS = kWave / W;
-    uint32_t j[SEGS];
+    constexpr uint32_t SEGS = kWave / W, CH = kept_chunk<W>(), PER = kWave / CH, LOADS = SEGS / PER;
+    const uint32_t half = lane / CH, q = c0 + lane % CH;
+    uint32_t j[LOADS];
     // every index before the first state load (s_waitcnt vmcnt(0) alone: expcnt 7, lgkmcnt 15)
     __builtin_amdgcn_s_waitcnt(0x0F70);
 #pragma unroll
-    for (uint32_t e = 0; e < SEGS; e++) j[e] = c0 + lane < __builtin_amdgcn_readlane(nw, e * W) ? k[e] : 0u;
+    for (uint32_t i = 0; i < LOADS; i++) {
+        uint32_t m = 0;
+#pragma unroll
+        for (uint32_t h = 0; h < PER; h++) {
+            const uint32_t me = __builtin_amdgcn_readlane(nw, (i * PER + h) * W);
+            m = h == half ? me : m;
+        }
+        j[i] = q < m ? k[i] : 0u;
+    }
 #if defined(__HIP_DEVICE_COMPILE__)  // the LDS address space exists in the device pass only
 #pragma unroll
-    for (uint32_t e = 0; e < SEGS; e++) {
-        __builtin_amdgcn_global_load_lds(spos + j[e], (__attribute__((address_space(3))) void*)(wpos + e * kKeptChunk),
+    for (uint32_t i = 0; i < LOADS; i++) {
+        __builtin_amdgcn_global_load_lds(spos + j[i], (__attribute__((address_space(3))) void*)(wpos + i * kWave),
                                          16, 0, 0);
-        __builtin_amdgcn_global_load_lds(svel + j[e], (__attribute__((address_space(3))) void*)(wvel + e * kKeptChunk),
+        __builtin_amdgcn_global_load_lds(svel + j[i], (__attribute__((address_space(3))) void*)(wvel + i * kWave),
                                          16, 0, 0);
     }
 #endif
