@@ -6,14 +6,14 @@
 // tables (cell start / end) that keep their contents between steps.
 //
 // Step (oclFluid3D::compute, cpp:293-342):
-//   k_hash_counts   cell key per particle + reset of the first clear_count buckets + the sort's
-//                   high-digit counts (cl:57-64, cpp:309-313; nothing between the two reads the
-//                   tables, so the reset moves to the start of the step)
-//   sort            stable key/index sort over bits [0, 18) (cpp:299; keys < 2^18): one counted
-//                   pass by bits [9, 18), then k_bucket_runs orders each high digit's run by bits
-//                   [0, 9) and writes the inverse permutation and the start / end of each occupied
-//                   bucket (cl:85-113) from the same histograms; up to 2^14 particles there is no
-//                   pass: each k_bucket_runs block selects its high digit's keys from all of them
+//   k_hash_tiles    cell key per particle + reset of the first clear_count buckets (cl:57-64,
+//                   cpp:309-313; nothing between the two reads the tables, so the reset moves to the
+//                   start of the step), each tile of keys then ordered by its high digit in LDS
+//   k_bucket_runs   the rest of the stable key/index sort over bits [0, 18) (cpp:299; keys < 2^18):
+//                   one block per high digit assembles that digit's run from the tiles, orders it by
+//                   bits [0, 9) and writes the inverse permutation and the start / end of each occupied
+//                   bucket (cl:85-113) from the same histograms; up to 2^14 particles the hash kernel
+//                   (k_hash_keys) leaves the keys in caller order and each block selects its own
 //   k_scatter_sorted sorted pos/vel copy, written from the caller's order   (cl:66-72)
 //   density pass    27-bucket neighbourhood, poly6, pressure                (cl:150-200)
 //   force pass      27-bucket pressure + viscosity forces, clip box, clamp, Euler, gravity
@@ -25,6 +25,7 @@
 #include "fluid3d.hpp"
 #include "sph_device.hpp"
 #include "tile.hpp"
+#include "tile_rank.hpp"
 
 #include <cstring>
 
@@ -52,46 +53,112 @@ __global__ __launch_bounds__(kThreads) void k_hash(const float4* __restrict__ po
     if (WRITE_INDEX) index[i] = i;
 }
 
-// k_hash<false, true> that also fills the sort's first-pass count table (bits [shift, shift + bits) of the
-// keys; the layout of RadixSort::first_pass_plan): block b hashes the keys of sort tile xcd_tile(), ITEMS per
-// thread at tile offsets k * 256 + thread, and histograms them in LDS. The grid is one block per tile.
+// The step's hash kernel for at most kSelectMax particles: cell key per particle (cl:57-64) in caller
+// order, the reset of the first `clear` buckets (cpp:309-313; nothing between the two reads the tables,
+// so the reset moves to the start of the step) and the parameter copy. ITEMS keys per thread, every
+// load issued before the first hash.
 template <int ITEMS>
-__global__ __launch_bounds__(kThreads) void k_hash_counts(const float4* __restrict__ pos, uint32_t n, float h,
-                                                          uint32_t* __restrict__ cell, mfl_params* __restrict__ params_buf,
-                                                          mfl_params p, uint32_t* __restrict__ cell_start,
-                                                          uint32_t* __restrict__ cell_end, uint32_t clear,
-                                                          uint32_t* __restrict__ counts, uint32_t ntiles, uint32_t shift,
-                                                          uint32_t bits)
+__global__ __launch_bounds__(kThreads) void k_hash_keys(const float4* __restrict__ pos, uint32_t n, float h,
+                                                        uint32_t* __restrict__ cell, mfl_params* __restrict__ params_buf,
+                                                        mfl_params p, uint32_t* __restrict__ cell_start,
+                                                        uint32_t* __restrict__ cell_end, uint32_t clear)
 {
-    __shared__ uint32_t hist[512];
-    const uint32_t radix = 1u << bits, mask = radix - 1u;
-    for (uint32_t d = threadIdx.x; d < radix; d += kThreads) hist[d] = 0;
     if (blockIdx.x == 0 && threadIdx.x == 0 && params_buf) *params_buf = p;  // getParamBuffer() in step
     for (uint32_t c = blockIdx.x * kThreads + threadIdx.x; c < clear; c += gridDim.x * kThreads) {
         cell_start[c] = kEmpty;
         cell_end[c] = kEmpty;
     }
-    __syncthreads();
-    const uint32_t tile = xcd_tile();
-    const size_t base = (size_t)tile * (kThreads * ITEMS);
-    // every load issued before the first count (a clamped index keeps the loads unconditional)
+    const size_t base = (size_t)blockIdx.x * (kThreads * ITEMS);
     float4 q[ITEMS];
 #pragma unroll
     for (int k = 0; k < ITEMS; k++) {
         const size_t i = base + (size_t)k * kThreads + threadIdx.x;
-        q[k] = pos[i < n ? i : n - 1];
+        q[k] = pos[i < n ? i : n - 1];  // a clamped index keeps the loads unconditional
     }
 #pragma unroll
     for (int k = 0; k < ITEMS; k++) {
         const size_t i = base + (size_t)k * kThreads + threadIdx.x;
+        if (i < n) cell[i] = cell_hash(grid_coord(q[k].x, h), grid_coord(q[k].y, h), grid_coord(q[k].z, h));
+    }
+}
+
+// The step's hash kernel for more than kSelectMax particles: k_hash_keys's keys, table reset and
+// parameter copy, and then the tile (kThreads x ITEMS particles, sort tile xcd_tile()) ordered stably by
+// the keys' high digit (bits [9, 18)) in LDS and written back in place: tkeys / tvals[tile * TILE + p]
+// hold the tile's keys and their particle indices in (high digit, index) order, counts[d ntiles + tile]
+// the tile's number of keys of digit d and starts[d ntiles + tile] where they begin in the tile. Item r
+// of wave w, lane l is particle tile * TILE + (w ITEMS + r) 64 + l ((wave, round, lane) order is input
+// order, as tile_rank.hpp ranks).
+template <int ITEMS>
+__global__ __launch_bounds__(kThreads) void k_hash_tiles(const float4* __restrict__ pos, uint32_t n, float h,
+                                                         mfl_params* __restrict__ params_buf, mfl_params p,
+                                                         uint32_t* __restrict__ cell_start,
+                                                         uint32_t* __restrict__ cell_end, uint32_t clear,
+                                                         uint32_t* __restrict__ tkeys, uint32_t* __restrict__ tvals,
+                                                         uint32_t* __restrict__ counts, uint32_t* __restrict__ starts,
+                                                         uint32_t ntiles)
+{
+    constexpr int TILE = kThreads * ITEMS;
+    constexpr uint32_t R = 512;
+    constexpr int WAVES = kThreads / kWave;
+    __shared__ uint32_t s_cnt[WAVES * R];
+    __shared__ uint32_t s_start[R];
+    __shared__ uint32_t s_keys[TILE];
+    __shared__ uint32_t s_vals[TILE];
+    __shared__ uint32_t s_wave[WAVES];
+    const uint32_t t = threadIdx.x, lane = t & (kWave - 1), wave = t / kWave;
+    for (uint32_t i = t; i < WAVES * R; i += kThreads) s_cnt[i] = 0;
+    if (blockIdx.x == 0 && t == 0 && params_buf) *params_buf = p;  // getParamBuffer() in step
+    for (uint32_t c = blockIdx.x * kThreads + t; c < clear; c += gridDim.x * kThreads) {
+        cell_start[c] = kEmpty;
+        cell_end[c] = kEmpty;
+    }
+    const uint32_t tile = xcd_tile();
+    const size_t tile_base = (size_t)tile * TILE;
+    const uint32_t first = (wave * ITEMS) * kWave + lane;
+    // every load issued before the first hash (a clamped index keeps the loads unconditional)
+    float4 q[ITEMS];
+#pragma unroll
+    for (int r = 0; r < ITEMS; r++) {
+        const size_t i = tile_base + first + (size_t)r * kWave;
+        q[r] = pos[i < n ? i : n - 1];
+    }
+    uint32_t key[ITEMS], rank[ITEMS];
+#pragma unroll
+    for (int r = 0; r < ITEMS; r++) key[r] = cell_hash(grid_coord(q[r].x, h), grid_coord(q[r].y, h), grid_coord(q[r].z, h));
+    __syncthreads();
+    rank_tile<ITEMS, 9, R, 9>(key, n, tile_base, first, 9, 9, s_cnt, rank);
+    __syncthreads();
+    const uint32_t d0 = 2u * t, d1 = d0 + 1u;  // thread t owns high digits 2t and 2t + 1
+    const uint32_t c0 = wave_offsets<R, WAVES>(d0, R, s_cnt), c1 = wave_offsets<R, WAVES>(d1, R, s_cnt);
+    {
+        uint32_t all;
+        const uint32_t st = block_scan<kThreads>(c0 + c1, s_wave, &all);
+        s_start[d0] = st;
+        s_start[d1] = st + c0;
+        counts[(size_t)d0 * ntiles + tile] = c0;
+        counts[(size_t)d1 * ntiles + tile] = c1;
+        starts[(size_t)d0 * ntiles + tile] = st;
+        starts[(size_t)d1 * ntiles + tile] = st + c0;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int r = 0; r < ITEMS; r++) {
+        const size_t i = tile_base + first + (size_t)r * kWave;
         if (i < n) {
-            const uint32_t key = cell_hash(grid_coord(q[k].x, h), grid_coord(q[k].y, h), grid_coord(q[k].z, h));
-            cell[i] = key;
-            atomicAdd(&hist[(key >> shift) & mask], 1u);
+            const uint32_t d = key[r] >> 9;
+            const uint32_t at = s_start[d] + s_cnt[wave * R + d] + rank[r];
+            s_keys[at] = key[r];
+            s_vals[at] = (uint32_t)i;
         }
     }
     __syncthreads();
-    for (uint32_t d = threadIdx.x; d < radix; d += kThreads) counts[(size_t)d * ntiles + tile] = hist[d];
+    const size_t remain = n - tile_base;
+    const uint32_t tile_n = remain < (size_t)TILE ? (uint32_t)remain : (uint32_t)TILE;
+    for (uint32_t at = t; at < tile_n; at += kThreads) {
+        tkeys[tile_base + at] = s_keys[at];
+        tvals[tile_base + at] = s_vals[at];
+    }
 }
 
 // Sorted gather (cl:66-72) over [0, n) and the per-step bucket reset (cpp:309-313) over [0, clear).
@@ -127,26 +194,33 @@ constexpr uint32_t kScatterCopyMin = 1u << 18;
 // own, which costs it less than the pass's two launches (digit scan, scatter) cost the step.
 constexpr uint32_t kSelectMax = 1u << 14;
 
-// Second half of the step's sort. RadixSort::counted_pass has ordered the keys (< 2^18) by their high
-// digit, bits [9, 18); block `c` (one per high digit, dealt to the XCDs in contiguous ranges) takes the
-// contiguous run of keys with high digit c and orders it stably by bits [0, 9): chunk by chunk in input
-// order, ranks by wave ballot matching (the scatter of radix_sort.hip), the chunk reordered by digit in
-// LDS and written out in runs at each digit's running offset (a run longer than one chunk takes a digit
-// histogram first; one chunk long, as usual, its histogram is the chunk's). High digit c then low digit d is bucket
+// Second half of the step's sort, one block per high digit c (bits [9, 18) of the keys, which are
+// < 2^18; blocks dealt to the XCDs in contiguous ranges): the block assembles the run of keys with high
+// digit c in input order and orders it stably by bits [0, 9): chunk by chunk in input order, ranks by
+// wave ballot matching (tile_rank.hpp's scheme), the chunk reordered by digit in LDS and written out in
+// runs at each digit's running offset (a run longer than one chunk takes a digit histogram first; one
+// chunk long, as usual, its histogram is the chunk's). High digit c then low digit d is bucket
 // c * 512 + d, so the run holds every key of buckets c * 512 .. c * 512 + 511, and the histogram gives
 // their bounds directly (cl:85-113: start and end written for occupied buckets only). The block also
 // writes each particle's sorted slot (inv) and, GATHER, copies the sorted state by gathering through the
 // index (for steps whose pos / vel sit in L2 anyway; larger steps copy with k_scatter_sorted).
+// MODE kRunsTiles: k_hash_tiles left each tile ordered by high digit; the run is the concatenation, in
+// tile order, of every tile's digit-c segment (counts / starts give each segment's length and place),
+// and its base is the sum over the tiles of their keys below digit c, which is the sum of the segments'
+// starts. MODE kRunsSelect (at most kSelectMax particles): the keys in caller order, selected below.
 constexpr int kRunThreads = 512;     // one thread per low digit
 constexpr int kRunWaves = kRunThreads / kWave;
 constexpr int kRunItems = 8;         // keys per thread and chunk
 constexpr uint32_t kRunDigits = 512;
 constexpr uint32_t kRunChunk = kRunThreads * kRunItems;
+enum : int { kRunsTiles = 0, kRunsSelect = 1 };
 
-template <bool GATHER, bool SELECT>
+template <bool GATHER, int MODE>
 __global__ __launch_bounds__(kRunThreads) void k_bucket_runs(const uint32_t* __restrict__ kin,
                                                              const uint32_t* __restrict__ vin,
-                                                             const uint32_t* __restrict__ totals,
+                                                             const uint32_t* __restrict__ counts,
+                                                             const uint32_t* __restrict__ starts, uint32_t ntiles,
+                                                             uint32_t tile_keys,
                                                              uint32_t* __restrict__ cell, uint32_t* __restrict__ index,
                                                              uint32_t* __restrict__ cell_start,
                                                              uint32_t* __restrict__ cell_end, uint32_t* __restrict__ inv,
@@ -154,16 +228,44 @@ __global__ __launch_bounds__(kRunThreads) void k_bucket_runs(const uint32_t* __r
                                                              const float4* __restrict__ vel, float4* __restrict__ spos,
                                                              float4* __restrict__ svel)
 {
+    constexpr bool SELECT = MODE == kRunsSelect;
     __shared__ uint32_t run_off[kRunDigits];          // histogram, then each digit's next slot in the run
     __shared__ uint32_t chunk_start[kRunDigits];      // each digit's first slot in the chunk's order
     __shared__ uint32_t s_cnt[kRunWaves * kRunDigits];  // a chunk's per-wave digit counts
     __shared__ uint32_t s_keys[kRunChunk];            // the chunk ordered by digit
     __shared__ uint32_t s_vals[kRunChunk];
     __shared__ uint32_t s_wave[kRunWaves];
-    __shared__ uint32_t s_base;
     const uint32_t c = xcd_tile();
     const uint32_t t = threadIdx.x, lane = t & (kWave - 1), wave = t / kWave;  // thread t owns low digit t
     const uint64_t lower_lanes = (1ull << lane) - 1ull;
+    // MODE kRunsTiles: thread t's tiles are tt0 .. tt1 - 1, its first segment starts at seg_off in the
+    // run; tiles_each(lo, hi, f) calls f(run position, key index in tkeys / tvals) for the run positions
+    // in [lo, hi) that its segments hold, four loads in flight at a time
+    const uint32_t tpt = MODE == kRunsTiles ? (ntiles + kRunThreads - 1) / kRunThreads : 0u;
+    const uint32_t tt0 = min(t * tpt, ntiles), tt1 = min(tt0 + tpt, ntiles);
+    uint32_t seg_off = 0;
+    uint32_t ct0 = 0, st0 = 0;  // the first tile's count and start, kept from the setup
+    auto tiles_each = [&](uint32_t lo, uint32_t hi, auto&& f) {
+        uint32_t off = seg_off;
+        for (uint32_t tt = tt0; tt < tt1; tt++) {
+            const uint32_t ct = tt == tt0 ? ct0 : counts[(size_t)c * ntiles + tt];
+            const size_t at = (size_t)tt * tile_keys + (tt == tt0 ? st0 : starts[(size_t)c * ntiles + tt]);
+            const uint32_t i_lo = lo > off ? min(lo - off, ct) : 0u, i_hi = hi > off ? min(hi - off, ct) : 0u;
+            for (uint32_t i = i_lo; i < i_hi; i += 4) {
+                uint32_t kk[4], vv[4];
+#pragma unroll
+                for (uint32_t r = 0; r < 4; r++) {
+                    const size_t j = at + min(i + r, i_hi - 1u);
+                    kk[r] = kin[j];
+                    vv[r] = vin[j];
+                }
+#pragma unroll
+                for (uint32_t r = 0; r < 4; r++)
+                    if (i + r < i_hi) f(off + i + r, kk[r], vv[r]);
+            }
+            off += ct;
+        }
+    };
     // SELECT: the keys are the hash kernel's, in input order (kin[0 .. n), n <= kSelectMax, followed by
     // 0xFFFFFFFF up to kSelectMax, which no digit matches); every block reads all of them and takes
     // those of its high digit. Wave w reads the contiguous keys w kSelectPer .. (w + 1) kSelectPer - 1
@@ -197,7 +299,7 @@ __global__ __launch_bounds__(kRunThreads) void k_bucket_runs(const uint32_t* __r
             }
         }
     };
-    uint32_t base, cnt, first_rank = 0;
+    uint32_t base = 0, cnt = 0, first_rank = 0;
     bool staged = true;  // every match of the run in s_keys / s_vals after the select (else sel_store)
     if constexpr (SELECT) {
         run_off[t] = 0;
@@ -269,17 +371,25 @@ __global__ __launch_bounds__(kRunThreads) void k_bucket_runs(const uint32_t* __r
         }
         __syncthreads();  // run_off complete, s_sel read, the staged chunk in place
     } else {
+        // this thread's tiles: a contiguous range, so that its segments follow each other in the run
+        uint32_t mine = 0, below = 0;
+        for (uint32_t tt = tt0; tt < tt1; tt++) {
+            const uint32_t ct = counts[(size_t)c * ntiles + tt], st = starts[(size_t)c * ntiles + tt];
+            if (tt == tt0) {
+                ct0 = ct;
+                st0 = st;
+            }
+            mine += ct;
+            below += st;
+        }
         uint32_t all;
-        const uint32_t ex = block_scan<kRunThreads>(totals[t], s_wave, &all);
-        if (t == c) s_base = ex;
+        seg_off = block_scan<kRunThreads>(mine, s_wave, &cnt);
+        (void)block_scan<kRunThreads>(below, s_wave, &all);
+        base = all;
         run_off[t] = 0;
         __syncthreads();
-        base = s_base;
-        cnt = totals[c];
         if (cnt == 0) return;  // block-uniform
     }
-    const uint32_t* rk = kin + base;
-    const uint32_t* rv = vin + base;
     // bounds of the run's buckets from its digit histogram, whose exclusive scan starts each digit's slots
     // (block-uniform call); returns the scan
     auto bounds = [&](uint32_t h) {
@@ -298,18 +408,7 @@ __global__ __launch_bounds__(kRunThreads) void k_bucket_runs(const uint32_t* __r
     if (SELECT) {
         bounds(run_off[t]);
     } else if (!one) {
-        // four loads in flight per thread (a clamped index keeps them unconditional)
-        for (uint32_t i0 = t; i0 < cnt; i0 += 4 * kRunThreads) {
-            uint32_t k4[4];
-#pragma unroll
-            for (int r = 0; r < 4; r++) {
-                const uint32_t i = i0 + r * kRunThreads;
-                k4[r] = rk[i < cnt ? i : cnt - 1];
-            }
-#pragma unroll
-            for (int r = 0; r < 4; r++)
-                if (i0 + r * kRunThreads < cnt) atomicAdd(&run_off[k4[r] & (kRunDigits - 1u)], 1u);
-        }
+        tiles_each(0u, cnt, [&](uint32_t, uint32_t k, uint32_t) { atomicAdd(&run_off[k & (kRunDigits - 1u)], 1u); });
         __syncthreads();
         bounds(run_off[t]);
     }
@@ -317,6 +416,11 @@ __global__ __launch_bounds__(kRunThreads) void k_bucket_runs(const uint32_t* __r
         for (uint32_t i = t; i < kRunWaves * kRunDigits; i += kRunThreads) s_cnt[i] = 0;
         if constexpr (SELECT) {  // this chunk's matches into s_keys / s_vals, unless staged there
             if (!staged) sel_store(first_rank, c0);
+        } else {  // this chunk's part of every tile segment
+            tiles_each(c0, c0 + kRunChunk, [&](uint32_t q, uint32_t k, uint32_t v) {
+                s_keys[q - c0] = k;
+                s_vals[q - c0] = v;
+            });
         }
         __syncthreads();
         // item r of wave w, lane l is key c0 + (w kRunItems + r) 64 + l: (wave, round, lane) order is input order
@@ -325,15 +429,10 @@ __global__ __launch_bounds__(kRunThreads) void k_bucket_runs(const uint32_t* __r
         for (int r = 0; r < kRunItems; r++) {
             const uint32_t q = c0 + (wave * kRunItems + r) * kWave + lane;
             const bool valid = q < cnt;
-            if constexpr (SELECT) {  // the chunk selected into LDS
-                key[r] = valid ? s_keys[q - c0] : 0u;
-                val[r] = valid ? s_vals[q - c0] : 0u;
-            } else {
-                key[r] = valid ? rk[q] : 0u;
-                val[r] = valid ? rv[q] : 0u;
-            }
+            key[r] = valid ? s_keys[q - c0] : 0u;  // the chunk, gathered into LDS above
+            val[r] = valid ? s_vals[q - c0] : 0u;
         }
-        if (SELECT) __syncthreads();  // read before the reorder below overwrites s_keys / s_vals
+        __syncthreads();  // read before the reorder below overwrites s_keys / s_vals
 #pragma unroll
         for (int r = 0; r < kRunItems; r++) {
             const uint32_t q = c0 + (wave * kRunItems + r) * kWave + lane;
@@ -1285,8 +1384,16 @@ bool Fluid3D::allocate_internal()
     if (n_ <= kSelectMax) {  // the selecting run pass reads kSelectMax keys: those past n never match
         if (!skey_.allocate(kSelectMax * 4) || !hip_ok(hipMemset(skey_.ptr, 0xFF, kSelectMax * 4), "hipMemset"))
             return false;
-    } else {
+        tkey_.release();
+        tval_.release();
+        tcount_.release();
+        tstart_.release();
+    } else {  // the hash kernel's tile-ordered keys and indices, and per tile and high digit counts / starts
         skey_.release();
+        const size_t max_tiles = (n_ + kThreads * 4 - 1) / (kThreads * 4);  // the smallest tiles
+        if (!tkey_.allocate(u1) || !tval_.allocate(u1) || !tcount_.allocate(max_tiles * 512 * 4) ||
+            !tstart_.allocate(max_tiles * 512 * 4))
+            return false;
     }
     porder_valid_ = false;
     steps_since_refresh_ = 0;
@@ -1416,6 +1523,14 @@ bool Fluid3D::check_user_buffers()
     return true;
 }
 
+// Particles per thread of the hash kernel's tile (k_hash_tiles): the sort's tile-size knob, else by size.
+int Fluid3D::tile_items(size_t n) const
+{
+    const int r = sort.tuning.items;
+    if (r == 4 || r == 8 || r == 16) return r;
+    return n >= ((size_t)8 << 20) ? 16 : n >= ((size_t)1 << 20) ? 8 : 4;
+}
+
 int Fluid3D::enqueue_front(hipStream_t s)
 {
     const uint32_t n = (uint32_t)n_;
@@ -1430,43 +1545,48 @@ int Fluid3D::enqueue_front(hipStream_t s)
     auto* ce = cell_end_.as<uint32_t>();
     const uint32_t clear = clear_count < kBuckets ? clear_count : kBuckets;
 
-    // The sort orders the keys (< 2^18) by their high 9 bits in one counted pass, whose digit counts
-    // the hash kernel takes per tile, then k_bucket_runs orders each high digit's run by the low 9 bits
-    // and writes the bucket bounds and the inverse permutation from the same histograms.
-    SortCountPlan plan;
-    if (!sort.first_pass_plan(n, 9, 18, &plan)) return 0;
-    // small steps: no sort pass; every block of the run kernel selects its high digit's keys itself
+    // The keys (< 2^18) are ordered by their high 9 bits tile by tile inside the hash kernel, then
+    // k_bucket_runs assembles each high digit's run from the tiles' segments, orders it by the low 9 bits
+    // and writes the bucket bounds and the inverse permutation from the same histograms. Small steps
+    // keep the keys in caller order and every run block selects its own.
     const bool select = n <= kSelectMax;
-    uint32_t* keys = select ? skey_.as<uint32_t>() : cell;
+    const int items = tile_items(n);
+    const uint32_t tile_keys = kThreads * (uint32_t)items, ntiles = (n + tile_keys - 1) / tile_keys;
+    auto* tkey = tkey_.as<uint32_t>();
+    auto* tval = tval_.as<uint32_t>();
+    auto* tcnt = tcount_.as<uint32_t>();
+    auto* tstart = tstart_.as<uint32_t>();
+    auto* skey = skey_.as<uint32_t>();
     timer.begin(MFL_STAGE_HASH, s);
+    if (select) {
+        hipLaunchKernelGGL(k_hash_keys<4>, dim3((n + kThreads * 4 - 1) / (kThreads * 4)), b, 0, s, pos, n,
+                           params.cellSize, skey, params_buf_.as<mfl_params>(), params, cs, ce, clear);
+    } else {
 #define MFL_HASH(IT)                                                                                              \
-    hipLaunchKernelGGL(k_hash_counts<IT>, dim3(plan.ntiles), b, 0, s, pos, n, params.cellSize, keys,              \
-                       params_buf_.as<mfl_params>(), params, cs, ce, clear, plan.counts, plan.ntiles, 9u, plan.bits)
-    switch (plan.items) {
-    case 16: MFL_HASH(16); break;
-    case 8: MFL_HASH(8); break;
-    default: MFL_HASH(4); break;
-    }
+    hipLaunchKernelGGL(k_hash_tiles<IT>, dim3(ntiles), b, 0, s, pos, n, params.cellSize, params_buf_.as<mfl_params>(), \
+                       params, cs, ce, clear, tkey, tval, tcnt, tstart, ntiles)
+        switch (items) {
+        case 16: MFL_HASH(16); break;
+        case 8: MFL_HASH(8); break;
+        default: MFL_HASH(4); break;
+        }
 #undef MFL_HASH
+    }
     timer.end(MFL_STAGE_HASH, s);
     MFL_HIP(hipGetLastError());
-    // the index buffer starts as the identity (cl:63) and is generated inside the pass
+    // the index buffer starts as the identity (cl:63) and is generated by the run pass
     timer.begin(MFL_STAGE_SORT, s);
     const bool gather = n < kScatterCopyMin;
     auto* inv = inv_.as<uint32_t>();
-    if (select) {
-        hipLaunchKernelGGL((k_bucket_runs<true, true>), dim3(kRunDigits), dim3(kRunThreads), 0, s, keys, nullptr,
-                           nullptr, cell, index, cs, ce, inv, pos, vel, spos, svel);
-    } else {
-        SortPassOut high;
-        if (!sort.counted_pass(cell, n, 9, 9, s, &high)) return 0;
-        if (gather)
-            hipLaunchKernelGGL((k_bucket_runs<true, false>), dim3(kRunDigits), dim3(kRunThreads), 0, s, high.keys,
-                               high.vals, high.totals, cell, index, cs, ce, inv, pos, vel, spos, svel);
-        else
-            hipLaunchKernelGGL((k_bucket_runs<false, false>), dim3(kRunDigits), dim3(kRunThreads), 0, s, high.keys,
-                               high.vals, high.totals, cell, index, cs, ce, inv, pos, vel, spos, svel);
-    }
+    if (select)
+        hipLaunchKernelGGL((k_bucket_runs<true, kRunsSelect>), dim3(kRunDigits), dim3(kRunThreads), 0, s, skey,
+                           nullptr, nullptr, nullptr, 0u, 0u, cell, index, cs, ce, inv, pos, vel, spos, svel);
+    else if (gather)
+        hipLaunchKernelGGL((k_bucket_runs<true, kRunsTiles>), dim3(kRunDigits), dim3(kRunThreads), 0, s, tkey, tval,
+                           tcnt, tstart, ntiles, tile_keys, cell, index, cs, ce, inv, pos, vel, spos, svel);
+    else
+        hipLaunchKernelGGL((k_bucket_runs<false, kRunsTiles>), dim3(kRunDigits), dim3(kRunThreads), 0, s, tkey, tval,
+                           tcnt, tstart, ntiles, tile_keys, cell, index, cs, ce, inv, pos, vel, spos, svel);
     MFL_HIP(hipGetLastError());
     timer.end(MFL_STAGE_SORT, s);
     // bucket bounds come out of the sort (the BOUNDS stage stays empty)

@@ -70,6 +70,7 @@ public:
 
     // hash, sort, reorder + bucket clear and bounds of one step (the part before the density pass)
     int enqueue_front(hipStream_t s);
+    int tile_items(size_t n) const;
     DeviceBuffer* select(int kind);
     bool check_user_buffers();
     int get_tuning(mfl_tuning* out) const;
@@ -93,6 +94,8 @@ private:
     DeviceBuffer counts_;        // accepted visits per sorted particle (written when record_counts)
     DeviceBuffer porder_, mkey_; // Morton traversal order (caller indices) and its sort keys
     DeviceBuffer skey_;          // small steps: the cell keys in caller order, for the selecting run pass
+    DeviceBuffer tkey_, tval_;   // larger steps: keys and particle indices, each hash tile ordered by high digit
+    DeviceBuffer tcount_, tstart_;  // per high digit and tile: the tile's count of that digit, its start
     bool porder_valid_ = false;
     uint32_t steps_since_refresh_ = 0;
     int refresh_order(hipStream_t s);

@@ -3,13 +3,13 @@
 // Same contract as oclRadixSort::compute (libCL/sort/oclRadixSort.cpp:62-144): after the call the
 // pairs are stably ordered by bits [start_bit, end_bit) of the key. The reference walks 4-bit digits
 // with a serial 256-long scan per 256-item block (libCL/sort/oclRadixSort.cl:20-222); here each pass
-// takes up to 9 bits (so the 18-bit cell keys of the fluid step sort in two passes) and has three
-// kernels:
+// takes up to 9 bits (an 18-bit key sorts in two passes) and has three kernels:
 //   k_tile_counts : per-tile digit histogram in LDS            (reads 4 B/key)
 //   k_scan_digits : per-digit exclusive scan over tiles          (tiny)
 //   k_scatter     : stable in-tile ranking with wave64 ballot matching (no serial scans), tile
 //                   reordered by digit in LDS, then runs written contiguously (reads 8, writes 8 B/key)
 #include "radix_sort.hpp"
+#include "tile_rank.hpp"
 
 #include <mutex>
 
@@ -135,62 +135,6 @@ __device__ __forceinline__ void load_tile(const uint32_t* __restrict__ kin, cons
     }
 }
 
-// rank[r] = the key's rank among its wave's keys of the same digit (in input order); s_cnt[wave * R + d]
-// (zeroed beforehand) ends as the wave's count of digit d.
-template <int ITEMS, int BITS, uint32_t R>
-__device__ __forceinline__ void rank_tile(const uint32_t* key, uint32_t n, size_t tile_base, uint32_t first, int shift,
-                                 int bits, uint32_t* s_cnt, uint32_t* rank)
-{
-    const int lane = threadIdx.x & (kWave - 1);
-    const int wave = threadIdx.x / kWave;
-    const uint32_t mask = (1u << bits) - 1u;
-    const uint64_t lower_lanes = (1ull << lane) - 1ull;
-#pragma unroll
-    for (int r = 0; r < ITEMS; r++) {
-        const size_t idx = tile_base + first + (size_t)r * kWave;
-        const bool valid = idx < n;
-        const uint32_t d = (key[r] >> shift) & mask;
-        // lanes whose digit differs from this lane's in some bit (or that hold no key): per bit b, the
-        // ballot of bit b XOR this lane's bit b replicated over 64 bits
-        const uint64_t live = __ballot(valid);
-        uint32_t differ_lo = ~(uint32_t)live, differ_hi = ~(uint32_t)(live >> 32);
-#pragma unroll
-        for (int b = 0; b < (BITS > 0 ? BITS : kMaxDigitBits); b++) {
-            if (BITS > 0 || b < bits) {
-                const uint32_t rep = (uint32_t)((int)(d << (31 - b)) >> 31);  // 0 or all ones
-                const uint64_t bal = __ballot(rep != 0u);
-                differ_lo |= (uint32_t)bal ^ rep;
-                differ_hi |= (uint32_t)(bal >> 32) ^ rep;
-            }
-        }
-        const uint64_t same = ~((uint64_t)differ_hi << 32 | differ_lo);
-        const int leader = __ffsll((unsigned long long)same) - 1;
-        uint32_t base = 0;
-        if (valid && lane == leader) {
-            base = s_cnt[wave * R + d];
-            s_cnt[wave * R + d] = base + (uint32_t)__popcll(same);
-        }
-        base = __shfl(base, leader & (kWave - 1), kWave);
-        rank[r] = base + (uint32_t)__popcll(same & lower_lanes);
-    }
-}
-
-// Turns column d of s_cnt (one count per wave) into offsets over the waves; returns the tile's count of
-// digit d (0 for d >= radix).
-template <uint32_t R>
-__device__ __forceinline__ uint32_t wave_offsets(uint32_t d, uint32_t radix, uint32_t* s_cnt)
-{
-    if (d >= radix) return 0u;
-    uint32_t run = 0;
-#pragma unroll
-    for (int w = 0; w < kWaves; w++) {
-        const uint32_t c = s_cnt[w * R + d];
-        s_cnt[w * R + d] = run;
-        run += c;
-    }
-    return run;
-}
-
 // The tile reordered by digit in LDS, then written out in runs: the key at tile position p, digit d,
 // goes to s_base[d] + (p - s_start[d]), s_start[d] being digit d's first position in the tile.
 template <int ITEMS, uint32_t R>
@@ -268,9 +212,9 @@ __global__ __launch_bounds__(kThreads) void k_scatter(const uint32_t* __restrict
     const uint32_t first = (uint32_t)(wave * ITEMS) * kWave + (uint32_t)lane;  // tile offset of item 0
     load_tile<ITEMS, IDENTITY_VALS>(kin, vin, n, tile_base, first, key, val);
     __syncthreads();
-    rank_tile<ITEMS, BITS, R>(key, n, tile_base, first, shift, bits, s_cnt, rank);
+    rank_tile<ITEMS, BITS, R, kMaxDigitBits>(key, n, tile_base, first, shift, bits, s_cnt, rank);
     __syncthreads();
-    const uint32_t tile_cnt0 = wave_offsets<R>(d0, radix, s_cnt), tile_cnt1 = wave_offsets<R>(d1, radix, s_cnt);
+    const uint32_t tile_cnt0 = wave_offsets<R, kWaves>(d0, radix, s_cnt), tile_cnt1 = wave_offsets<R, kWaves>(d1, radix, s_cnt);
     {
         uint32_t all;
         const uint32_t st = block_exclusive_scan(tile_cnt0 + tile_cnt1, s_wave, &all);
@@ -528,35 +472,6 @@ int RadixSort::compute_noalloc(uint32_t* keys, uint32_t* vals, size_t n, int sta
     b.counts = base + 4 * kNoAllocMaxPairs;
     b.totals = b.counts + kNoAllocCounts;
     return sort_passes(b, keys, vals, n, start_bit, end_bit, stream, false, false, items);
-}
-
-int RadixSort::counted_pass(const uint32_t* keys, size_t n, int start_bit, int bits, hipStream_t stream,
-                            SortPassOut* out)
-{
-    if (n == 0 || n > 0xFFFFFFFFull || start_bit < 0 || bits < 1 || bits > kMaxDigitBits || start_bit + bits > 32 ||
-        !keys) {
-        set_error("radix sort: bad single pass");
-        return 0;
-    }
-    if (!reserve(n, bits)) return 0;
-    uint32_t* kout = tmp_keys_[0].as<uint32_t>();
-    uint32_t* vout = tmp_vals_[0].as<uint32_t>();
-    uint32_t* counts = counts_.as<uint32_t>();
-    uint32_t* totals = totals_.as<uint32_t>();
-    bool ok;
-    switch (pick_items(n, tuning.items)) {
-    case 16: ok = run_pass<16>(keys, nullptr, kout, vout, (uint32_t)n, start_bit, bits, counts, totals, true, stream,
-                               true); break;
-    case 8: ok = run_pass<8>(keys, nullptr, kout, vout, (uint32_t)n, start_bit, bits, counts, totals, true, stream,
-                             true); break;
-    default: ok = run_pass<4>(keys, nullptr, kout, vout, (uint32_t)n, start_bit, bits, counts, totals, true, stream,
-                              true); break;
-    }
-    if (!ok) return 0;
-    out->keys = kout;
-    out->vals = vout;
-    out->totals = totals;
-    return 1;
 }
 
 int RadixSort::sort_bits(uint32_t* keys, uint32_t* vals, size_t n, int start_bit, int end_bit, hipStream_t stream,

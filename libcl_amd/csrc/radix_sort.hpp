@@ -10,14 +10,6 @@ struct SortTuning {
     int items = 0;        // 0 = pick by size
 };
 
-// Output of RadixSort::counted_pass: the pairs ordered by one digit (in the sort's own buffers, valid
-// until its next call) and the number of keys of each digit value.
-struct SortPassOut {
-    const uint32_t* keys;
-    const uint32_t* vals;
-    const uint32_t* totals;
-};
-
 struct SortCountPlan {
     uint32_t* counts;
     uint32_t ntiles, tile_keys, bits;
@@ -45,23 +37,17 @@ public:
     // overlap in time on one GPU share that array and race.
     int compute_noalloc(uint32_t* keys, uint32_t* vals, size_t n, int start_bit, int end_bit, hipStream_t stream);
 
-    // Same sort for any bit range in [0, 32] (no multiple-of-4 rule); used by the fluid step, whose
-    // keys are below 2^18 so that bits [0, 18) order them exactly as the reference's [0, 24).
+    // Same sort for any bit range in [0, 32] (no multiple-of-4 rule); used by the batched and
+    // oclFluid3Dnext steps and the fluid step's traversal order.
     // `first_counted`: the caller already filled the first pass's count table (first_pass_plan).
     int sort_bits(uint32_t* keys, uint32_t* vals, size_t n, int start_bit, int end_bit, hipStream_t stream,
                   bool identity_vals, bool first_counted = false);
 
-    // The first pass's count table, for a caller that fills it while producing the keys (the fluid
-    // step's hash kernel): tile t holds keys [t tile_keys, (t + 1) tile_keys) and counts[d ntiles + t]
-    // must be the number of its keys whose bits [start_bit, start_bit + bits) equal d. Reserves the
-    // sort's temporaries.
+    // The first pass's count table, for a caller that fills it while producing the keys (the batched
+    // and oclFluid3Dnext hash kernels): tile t holds keys [t tile_keys, (t + 1) tile_keys) and
+    // counts[d ntiles + t] must be the number of its keys whose bits [start_bit, start_bit + bits)
+    // equal d. Reserves the sort's temporaries.
     int first_pass_plan(size_t n, int start_bit, int end_bit, SortCountPlan* plan);
-
-    // One stable pass by the digit at bits [start_bit, start_bit + bits) (bits <= 9) of n keys with the
-    // values 0 .. n - 1 (the clHash index), whose count table the caller filled (first_pass_plan with the
-    // same bits): the ordered pairs go to the sort's own buffers, for a caller that finishes the order
-    // itself (the fluid step's per-bucket pass).
-    int counted_pass(const uint32_t* keys, size_t n, int start_bit, int bits, hipStream_t stream, SortPassOut* out);
 
     // Allocates every temporary a sort of n keys over `nbits` bits needs (so that a later sort
     // enqueues no allocation, e.g. under hipGraph capture).

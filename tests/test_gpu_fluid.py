@@ -237,6 +237,26 @@ def test_bucket_run_longer_than_a_chunk(oracle, total):
     _oracle_compare(f, fo)
 
 
+@pytest.mark.parametrize("items", [4, 16])
+def test_run_blocks_over_several_tiles(oracle, items):
+    """The run pass's threads each take a contiguous range of hash tiles when there are more tiles than
+    threads: 786,432 particles in tiles of 1,024 keys (two tiles per thread), and 1,048,576 in tiles of
+    4,096 (one tile for every other thread). Keys, permutation, tables and counts bit-exact; density and
+    pressure within 1e-5."""
+    import libcl_amd as L
+    sc = scenes.config_1m()
+    count = 786432 if items == 4 else sc.n
+    print(f"run pass over tiles of {256 * items} keys, {count} particles", flush=True)
+    pos, vel, force = sc.pos[:count].copy(), sc.vel[:count].copy(), sc.force[:count].copy()
+    prm = L.params_from(sc.params)
+    f = make_fluid(pos, vel, force, prm, sc.box_min, sc.box_max)
+    assert f.setTuning(sort_items=items) == 1, L.last_error()
+    run_forces_only(f)
+    fo = oracle.Fluid(pos, vel, force, oracle.Params(*prm.values()), box_min=sc.box_min, box_max=sc.box_max)
+    fo.step(integrate=False)
+    _oracle_compare(f, fo)
+
+
 @pytest.mark.parametrize("shape", ["tile_list", "tile_list8", "tile", "tile_kept", "tile_kept2", "list"])
 def test_1m_step_vs_oracle(oracle, shape):
     """1,048,576 particles in [0,4]^3: keys, permutation, bucket tables bit-exact; density, pressure
