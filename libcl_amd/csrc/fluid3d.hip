@@ -1224,14 +1224,16 @@ __global__ __launch_bounds__(kThreads) void k_forces_kept(const float4* __restri
     pos[o] = f4_add(pos[o], f4_scale(v, p.deltaTime));
 }
 
+// Morton code of each particle's cell, coordinates taken modulo 2^mbits (mbits <= 10).
 __global__ __launch_bounds__(kThreads) void k_morton_keys(const float4* __restrict__ pos, uint32_t n, float h,
-                                                          uint32_t* __restrict__ key)
+                                                          uint32_t mbits, uint32_t* __restrict__ key)
 {
     const uint32_t o = blockIdx.x * kThreads + threadIdx.x;
     if (o >= n) return;
     const float4 q = pos[o];
-    key[o] = spread_bits3((uint32_t)grid_coord(q.x, h)) | (spread_bits3((uint32_t)grid_coord(q.y, h)) << 1) |
-             (spread_bits3((uint32_t)grid_coord(q.z, h)) << 2);
+    const uint32_t m = (1u << mbits) - 1u;
+    key[o] = spread_bits3((uint32_t)grid_coord(q.x, h) & m) | (spread_bits3((uint32_t)grid_coord(q.y, h) & m) << 1) |
+             (spread_bits3((uint32_t)grid_coord(q.z, h) & m) << 2);
 }
 
 __global__ __launch_bounds__(kThreads) void k_clip_box(const float4* __restrict__ spos, const float4* __restrict__ svel,
@@ -2038,14 +2040,28 @@ int Fluid3D::launch_graph(hipStream_t s)
     return 1;
 }
 
+// Bits per axis of the traversal's Morton keys: 8 when the box lies in [0, 254 h] on every axis (its
+// cells' coordinates are below 256, where 8-bit keys order the cells exactly as 10-bit ones: 24-bit
+// keys, three sort passes instead of four), else 10. Any key is a valid order; a particle outside the
+// box only gets a less local one.
+uint32_t Fluid3D::morton_bits() const
+{
+    const float h = params.cellSize;
+    if (!(h > 0.f)) return 10;
+    for (int a = 0; a < 3; a++)
+        if (!(box_min[a] >= 0.f) || !(box_max[a] <= 254.f * h)) return 10;
+    return 8;
+}
+
 int Fluid3D::refresh_order(hipStream_t s)
 {
     const uint32_t n = (uint32_t)n_;
+    const uint32_t mbits = morton_bits();
     timer.begin(MFL_STAGE_ORDER, s);
-    hipLaunchKernelGGL(k_morton_keys, grid_for(n), dim3(kThreads), 0, s, pos_.as<float4>(), n, params.cellSize,
+    hipLaunchKernelGGL(k_morton_keys, grid_for(n), dim3(kThreads), 0, s, pos_.as<float4>(), n, params.cellSize, mbits,
                        mkey_.as<uint32_t>());
     MFL_HIP(hipGetLastError());
-    if (!sort.sort_bits(mkey_.as<uint32_t>(), porder_.as<uint32_t>(), n, 0, 30, s, true)) return 0;
+    if (!sort.sort_bits(mkey_.as<uint32_t>(), porder_.as<uint32_t>(), n, 0, (int)(3 * mbits), s, true)) return 0;
     timer.end(MFL_STAGE_ORDER, s);
     porder_valid_ = true;
     return 1;

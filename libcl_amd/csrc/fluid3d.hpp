@@ -98,6 +98,7 @@ private:
     DeviceBuffer tcount_, tstart_;  // per high digit and tile: the tile's count of that digit, its start
     bool porder_valid_ = false;
     uint32_t steps_since_refresh_ = 0;
+    uint32_t morton_bits() const;
     int refresh_order(hipStream_t s);
 
     hipGraph_t graph_ = nullptr;
