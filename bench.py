@@ -21,8 +21,11 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0  # MI355X HBM3E, vendor figure
-# VALU issue peak: 256 CUs x 4 SIMDs, one wave64 instruction per SIMD every 4 cycles at 2.4 GHz
-VALU_PEAK_G = 256 * 4 * 2.4 / 4  # G wave64 VALU instructions/s
+# VALU issue: 256 CUs x 4 SIMDs at 2.4 GHz. A wave64 instruction holds its SIMD for 4 cycles when it is an
+# FP32 add, mul or fma (or a 32-bit shift) and for 2 when it is a 32-bit integer add, bitwise, compare or
+# select (measured on MI355X with independent chains of one instruction: 603-608 against 1145-1195 G/s).
+VALU_SIMDS, VALU_GHZ = 256 * 4, 2.4
+VALU_PEAK_G = VALU_SIMDS * VALU_GHZ / 4  # G wave64 instructions/s, every instruction at 4 cycles
 
 # Algorithmic HBM bytes per particle of each stage: the reference dataflow with the minimum
 # materialisation its API exposes (sorted buffers, index and state), each array touched once;
@@ -67,17 +70,84 @@ def measured_traffic(stage: str, workload: str, tuning: dict):
 
 
 def measured_valu(stage: str, workload: str, tuning: dict):
-    """VALU wave64 instructions per launch of a stage's kernel from the committed rocprofv3 SQ counter
-    summary (profiles/valu.json: SQ_INSTS_VALU per dispatch); None when that kernel and workload were not
-    measured at this launch shape."""
+    """(VALU wave64 instructions per launch, FP32 share of them) of a stage's kernel from the committed
+    rocprofv3 SQ counter summary (profiles/valu.json: SQ_INSTS_VALU per dispatch and the FP32 classes'
+    share, SQ_INSTS_VALU_{ADD,MUL,FMA,TRANS}_F32); None when that kernel and workload were not measured at
+    this launch shape (the share is None when the class counters were not taken)."""
     t = _profiled("valu.json", workload, tuning)
     if not t or stage not in t:
         return None
-    return t[stage]["valu_per_wave"] * t[stage]["waves"]
+    return t[stage]["valu_per_wave"] * t[stage]["waves"], t[stage].get("fp32_share")
 
 
 def progress(msg: str) -> None:
     print(f"bench: {msg}", file=sys.stderr, flush=True)
+
+
+_HIP = None
+
+
+def _copy_d2d(dst: int, src: int, nbytes: int, stream) -> None:
+    """hipMemcpyAsync device to device on `stream`, through the HIP runtime torch has loaded."""
+    global _HIP
+    if _HIP is None:
+        import ctypes
+
+        import torch
+        lib = os.path.join(os.path.dirname(torch.__file__), "lib", "libamdhip64.so")
+        _HIP = ctypes.CDLL(lib if os.path.exists(lib) else "libamdhip64.so")
+        _HIP.hipMemcpyAsync.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_size_t, ctypes.c_int,
+                                        ctypes.c_void_p]
+        _HIP.hipMemcpyAsync.restype = ctypes.c_int
+    rc = _HIP.hipMemcpyAsync(dst, src, nbytes, 3, stream.cuda_stream)  # 3 = hipMemcpyDeviceToDevice
+    if rc != 0:
+        raise RuntimeError(f"hipMemcpyAsync failed with error {rc}")
+
+
+class DeviceSnapshot:
+    """Device buffers ((pointer, bytes) pairs) copied device to device, and put back the same way:
+    restoring them takes microseconds and no host round trip, so the card stays at the clocks of the
+    work before it. `after_restore` runs after each restore."""
+
+    def __init__(self, buffers, stream, after_restore=None):
+        import torch
+        self.bufs, self.stream, self.after = list(buffers), stream, after_restore
+        self.copies = [torch.empty(nbytes, dtype=torch.uint8, device="cuda") for _, nbytes in self.bufs]
+        for (ptr, nbytes), c in zip(self.bufs, self.copies):
+            _copy_d2d(c.data_ptr(), ptr, nbytes, stream)
+        stream.synchronize()
+
+    def restore(self) -> None:
+        for (ptr, nbytes), c in zip(self.bufs, self.copies):
+            _copy_d2d(ptr, c.data_ptr(), nbytes, self.stream)
+        if self.after:
+            self.after()
+        self.stream.synchronize()
+
+
+def program_snapshot(f, stream) -> DeviceSnapshot:
+    """A single-scene program's state: positions, velocities, forces and both bucket tables. A restore
+    also makes the next step rebuild the Morton order, so every leg started from the snapshot refreshes
+    it at the same steps."""
+    bufs = [(b.ptr, b.nbytes) for b in (f.bfPosition, f.bfVelocity, f.bfForce, f.bfCellStart, f.bfCellEnd)]
+    return DeviceSnapshot(bufs, stream, lambda: f.setTraversal(*f.getTraversal()))
+
+
+def hold_clocks(step, stream, reset=None, seconds: float = 0.3) -> int:
+    """Calls step() untimed for `seconds`, in rounds of 16 calls that each start with reset() (restoring
+    the state the timed steps start from, so the untimed steps cost what the timed ones do), and returns
+    the call count. The card lowers its clocks within milliseconds of idling and needs tens of
+    milliseconds of continuous work to raise them again: 20 steps of the 1M workload take 0.22-0.24 ms
+    each right after a host round trip or an idle pause, 0.21 ms after this."""
+    t0, k = time.perf_counter(), 0
+    while time.perf_counter() - t0 < seconds:
+        if reset:
+            reset()
+        for _ in range(16):
+            step()
+        stream.synchronize()
+        k += 16
+    return k
 
 
 def host_cores() -> tuple:
@@ -342,8 +412,9 @@ def next_leg(steps: int = 50, cpu: bool = True) -> dict:
 def small_leg(steps: int = 1000, reps: int = 3, cpu: bool = True) -> dict:
     """The 16,384-particle dam break in the unit box (sph_16k_unit_box), its 1000 steps from the initial
     state with the autotuned launch shape, as eager launches and as hipGraph replays on one stream,
-    alternating, `reps` times each (the best run of each counts): at this size a step is launch-bound,
-    so the figure is particle-steps/s and the graph gain."""
+    alternating, `reps` times each (the best run of each counts), each run after 0.3 s of untimed
+    stepping (hold_clocks): at this size a step is launch-bound, so the figure is particle-steps/s and
+    the graph gain."""
     import numpy as np
     import torch
 
@@ -370,11 +441,17 @@ def small_leg(steps: int = 1000, reps: int = 3, cpu: bool = True) -> dict:
         assert f.setUseGraph(mode == "graph") == 1
         progs[mode] = f
     empty = np.full(262144, 0xFFFFFFFF, np.uint32)
+    snaps = {}
+    for mode, f in progs.items():  # the initial state, restored device to device before each timed run
+        assert f.upload(sc.pos, sc.vel, sc.force) == 1
+        assert f.bfCellStart.write(empty) == 1 and f.bfCellEnd.write(empty) == 1
+        torch.cuda.synchronize()
+        snaps[mode] = program_snapshot(f, stream)
     best = {"eager": float("inf"), "graph": float("inf")}
     for _ in range(reps):
         for mode, f in progs.items():
-            assert f.upload(sc.pos, sc.vel, sc.force) == 1
-            assert f.bfCellStart.write(empty) == 1 and f.bfCellEnd.write(empty) == 1
+            hold_clocks(lambda f=f: f.compute(stream), stream, snaps[mode].restore)
+            snaps[mode].restore()
             torch.cuda.synchronize()
             t0 = time.perf_counter()
             for _ in range(steps):
@@ -408,7 +485,8 @@ def sort_leg(reps: int = 10, cpu: bool = True) -> dict:
     """oclRadixSort on 16,777,216 (key, value) pairs by bits [0, 24): keys over 2^24 cells, uniform
     (seed 3) and skewed (half the keys in a hot 10% of the cells, chosen with seed 4), values 0..N-1.
     Every timed sort starts from the same unsorted input, restored between sorts outside the HIP events
-    that time the sort; the sorted output is checked for order and for the value permutation."""
+    that time the sort, after 0.3 s of untimed sorts (hold_clocks); the sorted output is checked for order
+    and for the value permutation."""
     import numpy as np
     import torch
 
@@ -428,6 +506,13 @@ def sort_leg(reps: int = 10, cpu: bool = True) -> dict:
         k0 = torch.from_numpy(keys.view(np.int32).copy()).cuda()
         v0 = torch.arange(n, dtype=torch.int32, device="cuda")
         k, v = k0.clone(), v0.clone()
+
+        def one_sort():
+            k.copy_(k0)
+            v.copy_(v0)
+            assert s.compute(k, v, 0, 24) == 1, L.last_error()
+
+        hold_clocks(one_sort, torch.cuda.current_stream())
         times = []
         for _ in range(reps + 2):
             k.copy_(k0)
@@ -557,16 +642,24 @@ def main():
         tuned = {"ms_per_step": res[1], "pick": res[0]}
     tuning = f.getTuning()  # with the traversal order (the autotuner may change it)
 
+    # The card lowers its clocks within milliseconds of idling, so each timed leg below starts from a
+    # device-side snapshot right after 0.3 s of untimed stepping (hold_clocks): the timed steps run as
+    # in a simulation that steps continuously. `cold_ms_per_step` times the same steps after a 0.2 s pause.
+    empty = np.full(262144, 0xFFFFFFFF, np.uint32)
+    # one untimed step first (the program's first step sets up its buffers), then the initial state again
+    assert f.compute(stream) == 1, L.last_error()
+    assert f.upload(sc.pos, sc.vel, sc.force) == 1
+    assert f.bfCellStart.write(empty) == 1 and f.bfCellEnd.write(empty) == 1
+    torch.cuda.synchronize()
+    initial = program_snapshot(f, stream)
+    step = lambda: f.compute(stream)  # noqa: E731
+
     # the whole trajectory from the initial state: HIP events on the step stream at steps 0, 1, 10, 100
     # and the last (each step's cost grows as the flow develops)
     trajectory = None
-    empty = np.full(262144, 0xFFFFFFFF, np.uint32)
     if args.trajectory_steps > 0:
-        # one untimed step first (the program's first step sets up its buffers), then the initial state again
-        assert f.compute(stream) == 1, L.last_error()
-        assert f.upload(sc.pos, sc.vel, sc.force) == 1
-        assert f.bfCellStart.write(empty) == 1 and f.bfCellEnd.write(empty) == 1
-        torch.cuda.synchronize()
+        hold_clocks(step, stream, initial.restore)
+        initial.restore()
         marks = sorted({0, 1, 10, 100, args.trajectory_steps} & set(range(args.trajectory_steps + 1)))
         events = {}
         for k in range(args.trajectory_steps + 1):
@@ -581,21 +674,12 @@ def main():
                       "particle_steps_per_s": n * args.trajectory_steps / (total_ms / 1e3),
                       "segments_ms_per_step": {f"{a}-{b}": events[a].elapsed_time(events[b]) / (b - a)
                                                for a, b in zip(marks[:-1], marks[1:])}}
-        assert f.upload(sc.pos, sc.vel, sc.force) == 1
-        assert f.bfCellStart.write(empty) == 1 and f.bfCellEnd.write(empty) == 1
-        torch.cuda.synchronize()
 
+    initial.restore()
     for _ in range(args.warmup):
         assert f.compute(stream) == 1, L.last_error()
     stream.synchronize()
-    # every timed leg starts from this state
-    snap_state, snap_tables = f.state(), f.internals()
-
-    def restore():
-        assert f.upload(snap_state["pos"], snap_state["vel"], snap_state["force"]) == 1
-        assert f.bfCellStart.write(snap_tables["cell_start"]) == 1
-        assert f.bfCellEnd.write(snap_tables["cell_end"]) == 1
-        torch.cuda.synchronize()
+    start = program_snapshot(f, stream)  # every timed leg starts from this state
 
     # untimed stage profile: which kernel dominates the step
     f.setProfiling(L._native.STAGES)
@@ -607,48 +691,60 @@ def main():
     for s in L._native.STAGES:
         ms, cnt = f.stageTime(s)
         stage_ms[s] = ms / max(cnt, 1)
+    f.setProfiling([])
     f.stageTime("hash", reset=True)
     # the dominant per-step kernel ("order", the traversal refresh, runs every 16 steps, not per step)
     dominant = max((st for st in stage_ms if STAGE_BYTES_PER_PARTICLE.get(st)), key=stage_ms.get)
 
+    def run_steps() -> float:
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(args.steps):
+            f.compute(stream)
+        torch.cuda.synchronize()
+        return time.perf_counter() - t0
+
     # timed region: K steps, HIP events around the dominant kernel on its stream
-    restore()
+    hold_clocks(step, stream, start.restore)
+    start.restore()
     f.setProfiling([dominant])
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    for _ in range(args.steps):
-        f.compute(stream)
-    torch.cuda.synchronize()
-    t1 = time.perf_counter()
+    f.stageTime(dominant, reset=True)
+    elapsed = run_steps()
     dom_ms, dom_cnt = f.stageTime(dominant, reset=True)
     f.setProfiling([])
-    elapsed = t1 - t0
     ms_per_step = elapsed * 1e3 / args.steps
     value = n * args.steps / elapsed
 
+    # the same K steps from the same state after the card idled 0.2 s
+    start.restore()
+    time.sleep(0.2)
+    cold_ms = run_steps() * 1e3 / args.steps
+
     # the same K steps from the same state, replayed from a hipGraph (no events)
     f.setUseGraph(True)
-    f.compute(stream)
-    restore()
-    g0 = time.perf_counter()
-    for _ in range(args.steps):
-        f.compute(stream)
-    torch.cuda.synchronize()
-    graph_ms = (time.perf_counter() - g0) * 1e3 / args.steps
+    hold_clocks(step, stream, start.restore)  # its first step captures the graph
+    start.restore()
+    graph_ms = run_steps() * 1e3 / args.steps
 
     avg_s = dom_ms / max(dom_cnt, 1) / 1e3
     alg_bytes = STAGE_BYTES_PER_PARTICLE[dominant] * n + STAGE_BYTES_PER_LAUNCH.get(dominant, 0)
     hbm_achieved = alg_bytes / avg_s / 1e9
     step_bytes = sum(STAGE_BYTES_PER_PARTICLE.values()) * n + STEP_BYTES_PER_LAUNCH
     traffic = measured_traffic(dominant, sc.name, tuning)
-    valu_instr = measured_valu(dominant, sc.name, tuning)
+    valu = measured_valu(dominant, sc.name, tuning)
+    valu_instr, fp32_share = valu if valu else (None, None)
     hbm = {"achieved": hbm_achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": hbm_achieved / HBM_PEAK_GBS,
            "alg_bytes_per_launch": alg_bytes, "traffic": traffic}
-    if valu_instr is not None and valu_instr / avg_s / 1e9 / VALU_PEAK_G > hbm["frac"]:
+    # the issue rate the kernel's instruction mix allows: FP32 classes at 4 cycles, every other instruction
+    # at 2 (the fastest measured; shifts take 4, so this peak is an upper bound and `frac` a lower one)
+    cycles = 4.0 if fp32_share is None else 4.0 * fp32_share + 2.0 * (1.0 - fp32_share)
+    valu_peak = VALU_SIMDS * VALU_GHZ / cycles
+    if valu_instr is not None and valu_instr / avg_s / 1e9 / valu_peak > hbm["frac"]:
         # counters taken at this launch shape: the kernel's binding resource is VALU issue
         achieved = valu_instr / avg_s / 1e9
-        roofline = {"bound": "valu", "kernel": dominant, "achieved": achieved, "peak": VALU_PEAK_G,
-                    "unit": "G wave64 VALU instr/s", "frac": achieved / VALU_PEAK_G, "traffic": traffic,
+        roofline = {"bound": "valu", "kernel": dominant, "achieved": achieved, "peak": valu_peak,
+                    "unit": "G wave64 VALU instr/s", "frac": achieved / valu_peak, "traffic": traffic,
+                    "fp32_share": fp32_share, "frac_all_4_cycles": achieved / VALU_PEAK_G,
                     "valu_instr_per_launch": valu_instr, "avg_launch_us": avg_s * 1e6, "hbm": hbm}
     else:
         roofline = dict(hbm, bound="hbm", kernel=dominant, avg_launch_us=avg_s * 1e6,
@@ -677,6 +773,8 @@ def main():
         "step_hbm_frac": step_bytes / (ms_per_step / 1e3) / 1e9 / HBM_PEAK_GBS,
         "graph_ms_per_step": graph_ms,
         "graph_particle_steps_per_s": n / (graph_ms / 1e3),
+        "cold_ms_per_step": cold_ms,
+        "cold_particle_steps_per_s": n / (cold_ms / 1e3),
     }
     if trajectory:
         out["trajectory"] = trajectory

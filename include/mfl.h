@@ -181,7 +181,8 @@ int mfl_fluid3d_set_record_counts(mfl_fluid3d* f, int enable);
    walks a Morton order of the particles' cells rebuilt every `refresh_steps` steps (0 = once).
    Until this is called, mfl_fluid3d_set_particle_count picks the order by size: SORTED up to 131,072
    particles (their state fits one XCD's L2, so the hash order costs no locality, and it never goes
-   stale between refreshes), MORTON with a 16-step refresh above. */
+   stale between refreshes), MORTON with a 16-step refresh above. A call also makes the next compute
+   rebuild the Morton order (after a caller rewrites the state, for instance). */
 enum mfl_traversal { MFL_TRAVERSAL_SORTED = 0, MFL_TRAVERSAL_CALLER = 1, MFL_TRAVERSAL_MORTON = 2 };
 int mfl_fluid3d_set_traversal(mfl_fluid3d* f, int order, uint32_t refresh_steps);
 /* The current order and refresh period. */

@@ -308,6 +308,7 @@ int mfl_fluid3d_set_traversal(mfl_fluid3d* f, int order, uint32_t refresh_steps)
     f->impl.traversal = order;
     f->impl.refresh_interval = refresh_steps;
     f->impl.traversal_chosen = true;
+    f->impl.invalidate_order();
     return 1;
 }
 

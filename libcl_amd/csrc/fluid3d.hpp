@@ -76,6 +76,12 @@ public:
     int get_tuning(mfl_tuning* out) const;
     int set_tuning(const mfl_tuning& t);
     int autotune(hipStream_t s, uint32_t steps, mfl_tuning* best, double* ms_per_step);
+    // the Morton order is rebuilt by the next compute
+    void invalidate_order()
+    {
+        porder_valid_ = false;
+        steps_since_refresh_ = 0;
+    }
 
 private:
     bool allocate_internal();

@@ -92,3 +92,17 @@ def test_gpus_flag_rejected_above_one():
                        text=True, timeout=120)
     assert r.returncode == 2
     assert "one GPU" in r.stderr
+
+
+def test_committed_counter_profiles_match_the_timed_launch_shape():
+    """The bench's roofline reads the committed counter summaries only when they were taken at the launch
+    shape it times (the library's default shape at 1M particles, Morton traversal): they must be."""
+    sys.path.insert(0, ROOT)
+    import bench
+
+    shape = dict(neighbour_pass="tile_kept", density_lanes=2, tile_depth=96, block_waves=2, sort_items=0,
+                 force_chunks=2, traversal="morton")
+    instr, fp32_share = bench.measured_valu("density", "sph_1m_box4", shape)
+    assert instr > 0 and 0.0 < fp32_share < 1.0
+    assert bench.measured_traffic("density", "sph_1m_box4", shape) > 0
+    assert bench.measured_valu("density", "sph_1m_box4", dict(shape, tile_depth=64)) is None
