@@ -456,6 +456,37 @@ def test_traversal_orders_bit_identical():
             np.testing.assert_array_equal(it[k], states[0][1][k])
 
 
+def test_set_traversal_rebuilds_the_morton_order():
+    """setTraversal makes the next step rebuild the Morton order: a program whose order was built from
+    other positions (the same particles in a shuffled caller order) and then had its state rewritten
+    steps bit-identically to a fresh program from that state (with the tile passes the FP32 summation
+    order follows the traversal order, so a stale order shows)."""
+    import libcl_amd as L
+    sc = scenes.dam_break(32, 16, 16, 1.0, seed=41, steps=5)
+    prm = L.params_from(sc.params)
+    empty = np.full(262144, 0xFFFFFFFF, np.uint32)
+    perm = np.random.default_rng(7).permutation(sc.n)
+    used = make_fluid(sc.pos[perm], sc.vel[perm], sc.force[perm], prm, sc.box_min, sc.box_max, tables=(empty, empty))
+    assert used.setTraversal("morton", 16) == 1
+    for _ in range(3):
+        assert used.compute() == 1, L.last_error()
+    assert used.upload(sc.pos, sc.vel, sc.force) == 1
+    assert used.bfCellStart.write(empty) == 1 and used.bfCellEnd.write(empty) == 1
+    assert used.setTraversal("morton", 16) == 1
+    fresh = make_fluid(sc.pos, sc.vel, sc.force, prm, sc.box_min, sc.box_max, tables=(empty, empty))
+    assert fresh.setTraversal("morton", 16) == 1
+    for f in (used, fresh):
+        for _ in range(6):
+            assert f.compute() == 1, L.last_error()
+    L.load().mfl_device_finish()
+    a, b = used.state(), fresh.state()
+    for k in ("pos", "vel", "force"):
+        np.testing.assert_array_equal(a[k], b[k])
+    ia, ib = used.internals(), fresh.internals()
+    for k in ("spos", "svel", "cell_start", "cell_end", "index"):
+        np.testing.assert_array_equal(ia[k], ib[k])
+
+
 @pytest.mark.parametrize("order", ["sorted", "caller", "morton"])
 @pytest.mark.parametrize("shape", ["tile_list", "tile_list8", "tile", "tile64", "tile_list64", "tile_list4_64",
                                    "tile_list4_ch2", "tile_kept", "tile_kept8", "tile_kept64", "tile_kept2",
