@@ -60,20 +60,29 @@ __global__ __launch_bounds__(kThreads) void k_b_hash_counts(const float4* __rest
     for (uint32_t d = threadIdx.x; d < radix; d += kThreads) counts[(size_t)d * ntiles + tile] = hist[d];
 }
 
-// Inverse permutation (inv[index[i]] = i, for k_scatter_sorted's copy of the state, cl:66-72) and
-// clFindBounds (cl:85-113) per scene in scene-local indices (the tables were reset before the sort;
-// nothing between reads them).
+// The sorted state gathered through the index (clReorder, cl:66-72), the inverse permutation
+// (inv[index[i]] = i, for the neighbour passes' traversal) and clFindBounds (cl:85-113) per scene in
+// scene-local indices (the tables were reset before the sort; nothing between reads them). The gather
+// reads inside one scene's range (at most 65,536 particles, 2 MiB of pos + vel): blocks are dealt to
+// the XCDs in contiguous ranges of sorted slots (xcd_tile), so the blocks of one scene share an XCD's L2.
 __global__ __launch_bounds__(kThreads) void k_b_slots_bounds(const uint32_t* __restrict__ index, uint32_t N,
                                                              uint32_t* __restrict__ inv,
                                                              const uint32_t* __restrict__ cell,
                                                              const uint32_t* __restrict__ off,
                                                              uint32_t* __restrict__ cs_all,
-                                                             uint32_t* __restrict__ ce_all)
+                                                             uint32_t* __restrict__ ce_all,
+                                                             const float4* __restrict__ pos,
+                                                             const float4* __restrict__ vel,
+                                                             float4* __restrict__ spos, float4* __restrict__ svel)
 {
-    const uint32_t i = blockIdx.x * kThreads + threadIdx.x;
+    const uint32_t i = xcd_tile() * kThreads + threadIdx.x;
     if (i >= N) return;
     const uint32_t g = index[i];
-    if (g < N) inv[g] = i;  // index is a permutation of [0, N)
+    if (g < N) {  // index is a permutation of [0, N)
+        inv[g] = i;
+        spos[i] = pos[g];
+        svel[i] = vel[g];
+    }
     const uint32_t key = cell[i];
     const uint32_t s = key >> 18, h = key & (kBuckets - 1u);
     const uint32_t lo = off[s], n_s = off[s + 1] - lo, li = i - lo;
@@ -737,8 +746,8 @@ int FluidBatch::enqueue_step(hipStream_t s)
         hipLaunchKernelGGL(k_b_clear, dim3(bx, S_), b, 0, s, cs_all, ce_all, clear);
     }
     if (!sort.sort_bits(cell, index, N_, 0, 18 + (int)scene_bits_, s, true, true)) return 0;
-    hipLaunchKernelGGL(k_b_slots_bounds, g, b, 0, s, index, N_, inv_.as<uint32_t>(), cell, off, cs_all, ce_all);
-    hipLaunchKernelGGL(k_scatter_sorted, g, b, 0, s, inv_.as<uint32_t>(), pos, vel, spos, svel, N_);
+    hipLaunchKernelGGL(k_b_slots_bounds, g, b, 0, s, index, N_, inv_.as<uint32_t>(), cell, off, cs_all, ce_all, pos,
+                       vel, spos, svel);
     const bool tiles = (neighbour_pass == MFL_PASS_TILE_LIST || neighbour_pass == MFL_PASS_TILE_KEPT) && porder_valid_;
     const bool kept = tiles && neighbour_pass == MFL_PASS_TILE_KEPT;
     if (kept && !(kept_.ptr && hitmask_.ptr && hitx_.ptr && meta_.ptr)) {
