@@ -431,10 +431,13 @@ __global__ __launch_bounds__(kRunThreads) void k_bucket_runs(const uint32_t* __r
             const bool valid = q < cnt;
             key[r] = valid ? s_keys[q - c0] : 0u;  // the chunk, gathered into LDS above
             val[r] = valid ? s_vals[q - c0] : 0u;
+            rank[r] = 0;
         }
         __syncthreads();  // read before the reorder below overwrites s_keys / s_vals
 #pragma unroll
         for (int r = 0; r < kRunItems; r++) {
+            // rounds wholly past the run's end rank nothing (wave-uniform: the rest of the wave's rounds too)
+            if (c0 + (uint32_t)(wave * kRunItems + r) * kWave >= cnt) break;
             const uint32_t q = c0 + (wave * kRunItems + r) * kWave + lane;
             const bool valid = q < cnt;
             const uint32_t d = key[r] & (kRunDigits - 1u);
