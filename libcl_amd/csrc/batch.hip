@@ -11,6 +11,7 @@
 #include "sph_device.hpp"
 #include "tile.hpp"
 
+#include <chrono>
 #include <cstring>
 
 namespace mfl {
@@ -1017,6 +1018,16 @@ int FluidBatch::autotune(hipStream_t s, uint32_t steps, mfl_tuning* best, double
     use_graph = false;
     hipEvent_t e0 = nullptr, e1 = nullptr;
     int ok = hip_ok(hipEventCreate(&e0), "hipEventCreate") && hip_ok(hipEventCreate(&e1), "hipEventCreate");
+    // The card lowers its clocks within milliseconds of idling: step the starting shape untimed for 30 ms
+    // first, so that the first candidates are not timed at lower clocks than the last ones.
+    {
+        const auto t0 = std::chrono::steady_clock::now();
+        while (ok && std::chrono::steady_clock::now() - t0 < std::chrono::milliseconds(30)) {
+            ok = snapshot(true);
+            for (int j = 0; ok && j < 4; j++) ok = compute(s);
+            ok = ok && hip_ok(hipStreamSynchronize(s), "hipStreamSynchronize");
+        }
+    }
     double best_ms = 1e30;
     mfl_tuning pick = start;
     for (size_t c = 0; ok && c < cand.size(); c++) {

@@ -27,6 +27,8 @@
 #include "tile.hpp"
 #include "tile_rank.hpp"
 
+#include <algorithm>
+#include <chrono>
 #include <cstring>
 
 namespace mfl {
@@ -1952,29 +1954,58 @@ int Fluid3D::autotune(hipStream_t s, uint32_t steps, mfl_tuning* best, double* m
     timer.mask = 0;
     hipEvent_t e0 = nullptr, e1 = nullptr;
     int ok = hip_ok(hipEventCreate(&e0), "hipEventCreate") && hip_ok(hipEventCreate(&e1), "hipEventCreate");
-    double best_ms = 1e30;
-    mfl_tuning pick = start;
-    auto trial = [&](const mfl_tuning& t) {
-        ok = set_tuning(t) && snapshot(true) && compute(s);  // one untimed step per trial
-        for (uint32_t k = 0; ok && k < steps; k++) {
-            if (k == 0) ok = hip_ok(hipEventRecord(e0, s), "hipEventRecord");
+    // ms per step of shape t over k timed steps from the snapshot (after one untimed step), or a negative
+    // value when a call failed
+    auto trial = [&](const mfl_tuning& t, uint32_t k) -> double {
+        ok = set_tuning(t) && snapshot(true) && compute(s);
+        for (uint32_t j = 0; ok && j < k; j++) {
+            if (j == 0) ok = hip_ok(hipEventRecord(e0, s), "hipEventRecord");
             ok = ok && compute(s);
         }
         ok = ok && hip_ok(hipEventRecord(e1, s), "hipEventRecord") && hip_ok(hipEventSynchronize(e1), "sync");
         float ms = 0.f;
         ok = ok && hip_ok(hipEventElapsedTime(&ms, e0, e1), "hipEventElapsedTime");
-        if (ok && ms / steps < best_ms) {
-            best_ms = ms / steps;
-            pick = t;
-        }
+        return ok ? (double)ms / k : -1.0;
     };
-    for (size_t c = 0; ok && c < cand.size(); c++) trial(cand[c]);
-    // then the sort's tile size, with the best neighbour passes
+    // The card lowers its clocks within milliseconds of idling and needs tens of milliseconds of work to
+    // raise them: step the starting shape untimed for 30 ms first, so that the first candidates are not
+    // timed at lower clocks than the last ones.
+    {
+        const auto t0 = std::chrono::steady_clock::now();
+        ok = ok && set_tuning(start);
+        while (ok && std::chrono::steady_clock::now() - t0 < std::chrono::milliseconds(30)) {
+            ok = snapshot(true);
+            for (int j = 0; ok && j < 16; j++) ok = compute(s);
+            ok = ok && hip_ok(hipStreamSynchronize(s), "hipStreamSynchronize");
+        }
+    }
+    // every candidate for `steps` steps, then the four fastest again for four times as many (a short
+    // trial's ranking is noisy), then the sort's tile size with the best neighbour passes
+    std::vector<std::pair<double, size_t>> times;
+    for (size_t c = 0; ok && c < cand.size(); c++) {
+        const double ms = trial(cand[c], steps);
+        if (ok) times.emplace_back(ms, c);
+    }
+    std::sort(times.begin(), times.end());
+    double best_ms = 1e30;
+    mfl_tuning pick = start;
+    for (size_t r = 0; ok && r < times.size() && r < 4; r++) {
+        const double ms = trial(cand[times[r].second], 4 * steps);
+        if (ok && ms < best_ms) {
+            best_ms = ms;
+            pick = cand[times[r].second];
+        }
+    }
     const mfl_tuning passes = pick;
     for (uint32_t items : {4u, 8u, 16u}) {
         mfl_tuning t = passes;
         t.sort_items = items;
-        if (ok && items != passes.sort_items) trial(t);
+        if (!ok || items == passes.sort_items) continue;
+        const double ms = trial(t, 4 * steps);
+        if (ok && ms < best_ms) {
+            best_ms = ms;
+            pick = t;
+        }
     }
     if (e0) (void)hipEventDestroy(e0);
     if (e1) (void)hipEventDestroy(e1);
