@@ -21,13 +21,18 @@ using namespace sph;
 
 // Keys s << 18 | hash per particle, and the sort's first-pass digit counts (RadixSort::first_pass_plan):
 // block b hashes the keys of sort tile xcd_tile(), ITEMS per thread at tile offsets k * 256 + thread,
-// and histograms their low `bits` bits in LDS. The grid is one block per tile.
+// and histograms their low `bits` bits in LDS. The grid is one block per tile. The same launch resets
+// the first `clear` entries of every scene's tables (cpp:309-313; the reference resets them between the
+// reorder and the bounds, and nothing before the bounds kernel reads them): block b takes its share of
+// the S x clear entries, 16 bytes per store when clear is a multiple of 4.
 template <int ITEMS>
 __global__ __launch_bounds__(kThreads) void k_b_hash_counts(const float4* __restrict__ pos,
                                                             const uint32_t* __restrict__ pscene,
                                                             const mfl_params* __restrict__ params, uint32_t N,
                                                             uint32_t* __restrict__ cell, uint32_t* __restrict__ counts,
-                                                            uint32_t ntiles, uint32_t bits)
+                                                            uint32_t ntiles, uint32_t bits,
+                                                            uint32_t* __restrict__ cs_all,
+                                                            uint32_t* __restrict__ ce_all, uint32_t clear, uint32_t S)
 {
     __shared__ uint32_t hist[512];
     const uint32_t radix = 1u << bits, mask = radix - 1u;
@@ -44,6 +49,25 @@ __global__ __launch_bounds__(kThreads) void k_b_hash_counts(const float4* __rest
         const size_t gc = g < N ? g : N - 1;
         q[k] = pos[gc];
         sc[k] = pscene[gc];
+    }
+    if (clear) {  // while the loads are in flight
+        const uint32_t unit = (clear & 3u) == 0u ? 4u : 1u, per = clear / unit;  // units per scene
+        const size_t total = (size_t)S * per, chunk = (total + gridDim.x - 1) / gridDim.x;
+        const size_t lo = (size_t)blockIdx.x * chunk, hi = lo + chunk < total ? lo + chunk : total;
+        size_t u = lo + threadIdx.x;
+        uint32_t s = (uint32_t)(u / per), k = (uint32_t)(u % per);
+        for (; u < hi; u += kThreads) {
+            const size_t at = (size_t)s * kBuckets + (size_t)k * unit;
+            if (unit == 4u) {
+                const uint4 e = make_uint4(kEmpty, kEmpty, kEmpty, kEmpty);
+                *reinterpret_cast<uint4*>(cs_all + at) = e;
+                *reinterpret_cast<uint4*>(ce_all + at) = e;
+            } else {
+                cs_all[at] = kEmpty;
+                ce_all[at] = kEmpty;
+            }
+            for (k += kThreads; k >= per; k -= per) s++;
+        }
     }
 #pragma unroll
     for (int k = 0; k < ITEMS; k++) {
@@ -119,28 +143,6 @@ __global__ __launch_bounds__(kThreads) void k_b_morton_keys(const float4* __rest
                           (spread_bits3((uint32_t)grid_coord(q.z, h) & m) << 2);
     key[g] = (s << (3 * mbits)) | code;
 }
-
-// the reference's per-step reset of the first `clear` entries of each scene's tables (cpp:309-313):
-// scene blockIdx.y, four entries per thread (16-byte stores; the tables start on 16-byte boundaries)
-__global__ __launch_bounds__(kThreads) void k_b_clear(uint32_t* __restrict__ cs_all, uint32_t* __restrict__ ce_all,
-                                                      uint32_t clear)
-{
-    const uint32_t b = (blockIdx.x * kThreads + threadIdx.x) * 4u;
-    if (b >= clear) return;
-    uint32_t* cs = cs_all + (size_t)blockIdx.y * kBuckets;
-    uint32_t* ce = ce_all + (size_t)blockIdx.y * kBuckets;
-    if (b + 4u <= clear) {
-        const uint4 e = make_uint4(kEmpty, kEmpty, kEmpty, kEmpty);
-        *reinterpret_cast<uint4*>(cs + b) = e;
-        *reinterpret_cast<uint4*>(ce + b) = e;
-    } else {
-        for (uint32_t k = b; k < clear; k++) {
-            cs[k] = kEmpty;
-            ce[k] = kEmpty;
-        }
-    }
-}
-
 
 // One group of eight lanes per particle (k_density_grp of the single-scene program) on scene s.
 __device__ inline void batch_density_home(uint32_t s, int gx, int gy, int gz, const float4& pi, uint32_t i, uint32_t slot,
@@ -732,7 +734,7 @@ int FluidBatch::enqueue_step(hipStream_t s)
     if (!sort.first_pass_plan(N_, 0, 18 + (int)scene_bits_, &plan)) return 0;
 #define MFL_B_HASH(IT)                                                                                            \
     hipLaunchKernelGGL(k_b_hash_counts<IT>, dim3(plan.ntiles), b, 0, s, pos, pscene_.as<uint32_t>(), params, N_, cell, \
-                       plan.counts, plan.ntiles, plan.bits)
+                       plan.counts, plan.ntiles, plan.bits, cs_all, ce_all, clear, S_)
     switch (plan.items) {
     case 16: MFL_B_HASH(16); break;
     case 8: MFL_B_HASH(8); break;
@@ -740,12 +742,6 @@ int FluidBatch::enqueue_step(hipStream_t s)
     }
 #undef MFL_B_HASH
     MFL_HIP(hipGetLastError());
-    // the reference resets the tables between the reorder and the bounds (cpp:309-313); nothing before
-    // the bounds reads them, so the reset goes first and the bounds come out of the reorder pass
-    if (clear) {
-        const unsigned bx = (clear + 4u * kThreads - 1u) / (4u * kThreads);
-        hipLaunchKernelGGL(k_b_clear, dim3(bx, S_), b, 0, s, cs_all, ce_all, clear);
-    }
     if (!sort.sort_bits(cell, index, N_, 0, 18 + (int)scene_bits_, s, true, true)) return 0;
     hipLaunchKernelGGL(k_b_slots_bounds, g, b, 0, s, index, N_, inv_.as<uint32_t>(), cell, off, cs_all, ce_all, pos,
                        vel, spos, svel);
