@@ -197,20 +197,23 @@ __global__ __launch_bounds__(kThreads) void k_scatter(const uint32_t* __restrict
     const uint32_t tile = xcd_tile();
     const size_t tile_base = (size_t)tile * TILE;
 
+    // the tile's loads first: they are in flight while the digit bases are read and scanned
+    uint32_t key[ITEMS], val[ITEMS], rank[ITEMS];
+    const uint32_t first = (uint32_t)(wave * ITEMS) * kWave + (uint32_t)lane;  // tile offset of item 0
+    load_tile<ITEMS, IDENTITY_VALS>(kin, vin, n, tile_base, first, key, val);
+
     // thread t owns digits 2t and 2t + 1 in the per-digit loops
     const uint32_t d0 = 2u * (uint32_t)t, d1 = d0 + 1u;
     for (uint32_t i = t; i < kWaves * radix; i += kThreads) s_cnt[(i / radix) * R + i % radix] = 0;
     {
+        const uint32_t c0 = d0 < radix ? counts[(size_t)d0 * ntiles + tile] : 0u;
+        const uint32_t c1 = d1 < radix ? counts[(size_t)d1 * ntiles + tile] : 0u;
         const uint32_t t0 = d0 < radix ? totals[d0] : 0u, t1 = d1 < radix ? totals[d1] : 0u;
         uint32_t all;
         const uint32_t digit_base = block_exclusive_scan(t0 + t1, s_wave, &all);
-        if (d0 < radix) s_base[d0] = digit_base + counts[(size_t)d0 * ntiles + tile];
-        if (d1 < radix) s_base[d1] = digit_base + t0 + counts[(size_t)d1 * ntiles + tile];
+        if (d0 < radix) s_base[d0] = digit_base + c0;
+        if (d1 < radix) s_base[d1] = digit_base + t0 + c1;
     }
-
-    uint32_t key[ITEMS], val[ITEMS], rank[ITEMS];
-    const uint32_t first = (uint32_t)(wave * ITEMS) * kWave + (uint32_t)lane;  // tile offset of item 0
-    load_tile<ITEMS, IDENTITY_VALS>(kin, vin, n, tile_base, first, key, val);
     __syncthreads();
     rank_tile<ITEMS, BITS, R, kMaxDigitBits>(key, n, tile_base, first, shift, bits, s_cnt, rank);
     __syncthreads();
