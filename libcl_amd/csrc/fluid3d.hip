@@ -248,6 +248,7 @@ __global__ __launch_bounds__(kRunThreads) void k_bucket_runs(const uint32_t* __r
     const uint32_t tt0 = min(t * tpt, ntiles), tt1 = min(tt0 + tpt, ntiles);
     uint32_t seg_off = 0;
     uint32_t ct0 = 0, st0 = 0;  // the first tile's count and start, kept from the setup
+    uint32_t pk[4] = {0u, 0u, 0u, 0u}, pv[4] = {0u, 0u, 0u, 0u};  // its first four entries, loaded there too
     auto tiles_each = [&](uint32_t lo, uint32_t hi, auto&& f) {
         uint32_t off = seg_off;
         for (uint32_t tt = tt0; tt < tt1; tt++) {
@@ -256,11 +257,12 @@ __global__ __launch_bounds__(kRunThreads) void k_bucket_runs(const uint32_t* __r
             const uint32_t i_lo = lo > off ? min(lo - off, ct) : 0u, i_hi = hi > off ? min(hi - off, ct) : 0u;
             for (uint32_t i = i_lo; i < i_hi; i += 4) {
                 uint32_t kk[4], vv[4];
+                const bool pre = tt == tt0 && i == 0u;  // entries 0-3 of the first tile: prefetched
 #pragma unroll
                 for (uint32_t r = 0; r < 4; r++) {
                     const size_t j = at + min(i + r, i_hi - 1u);
-                    kk[r] = kin[j];
-                    vv[r] = vin[j];
+                    kk[r] = pre ? pk[r] : kin[j];
+                    vv[r] = pre ? pv[r] : vin[j];
                 }
 #pragma unroll
                 for (uint32_t r = 0; r < 4; r++)
@@ -381,6 +383,14 @@ __global__ __launch_bounds__(kRunThreads) void k_bucket_runs(const uint32_t* __r
             if (tt == tt0) {
                 ct0 = ct;
                 st0 = st;
+                if (ct > 0u) {  // the segment's first entries, in flight during the block scans below
+                    const size_t at = (size_t)tt * tile_keys + st;
+#pragma unroll
+                    for (uint32_t r = 0; r < 4; r++) {
+                        pk[r] = kin[at + min(r, ct - 1u)];
+                        pv[r] = vin[at + min(r, ct - 1u)];
+                    }
+                }
             }
             mine += ct;
             below += st;
